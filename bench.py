@@ -1,0 +1,194 @@
+"""Benchmark: cell-steps/s of the quasi-static STZ step at 1024^2 on one GPU.
+
+Workload: simple shear 1024x1024 cells (x periodic, walls at y = +/-1 moving
+at +/-U), chi = 600 K + 25 K seeded Gaussian field (correlation 3 cells),
+zero initial stress, dt = 0.2 (5000 steps reach T = 1000), V(2,2) cycles to
+the reference's own residual threshold.  A step is one
+shear_sim::step_forward_quasistatic: explicit ENO/STZ update, projection
+source, adaptive multigrid solve, stress correction and boundaries.
+
+The headline value is the mixed-precision form (FP32 V-cycles inside FP64
+iterative refinement); the plain FP64 form runs the same steps from the same
+state, and the two are compared field by field.  The roofline object is for
+the dominant kernel (the top-level smoother colour pass of the mixed form),
+timed live with HIP events on the simulation's own stream.  The CPU baseline
+is the reference's own code (oracle/_ref) on a bounded sample of the same
+workload, or the FP64 restatement when the reference build is absent.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-steps/s at 1024^2: FP32-MG+FP64 refine vs FP64 @1e-8 rel; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def group_err(a, b, ix):
+    return float(np.max(np.abs(a[ix] - b[ix])) / max(np.max(np.abs(b[ix])), 1e-300))
+
+
+def run_form(params, planes, form, dt, warmup, steps, nu, stream):
+    import torch
+    from stz_amd import ShearSim
+    sim = ShearSim(params, form)
+    sim.set_stream(stream.cuda_stream)
+    sim.set_fields(planes)
+    sim.set_vel(np.zeros((params["n"] + 1, params["m"], 2)))
+    sim.time = 0.0
+    sim.set_mg_config(nu1=nu, nu2=nu)
+    sim.step_forward_quasistatic(dt, warmup)
+    sim.avg_iters()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step_forward_quasistatic(dt, steps)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    cycles = sim.avg_iters()
+    info = sim.last_solve()
+    return sim, elapsed, cycles, info
+
+
+def smoother_roofline(sim, nodes, bytes_per_node_sweep, stream, sweeps=50):
+    """average duration of one top-level colour-pass launch, from HIP events
+    recorded on the stream the kernels run on"""
+    import torch
+    sim.mg_smooth(0, 2)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    sim.mg_smooth(0, sweeps)
+    e1.record(stream)
+    e1.synchronize()
+    launches = 4 * sweeps
+    per_launch_s = e0.elapsed_time(e1) * 1e-3 / launches
+    algo_bytes = bytes_per_node_sweep * nodes / 4.0
+    achieved = algo_bytes / per_launch_s / 1e9
+    return per_launch_s, algo_bytes, achieved
+
+
+def cpu_baseline(params, planes, dt, budget_s=20.0, max_steps=40):
+    from oracle import pyoracle
+    kind = "reference" if pyoracle.reference_available() else "port"
+    cls = pyoracle.Reference if kind == "reference" else pyoracle.Oracle
+    sim = cls(params)
+    sim.set_fields(planes)
+    sim.set_vel(np.zeros((params["n"] + 1, params["m"], 2)))
+    sim.time = 0.0
+    sim.step(dt, 1)  # multigrid setup and OpenMP thread tuning
+    steps, t0 = 0, time.perf_counter()
+    while steps < max_steps and time.perf_counter() - t0 < budget_s:
+        sim.step(dt, 1)
+        steps += 1
+    el = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    if kind == "port":
+        cores = 1
+    cells = params["m"] * params["n"]
+    return {"value": cells * steps / el, "unit": "cell-steps/s", "cores": cores, "kind": kind,
+            "sample": "%dx%d simple shear, steps 2..%d of the same run (%.1f s)" %
+                      (params["m"], params["n"], steps + 1, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--nu", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (N=%d requested); the step "
+              "would shard into y-slabs across GPUs, which is not built" % args.gpus)
+        sys.exit(2)
+
+    import torch
+    from stz_amd import problems
+    from stz_amd.initial import simple_shear_state
+
+    N = args.size
+    params = problems.simple_shear(N)
+    dt = problems.T_END / 5000
+    planes = simple_shear_state(params, 600.0, 25.0, 3.0, seed=1)
+    stream = torch.cuda.Stream()
+    cells = N * N
+
+    log("bench: %dx%d, warmup %d, steps %d, V(%d,%d)" % (N, N, args.warmup, args.steps, args.nu, args.nu))
+    res = {}
+    for form in ("fp64", "mixed"):
+        sim, el, cyc, info = run_form(params, planes, form, dt, args.warmup, args.steps, args.nu, stream)
+        res[form] = dict(sim=sim, elapsed=el, cycles=cyc, info=info,
+                         value=cells * args.steps / el, ms=1e3 * el / args.steps)
+        log("bench: %-5s %.3f ms/step, %.3e cell-steps/s, %.2f cycles/solve, last %s" %
+            (form, res[form]["ms"], res[form]["value"], cyc, info))
+
+    fa, fb = res["mixed"]["sim"].get_fields(), res["fp64"]["sim"].get_fields()
+    errs = {"stress": group_err(fa, fb, [2, 3, 4, 5]), "chi": group_err(fa, fb, [6]),
+            "velocity": group_err(fa, fb, [0, 1]), "refmap": group_err(fa, fb, [7, 8])}
+    log("bench: mixed vs fp64 after %d steps: %s" % (args.warmup + args.steps, errs))
+
+    nodes = N * (N + 1)
+    per_launch, algo_bytes, achieved = smoother_roofline(res["mixed"]["sim"], nodes, 24.0, stream)
+    per_launch64, algo_bytes64, achieved64 = smoother_roofline(res["fp64"]["sim"], nodes, 48.0, stream)
+    log("bench: smoother colour pass fp32 %.2f us (%.0f GB/s), fp64 %.2f us (%.0f GB/s)" %
+        (per_launch * 1e6, achieved, per_launch64 * 1e6, achieved64))
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("colour_kernel_float_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(params, planes, dt)
+        log("bench: cpu baseline %s" % cpu)
+
+    mixed, fp64 = res["mixed"], res["fp64"]
+    line = {
+        "metric": METRIC,
+        "value": mixed["value"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "simple shear %dx%d, chi 600K+25K seeded (corr 3 cells), dt %.3g, V(%d,%d), "
+                        "reference residual threshold" % (N, N, dt, args.nu, args.nu),
+            "form": "FP32 V-cycles in FP64 iterative refinement",
+            "fp64_value": fp64["value"],
+            "fp64_ms_per_step": fp64["ms"],
+            "mixed_over_fp64": mixed["value"] / fp64["value"],
+            "mixed_vs_fp64_rel_err": errs,
+            "cycles_per_solve": {"mixed": mixed["cycles"], "fp64": fp64["cycles"]},
+            "smoother_fp64_GBps": achieved64,
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "colour_kernel<float> (top level, one colour)",
+                     "algorithmic_bytes_per_launch": algo_bytes, "avg_launch_us": per_launch * 1e6},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

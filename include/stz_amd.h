@@ -1,0 +1,220 @@
+/*
+ * stz_amd.h -- C ABI of the MI355X (gfx950) quasi-static STZ timestep.
+ *
+ * The reference exposes this path only as C++ classes (no FFI): the
+ * simulation class `shear_sim` (sb/main/shear_sim.hh:20-151), the multigrid
+ * template `tgmg<S,V,M>` (tgmg/tgmg.hh:286-495) with its problem class
+ * `qs_multi` (sb/main/qs_multi.hh:11-57), and the plasticity law
+ * `stz_dynamics` (sb/main/stz_model.hh:9-170).  Every entry point below is
+ * what a thin C++ facade keeping those signatures forwards to; the facade
+ * lives in stz_amd/host/ and the per-entry-point mapping is given in each
+ * comment.  Plain C types only: pointers, sizes, doubles, ints.
+ *
+ * Conventions
+ *  - All functions return an int status (STZ_OK on success).  The reference
+ *    has no status codes: its failures call exit(1) (qs_multi.hh:52,
+ *    tgmg.hh:29-32); the C++ facade maps a non-zero status back to that.
+ *  - Field planes are FP64, row-major, m columns, n+2 rows: plane row r holds
+ *    grid row j=r-1 for j in [-1,n].  This is the reference's `fm` array
+ *    (shear_sim.cc:37, c_field[m*(n+2)] offset by m) split into one plane per
+ *    c_field member (fields.hh:7-52).  Node fields (u,v,X,Y) use rows 0..n,
+ *    cell fields (p,q,s,tau,chi,ddev) use rows -1..n (ghost rows -1 and n).
+ *  - The multigrid solution/velocity `vel` and source `src` are vec2 arrays of
+ *    m*(n+1) nodes, interleaved (x,y) (shear_sim.cc:37-38, vec.hh:6-8).
+ *  - Host pointers are copied synchronously; device pointers returned by the
+ *    library stay valid until stz_sim_destroy.
+ */
+#ifndef STZ_AMD_H
+#define STZ_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    STZ_OK = 0,
+    STZ_ERR_ARG = 1,            /* bad argument (size, null, enum) */
+    STZ_ERR_HIP = 2,            /* a HIP runtime call failed */
+    STZ_ERR_NOT_CONVERGED = 3,  /* V-cycle budget exceeded (tgmg.in.cc:285-288) */
+    STZ_ERR_NAN = 4,            /* residual is NaN (tgmg.in.cc:300) */
+    STZ_ERR_NO_DEVICE = 5,      /* no gfx950 device visible */
+    STZ_ERR_STATE = 6           /* call out of order (e.g. solve before setup) */
+};
+
+/* ---- STZ plasticity law (stz_model.hh) ---------------------------------- */
+enum {
+    STZ_MODEL_LINEAR_ATHERMAL = 0,    /* stz_model.hh:90-170 (used by drivers) */
+    STZ_MODEL_LINEAR = 1,             /* stz_model.hh:50-68 */
+    STZ_MODEL_NONLINEAR = 2,          /* stz_model.hh:70-88 */
+    STZ_MODEL_NONLINEAR_ATHERMAL = 3  /* stz_model.hh:172-207 */
+};
+
+/* Constructor arguments of stz_dynamics (stz_model.hh:22-26, 98-101).
+ * theta is passed as given; the law uses 1/theta.  t_model, t_icpt and
+ * t_grad are the truncation settings of the linear athermal law
+ * (stz_model.hh:93-97); set t_grad = 0.5*Omegaeps0/Delta, t_icpt = 0 for the
+ * reference defaults. */
+typedef struct stz_model_params {
+    int32_t kind;
+    int32_t t_model;
+    double TZ, c0, tau0, kappa, Delta, Omegaeps0, chi_inf, theta, rho;
+    double t_icpt, t_grad;
+} stz_model_params;
+
+/* Constructor arguments of shear_sim (shear_sim.hh:95-99, shear_sim.cc:26-44). */
+typedef struct stz_sim_params {
+    int32_t m, n;                 /* cells in x (periodic) and y (walls) */
+    double ax, bx, ay, by;        /* domain bounds */
+    double mu, K, viscosity;      /* shear/bulk modulus, viscous damping */
+    double chi_diff, chi_len;     /* chi diffusion (constant, Dpl-mediated) */
+    double t_scale, tmult, adapt_fac, u_bdry;
+    stz_model_params stz;
+} stz_sim_params;
+
+/* ---- field planes (fields.hh:7-52 member order) ------------------------- */
+enum {
+    STZ_F_U = 0, STZ_F_V, STZ_F_P, STZ_F_Q, STZ_F_S, STZ_F_TAU, STZ_F_CHI,
+    STZ_F_X, STZ_F_Y, STZ_F_DDEV, STZ_NFIELDS
+};
+
+/* ---- precision of the projection solve --------------------------------- */
+enum {
+    STZ_PREC_FP64 = 0,   /* all multigrid work in FP64 */
+    STZ_PREC_MIXED = 1   /* FP32 V-cycles inside FP64 iterative refinement */
+};
+
+/* Multigrid settings.  Defaults reproduce the reference's solve_v_cycle
+ * (tgmg.hh:439): one pre/post sweep, 10+10 bottom sweeps, the adaptive
+ * V-cycle predictor of tgmg_predict.hh:6-33 and the tolerance
+ * tgmg_accuracy(2*(tmufac1+tmufac2),1e4) of qs_multi.cc:14 scaled by
+ * tol_mult (mean-square residual threshold = tol_mult * that value). */
+typedef struct stz_mg_config {
+    int32_t nu1, nu2;            /* pre/post smoothing sweeps per level */
+    int32_t bottom_sweeps;       /* half the sweeps on the coarsest level */
+    int32_t extra_cycles;        /* cycles after convergence (reference: 1) */
+    int32_t predict_mult, predict_decay, predict_init, predict_max_iters;
+    double tol_mult;             /* multiplier on the mean-square tolerance */
+    int32_t max_refine;          /* mixed: cap on FP64 refinement iterations */
+    int32_t reserved[7];
+} stz_mg_config;
+
+/* Result of the last projection solve. */
+typedef struct stz_solve_info {
+    int32_t status;              /* STZ_OK / STZ_ERR_NOT_CONVERGED / STZ_ERR_NAN */
+    int32_t cycles;              /* V-cycles counted as tgmg_predict does */
+    int32_t total_cycles;        /* V-cycles actually run (incl. extra) */
+    int32_t refinements;         /* FP64 residual evaluations (mixed mode) */
+    double residual;             /* mean-square residual (tgmg.hh:146) */
+    double tolerance;            /* threshold it was compared against */
+} stz_solve_info;
+
+typedef struct stz_sim stz_sim;   /* opaque */
+
+/* Last error message of the calling thread ("" if none). */
+const char *stz_last_error(void);
+
+/* Number of visible devices (0 without a GPU; never fails). */
+int stz_device_count(int *count);
+
+/* shear_sim::shear_sim (shear_sim.cc:26-44) + qs_multi/tgmg construction
+ * (qs_multi.cc:9-15, tgmg.in.cc:10-30).  Fields start zeroed. */
+int stz_sim_create(const stz_sim_params *p, int precision, stz_sim **out);
+/* shear_sim::~shear_sim (shear_sim.cc:47-52). */
+int stz_sim_destroy(stz_sim *sim);
+
+/* Run all following work on this HIP stream (hipStream_t passed as void*;
+ * NULL = the library's own stream). */
+int stz_sim_set_stream(stz_sim *sim, void *hip_stream);
+int stz_sim_synchronize(stz_sim *sim);
+
+/* Field access (public `fm`, shear_sim.hh:79).  Plane = m*(n+2) doubles. */
+int stz_sim_upload_field(stz_sim *sim, int field, const double *host_plane);
+int stz_sim_download_field(stz_sim *sim, int field, double *host_plane);
+int stz_sim_field_device_ptr(stz_sim *sim, int field, double **dev_plane);
+/* Public `vel` / `src` (shear_sim.hh:82-85): m*(n+1) interleaved vec2. */
+int stz_sim_upload_vel(stz_sim *sim, const double *host_vec2);
+int stz_sim_download_vel(stz_sim *sim, double *host_vec2);
+int stz_sim_download_src(stz_sim *sim, double *host_vec2);
+int stz_sim_vel_device_ptr(stz_sim *sim, double **dev_vec2);
+
+/* Public `time` (shear_sim.hh:87) and `chi_inf` (shear_sim.hh:92). */
+int stz_sim_set_time(stz_sim *sim, double t);
+int stz_sim_get_time(stz_sim *sim, double *t);
+int stz_sim_set_chi_inf(stz_sim *sim, double chi_inf);
+
+/* shear_sim::init_fields (shear_sim.cc:60-114), computed on the device. */
+int stz_sim_init_fields(stz_sim *sim, int chi_case, double tem_base, double tem_delta);
+
+/* shear_sim::step_forward_quasistatic (shear_sim.cc:280-291), nsteps times.
+ * Stops at the first failed solve and returns its status. */
+int stz_sim_step_quasistatic(stz_sim *sim, double dt, int nsteps);
+/* shear_sim::advection_step (shear_sim.cc:296-361), chi_diffusion included. */
+int stz_sim_advection_step(stz_sim *sim, double dt);
+/* shear_sim::projection_step (shear_sim.cc:391-438). */
+int stz_sim_projection_step(stz_sim *sim, double dt);
+/* shear_sim::set_boundaries (shear_sim.cc:552-581). */
+int stz_sim_set_boundaries(stz_sim *sim);
+
+/* Multigrid controls (tgmg.hh:331-441, tgmg_predict.hh). */
+int stz_sim_get_mg_config(stz_sim *sim, stz_mg_config *cfg);
+int stz_sim_set_mg_config(stz_sim *sim, const stz_mg_config *cfg);
+int stz_sim_last_solve(stz_sim *sim, stz_solve_info *info);
+/* shear_sim::avg_iters -> tgmg_predict::avg_iters (tgmg_predict.hh:28-33):
+ * mean V-cycles per solve since the last call; resets the counters. */
+int stz_sim_avg_iters(stz_sim *sim, double *avg);
+/* Hierarchy: number of levels (top included) and their node dimensions
+ * (tgmg.in.cc:10-30); dims holds 2*max_levels ints. */
+int stz_sim_mg_hierarchy(stz_sim *sim, int max_levels, int *nlevels, int *dims);
+
+/* qs_multi::init -> tgmg::setup (qs_multi.cc:31-54, tgmg.hh:331-334):
+ * rebuild the operator hierarchy for viscosity/dt = viscdt if it changed. */
+int stz_mg_setup(stz_sim *sim, double viscdt);
+/* tgmg::solve_v_cycle(tgmg_predict&) on the current src/vel
+ * (tgmg.in.cc:266-301): adaptive V-cycles until the tolerance is met. */
+int stz_mg_solve(stz_sim *sim, stz_solve_info *info);
+/* One V-cycle (tgmg.in.cc:313-328) on the current src/vel. */
+int stz_mg_v_cycle(stz_sim *sim);
+/* tgmg_base::l2_error (tgmg.hh:146): mean-square residual of src - A vel. */
+int stz_mg_l2_error(stz_sim *sim, double *err);
+/* tgmg_base::gauss_seidel (tgmg.in.cc:127-165): `sweeps` smoothing sweeps
+ * on hierarchy level `level` (0 = top) of the hierarchy the solve uses (the
+ * FP32 one in the mixed form).  Asynchronous on the simulation's stream, so
+ * a caller can bracket it with its own events. */
+int stz_mg_smooth(stz_sim *sim, int level, int sweeps);
+/* Upload a source array (m*(n+1) vec2) for a standalone solve. */
+int stz_mg_upload_src(stz_sim *sim, const double *host_vec2);
+
+/* Per-cell decision bits of the last advection step (shear_sim.cc:296-349,
+ * 513-548), one uint32 per cell (i,j), j in [0,n), i fastest.  "ENO" bits are
+ * set when eno2 (shear_sim.cc:653-655) took the one-sided second-order
+ * branch 3p1-4p2+p3 rather than the centred p0-p2. */
+enum {
+    STZ_D_NODE_UPX = 1u << 0,    /* j>0: uc=f.u > 0 (x upwind side) */
+    STZ_D_NODE_UPY = 1u << 1,    /* j>0: vc=f.v > 0 */
+    STZ_D_NODE_ENO_XX = 1u << 2, /* eno2 branch, X along x */
+    STZ_D_NODE_ENO_YX = 1u << 3, /* eno2 branch, Y along x */
+    STZ_D_NODE_ENO_XY = 1u << 4, /* eno2 branch, X along y */
+    STZ_D_NODE_ENO_YY = 1u << 5, /* eno2 branch, Y along y */
+    STZ_D_NODE_ONESIDED = 1u << 6, /* y derivative first-order one-sided (wall) */
+    STZ_D_CELL_UPX = 1u << 7,    /* staggered uc > 0 */
+    STZ_D_CELL_UPY = 1u << 8,    /* staggered vc > 0 */
+    STZ_D_CELL_ENO_X0 = 9,       /* bit index: eno2 branch along x, p,q,s,tau,chi -> bits 9..13 */
+    STZ_D_CELL_ENO_Y0 = 14,      /* along y -> bits 14..18 */
+    STZ_D_CELL_ONESIDED = 1u << 19, /* staggered y derivative one-sided (wall) */
+    STZ_D_YIELD = 1u << 20,      /* Dplastic took the sbar>=1 branch on entry */
+    STZ_D_SMALLDEV = 1u << 21,   /* sbar < 1e-12 cut-off (shear_sim.cc:522) */
+    STZ_D_SUBSTEP_SHIFT = 24     /* bits 24..31: plastic loop trips, saturating */
+};
+
+/* Recording is off by default (it costs one extra store per cell). */
+int stz_sim_record_decisions(stz_sim *sim, int enable);
+int stz_sim_download_decisions(stz_sim *sim, uint32_t *host_cells);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STZ_AMD_H */

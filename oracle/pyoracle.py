@@ -1,0 +1,331 @@
+"""ctypes access to the CPU checkers -- TEST INFRASTRUCTURE ONLY.
+
+Two libraries, both built by oracle/Makefile:
+  * ``liboracle.so``      -- this repository's FP64 restatement of the
+    reference step (oracle/stz_oracle.c);
+  * ``_ref/libstzref.so`` -- the reference's own C++ sources compiled with
+    oracle/ref_harness.cc (only where the reference tree was available at
+    build time; it is optional everywhere).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use
+this module, and only as the checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libstzref.so")
+
+NFIELDS = 10
+FIELD_NAMES = ("u", "v", "p", "q", "s", "tau", "chi", "X", "Y", "ddev")
+
+_MODEL_DOUBLES = ("TZ", "c0", "tau0", "kappa", "Delta", "Omegaeps0", "chi_inf",
+                  "theta", "rho", "t_icpt", "t_grad")
+_SIM_DOUBLES = ("ax", "bx", "ay", "by", "mu", "K", "viscosity", "chi_diff",
+                "chi_len", "t_scale", "tmult", "adapt_fac", "u_bdry")
+
+
+class ModelParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("t_model", C.c_int32)] + \
+        [(k, C.c_double) for k in _MODEL_DOUBLES]
+
+
+class SimParams(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32)] + \
+        [(k, C.c_double) for k in _SIM_DOUBLES] + [("stz", ModelParams)]
+
+
+def make_params(d):
+    """Fill SimParams from a flat dict (keys of SimParams plus 'stz' dict)."""
+    p = SimParams()
+    p.m, p.n = int(d["m"]), int(d["n"])
+    for k in _SIM_DOUBLES:
+        setattr(p, k, float(d[k]))
+    mp = d["stz"]
+    p.stz.kind = int(mp.get("kind", 0))
+    p.stz.t_model = int(mp.get("t_model", -1))
+    for k in _MODEL_DOUBLES:
+        setattr(p.stz, k, float(mp[k]))
+    return p
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def _bind(lib, prefix):
+    vp, d, i = C.c_void_p, C.c_double, C.c_int
+    sig = {
+        "create": (vp, [C.POINTER(SimParams)]),
+        "destroy": (None, [vp]),
+        "set_field": (None, [vp, i, _dp]),
+        "get_field": (None, [vp, i, _dp]),
+        "set_vel": (None, [vp, _dp]),
+        "get_vel": (None, [vp, _dp]),
+        "set_src": (None, [vp, _dp]),
+        "get_src": (None, [vp, _dp]),
+        "set_time": (None, [vp, d]),
+        "get_time": (d, [vp]),
+        "set_chi_inf": (None, [vp, d]),
+        "set_tol_mult": (None, [vp, d]),
+        "tolerance": (d, [vp]),
+        "init_fields": (None, [vp, i, d, d]),
+        "initialize_random": (None, [vp, d, d, d, C.c_uint]),
+        "advection_step": (None, [vp, d]),
+        "mg_setup": (None, [vp, d]),
+        "mg_v_cycle": (None, [vp]),
+        "mg_l2_error": (d, [vp]),
+        "mg_levels": (i, [vp, _ip, i]),
+        "mg_stencil": (i, [vp, i, _dp]),
+        "avg_iters": (d, [vp]),
+        "tractions": (None, [vp, _dp]),
+        "qs_measure": (None, [vp, i, C.POINTER(d), C.POINTER(d)]),
+    }
+    out = {}
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+        out[name] = fn
+    return out
+
+
+class _Sim:
+    """Common driver over either checker library."""
+
+    def __init__(self, params):
+        self.params = params
+        self._p = make_params(params)
+        self.m, self.n = self._p.m, self._p.n
+        self._h = self._f["create"](C.byref(self._p))
+        if not self._h:
+            raise RuntimeError("checker create failed")
+
+    def close(self):
+        if self._h:
+            self._f["destroy"](self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- fields
+    @property
+    def plane_shape(self):
+        return (self.n + 2, self.m)
+
+    def set_field(self, k, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.plane_shape)
+        self._f["set_field"](self._h, int(k), a)
+
+    def get_field(self, k):
+        a = np.empty(self.plane_shape, dtype=np.float64)
+        self._f["get_field"](self._h, int(k), a)
+        return a
+
+    def set_fields(self, planes):
+        for k in range(NFIELDS):
+            self.set_field(k, planes[k])
+
+    def get_fields(self):
+        return np.stack([self.get_field(k) for k in range(NFIELDS)])
+
+    def set_vel(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.n + 1, self.m, 2)
+        self._f["set_vel"](self._h, a)
+
+    def get_vel(self):
+        a = np.empty((self.n + 1, self.m, 2), dtype=np.float64)
+        self._f["get_vel"](self._h, a)
+        return a
+
+    def set_src(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.n + 1, self.m, 2)
+        self._f["set_src"](self._h, a)
+
+    def get_src(self):
+        a = np.empty((self.n + 1, self.m, 2), dtype=np.float64)
+        self._f["get_src"](self._h, a)
+        return a
+
+    time = property(lambda self: self._f["get_time"](self._h),
+                    lambda self, t: self._f["set_time"](self._h, float(t)))
+
+    def set_chi_inf(self, c):
+        self._f["set_chi_inf"](self._h, float(c))
+
+    def set_tol_mult(self, mult):
+        self._f["set_tol_mult"](self._h, float(mult))
+
+    def tolerance(self):
+        return self._f["tolerance"](self._h)
+
+    def init_fields(self, case, tem_base=600.0, tem_delta=200.0):
+        self._f["init_fields"](self._h, int(case), float(tem_base), float(tem_delta))
+
+    def initialize_random(self, l, tem_avg, tem_stdev, seed):
+        self._f["initialize_random"](self._h, float(l), float(tem_avg), float(tem_stdev), int(seed))
+
+    def advection_step(self, dt):
+        self._f["advection_step"](self._h, float(dt))
+
+    def mg_setup(self, viscdt):
+        self._f["mg_setup"](self._h, float(viscdt))
+
+    def mg_v_cycle(self):
+        self._f["mg_v_cycle"](self._h)
+
+    def mg_l2_error(self):
+        return self._f["mg_l2_error"](self._h)
+
+    def mg_levels(self):
+        dims = np.zeros(2 * 32, dtype=np.int32)
+        nl = self._f["mg_levels"](self._h, dims, 32)
+        return [(int(dims[2 * k]), int(dims[2 * k + 1])) for k in range(nl)]
+
+    def mg_stencil(self, level):
+        m, n = self.mg_levels()[level]
+        out = np.empty((n, m, 10, 4), dtype=np.float64)
+        if self._f["mg_stencil"](self._h, int(level), out) != 0:
+            raise ValueError("bad level")
+        return out
+
+    def avg_iters(self):
+        return self._f["avg_iters"](self._h)
+
+    def tractions(self):
+        o = np.zeros(4)
+        self._f["tractions"](self._h, o)
+        return o
+
+    def qs_measure(self, staggered=False):
+        q, mq = C.c_double(), C.c_double()
+        self._f["qs_measure"](self._h, 1 if staggered else 0, C.byref(q), C.byref(mq))
+        return q.value, mq.value
+
+
+class Oracle(_Sim):
+    """The FP64 restatement (oracle/stz_oracle.c)."""
+    _lib = None
+
+    def __init__(self, params):
+        if Oracle._lib is None:
+            lib = C.CDLL(ORACLE_SO)
+            Oracle._lib = lib
+            f = _bind(lib, "orc_")
+            for name, res, args in (
+                ("step_quasistatic", C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+                ("projection_step", C.c_int, [C.c_void_p, C.c_double]),
+                ("set_boundaries", None, [C.c_void_p]),
+                ("mg_solve", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+                ("set_decisions", None, [C.c_void_p, C.c_int]),
+                ("get_decisions", None, [C.c_void_p, _up]),
+                ("last_cycles", C.c_int, [C.c_void_p]),
+                ("eno2", C.c_double, [C.c_double] * 4),
+                ("dplastic", C.c_double, [C.POINTER(ModelParams), C.c_double, C.c_double,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+                ("plastic_term", C.c_double, [C.c_void_p, C.c_double, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_int)]),
+            ):
+                fn = getattr(lib, "orc_" + name)
+                fn.restype = res
+                fn.argtypes = args
+                f[name] = fn
+            Oracle._fns = f
+        self._f = Oracle._fns
+        super().__init__(params)
+
+    def step(self, dt, nsteps=1):
+        return self._f["step_quasistatic"](self._h, float(dt), int(nsteps))
+
+    def projection_step(self, dt):
+        return self._f["projection_step"](self._h, float(dt))
+
+    def set_boundaries(self):
+        self._f["set_boundaries"](self._h)
+
+    def mg_solve(self):
+        c, r = C.c_int(), C.c_double()
+        st = self._f["mg_solve"](self._h, C.byref(c), C.byref(r))
+        return st, c.value, r.value
+
+    def record_decisions(self, on=True):
+        self._f["set_decisions"](self._h, 1 if on else 0)
+
+    def get_decisions(self):
+        a = np.empty((self.n, self.m), dtype=np.uint32)
+        self._f["get_decisions"](self._h, a)
+        return a
+
+    def last_cycles(self):
+        return self._f["last_cycles"](self._h)
+
+    @staticmethod
+    def eno2(p0, p1, p2, p3):
+        lib = C.CDLL(ORACLE_SO)
+        lib.orc_eno2.restype = C.c_double
+        lib.orc_eno2.argtypes = [C.c_double] * 4
+        return lib.orc_eno2(float(p0), float(p1), float(p2), float(p3))
+
+    def dplastic(self, sbar, chi):
+        d1, d2 = C.c_double(), C.c_double()
+        r = self._f["dplastic"](C.byref(self._p.stz), float(sbar), float(chi), C.byref(d1), C.byref(d2))
+        return r, d1.value, d2.value
+
+    def plastic_term(self, sbar, chi, dt):
+        c, dd, t = C.c_double(chi), C.c_double(), C.c_int()
+        r = self._f["plastic_term"](self._h, float(sbar), C.byref(c), C.byref(dd), float(dt), C.byref(t))
+        return r, c.value, dd.value, t.value
+
+
+def reference_available():
+    return os.path.exists(REF_SO)
+
+
+class Reference(_Sim):
+    """The reference's own compiled code (oracle/_ref/libstzref.so)."""
+    _fns = None
+
+    def __init__(self, params):
+        if Reference._fns is None:
+            lib = C.CDLL(REF_SO)
+            f = _bind(lib, "ref_")
+            for name, res, args in (
+                ("step", None, [C.c_void_p, C.c_double, C.c_int]),
+                ("projection_step", None, [C.c_void_p, C.c_double]),
+                ("mg_solve", C.c_int, [C.c_void_p]),
+                ("mg_lim", C.c_int, [C.c_void_p]),
+                ("set_threads", None, [C.c_int]),
+            ):
+                fn = getattr(lib, "ref_" + name)
+                fn.restype = res
+                fn.argtypes = args
+                f[name] = fn
+            Reference._fns = f
+        self._f = Reference._fns
+        super().__init__(params)
+
+    def step(self, dt, nsteps=1):
+        self._f["step"](self._h, float(dt), int(nsteps))
+        return 0
+
+    def projection_step(self, dt):
+        self._f["projection_step"](self._h, float(dt))
+        return 0
+
+    def mg_solve(self):
+        return self._f["mg_solve"](self._h)
+
+    def mg_lim(self):
+        return self._f["mg_lim"](self._h)
+
+    @staticmethod
+    def set_threads(n):
+        Reference._fns["set_threads"](int(n))

@@ -1,0 +1,534 @@
+// explicit.hip -- gfx950 kernels for the explicit part of the quasi-static STZ
+// step (shear_sim::advection_step, chi_diffusion, the projection source and
+// correction, set_boundaries, init_fields).
+//
+// Layout: structure-of-arrays FP64 planes, m columns by n+2 rows (rows -1..n),
+// one thread per cell, 64 threads of a wavefront along a row so every plane
+// access is a coalesced 512-byte row segment.  The advection kernel reads the
+// old planes and writes a second plane set, which replaces the reference's
+// change slots cp..cY (fields.hh:28-51) and its separate update loop
+// (shear_sim.cc:356-360): the state after the kernel is the same, with seven
+// fewer plane reads and writes.
+//
+// Every arithmetic expression keeps the reference's evaluation order, and FMA
+// contraction is off in this file, so results agree bit for bit with the
+// reference wherever no transcendental function is involved (the ENO, upwind
+// and yield decisions in particular); exp/tanh/cosh differ from glibc by at
+// most a few ulp.
+#include "device.h"
+
+#pragma clang fp contract(off)
+
+namespace stz {
+
+#define PL(F, i, j) (F)[(i) + m * ((j) + 1)]
+
+// ---------------------------------------------------------------------------
+// STZ law (stz_model.hh:9-207)
+// ---------------------------------------------------------------------------
+
+__device__ static double dpl_common(const stz_model_params &mp, double sbar, double chi, double &dchi1,
+                                    double &dchi2, double sC1, double sC2, double sC, double sT) {
+    // stz_model.hh:30-47
+    const double s0 = 1;
+    double sbart = sbar > 1e-12 ? sbar : 1e-12, bfac = exp(-1 / chi),
+           pM = 1 + sbart * sT / s0 + mp.rho / (2 * sC2),
+           M = s0 / (2 * sbart) * (pM - sqrt(pM * pM - 4 * sbart * sT / s0)),
+           Dpl = bfac * sC * (sT - M) / mp.tau0,
+           Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
+    bfac /= mp.tau0 * mp.c0;
+    dchi1 = bfac * Gamma;
+    dchi2 = bfac * mp.kappa * sC1 * mp.rho;
+    return Dpl;
+}
+
+__device__ static double dplastic(const stz_model_params &mp, double thetainv, double sbar, double chi,
+                                  double &dchi1, double &dchi2) {
+    const double s0 = 1;
+    switch (mp.kind) {
+    case STZ_MODEL_LINEAR: {  // stz_model.hh:58-67
+        double sC1 = exp(-mp.Delta * thetainv);
+        double sC2 = cosh(mp.Omegaeps0 * sbar * thetainv), sC = sC1 * sC2,
+               sT = tanh(mp.Omegaeps0 * sbar * thetainv);
+        return dpl_common(mp, sbar, chi, dchi1, dchi2, sC1, sC2, sC, sT);
+    }
+    case STZ_MODEL_NONLINEAR: {  // stz_model.hh:76-87
+        double x = mp.Omegaeps0 * sbar / mp.Delta, sC1 = exp(-mp.Delta * thetainv),
+               sC = exp(-mp.Delta * cosh(x) * thetainv) * cosh(mp.Delta * sinh(x) * thetainv), sC2 = sC / sC1,
+               sT = tanh(mp.Delta * sinh(x) * thetainv);
+        return dpl_common(mp, sbar, chi, dchi1, dchi2, sC1, sC2, sC, sT);
+    }
+    case STZ_MODEL_NONLINEAR_ATHERMAL: {  // stz_model.hh:179-206
+        dchi2 = 0;
+        if (sbar < 1) { dchi1 = 0; return 0; }
+        double x = mp.Omegaeps0 * sbar / mp.Delta,
+               sC = 0.5 * (exp(-mp.Delta * thetainv * exp(x)) + exp(-mp.Delta * thetainv * exp(-x))),
+               sT = tanh(mp.Delta * sinh(x) * thetainv);
+        double bfac = exp(-1 / chi), Dpl = bfac * sC * (sT - s0 / sbar) / mp.tau0,
+               Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
+        dchi1 = bfac * Gamma / (mp.tau0 * mp.c0);
+        return Dpl;
+    }
+    default: {  // linear athermal, stz_model.hh:103-151
+        double tmp, sC, sT;
+        dchi2 = 0;
+        if (sbar < 1) { dchi1 = 0; return 0; }
+        if (mp.t_model == -1 || mp.Delta >= mp.Omegaeps0 * sbar)
+            sC = 0.5 * (exp((mp.Omegaeps0 * sbar - mp.Delta) * thetainv) +
+                        exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv));
+        else switch (mp.t_model) {
+            case 0: sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) + 1); break;
+            case 1:
+                tmp = mp.Omegaeps0 * sbar / mp.Delta;
+                sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) + sqrt(0.5 * (1 + tmp * tmp)));
+                break;
+            case 2:
+                tmp = mp.Omegaeps0 * sbar / mp.Delta;
+                sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) + (0.5 * (1 + tmp * tmp)));
+                break;
+            case 3: sC = mp.t_icpt + sbar * mp.t_grad; break;
+            case 4:
+                sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) +
+                            (mp.Omegaeps0 * sbar - mp.Delta) * thetainv + 1);
+                break;
+            default: sC = 0;
+        }
+        sT = mp.t_model != 3 ? tanh(mp.Omegaeps0 * sbar * thetainv) : 1;
+        double bfac = exp(-1 / chi), Dpl = bfac * sC * (sT - s0 / sbar) / mp.tau0,
+               Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
+        dchi1 = bfac * Gamma / (mp.tau0 * mp.c0);
+        return Dpl;
+    }
+    }
+}
+
+// shear_sim.cc:513-548 (adaptive_plastic_term); trips counts loop iterations
+__device__ static double plastic_term(const SimConst &c, double sbar, double &chiv, double &ddev, double dt,
+                                      int &trips) {
+    const double small_dev_cutoff = 1e-12;
+    const double theta = 0;
+    bool adapt = true;
+    int t = 0;
+    double dtl = dt, osbar = sbar, dplas, dchi1, dchi2, adt;
+    if (sbar < small_dev_cutoff) { ddev = 0; trips = 0; return 0; }
+    do {
+        dplas = 2 * c.t_scale * c.mu * dplastic(c.stz, c.thetainv, sbar, chiv, dchi1, dchi2);
+        if (fabs(dtl * dplas) > c.adapt_fac) {
+            adt = c.adapt_fac / fabs(dplas);
+            dtl -= adt;
+        } else {
+            adapt = false;
+            adt = dtl;
+        }
+        sbar -= adt * dplas;
+        chiv += (dchi1 * (c.chi_inf - chiv) + dchi2 * (theta - chiv)) * c.t_scale * adt;
+        t++;
+    } while (adapt);
+    trips = t;
+    ddev = osbar - sbar;
+    return ddev / osbar;
+}
+
+// shear_sim.cc:653-655; br = 1 when the one-sided second-order branch is taken
+__device__ __forceinline__ static double eno2(double p0, double p1, double p2, double p3, uint32_t &br) {
+    bool b = fabs(p0 - 2 * p1 + p2) > fabs(p1 - 2 * p2 + p3);
+    br = b ? 1u : 0u;
+    return b ? 3 * p1 - 4 * p2 + p3 : p0 - p2;
+}
+
+// ---------------------------------------------------------------------------
+// advection + plasticity: shear_sim.cc:296-349 (+ update, :356-360)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) advect_kernel(SimConst c, FieldPtrs o, FieldPtrs w, double dt,
+                                                     uint32_t *dec) {
+    const int m = c.m, n = c.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= m || j >= n) return;
+    const double hx = 0.5 * c.xsp * dt, hy = 0.5 * c.ysp * dt, lx = c.lx;
+    const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
+    const int ill = i <= 1 ? i + m - 2 : i - 2, irr = i >= m - 2 ? i + 2 - m : i + 2;
+    uint32_t bits = 0, br;
+    double uc, vc, two_omega, dtmp;
+
+    if (j > 0) {
+        double Xx, Xy, Yx, Yy;
+        uc = PL(o.u, i, j);
+        vc = PL(o.v, i, j);
+        const double X0 = PL(o.X, i, j), Y0 = PL(o.Y, i, j);
+        if (uc > 0) {
+            bits |= STZ_D_NODE_UPX;
+            Xx = hx * eno2(PL(o.X, ir, j) + (i == m - 1 ? lx : 0), X0, PL(o.X, il, j) + (i == 0 ? -lx : 0),
+                           PL(o.X, ill, j) + (i <= 1 ? -lx : 0), br);
+            bits |= br ? STZ_D_NODE_ENO_XX : 0;
+            Yx = hx * eno2(PL(o.Y, ir, j), Y0, PL(o.Y, il, j), PL(o.Y, ill, j), br);
+            bits |= br ? STZ_D_NODE_ENO_YX : 0;
+        } else {
+            Xx = -hx * eno2(PL(o.X, il, j) + (i == 0 ? -lx : 0), X0, PL(o.X, ir, j) + (i == m - 1 ? lx : 0),
+                            PL(o.X, irr, j) + (i >= m - 2 ? lx : 0), br);
+            bits |= br ? STZ_D_NODE_ENO_XX : 0;
+            Yx = -hx * eno2(PL(o.Y, il, j), Y0, PL(o.Y, ir, j), PL(o.Y, irr, j), br);
+            bits |= br ? STZ_D_NODE_ENO_YX : 0;
+        }
+        if (vc > 0) {
+            bits |= STZ_D_NODE_UPY;
+            if (j > 1) {
+                Xy = hy * eno2(PL(o.X, i, j + 1), X0, PL(o.X, i, j - 1), PL(o.X, i, j - 2), br);
+                bits |= br ? STZ_D_NODE_ENO_XY : 0;
+                Yy = hy * eno2(PL(o.Y, i, j + 1), Y0, PL(o.Y, i, j - 1), PL(o.Y, i, j - 2), br);
+                bits |= br ? STZ_D_NODE_ENO_YY : 0;
+            } else {
+                const double hs = hy * 2;
+                Xy = hs * (X0 - PL(o.X, i, j - 1));
+                Yy = hs * (Y0 - PL(o.Y, i, j - 1));
+                bits |= STZ_D_NODE_ONESIDED;
+            }
+        } else {
+            if (j < n - 1) {
+                Xy = -hy * eno2(PL(o.X, i, j - 1), X0, PL(o.X, i, j + 1), PL(o.X, i, j + 2), br);
+                bits |= br ? STZ_D_NODE_ENO_XY : 0;
+                Yy = -hy * eno2(PL(o.Y, i, j - 1), Y0, PL(o.Y, i, j + 1), PL(o.Y, i, j + 2), br);
+                bits |= br ? STZ_D_NODE_ENO_YY : 0;
+            } else {
+                const double hs = -hy * 2;
+                Xy = hs * (X0 - PL(o.X, i, j + 1));
+                Yy = hs * (Y0 - PL(o.Y, i, j + 1));
+                bits |= STZ_D_NODE_ONESIDED;
+            }
+        }
+        // X += cX, Y += cY (fields.hh:68-71)
+        PL(w.X, i, j) = X0 + (-uc * Xx - vc * Xy);
+        PL(w.Y, i, j) = Y0 + (-uc * Yx - vc * Yy);
+    } else {
+        // row 0 reference map is left to set_boundaries; keep it, and the
+        // ghost row -1, in the new plane set
+        PL(w.X, i, 0) = PL(o.X, i, 0);
+        PL(w.Y, i, 0) = PL(o.Y, i, 0);
+        PL(w.p, i, -1) = PL(o.p, i, -1);
+        PL(w.q, i, -1) = PL(o.q, i, -1);
+        PL(w.s, i, -1) = PL(o.s, i, -1);
+        PL(w.tau, i, -1) = PL(o.tau, i, -1);
+        PL(w.chi, i, -1) = PL(o.chi, i, -1);
+    }
+    if (j == n - 1) {
+        PL(w.X, i, n) = PL(o.X, i, n);
+        PL(w.Y, i, n) = PL(o.Y, i, n);
+        PL(w.p, i, n) = PL(o.p, i, n);
+        PL(w.q, i, n) = PL(o.q, i, n);
+        PL(w.s, i, n) = PL(o.s, i, n);
+        PL(w.tau, i, n) = PL(o.tau, i, n);
+        PL(w.chi, i, n) = PL(o.chi, i, n);
+    }
+
+    // staggered velocity and spin, shear_sim.cc:330-333
+    {
+        const double u00 = PL(o.u, i, j), u10 = PL(o.u, ir, j), u01 = PL(o.u, i, j + 1), u11 = PL(o.u, ir, j + 1);
+        const double v00 = PL(o.v, i, j), v10 = PL(o.v, ir, j), v01 = PL(o.v, i, j + 1), v11 = PL(o.v, ir, j + 1);
+        uc = 0.25 * (u00 + u10 + u01 + u11);
+        vc = 0.25 * (v00 + v10 + v01 + v11);
+        two_omega = hx * (-v00 + v10 - v01 + v11) - hy * (-u00 - u10 + u01 + u11);
+    }
+
+    // ENO derivatives of p,q,s,tau,chi, shear_sim.cc:336-338
+    const double *F[5] = {o.p, o.q, o.s, o.tau, o.chi};
+    double f0[5], f1[5], f2[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) f0[k] = PL(F[k], i, j);
+    if (uc > 0) {
+        bits |= STZ_D_CELL_UPX;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            f1[k] = hx * eno2(PL(F[k], ir, j), f0[k], PL(F[k], il, j), PL(F[k], ill, j), br);
+            bits |= br << (STZ_D_CELL_ENO_X0 + k);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            f1[k] = -hx * eno2(PL(F[k], il, j), f0[k], PL(F[k], ir, j), PL(F[k], irr, j), br);
+            bits |= br << (STZ_D_CELL_ENO_X0 + k);
+        }
+    }
+    if (vc > 0) {
+        bits |= STZ_D_CELL_UPY;
+        if (j > 0) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                f2[k] = hy * eno2(PL(F[k], i, j + 1), f0[k], PL(F[k], i, j - 1), PL(F[k], i, j - 2), br);
+                bits |= br << (STZ_D_CELL_ENO_Y0 + k);
+            }
+        } else {
+            const double hs = hy * 2;
+#pragma unroll
+            for (int k = 0; k < 5; k++) f2[k] = hs * (f0[k] - PL(F[k], i, j - 1));
+            bits |= STZ_D_CELL_ONESIDED;
+        }
+    } else {
+        if (j < n - 1) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                f2[k] = -hy * eno2(PL(F[k], i, j - 1), f0[k], PL(F[k], i, j + 1), PL(F[k], i, j + 2), br);
+                bits |= br << (STZ_D_CELL_ENO_Y0 + k);
+            }
+        } else {
+            const double hs = -hy * 2;
+#pragma unroll
+            for (int k = 0; k < 5; k++) f2[k] = hs * (f0[k] - PL(F[k], i, j + 1));
+            bits |= STZ_D_CELL_ONESIDED;
+        }
+    }
+
+    // chi advection and plasticity, shear_sim.cc:341-342; fields.hh:85-93
+    double cchi = f0[4] - uc * f1[4] - vc * f2[4];
+    const double sbar = sqrt(3 * f0[1] * f0[1] + f0[2] * f0[2] + f0[3] * f0[3]);
+    double ddev;
+    int trips;
+    dtmp = plastic_term(c, sbar, cchi, ddev, dt, trips);
+    PL(w.ddev, i, j) = ddev;
+    PL(w.chi, i, j) = cchi;
+
+    // stress updates, shear_sim.cc:345-348, applied as in fields.hh:61-67
+    PL(w.p, i, j) = f0[0] + (-uc * f1[0] - vc * f2[0]);
+    PL(w.q, i, j) = f0[1] + (-uc * f1[1] - vc * f2[1] - dtmp * f0[1]);
+    PL(w.s, i, j) = f0[2] + (-uc * f1[2] - vc * f2[2] + two_omega * f0[3] - dtmp * f0[2]);
+    PL(w.tau, i, j) = f0[3] + (-uc * f1[3] - vc * f2[3] - two_omega * f0[2] - dtmp * f0[3]);
+
+    if (dec) {
+        if (sbar < 1e-12) bits |= STZ_D_SMALLDEV;
+        else if (!(sbar < 1) && (c.stz.kind == STZ_MODEL_LINEAR_ATHERMAL ||
+                                 c.stz.kind == STZ_MODEL_NONLINEAR_ATHERMAL))
+            bits |= STZ_D_YIELD;
+        bits |= (uint32_t)(trips > 255 ? 255 : trips) << STZ_D_SUBSTEP_SHIFT;
+        dec[i + m * j] = bits;
+    }
+}
+
+// chi_diffusion, shear_sim.cc:364-386: w.chi holds the post-plastic chi
+// (cchi) and receives the diffused value; o.chi is the previous chi.
+__global__ void __launch_bounds__(256) chi_diffusion_kernel(SimConst c, const double *__restrict__ chi_old,
+                                                            const double *__restrict__ ddev, double *chi_new,
+                                                            double dt) {
+    const int m = c.m, n = c.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= m || j >= n) return;
+    const double hx = 0.5 * c.xsp * dt, hy = 0.5 * c.ysp * dt;
+    const double hxxcd = 2 * c.xsp * hx * c.chi_diff, hyycd = 2 * c.ysp * hy * c.chi_diff,
+                 hccd = 2 * (hxxcd + hyycd);
+    const double dfac = 0.25 * c.chi_len * c.chi_len * c.mu_inv, dfacx = dfac * c.xsp * c.xsp,
+                 dfacy = dfac * c.ysp * c.ysp;
+    const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
+    const double D0 = PL(ddev, i, j);
+    const double Dd = D0 + PL(ddev, i, j - 1), Dl = D0 + PL(ddev, il, j), Dr = D0 + PL(ddev, ir, j),
+                 Du = D0 + PL(ddev, i, j + 1);
+    const double cf = PL(chi_old, i, j), cl = PL(chi_old, il, j), cr = PL(chi_old, ir, j),
+                 cd = PL(chi_old, i, j - 1), cu = PL(chi_old, i, j + 1);
+    PL(chi_new, i, j) += hxxcd * (cr + cl) + hyycd * (cu + cd) - hccd * cf
+                         + dfacx * (Dr * cr - (Dr + Dl) * cf + Dl * cl)
+                         + dfacy * (Du * cu - (Du + Dd) * cf + Dd * cd);
+}
+
+// ---------------------------------------------------------------------------
+// projection source b = div(sigma)/dt and wall rows, shear_sim.cc:399-409,
+// net_force :447-453.  One thread per node (i,j), j in [0,n].
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) source_kernel(SimConst c, FieldPtrs f, double xfac, double yfac,
+                                                     double xbc, double bvel, double *src) {
+    const int m = c.m, n = c.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= m || j > n) return;
+    double fx, fy;
+    if (j == 0) {
+        fx = -xbc * bvel;
+        fy = 0;
+    } else if (j == n) {
+        fx = xbc * bvel;
+        fy = 0;
+    } else {
+        const int il = i == 0 ? m - 1 : i - 1;
+        // f0=(i-1,j-1) f1=(i,j-1) f2=(i-1,j) f3=(i,j)
+        const double p0 = PL(f.p, il, j - 1), p1 = PL(f.p, i, j - 1), p2 = PL(f.p, il, j), p3 = PL(f.p, i, j);
+        const double q0 = PL(f.q, il, j - 1), q1 = PL(f.q, i, j - 1), q2 = PL(f.q, il, j), q3 = PL(f.q, i, j);
+        const double s0 = PL(f.s, il, j - 1), s1 = PL(f.s, i, j - 1), s2 = PL(f.s, il, j), s3 = PL(f.s, i, j);
+        const double t0 = PL(f.tau, il, j - 1), t1 = PL(f.tau, i, j - 1), t2 = PL(f.tau, il, j),
+                     t3 = PL(f.tau, i, j);
+        fx = (-p0 + p1 - p2 + p3 - q0 + q1 - q2 + q3 + s0 - s1 + s2 - s3) * xfac + (t0 + t1 - t2 - t3) * yfac;
+        fy = (t0 - t1 + t2 - t3) * xfac + (-p0 - p1 + p2 + p3 - q0 - q1 + q2 + q3 - s0 - s1 + s2 + s3) * yfac;
+    }
+    const int ij = i + m * j;
+    src[2 * ij] = fx;
+    src[2 * ij + 1] = fy;
+}
+
+// ---------------------------------------------------------------------------
+// stress correction from the solved velocity, shear_sim.cc:418-434, fused
+// with set_boundaries (shear_sim.cc:552-581).  Rows 0/1 are done by the row-0
+// threads and rows n-1/n-2 by the row-(n-1) threads, so that the ghost-row
+// extrapolation reads corrected values without a second kernel (n >= 4).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ static void correct_cell(const SimConst &c, const FieldPtrs &f, const double *vel,
+                                                    int i, int j, double hx, double hy, double &p, double &q,
+                                                    double &s, double &t) {
+    const int m = c.m;
+    const double third = 1 / 3.0;
+    const int io = i == m - 1 ? 0 : i + 1;
+    const double *vp = vel + 2 * (i + m * j), *vo = vel + 2 * (io + m * j);
+    const double vpx = vp[0], vpy = vp[1], vox = vo[0], voy = vo[1];
+    const double vpux = vp[2 * m], vpuy = vp[2 * m + 1], voux = vo[2 * m], vouy = vo[2 * m + 1];
+    const double ux = hx * (-vpx + vox - vpux + voux), uy = hy * (-vpx - vox + vpux + voux),
+                 vx = hx * (-vpy + voy - vpuy + vouy), vy = hy * (-vpy - voy + vpuy + vouy);
+    p = PL(f.p, i, j) + -c.K * (ux + vy);
+    q = PL(f.q, i, j) + -c.mu * third * (ux + vy);
+    s = PL(f.s, i, j) + c.mu * (ux - vy);
+    t = PL(f.tau, i, j) + c.mu * (uy + vx);
+    PL(f.u, i, j) = vpx;
+    PL(f.v, i, j) = vpy;
+}
+
+__global__ void __launch_bounds__(256) correct_kernel(SimConst c, FieldPtrs f, const double *vel, double dt,
+                                                      double bvel, double bpos) {
+    const int m = c.m, n = c.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= m || j >= n) return;
+    if (j == 1 || j == n - 2) return;  // done by the wall-row threads
+    const double hx = 0.5 * c.xsp * dt, hy = 0.5 * c.ysp * dt;
+    double p, q, s, t;
+    correct_cell(c, f, vel, i, j, hx, hy, p, q, s, t);
+    if (j == 0 || j == n - 1) {
+        const int j1 = j == 0 ? 1 : n - 2, jg = j == 0 ? -1 : n;
+        double p1, q1, s1, t1;
+        correct_cell(c, f, vel, i, j1, hx, hy, p1, q1, s1, t1);
+        PL(f.p, i, j1) = p1;
+        PL(f.q, i, j1) = q1;
+        PL(f.s, i, j1) = s1;
+        PL(f.tau, i, j1) = t1;
+        PL(f.p, i, jg) = 2 * p - p1;
+        PL(f.q, i, jg) = 2 * q - q1;
+        PL(f.s, i, jg) = 2 * s - s1;
+        PL(f.tau, i, jg) = 2 * t - t1;
+        PL(f.chi, i, jg) = 2 * PL(f.chi, i, j) - PL(f.chi, i, j1);
+        const int jw = j == 0 ? 0 : n;
+        PL(f.u, i, jw) = j == 0 ? -bvel : bvel;
+        PL(f.v, i, jw) = 0;
+        PL(f.X, i, jw) = j == 0 ? c.ax + i * c.dx + bpos : c.ax + i * c.dx - bpos;
+        PL(f.Y, i, jw) = j == 0 ? c.ay : c.by;
+    }
+    PL(f.p, i, j) = p;
+    PL(f.q, i, j) = q;
+    PL(f.s, i, j) = s;
+    PL(f.tau, i, j) = t;
+}
+
+// set_boundaries on its own, shear_sim.cc:552-581
+__global__ void boundaries_kernel(SimConst c, FieldPtrs f, double bvel, double bpos) {
+    const int m = c.m, n = c.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    PL(f.u, i, 0) = -bvel;
+    PL(f.v, i, 0) = 0;
+    PL(f.X, i, 0) = c.ax + i * c.dx + bpos;
+    PL(f.Y, i, 0) = c.ay;
+    double *F[5] = {f.p, f.q, f.s, f.tau, f.chi};
+#pragma unroll
+    for (int k = 0; k < 5; k++) PL(F[k], i, -1) = 2 * PL(F[k], i, 0) - PL(F[k], i, 1);
+    PL(f.u, i, n) = bvel;
+    PL(f.v, i, n) = 0;
+    PL(f.X, i, n) = c.ax + i * c.dx - bpos;
+    PL(f.Y, i, n) = c.by;
+#pragma unroll
+    for (int k = 0; k < 5; k++) PL(F[k], i, n) = 2 * PL(F[k], i, n - 1) - PL(F[k], i, n - 2);
+}
+
+// shear_sim.cc:60-114
+__global__ void init_fields_kernel(SimConst c, FieldPtrs f, int chi_case, double chi0, double chi1) {
+    const int m = c.m, n = c.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y - 1;
+    if (i >= m || j > n) return;
+    const double pi = 3.1415926535897932384626433832795;
+    double Y = c.ay + j * c.dy, Ys = Y + 0.5 * c.dy;
+    double X = c.ax + i * c.dx, Xs = X + 0.5 * c.dx, chi;
+    PL(f.u, i, j) = 0;
+    PL(f.v, i, j) = 0;
+    PL(f.ddev, i, j) = 0;
+    PL(f.p, i, j) = 0;
+    PL(f.q, i, j) = 0;
+    PL(f.s, i, j) = 0;
+    PL(f.tau, i, j) = 0;
+    if (chi_case == 0) chi = chi0 + chi1 * exp(-200 * (Xs * Xs + Ys * Ys));
+    else if (chi_case == 1) {
+        const double csa = cos(30 * pi / 180.), sia = sin(30 * pi / 180.);
+        double Xr = csa * Xs + sia * Ys, Yr = -sia * Xs + csa * Ys;
+        Xr = Xr < -1 ? Xr + 1 : (Xr > 1 ? Xr - 1 : 0);
+        chi = chi0 + chi1 * exp(-200 * (Xr * Xr + Yr * Yr));
+    } else {
+        double R = 1e100, Rt, Xr, Yr;
+        for (int k = 0; k < 16; k++) {
+            double sxk = c.ax + c.lx * (k + 0.5) * 0.0625, syk = -0.2 * sin(pi * 0.0625 * (2 * k + 1));
+            Xr = sxk - Xs;
+            if (Xr < -c.lx * 0.5) Xr += c.lx;
+            if (Xr > c.lx * 0.5) Xr -= c.lx;
+            Yr = syk - Ys;
+            Rt = Xr * Xr + Yr * Yr;
+            if (Rt < R) R = Rt;
+        }
+        chi = chi0 + chi1 * exp(-200 * R);
+    }
+    PL(f.chi, i, j) = chi;
+    PL(f.X, i, j) = X;
+    PL(f.Y, i, j) = Y;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+int launch_advect(const SimConst &c, const FieldPtrs &o, const FieldPtrs &w, double dt, uint32_t *dec,
+                  hipStream_t st) {
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(advect_kernel, grid2(c.m, c.n, b), b, 0, st, c, o, w, dt, dec);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int launch_chi_diffusion(const SimConst &c, const FieldPtrs &o, const FieldPtrs &w, double dt, hipStream_t st) {
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(chi_diffusion_kernel, grid2(c.m, c.n, b), b, 0, st, c, o.chi, w.ddev, w.chi, dt);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int launch_source(const SimConst &c, const FieldPtrs &f, double dt, double xbc, double bvel, double *src,
+                  hipStream_t st) {
+    dim3 b(64, 4);
+    const double xfac = 0.5 / dt * c.xsp, yfac = 0.5 / dt * c.ysp;
+    hipLaunchKernelGGL(source_kernel, grid2(c.m, c.n + 1, b), b, 0, st, c, f, xfac, yfac, xbc, bvel, src);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int launch_correct(const SimConst &c, const FieldPtrs &f, const double *vel, double dt, double bvel, double bpos,
+                   hipStream_t st) {
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(correct_kernel, grid2(c.m, c.n, b), b, 0, st, c, f, vel, dt, bvel, bpos);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int launch_boundaries(const SimConst &c, const FieldPtrs &f, double bvel, double bpos, hipStream_t st) {
+    hipLaunchKernelGGL(boundaries_kernel, dim3((c.m + 255) / 256), dim3(256), 0, st, c, f, bvel, bpos);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int launch_init_fields(const SimConst &c, const FieldPtrs &f, int chi_case, double chi0, double chi1,
+                       hipStream_t st) {
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(init_fields_kernel, grid2(c.m, c.n + 2, b), b, 0, st, c, f, chi_case, chi0, chi1);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+}  // namespace stz

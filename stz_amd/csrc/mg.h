@@ -1,0 +1,43 @@
+// mg.h -- device-side multigrid structures and launchers (mg.hip).
+#pragma once
+
+#include "device.h"
+
+namespace stz {
+
+constexpr int kMaxLevels = 16;
+
+// One level of the hierarchy: m*n nodes, periodic in x, bounded in y.  Its
+// operator is a 9-point 2x2-block stencil: entry k = 0..8 (dl,dc,dr,cl,cc,cr,
+// ul,uc,ur) plus k = 9 the inverse of cc, 4 values (a,b,c,d) each, stored for
+// node (i,j) at S[(j*rs + i*cs)*40 + 4k + c].  x-invariant levels use cs = 0,
+// rs = 1 (one stencil per row).  z and b are interleaved (x,y) pairs.
+template <typename T> struct LevelArgs {
+    int m, n;
+    T *z;
+    T *b;
+    const T *S;
+    int rs, cs;
+    int xodd;  // m odd: the seam column i = m-1 gets a colour of its own
+};
+
+// A whole V-cycle: levels 0..L-1 (0 = top); levels >= lc are run by a single
+// workgroup with z and b in LDS, the coarsest exactly through binv.
+template <typename T> struct CoarseArgs {
+    int lc, L;
+    int nu1, nu2;
+    const T *binv;  // dense inverse of the coarsest operator, (2mn)^2
+    LevelArgs<T> lv[kMaxLevels];
+};
+
+template <typename T> int mg_smooth(const LevelArgs<T> &L, int sweeps, hipStream_t st);
+template <typename T> int mg_vcycle(const CoarseArgs<T> &A, hipStream_t st);
+template <typename T> int mg_resid_norm(const LevelArgs<T> &L, double *partial, double *out, hipStream_t st);
+template <typename T> size_t coarse_lds_bytes(const CoarseArgs<T> &A);
+// allow the workgroup solver this much dynamic LDS
+template <typename T> int mg_set_lds_limit(size_t bytes);
+int mg_resid64_to32(const LevelArgs<double> &L, float *b32, float *z32, double *partial, double *out,
+                    hipStream_t st);
+int mg_correct64(double *z, const float *e, int count, hipStream_t st);
+
+}  // namespace stz

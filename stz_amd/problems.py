@@ -1,0 +1,80 @@
+"""Parameter sets for the quasi-static simple-shear workloads.
+
+The physical set follows the reference drivers: the linear athermal STZ law
+(stz_model.hh:90-170) constructed as in sb/main/shear_small.cc:30-40 and
+shear_random.cc:41-82 (9 constructor arguments, so rho = 0 and
+t_model = -1), moduli rescaled by the yield stress, and the walls at y = +/-1
+moving at +/-U (shear_sim.hh:120-124).
+
+Choices the reference leaves to the driver, recorded here:
+  * Omega = 300 A^3 and Omega*eps0 = Omega*1e-30*eps0*s_y/(T_Z k_B)
+    (shear_small.cc:40 form with s_y = 0.85 GPa);
+  * viscosity 0.02, adapt_fac 0.002, tmult 0.5, chi_diff = chi_len = 0;
+  * U = 1e-4: the wall-to-wall velocity difference 2U over height 2 gives
+    the nondimensional shear rate 1e-4;
+  * t_scale = 1 (one time unit is one second of STZ time), so the steady
+    flow stress sits near 1.2 s_y;
+  * the run lasts T = 1000 time units (shear strain 0.1, about twice the
+    yield strain mu^-1), dt = T / steps.
+"""
+import math
+
+TZ = 21000.0
+KB = 1.3806503e-23          # shear_small.cc:14
+S_Y = 0.85e9                # Pa
+MU = 20e9 / S_Y             # 20 GPa in units of s_y
+K_BULK = 143.2e9 / S_Y      # 143.2 GPa in units of s_y
+OMEGA = 300.0               # cubic Angstrom
+EPS0 = 0.3
+
+
+def stz_linear_athermal(c0=0.4, eps0=EPS0, chi_inf_K=900.0, tau0=1e-13,
+                        Delta_K=8000.0, theta_K=400.0, omega=OMEGA):
+    Omegaeps0 = omega * 1e-30 * eps0 * S_Y / (TZ * KB)
+    Delta = Delta_K / TZ
+    return {
+        "kind": 0, "t_model": -1,
+        "TZ": TZ, "c0": c0, "tau0": tau0, "kappa": 4.2,
+        "Delta": Delta, "Omegaeps0": Omegaeps0, "chi_inf": chi_inf_K / TZ,
+        "theta": theta_K / TZ, "rho": 0.0,
+        "t_icpt": 0.0, "t_grad": 0.5 * Omegaeps0 / Delta,
+    }
+
+
+def simple_shear(m, n=None, u_bdry=1e-4, t_scale=1.0, viscosity=0.02,
+                 adapt_fac=0.002, chi_diff=0.0, chi_len=0.0, **stz_kw):
+    """shear_sim constructor arguments for the periodic-x simple shear cell."""
+    n = m if n is None else n
+    return {
+        "m": m, "n": n, "ax": -1.0, "bx": 1.0, "ay": -1.0, "by": 1.0,
+        "mu": MU, "K": K_BULK, "viscosity": viscosity,
+        "chi_diff": chi_diff, "chi_len": chi_len, "t_scale": t_scale,
+        "tmult": 0.5, "adapt_fac": adapt_fac, "u_bdry": u_bdry,
+        "stz": stz_linear_athermal(**stz_kw),
+    }
+
+
+# Simple-shear workloads: (cells per side, steps, chi mean K, chi std K,
+# correlation length in cells)
+WORKLOADS = {
+    "C1": dict(N=128, steps=2000, chi_mean=600.0, chi_std=20.0, corr=2.0),
+    "C2": dict(N=1024, steps=5000, chi_mean=600.0, chi_std=25.0, corr=3.0),
+    "C5": dict(N=256, steps=400, chi_mean=620.0, chi_std=25.0, corr=3.0),
+}
+T_END = 1000.0
+
+
+def workload(name):
+    w = dict(WORKLOADS[name])
+    w["params"] = simple_shear(w["N"])
+    w["dt"] = T_END / w["steps"]
+    return w
+
+
+def describe(params):
+    return ("simple shear %dx%d, mu=%.4g K=%.4g eta=%.3g U=%.3g t_scale=%.3g" %
+            (params["m"], params["n"], params["mu"], params["K"],
+             params["viscosity"], params["u_bdry"], params["t_scale"]))
+
+
+assert math.isclose(simple_shear(8)["stz"]["Omegaeps0"], 0.2638, rel_tol=1e-3)

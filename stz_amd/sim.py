@@ -1,0 +1,211 @@
+"""Python mirror of the reference's ``shear_sim`` class (sb/main/shear_sim.hh:20-151)
+over the gfx950 C ABI.
+
+Method names, argument meaning and failure behaviour follow the reference:
+``step_forward_quasistatic(dt)``, ``advection_step(dt)``,
+``projection_step(dt)``, ``init_fields(case, T0, dT)``, ``avg_iters()``,
+``bdry_vel()``, ``bdry_pos()``, public ``time`` and ``chi_inf``.  Where the
+reference calls exit(1) on a failed solve (qs_multi.hh:52), this raises
+``StzError``.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELD_INDEX, FIELD_NAMES, NFIELDS, check
+
+
+class ShearSim:
+    """Quasi-static simple-shear STZ simulation resident on one GPU."""
+
+    def __init__(self, params, precision="fp64"):
+        lib = _lib.load()
+        self.params = dict(params)
+        self._p = _lib.make_params(params)
+        self.m, self.n = self._p.m, self._p.n
+        prec = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
+        self.precision = precision
+        h = C.c_void_p()
+        check(lib.stz_sim_create(C.byref(self._p), prec, C.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    # ---- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.stz_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- fields (public fm, vel, src)
+    @property
+    def plane_shape(self):
+        return (self.n + 2, self.m)
+
+    def _fid(self, k):
+        return FIELD_INDEX[k] if isinstance(k, str) else int(k)
+
+    def set_field(self, k, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.plane_shape)
+        check(self._lib.stz_sim_upload_field(self._h, self._fid(k), a))
+
+    def get_field(self, k):
+        a = np.empty(self.plane_shape, dtype=np.float64)
+        check(self._lib.stz_sim_download_field(self._h, self._fid(k), a))
+        return a
+
+    def set_fields(self, planes):
+        for k in range(NFIELDS):
+            self.set_field(k, planes[k])
+
+    def get_fields(self):
+        return np.stack([self.get_field(k) for k in range(NFIELDS)])
+
+    def set_vel(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.n + 1, self.m, 2)
+        check(self._lib.stz_sim_upload_vel(self._h, a))
+
+    def get_vel(self):
+        a = np.empty((self.n + 1, self.m, 2), dtype=np.float64)
+        check(self._lib.stz_sim_download_vel(self._h, a))
+        return a
+
+    def get_src(self):
+        a = np.empty((self.n + 1, self.m, 2), dtype=np.float64)
+        check(self._lib.stz_sim_download_src(self._h, a))
+        return a
+
+    def field_device_ptr(self, k):
+        p = C.c_void_p()
+        check(self._lib.stz_sim_field_device_ptr(self._h, self._fid(k), C.byref(p)))
+        return p.value
+
+    def vel_device_ptr(self):
+        p = C.c_void_p()
+        check(self._lib.stz_sim_vel_device_ptr(self._h, C.byref(p)))
+        return p.value
+
+    # ---- public scalars
+    @property
+    def time(self):
+        t = C.c_double()
+        check(self._lib.stz_sim_get_time(self._h, C.byref(t)))
+        return t.value
+
+    @time.setter
+    def time(self, t):
+        check(self._lib.stz_sim_set_time(self._h, float(t)))
+
+    def set_chi_inf(self, c):
+        check(self._lib.stz_sim_set_chi_inf(self._h, float(c)))
+
+    def bdry_vel(self):
+        """shear_sim.hh:120"""
+        t, u = self.time, self.params["u_bdry"]
+        return t * u if t < 1 else u
+
+    def bdry_pos(self):
+        """shear_sim.hh:124"""
+        t, u = self.time, self.params["u_bdry"]
+        return u * t * t * 0.5 if t < 1 else u * (t - 0.5)
+
+    # ---- stepping
+    def init_fields(self, chi_case, tem_base=600.0, tem_delta=200.0):
+        check(self._lib.stz_sim_init_fields(self._h, int(chi_case), float(tem_base), float(tem_delta)))
+
+    def step_forward_quasistatic(self, dt, nsteps=1):
+        check(self._lib.stz_sim_step_quasistatic(self._h, float(dt), int(nsteps)))
+
+    def advection_step(self, dt):
+        check(self._lib.stz_sim_advection_step(self._h, float(dt)))
+
+    def projection_step(self, dt):
+        check(self._lib.stz_sim_projection_step(self._h, float(dt)))
+
+    def set_boundaries(self):
+        check(self._lib.stz_sim_set_boundaries(self._h))
+
+    def synchronize(self):
+        check(self._lib.stz_sim_synchronize(self._h))
+
+    # ---- multigrid
+    def mg_config(self):
+        g = _lib.MGConfig()
+        check(self._lib.stz_sim_get_mg_config(self._h, C.byref(g)))
+        return g
+
+    def set_mg_config(self, **kw):
+        g = self.mg_config()
+        for k, v in kw.items():
+            setattr(g, k, v)
+        check(self._lib.stz_sim_set_mg_config(self._h, C.byref(g)))
+
+    def last_solve(self):
+        info = _lib.SolveInfo()
+        check(self._lib.stz_sim_last_solve(self._h, C.byref(info)))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def avg_iters(self):
+        a = C.c_double()
+        check(self._lib.stz_sim_avg_iters(self._h, C.byref(a)))
+        return a.value
+
+    def mg_hierarchy(self):
+        dims = np.zeros(64, dtype=np.int32)
+        nl = C.c_int()
+        check(self._lib.stz_sim_mg_hierarchy(self._h, 32, C.byref(nl), dims))
+        return [(int(dims[2 * k]), int(dims[2 * k + 1])) for k in range(nl.value)]
+
+    def mg_setup(self, viscdt):
+        check(self._lib.stz_mg_setup(self._h, float(viscdt)))
+
+    def mg_solve(self):
+        info = _lib.SolveInfo()
+        code = self._lib.stz_mg_solve(self._h, C.byref(info))
+        check(code)
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def mg_v_cycle(self):
+        check(self._lib.stz_mg_v_cycle(self._h))
+
+    def mg_l2_error(self):
+        e = C.c_double()
+        check(self._lib.stz_mg_l2_error(self._h, C.byref(e)))
+        return e.value
+
+    def mg_smooth(self, level, sweeps):
+        """asynchronous smoothing sweeps on one level (tgmg gauss_seidel)"""
+        check(self._lib.stz_mg_smooth(self._h, int(level), int(sweeps)))
+
+    def set_stream(self, stream_handle):
+        """run on a HIP stream given as an integer handle (e.g. a torch
+        stream's cuda_stream); None returns to the library's own stream"""
+        check(self._lib.stz_sim_set_stream(self._h, C.c_void_p(stream_handle) if stream_handle else None))
+
+    def mg_set_src(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.n + 1, self.m, 2)
+        check(self._lib.stz_mg_upload_src(self._h, a))
+
+    # ---- decisions
+    def record_decisions(self, on=True):
+        check(self._lib.stz_sim_record_decisions(self._h, 1 if on else 0))
+
+    def get_decisions(self):
+        a = np.empty((self.n, self.m), dtype=np.uint32)
+        check(self._lib.stz_sim_download_decisions(self._h, a))
+        return a
+
+
+__all__ = ["ShearSim", "FIELD_NAMES"]

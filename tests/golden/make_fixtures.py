@@ -1,0 +1,103 @@
+"""Write the golden fixtures from the reference's own compiled code.
+
+Run in a container that has the reference tree (oracle/_ref/libstzref.so is
+built by ``make -C oracle ref``):
+
+    python tests/golden/make_fixtures.py
+
+Each fixture holds seeded inputs (all ten c_field planes, rows -1..n, and the
+velocity/solution array) and the reference's outputs after k quasi-static
+steps (shear_sim::step_forward_quasistatic, shear_sim.cc:280-291), with the
+multigrid threshold qs_multi::acc tightened by ``tol_mult`` so the solve is
+converged to rounding.  The "traj" fixtures keep the reference's default
+threshold and record its mean V-cycle count.  "rand_init" holds a chi field
+from shear_sim::initialize_random after srand(seed) (shear_sim.cc:932-998).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from oracle.pyoracle import Reference  # noqa: E402
+from stz_amd.problems import simple_shear  # noqa: E402
+from stategen import random_state  # noqa: E402
+
+TIGHT = 1e-12
+
+
+def box(m, n, dx):
+    """domain with square cells: x in [-m dx/2, m dx/2], y in [-n dx/2, n dx/2]"""
+    return dict(ax=-0.5 * m * dx, bx=0.5 * m * dx, ay=-0.5 * n * dx, by=0.5 * n * dx)
+
+
+def run_case(name, params, seed, dt, t0, steps, tol_mult=TIGHT, vel_scale=3e-3):
+    planes, vel = random_state(params, seed, vel=vel_scale)
+    r = Reference(params)
+    r.set_fields(planes)
+    r.set_vel(vel)
+    r.time = t0
+    r.set_tol_mult(tol_mult)
+    out = {"params": json.dumps(params), "seed": seed, "dt": dt, "t0": t0, "tol_mult": tol_mult,
+           "steps": np.array(steps), "in_planes": planes, "in_vel": vel}
+    done = 0
+    for k in steps:
+        r.step(dt, k - done)
+        done = k
+        out["planes_%d" % k] = r.get_fields()
+        out["vel_%d" % k] = r.get_vel()
+    out["avg_iters"] = r.avg_iters()
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "avg V-cycles %.3f" % out["avg_iters"])
+
+
+def traj_case(name, N, dt, steps, seed):
+    params = simple_shear(N)
+    r = Reference(params)
+    r.init_fields(0, 600.0, 200.0)
+    r.initialize_random(2.0, 600.0, 20.0, seed)
+    planes = r.get_fields()
+    vel = np.zeros((N + 1, N, 2))
+    r.set_vel(vel)
+    r.time = 0.0
+    r.step(dt, steps)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), params=json.dumps(params), seed=seed, dt=dt,
+                        steps=steps, in_planes=planes, in_vel=vel, out_planes=r.get_fields(),
+                        out_vel=r.get_vel(), avg_iters=r.avg_iters())
+    print(name, "done")
+
+
+def rand_init_case(name, N, l, avg, sd, seed):
+    params = simple_shear(N)
+    r = Reference(params)
+    r.init_fields(0, 600.0, 200.0)
+    r.initialize_random(l, avg, sd, seed)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), params=json.dumps(params), l=l, avg=avg, sd=sd,
+                        seed=seed, chi=r.get_field(6))
+    print(name, "done")
+
+
+def main():
+    shear = dict(u_bdry=2e-3)
+    run_case("rand32", simple_shear(32, **shear), 1, 0.5, 0.8, [1, 2, 10])
+    run_case("rand64", simple_shear(64, **shear), 2, 0.5, 0.8, [1, 3])
+    run_case("rand128", simple_shear(128, **shear), 3, 0.25, 0.9, [1])
+    p = simple_shear(24, 31, **shear)
+    p.update(box(24, 31, 2.0 / 24))
+    run_case("odd24x31", p, 4, 0.5, 0.5, [1, 2])
+    p = simple_shear(25, 20, **shear)
+    p.update(box(25, 20, 2.0 / 25))
+    run_case("odd25x20", p, 5, 0.5, 2.0, [1, 2])
+    p = simple_shear(32, chi_diff=1e-4, chi_len=0.05, **shear)
+    run_case("diff32", p, 6, 0.5, 1.5, [1, 2])
+    traj_case("traj64", 64, 0.5, 40, 11)
+    rand_init_case("rand_init48", 48, 2.0, 600.0, 20.0, 7)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,79 @@
+"""The C ABI library: it loads, exports every entry point include/stz_amd.h
+declares, its structs have the layout the header gives them, and without a
+device it fails loudly instead of falling back to the CPU."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+import stz_amd
+from stz_amd import _lib
+from oracle import pyoracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "stz_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(stz_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding covers exactly the declared set
+    assert sorted(_lib.exported_symbols()) == names
+
+
+def test_struct_layout_matches_header(tmp_path):
+    probe = tmp_path / "probe.c"
+    structs = {
+        "stz_model_params": _lib.ModelParams,
+        "stz_sim_params": _lib.SimParams,
+        "stz_mg_config": _lib.MGConfig,
+        "stz_solve_info": _lib.SolveInfo,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, fname, cname, fname))
+    lines.append("return 0; }")
+    probe.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.check_call(["gcc", "-std=c99", "-o", str(exe), str(probe)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)], text=True).splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got["%s.%s" % (cname, fname)]) == getattr(py, fname).offset, (cname, fname)
+    # the checker's own copy of the parameter structs agrees too
+    assert C.sizeof(pyoracle.SimParams) == C.sizeof(_lib.SimParams)
+    for fname, _ in _lib.SimParams._fields_:
+        assert getattr(pyoracle.SimParams, fname).offset == getattr(_lib.SimParams, fname).offset
+
+
+def test_no_device_fails_loudly():
+    if stz_amd.device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(stz_amd.StzError) as e:
+        stz_amd.ShearSim(stz_amd.problems.simple_shear(16))
+    assert e.value.code == _lib.ERR_NO_DEVICE
+
+
+def test_bad_arguments_rejected_before_device_use():
+    lib = _lib.load()
+    p = _lib.make_params(stz_amd.problems.simple_shear(2))
+    h = C.c_void_p()
+    assert lib.stz_sim_create(C.byref(p), 0, C.byref(h)) == _lib.ERR_ARG
+    assert b"m >= 4" in lib.stz_last_error()
+    p = _lib.make_params(stz_amd.problems.simple_shear(16))
+    assert lib.stz_sim_create(C.byref(p), 7, C.byref(h)) == _lib.ERR_ARG
+    assert lib.stz_sim_destroy(None) == _lib.OK
+    assert lib.stz_sim_set_time(None, 1.0) == _lib.ERR_ARG

@@ -1,0 +1,153 @@
+"""GPU parity: the gfx950 path (through the C ABI) against the reference's
+golden fixtures and the FP64 restatement, on identical seeded inputs.
+
+Bars: decision bits (upwind side, ENO branch, wall rule, yield branch)
+identical; explicit-step fields bit-identical where no transcendental enters
+(X, Y, p) and within 1e-13 elsewhere; whole steps with converged solves
+within 1e-10 relative per field; the mixed-precision form within 1e-8 on
+stress and chi."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import field_errors, group_errors, load_fixture
+from dense import top_dense
+from oracle.pyoracle import Oracle
+from stz_amd import ShearSim, StzError
+from stz_amd.problems import simple_shear
+
+pytestmark = pytest.mark.gpu
+
+STEP_CASES = ["rand32", "rand64", "rand128", "odd24x31", "odd25x20", "diff32"]
+SUBSTEP_MASK = np.uint32(0xFF << 24)
+
+
+def gpu_from_fixture(fx, precision="fp64"):
+    g = ShearSim(fx["params"], precision)
+    g.set_fields(fx["in_planes"])
+    g.set_vel(fx["in_vel"])
+    g.time = float(fx["t0"])
+    g.set_mg_config(tol_mult=float(fx["tol_mult"]))
+    return g
+
+
+@pytest.mark.parametrize("case", STEP_CASES)
+def test_advection_decisions_and_fields(case):
+    fx = load_fixture(case)
+    print("case", case, flush=True)
+    o = Oracle(fx["params"])
+    o.set_fields(fx["in_planes"])
+    o.record_decisions(True)
+    o.time = float(fx["t0"]) + float(fx["dt"])
+    o.advection_step(float(fx["dt"]))
+    g = gpu_from_fixture(fx)
+    g.record_decisions(True)
+    g.time = float(fx["t0"]) + float(fx["dt"])
+    g.advection_step(float(fx["dt"]))
+    do, dg = o.get_decisions(), g.get_decisions()
+    assert np.array_equal(do & ~SUBSTEP_MASK, dg & ~SUBSTEP_MASK)
+    trips_o, trips_g = do >> 24, dg >> 24
+    # substep counts come from exp/tanh: allow ulp-level threshold flips only
+    assert np.count_nonzero(trips_o != trips_g) <= max(1, do.size // 1000)
+    assert np.count_nonzero(do & np.uint32(1 << 20)) > 0, "no yielding cell exercised"
+    fo, fg = o.get_fields(), g.get_fields()
+    for k in (0, 1, 2, 7, 8):  # u, v, p, X, Y: no transcendental on their path
+        assert np.array_equal(fo[k], fg[k]), k
+    err = field_errors(fg, fo)
+    print(case, "advection errors", ["%.1e" % e for e in err])
+    assert max(err) < 1e-13, err
+
+
+@pytest.mark.parametrize("case", STEP_CASES)
+def test_step_matches_reference_fixture(case):
+    fx = load_fixture(case)
+    print("case", case, flush=True)
+    for k in [int(s) for s in fx["steps"]]:
+        g = gpu_from_fixture(fx)
+        g.step_forward_quasistatic(float(fx["dt"]), k)
+        err = field_errors(g.get_fields(), fx["planes_%d" % k])
+        print(case, k, "step errors", ["%.1e" % e for e in err], "cycles", g.last_solve()["cycles"])
+        assert max(err) < 1e-10, (case, k, err)
+
+
+@pytest.mark.parametrize("case", ["rand64", "odd24x31"])
+def test_mixed_precision_matches_reference(case):
+    fx = load_fixture(case)
+    print("case", case, flush=True)
+    k = int(fx["steps"][-1])
+    g = gpu_from_fixture(fx, "mixed")
+    g.step_forward_quasistatic(float(fx["dt"]), k)
+    ge = group_errors(g.get_fields(), fx["planes_%d" % k])
+    print(case, "mixed errors", {a: "%.1e" % b for a, b in ge.items()}, g.last_solve())
+    assert ge["stress"] < 1e-8 and ge["chi"] < 1e-8, ge
+    assert max(ge.values()) < 1e-8, ge
+
+
+def test_trajectory_default_tolerance():
+    """40 steps at the reference's own threshold from initialize_random.
+    Smoothers differ (four-colour vs zebra), so the solves stop at different
+    points inside the tolerance; the trajectories agree to that level."""
+    fx = load_fixture("traj64")
+    print("case traj64", flush=True)
+    g = ShearSim(fx["params"])
+    g.set_fields(fx["in_planes"])
+    g.set_vel(fx["in_vel"])
+    g.time = 0.0
+    g.step_forward_quasistatic(float(fx["dt"]), int(fx["steps"]))
+    ge = group_errors(g.get_fields(), fx["out_planes"])
+    print("traj64 errors", ge, "avg cycles gpu %.2f ref %.2f" % (g.avg_iters(), float(fx["avg_iters"])))
+    assert max(ge.values()) < 1e-6, ge
+
+
+def test_mg_solve_equals_dense_solve():
+    p = simple_shear(8)
+    print("case dense8", flush=True)
+    g = ShearSim(p)
+    viscdt = 0.05
+    g.mg_setup(viscdt)
+    g.set_mg_config(tol_mult=1e-14)
+    rng = np.random.default_rng(5)
+    b = rng.normal(size=(p["n"] + 1, p["m"], 2))
+    g.mg_set_src(b)
+    g.set_vel(np.zeros_like(b))
+    info = g.mg_solve()
+    z = np.linalg.solve(top_dense(p, viscdt), b.ravel())
+    assert info["status"] == 0
+    assert np.abs(g.get_vel().ravel() - z).max() <= 1e-12 * np.abs(z).max()
+
+
+def test_hierarchy_matches_reference_rule():
+    for m, n in ((1024, 1024), (384, 384), (24, 31), (25, 20)):
+        p = simple_shear(m, n)
+        g = ShearSim(p)
+        o = Oracle(p)
+        assert g.mg_hierarchy() == o.mg_levels(), (m, n)
+
+
+@pytest.mark.parametrize("chi_case", [0, 1, 2])
+def test_init_fields(chi_case):
+    p = simple_shear(48, 40)
+    o = Oracle(p)
+    o.init_fields(chi_case, 600.0, 200.0)
+    g = ShearSim(p)
+    g.init_fields(chi_case, 600.0, 200.0)
+    fo, fg = o.get_fields(), g.get_fields()
+    for k in range(10):
+        if k == 6:
+            assert np.max(np.abs(fo[k] - fg[k])) <= 4e-16 * np.max(np.abs(fo[k]))
+        else:
+            assert np.array_equal(fo[k], fg[k]), k
+
+
+def test_nan_source_reports_failure():
+    fx = load_fixture("rand32")
+    print("case nan32", flush=True)
+    planes = fx["in_planes"].copy()
+    planes[2][10, 10] = math.nan
+    g = ShearSim(fx["params"])
+    g.set_fields(planes)
+    g.set_vel(fx["in_vel"])
+    with pytest.raises(StzError) as e:
+        g.step_forward_quasistatic(0.5)
+    assert e.value.code in (3, 4)
