@@ -99,7 +99,10 @@ typedef struct stz_mg_config {
     int32_t predict_mult, predict_decay, predict_init, predict_max_iters;
     double tol_mult;             /* multiplier on the mean-square tolerance */
     int32_t max_refine;          /* mixed: cap on FP64 refinement iterations */
-    int32_t reserved[7];
+    int32_t kernels;             /* 0: LDS-tiled multi-sweep kernels where a
+                                    level allows them (default); 1: one
+                                    kernel per colour pass everywhere */
+    int32_t reserved[6];
 } stz_mg_config;
 
 /* Result of the last projection solve. */

@@ -40,7 +40,7 @@ class MGConfig(C.Structure):
                 ("extra_cycles", C.c_int32), ("predict_mult", C.c_int32),
                 ("predict_decay", C.c_int32), ("predict_init", C.c_int32),
                 ("predict_max_iters", C.c_int32), ("tol_mult", C.c_double),
-                ("max_refine", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("max_refine", C.c_int32), ("kernels", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class SolveInfo(C.Structure):

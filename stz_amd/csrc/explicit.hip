@@ -204,6 +204,8 @@ __global__ void __launch_bounds__(256) advect_kernel(SimConst c, FieldPtrs o, Fi
         // ghost row -1, in the new plane set
         PL(w.X, i, 0) = PL(o.X, i, 0);
         PL(w.Y, i, 0) = PL(o.Y, i, 0);
+        PL(w.X, i, -1) = PL(o.X, i, -1);
+        PL(w.Y, i, -1) = PL(o.Y, i, -1);
         PL(w.p, i, -1) = PL(o.p, i, -1);
         PL(w.q, i, -1) = PL(o.q, i, -1);
         PL(w.s, i, -1) = PL(o.s, i, -1);

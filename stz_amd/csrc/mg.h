@@ -14,7 +14,8 @@ constexpr int kMaxLevels = 16;
 // rs = 1 (one stencil per row).  z and b are interleaved (x,y) pairs.
 template <typename T> struct LevelArgs {
     int m, n;
-    T *z;
+    T *z;   // current solution
+    T *z2;  // second buffer: tiled sweeps read z and write z2, then swap
     T *b;
     const T *S;
     int rs, cs;
@@ -31,6 +32,9 @@ template <typename T> struct CoarseArgs {
 };
 
 template <typename T> int mg_smooth(const LevelArgs<T> &L, int sweeps, hipStream_t st);
+template <typename T> int mg_restrict(const LevelArgs<T> &F, const LevelArgs<T> &C, hipStream_t st);
+template <typename T> int mg_prolong(const LevelArgs<T> &C, const LevelArgs<T> &F, hipStream_t st);
+template <typename T> int mg_coarse(const CoarseArgs<T> &A, hipStream_t st);
 template <typename T> int mg_vcycle(const CoarseArgs<T> &A, hipStream_t st);
 template <typename T> int mg_resid_norm(const LevelArgs<T> &L, double *partial, double *out, hipStream_t st);
 template <typename T> size_t coarse_lds_bytes(const CoarseArgs<T> &A);
@@ -39,5 +43,19 @@ template <typename T> int mg_set_lds_limit(size_t bytes);
 int mg_resid64_to32(const LevelArgs<double> &L, float *b32, float *z32, double *partial, double *out,
                     hipStream_t st);
 int mg_correct64(double *z, const float *e, int count, hipStream_t st);
+int mg_sum_partials(const double *partial, int np, double *out, hipStream_t st);
+template <typename T> int mg_fill_zero(T *p, size_t count, hipStream_t st);
+
+// tiled path (mg_tiled.hip)
+template <typename T> bool tiled_ok(const LevelArgs<T> &L);
+// one V-cycle with the tiled kernels where a level allows them; swaps the
+// z/z2 pointers of the levels it smooths; top_zero starts the top level
+// from z = 0 (correction equation)
+template <typename T> int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipStream_t st);
+// `sweeps` smoothing sweeps on one level (tiled where possible; swaps z/z2)
+template <typename T> int mg_smooth_level(LevelArgs<T> &L, int sweeps, hipStream_t st);
+// residual of the FP64 top level -> FP32 source, |r|^2 partials summed to *out
+int mg_refine(const LevelArgs<double> &L, float *b32, double *partial, double *out, hipStream_t st);
+int mg_refine_blocks(const LevelArgs<double> &L);
 
 }  // namespace stz

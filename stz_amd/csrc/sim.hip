@@ -324,10 +324,15 @@ template <typename T> static int alloc_levels(stz_sim *s, DevMG<T> &mg, bool own
         A.xodd = A.m & 1;
         const size_t cnt = 2 * (size_t)A.m * A.n;
         if (l > 0 || own_top) {
-            STZ_HIP(hipMalloc(&mg.zb[l], 2 * cnt * sizeof(T)));
-            STZ_HIP(hipMemset(mg.zb[l], 0, 2 * cnt * sizeof(T)));
+            STZ_HIP(hipMalloc(&mg.zb[l], 3 * cnt * sizeof(T)));
+            STZ_HIP(hipMemset(mg.zb[l], 0, 3 * cnt * sizeof(T)));
             A.z = mg.zb[l];
             A.b = mg.zb[l] + cnt;
+            A.z2 = mg.zb[l] + 2 * cnt;
+        } else {  // FP64 top level: z is the simulation's vel, z2 its twin
+            STZ_HIP(hipMalloc(&mg.zb[l], cnt * sizeof(T)));
+            STZ_HIP(hipMemset(mg.zb[l], 0, cnt * sizeof(T)));
+            A.z2 = mg.zb[l];
         }
     }
     // first level run by the workgroup solver: the largest suffix of levels
@@ -384,13 +389,20 @@ static int mg_build(stz_sim *s, double viscdt) {
 
 template <typename T> static int set_lds_attr(const DevMG<T> &mg) { return mg_set_lds_limit<T>(mg.lds); }
 
+// mixed form: FP64 residual of the top level -> FP32 source of the next
+// correction equation (and zero correction for the per-colour path)
+static int refine_resid(stz_sim *s) {
+    LevelArgs<double> t = s->top64();
+    LevelArgs<float> &f = s->mg32.args.lv[0];
+    if (s->cfg.kernels == 0) return mg_refine(t, f.b, s->partial, s->dnorm, s->stream);
+    return mg_resid64_to32(t, f.b, f.z, s->partial, s->dnorm, s->stream);
+}
+
 // mean-square residual of the top level (tgmg.hh:146) -> host
 static int top_l2(stz_sim *s, double *out) {
     int e;
     if (s->precision == STZ_PREC_MIXED) {
-        LevelArgs<double> t = s->top64();
-        if ((e = mg_resid64_to32(t, s->mg32.args.lv[0].b, s->mg32.args.lv[0].z, s->partial, s->dnorm, s->stream)))
-            return e;
+        if ((e = refine_resid(s))) return e;
     } else {
         if ((e = mg_resid_norm(s->top64(), s->partial, s->dnorm, s->stream))) return e;
     }
@@ -400,24 +412,43 @@ static int top_l2(stz_sim *s, double *out) {
     return STZ_OK;
 }
 
+// the tiled V-cycle may leave the top FP64 level's result in its second
+// buffer; move it back into vel so the public pointer stays valid
+static int top_home(stz_sim *s) {
+    LevelArgs<double> &t = s->mg64.args.lv[0];
+    if (t.z != s->vel) {
+        STZ_HIP(hipMemcpyAsync(s->vel, t.z, 2 * sizeof(double) * t.m * t.n, hipMemcpyDeviceToDevice, s->stream));
+        t.z2 = t.z;
+        t.z = s->vel;
+    }
+    return STZ_OK;
+}
+
 // one cycle: FP64 V-cycle on vel, or (mixed) FP32 V-cycle on the correction
 // equation whose source the last residual evaluation wrote, then vel += e
 // and a fresh FP64 residual (the next cycle's source)
 static int one_cycle(stz_sim *s, bool need_resid) {
     int e;
-    s->mg64.args.nu1 = s->cfg.nu1;
-    s->mg64.args.nu2 = s->cfg.nu2;
-    if (s->precision == STZ_PREC_FP64) return mg_vcycle(s->mg64.args, s->stream);
-    s->mg32.args.nu1 = s->cfg.nu1;
-    s->mg32.args.nu2 = s->cfg.nu2;
-    if ((e = mg_vcycle(s->mg32.args, s->stream))) return e;
-    if ((e = mg_correct64(s->vel, s->mg32.args.lv[0].z, 2 * s->c.m * (s->c.n + 1), s->stream))) return e;
-    s->last.refinements++;
-    if (need_resid) {
-        LevelArgs<double> t = s->top64();
-        if ((e = mg_resid64_to32(t, s->mg32.args.lv[0].b, s->mg32.args.lv[0].z, s->partial, s->dnorm, s->stream)))
-            return e;
+    const bool tiled = s->cfg.kernels == 0;
+    if (s->precision == STZ_PREC_FP64) {
+        CoarseArgs<double> &A = s->mg64.args;
+        A.nu1 = s->cfg.nu1;
+        A.nu2 = s->cfg.nu2;
+        if (!tiled) return mg_vcycle(A, s->stream);
+        if ((e = mg_vcycle_tiled(A, false, s->stream))) return e;
+        return top_home(s);
     }
+    CoarseArgs<float> &A = s->mg32.args;
+    A.nu1 = s->cfg.nu1;
+    A.nu2 = s->cfg.nu2;
+    if (tiled) {
+        if ((e = mg_vcycle_tiled(A, true, s->stream))) return e;
+    } else {
+        if ((e = mg_vcycle(A, s->stream))) return e;
+    }
+    if ((e = mg_correct64(s->vel, A.lv[0].z, 2 * s->c.m * (s->c.n + 1), s->stream))) return e;
+    s->last.refinements++;
+    if (need_resid) return refine_resid(s);
     return STZ_OK;
 }
 
@@ -439,9 +470,7 @@ static int mg_solve_impl(stz_sim *s) {
     s->last.tolerance = tol;
     double acc;
     if (s->precision == STZ_PREC_MIXED) {  // first source of the correction equation
-        LevelArgs<double> t = s->top64();
-        if ((e = mg_resid64_to32(t, s->mg32.args.lv[0].b, s->mg32.args.lv[0].z, s->partial, s->dnorm, s->stream)))
-            return e;
+        if ((e = refine_resid(s))) return e;
         s->last.refinements = 0;
     }
     int n = s->lim / g.predict_mult;
@@ -609,7 +638,8 @@ int stz_sim_create(const stz_sim_params *p, int precision, stz_sim **out) {
     s->src = s->vel + 2 * nodes;
     if (hipMalloc(&s->dec, (size_t)c.m * c.n * sizeof(uint32_t)) != hipSuccess) return fail(STZ_ERR_HIP);
     (void)hipMemset(s->dec, 0, (size_t)c.m * c.n * sizeof(uint32_t));
-    s->nblocks_norm = (int)((nodes + 255) / 256);
+    s->nblocks_norm = (int)std::max<size_t>((nodes + 255) / 256,
+                                            (size_t)((c.m + 63) / 64) * ((c.n + 1 + 15) / 16));
     if (hipMalloc(&s->partial, (s->nblocks_norm + 1) * sizeof(double)) != hipSuccess) return fail(STZ_ERR_HIP);
     s->dnorm = s->partial + s->nblocks_norm;
     if (hipHostMalloc(&s->hnorm, sizeof(double), hipHostMallocDefault) != hipSuccess) return fail(STZ_ERR_HIP);
@@ -855,11 +885,7 @@ int stz_mg_v_cycle(stz_sim *s) {
     CHECK_SIM(s);
     if (s->viscdt_prev < 0) { set_error("stz_mg_setup has not been called"); return STZ_ERR_STATE; }
     int e;
-    if (s->precision == STZ_PREC_MIXED) {
-        LevelArgs<double> t = s->top64();
-        if ((e = mg_resid64_to32(t, s->mg32.args.lv[0].b, s->mg32.args.lv[0].z, s->partial, s->dnorm, s->stream)))
-            return e;
-    }
+    if (s->precision == STZ_PREC_MIXED && (e = refine_resid(s))) return e;
     if ((e = one_cycle(s, false))) return e;
     STZ_HIP(hipStreamSynchronize(s->stream));
     return STZ_OK;
@@ -884,8 +910,15 @@ int stz_mg_smooth(stz_sim *s, int level, int sweeps) {
         set_error("bad level %d or sweeps %d", level, sweeps);
         return STZ_ERR_ARG;
     }
-    if (s->precision == STZ_PREC_MIXED) return mg_smooth(s->mg32.args.lv[level], sweeps, s->stream);
-    return mg_smooth(s->mg64.args.lv[level], sweeps, s->stream);
+    int e;
+    if (s->precision == STZ_PREC_MIXED) {
+        LevelArgs<float> &L = s->mg32.args.lv[level];
+        return s->cfg.kernels == 0 ? mg_smooth_level(L, sweeps, s->stream) : mg_smooth(L, sweeps, s->stream);
+    }
+    LevelArgs<double> &L = s->mg64.args.lv[level];
+    if (s->cfg.kernels != 0) return mg_smooth(L, sweeps, s->stream);
+    if ((e = mg_smooth_level(L, sweeps, s->stream))) return e;
+    return level == 0 ? top_home(s) : STZ_OK;
 }
 
 int stz_mg_upload_src(stz_sim *s, const double *h) {
