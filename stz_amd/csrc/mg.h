@@ -20,6 +20,10 @@ template <typename T> struct LevelArgs {
     const T *S;
     int rs, cs;
     int xodd;  // m odd: the seam column i = m-1 gets a colour of its own
+    // x-invariant levels: every row j with nb <= j < n-nb has the stencil Si
+    // (the rows near the walls differ), so kernels can keep it in registers
+    const T *Si;
+    int nb;
 };
 
 // A whole V-cycle: levels 0..L-1 (0 = top); levels >= lc are run by a single
@@ -29,7 +33,14 @@ template <typename T> struct CoarseArgs {
     int nu1, nu2;
     const T *binv;  // dense inverse of the coarsest operator, (2mn)^2
     LevelArgs<T> lv[kMaxLevels];
+    // LDS layout of the workgroup solver, in elements of T
+    int zoff[kMaxLevels], boff[kMaxLevels], soff[kMaxLevels], binv_off;
+    int lds_elems;
 };
+
+// LDS layout for levels lc..L-1 (z, b, stencils, bottom inverse); returns
+// the number of elements of T
+template <typename T> int coarse_layout(CoarseArgs<T> &A);
 
 template <typename T> int mg_smooth(const LevelArgs<T> &L, int sweeps, hipStream_t st);
 template <typename T> int mg_restrict(const LevelArgs<T> &F, const LevelArgs<T> &C, hipStream_t st);

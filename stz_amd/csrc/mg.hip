@@ -129,91 +129,224 @@ __global__ void __launch_bounds__(256) correct64_kernel(double *z, const float *
 }
 
 // ---------------------------------------------------------------------------
-// workgroup coarse solver: levels lc..L-1, z and b in LDS
+// workgroup coarse solver: levels lc..L-1 entirely in LDS (z, b, stencils and
+// the coarsest level's dense inverse), 256 threads, one barrier per pass.
+// z and b of a level are stored colour-split: node (i,j) in parity array
+// (i&1) + 2(j&1) at ((j>>1), (i>>1)), so the lanes of a colour pass, and the
+// neighbours they read, are at consecutive addresses (no bank conflicts).
+// All addressing is relative to the shared array, so accesses are LDS ones.
 // ---------------------------------------------------------------------------
+template <typename T> struct WGLevel {
+    int m, n, rs, cs, xodd, hm, hn;
+    int zo, bo, so;
+};
+
+template <typename T> __device__ __forceinline__ static WGLevel<T> wg_level(const CoarseArgs<T> &A, int l) {
+    WGLevel<T> W;
+    W.m = A.lv[l].m;
+    W.n = A.lv[l].n;
+    W.rs = A.lv[l].rs;
+    W.cs = A.lv[l].cs;
+    W.xodd = A.lv[l].xodd;
+    W.hm = (W.m + 1) >> 1;
+    W.hn = (W.n + 1) >> 1;
+    W.zo = A.zoff[l];
+    W.bo = A.boff[l];
+    W.so = A.soff[l];
+    return W;
+}
+
+// element offset of node (i,j)'s x component in a colour-split level array
+template <typename T> __device__ __forceinline__ static int cidx(const WGLevel<T> &W, int i, int j) {
+    return 2 * ((((i & 1) + 2 * (j & 1)) * W.hn + (j >> 1)) * W.hm + (i >> 1));
+}
+
 template <typename T>
-__device__ static void wg_smooth(const LevelArgs<T> &L, int sweeps) {
-    const int hm = (L.m + 1) >> 1, hn = (L.n + 1) >> 1;
-    for (int sw = 0; sw < sweeps; sw++) {
-        for (int cy = 0; cy < 2; cy++)
-            for (int cx = 0; cx < 2 + L.xodd; cx++) {
-                const int cnt = cx == 2 ? hn : hm * hn;
-                for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-                    int i, j;
-                    if (cx == 2) { i = L.m - 1; j = 2 * t + cy; }
-                    else { const int ty = t / hm; i = 2 * (t - ty * hm) + cx; j = 2 * ty + cy; }
-                    if (i >= L.m || j >= L.n) continue;
-                    if (cx < 2 && L.xodd && i == L.m - 1) continue;
-                    gs_node(L, i, j);
-                }
-                __syncthreads();
-            }
+__device__ __forceinline__ static void wg_offdiag(const T *sm, const WGLevel<T> &W, int i, int j, const T *s, T &ax,
+                                                  T &ay) {
+    const int m = W.m;
+    const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
+    const T *z = sm + W.zo;
+    ax = 0;
+    ay = 0;
+    if (j > 0) {
+        acc2(ax, ay, s + 0, z + cidx(W, il, j - 1));
+        acc2(ax, ay, s + 4, z + cidx(W, i, j - 1));
+        acc2(ax, ay, s + 8, z + cidx(W, ir, j - 1));
+    }
+    acc2(ax, ay, s + 12, z + cidx(W, il, j));
+    acc2(ax, ay, s + 20, z + cidx(W, ir, j));
+    if (j < W.n - 1) {
+        acc2(ax, ay, s + 24, z + cidx(W, il, j + 1));
+        acc2(ax, ay, s + 28, z + cidx(W, i, j + 1));
+        acc2(ax, ay, s + 32, z + cidx(W, ir, j + 1));
     }
 }
 
-template <typename T> __global__ void __launch_bounds__(1024) coarse_solve_kernel(CoarseArgs<T> A) {
+template <typename T> __device__ __forceinline__ static void wg_resid(const T *sm, const WGLevel<T> &W, int i, int j,
+                                                                      T &rx, T &ry) {
+    const T *s = sm + W.so + (j * W.rs + i * W.cs) * 40;
+    T ax, ay;
+    wg_offdiag(sm, W, i, j, s, ax, ay);
+    const int k = cidx(W, i, j);
+    const T zx = sm[W.zo + k], zy = sm[W.zo + k + 1];
+    rx = sm[W.bo + k] - ax - (s[16] * zx + s[17] * zy);
+    ry = sm[W.bo + k + 1] - ay - (s[18] * zx + s[19] * zy);
+}
+
+template <typename T> __device__ static void wg_smooth(T *sm, const WGLevel<T> &W, int sweeps) {
+    const int hm = W.hm, hn = W.hn;
+    for (int sw = 0; sw < sweeps; sw++)
+        for (int cy = 0; cy < 2; cy++)
+            for (int cx = 0; cx < 2 + W.xodd; cx++) {
+                const int cnt = cx == 2 ? hn : hm * hn;
+                for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+                    int i, j;
+                    if (cx == 2) { i = W.m - 1; j = 2 * t + cy; }
+                    else { const int ty = t / hm; i = 2 * (t - ty * hm) + cx; j = 2 * ty + cy; }
+                    if (i >= W.m || j >= W.n) continue;
+                    if (cx < 2 && W.xodd && i == W.m - 1) continue;
+                    const T *s = sm + W.so + (j * W.rs + i * W.cs) * 40;
+                    T ax, ay;
+                    wg_offdiag(sm, W, i, j, s, ax, ay);
+                    const int k = cidx(W, i, j);
+                    const T rx = sm[W.bo + k] - ax, ry = sm[W.bo + k + 1] - ay;
+                    sm[W.zo + k] = s[36] * rx + s[37] * ry;
+                    sm[W.zo + k + 1] = s[38] * rx + s[39] * ry;
+                }
+                __syncthreads();
+            }
+}
+
+template <typename T> __global__ void __launch_bounds__(256) coarse_solve_kernel(CoarseArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *smem = reinterpret_cast<T *>(smem_raw);
-    // level l with z and b redirected to its LDS slice
-    auto lds = [&](int l) {
-        size_t off = 0;
-        for (int k = A.lc; k < l; k++) off += 4 * (size_t)A.lv[k].m * A.lv[k].n;
-        LevelArgs<T> L = A.lv[l];
-        L.z = smem + off;
-        L.b = smem + off + 2 * L.m * L.n;
-        return L;
-    };
-    // load the restricted source of level lc, zero its solution
-    {
-        const LevelArgs<T> L = lds(A.lc);
-        const T *gb = A.lv[A.lc].b;
-        const int cnt = 2 * L.m * L.n;
-        for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-            L.b[t] = gb[t];
-            L.z[t] = 0;
-        }
-        __syncthreads();
+    T *sm = reinterpret_cast<T *>(smem_raw);
+    const int tid = threadIdx.x, NT = blockDim.x;
+    // prologue: stencils of every level, source of level lc, bottom inverse
+    for (int l = A.lc; l < A.L; l++) {
+        const int cnt = (A.lv[l].cs ? A.lv[l].m * A.lv[l].n : A.lv[l].n) * 40;
+        const T *src = A.lv[l].S;
+        for (int t = tid; t < cnt; t += NT) sm[A.soff[l] + t] = src[t];
     }
+    {
+        const WGLevel<T> W = wg_level(A, A.lc);
+        const T *gb = A.lv[A.lc].b;
+        for (int t = tid; t < W.m * W.n; t += NT) {
+            const int j = t / W.m, i = t - j * W.m, k = cidx(W, i, j);
+            sm[W.bo + k] = gb[2 * t];
+            sm[W.bo + k + 1] = gb[2 * t + 1];
+            sm[W.zo + k] = 0;
+            sm[W.zo + k + 1] = 0;
+        }
+        const int N = 2 * A.lv[A.L - 1].m * A.lv[A.L - 1].n;
+        for (int t = tid; t < N * N; t += NT) sm[A.binv_off + t] = A.binv[t];
+    }
+    __syncthreads();
     // down
     for (int l = A.lc; l < A.L - 1; l++) {
-        const LevelArgs<T> F = lds(l), C = lds(l + 1);
-        wg_smooth(F, A.nu1);
-        for (int t = threadIdx.x; t < C.m * C.n; t += blockDim.x) {
+        const WGLevel<T> F = wg_level(A, l), C = wg_level(A, l + 1);
+        wg_smooth(sm, F, A.nu1);
+        for (int t = tid; t < C.m * C.n; t += NT) {
             const int J = t / C.m, I = t - J * C.m;
-            restrict_node(F, C, I, J);
+            const int fx0 = 2 * I, fy0 = ((F.n & 1) == 0 && J == C.n - 1) ? F.n - 1 : 2 * J;
+            T bx = 0, by = 0;
+            for (int b = -1; b <= 1; b++) {
+                const int fy = fy0 + b;
+                const float wy = pw_bounded(fy, F.n, C.n, J);
+                if (wy == 0.f) continue;
+                for (int a = -1; a <= 1; a++) {
+                    const int fx = fx0 + a;
+                    const float wx = pw_periodic(fx, F.m, C.m, I);
+                    if (wx == 0.f) continue;
+                    const int fi = fx < 0 ? fx + F.m : (fx >= F.m ? fx - F.m : fx);
+                    T rx, ry;
+                    wg_resid(sm, F, fi, fy, rx, ry);
+                    const T w = (T)(wx * wy);
+                    bx += w * rx;
+                    by += w * ry;
+                }
+            }
+            const int k = cidx(C, I, J);
+            sm[C.bo + k] = bx;
+            sm[C.bo + k + 1] = by;
+            sm[C.zo + k] = 0;
+            sm[C.zo + k + 1] = 0;
         }
         __syncthreads();
     }
-    // bottom: exact solve with the dense inverse
+    // bottom: exact solve with the dense inverse (natural unknown order)
     {
-        const LevelArgs<T> B = lds(A.L - 1);
+        const WGLevel<T> B = wg_level(A, A.L - 1);
         const int N = 2 * B.m * B.n;
-        for (int r = threadIdx.x; r < N; r += blockDim.x) {
-            const T *row = A.binv + (size_t)r * N;
+        for (int r = tid; r < N; r += NT) {
+            const T *row = sm + A.binv_off + r * N;
             T acc = 0;
-            for (int k = 0; k < N; k++) acc += row[k] * B.b[k];
-            B.z[r] = acc;
+            for (int k = 0; k < N; k++) {
+                const int node = k >> 1, j = node / B.m, i = node - j * B.m;
+                acc += row[k] * sm[B.bo + cidx(B, i, j) + (k & 1)];
+            }
+            const int node = r >> 1, j = node / B.m, i = node - j * B.m;
+            sm[B.zo + cidx(B, i, j) + (r & 1)] = acc;
         }
         __syncthreads();
     }
     // up
     for (int l = A.L - 2; l >= A.lc; l--) {
-        const LevelArgs<T> F = lds(l), C = lds(l + 1);
-        for (int t = threadIdx.x; t < F.m * F.n; t += blockDim.x) {
+        const WGLevel<T> F = wg_level(A, l), C = wg_level(A, l + 1);
+        for (int t = tid; t < F.m * F.n; t += NT) {
             const int j = t / F.m, i = t - j * F.m;
-            prolong_node(C, F, i, j);
+            const int sc = C.m;
+            int cx0, cx1, cy0, cy1;
+            T wx0, wx1, wy0, wy1;
+            if ((i & 1) == 0) { cx0 = i >> 1; wx0 = 1; cx1 = cx0; wx1 = 0; }
+            else { cx0 = i >> 1; cx1 = cx0 + 1 == sc ? 0 : cx0 + 1; wx0 = wx1 = (T)0.5; }
+            if ((F.n & 1) == 0 && j == F.n - 1) { cy0 = C.n - 1; wy0 = 1; cy1 = cy0; wy1 = 0; }
+            else if ((j & 1) == 0) { cy0 = j >> 1; wy0 = 1; cy1 = cy0; wy1 = 0; }
+            else { cy0 = j >> 1; cy1 = cy0 + 1; wy0 = wy1 = (T)0.5; }
+            const T *zc = sm + C.zo;
+            const int a00 = cidx(C, cx0, cy0), a10 = cidx(C, cx1, cy0), a01 = cidx(C, cx0, cy1),
+                      a11 = cidx(C, cx1, cy1);
+            const int k = cidx(F, i, j);
+            sm[F.zo + k] += wy0 * (wx0 * zc[a00] + wx1 * zc[a10]) + wy1 * (wx0 * zc[a01] + wx1 * zc[a11]);
+            sm[F.zo + k + 1] +=
+                wy0 * (wx0 * zc[a00 + 1] + wx1 * zc[a10 + 1]) + wy1 * (wx0 * zc[a01 + 1] + wx1 * zc[a11 + 1]);
         }
         __syncthreads();
-        wg_smooth(F, A.nu2);
+        wg_smooth(sm, F, A.nu2);
     }
-    // store the level-lc solution
+    // store the level-lc solution in natural order
     {
-        const LevelArgs<T> L = lds(A.lc);
+        const WGLevel<T> W = wg_level(A, A.lc);
         T *gz = A.lv[A.lc].z;
-        const int cnt = 2 * L.m * L.n;
-        for (int t = threadIdx.x; t < cnt; t += blockDim.x) gz[t] = L.z[t];
+        for (int t = tid; t < W.m * W.n; t += NT) {
+            const int j = t / W.m, i = t - j * W.m, k = cidx(W, i, j);
+            gz[2 * t] = sm[W.zo + k];
+            gz[2 * t + 1] = sm[W.zo + k + 1];
+        }
     }
 }
+
+template <typename T> int coarse_layout(CoarseArgs<T> &A) {
+    int off = 0;
+    for (int l = A.lc; l < A.L; l++) {
+        // colour-split storage: four parity arrays of ceil(m/2) x ceil(n/2)
+        const int cnt = 8 * ((A.lv[l].m + 1) >> 1) * ((A.lv[l].n + 1) >> 1);
+        A.zoff[l] = off;
+        off += cnt;
+        A.boff[l] = off;
+        off += cnt;
+        A.soff[l] = off;
+        off += (A.lv[l].cs ? A.lv[l].m * A.lv[l].n : A.lv[l].n) * 40;
+        off = (off + 3) & ~3;
+    }
+    const int N = 2 * A.lv[A.L - 1].m * A.lv[A.L - 1].n;
+    A.binv_off = off;
+    off += N * N;
+    A.lds_elems = off;
+    return off;
+}
+template int coarse_layout<double>(CoarseArgs<double> &);
+template int coarse_layout<float>(CoarseArgs<float> &);
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -245,14 +378,10 @@ template <typename T> int mg_prolong(const LevelArgs<T> &C, const LevelArgs<T> &
     return STZ_OK;
 }
 
-template <typename T> size_t coarse_lds_bytes(const CoarseArgs<T> &A) {
-    size_t cnt = 0;
-    for (int l = A.lc; l < A.L; l++) cnt += 4 * (size_t)A.lv[l].m * A.lv[l].n;
-    return cnt * sizeof(T);
-}
+template <typename T> size_t coarse_lds_bytes(const CoarseArgs<T> &A) { return (size_t)A.lds_elems * sizeof(T); }
 
 template <typename T> int mg_coarse(const CoarseArgs<T> &A, hipStream_t st) {
-    hipLaunchKernelGGL(coarse_solve_kernel<T>, dim3(1), dim3(1024), coarse_lds_bytes(A), st, A);
+    hipLaunchKernelGGL(coarse_solve_kernel<T>, dim3(1), dim3(256), coarse_lds_bytes(A), st, A);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }

@@ -8,11 +8,12 @@
 //   interior is still exact, and only it is written back.  S sweeps therefore
 //   cost one read and one write of z and one read of b, instead of 4*S
 //   kernel launches each reading all of z.  Each thread owns 2x2 blocks of
-//   nodes (one node of every colour) and keeps their b values in registers;
-//   the per-row operator stencils live in LDS.  Optionally the kernel starts
-//   from z = 0 (coarse levels, correction equation) or adds the bilinear
-//   prolongation of the coarser level's z while loading (fused coarse-grid
-//   correction, tgmg.in.cc:552-621).
+//   nodes (one node of every colour) and keeps their b values in registers.
+//   Rows away from the walls all share one stencil, which every thread keeps
+//   in registers; only the few wall-adjacent rows read their stencil from
+//   LDS.  Optionally the kernel starts from z = 0 (coarse levels, correction
+//   equation) or adds the bilinear prolongation of the coarser level's z
+//   while loading (fused coarse-grid correction, tgmg.in.cc:552-621).
 // restrict_tile_kernel: one workgroup per CX x CY coarse tile; computes every
 //   fine residual it needs exactly once into LDS, then the P^T-weighted sums
 //   (tgmg.in.cc:626-647).
@@ -35,23 +36,54 @@ enum { TILE_LOAD = 0, TILE_ZERO = 1, TILE_PROLONG = 2 };
 
 template <typename T> struct alignas(2 * sizeof(T)) vec2s { T x, y; };
 
+// (ax,ay) = A' z, the off-diagonal part, from the eight neighbours in
+// stencil order (dl,dc,dr,cl,cr,ul,uc,ur); SK(k) yields stencil value k
+#define STZ_ACC(SK, K, P, ax, ay)                                                  \
+    {                                                                              \
+        const auto p_ = (P);                                                       \
+        ax += SK(4 * K) * p_.x + SK(4 * K + 1) * p_.y;                             \
+        ay += SK(4 * K + 2) * p_.x + SK(4 * K + 3) * p_.y;                         \
+    }
+#define STZ_OFFDIAG8(SK, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay)                   \
+    {                                                                              \
+        ax = 0;                                                                    \
+        ay = 0;                                                                    \
+        STZ_ACC(SK, 0, dl, ax, ay) STZ_ACC(SK, 1, dc, ax, ay)                      \
+        STZ_ACC(SK, 2, dr, ax, ay) STZ_ACC(SK, 3, cl, ax, ay)                      \
+        STZ_ACC(SK, 5, cr, ax, ay) STZ_ACC(SK, 6, ul, ax, ay)                      \
+        STZ_ACC(SK, 7, uc, ax, ay) STZ_ACC(SK, 8, ur, ax, ay)                      \
+    }
+// the same for a row-major neighbourhood: rows zd, zc, zu around column 0
+#define STZ_OFFDIAG(SK, zd, zc, zu, ax, ay)                                        \
+    STZ_OFFDIAG8(SK, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay)
+
 template <typename T, int TX, int TY, int S, int MODE>
 __global__ void __launch_bounds__(256) smooth_tile_kernel(LevelArgs<T> L, T *__restrict__ zout, LevelArgs<T> Cz) {
     constexpr int H = 4 * S, RX = TX + 2 * H, RY = TY + 2 * H;
     constexpr int BX = RX / 2, BY = RY / 2, NB = BX * BY, NT = 256;
     constexpr int BPT = (NB + NT - 1) / NT;
     static_assert(RX % 2 == 0 && RY % 2 == 0, "even region");
-    __shared__ vec2s<T> zs[RY * RX];
+    // colour-split LDS: node (li,lj) lives in array c = (li&1) + 2(lj&1) at
+    // block index (lj>>1)*BX + (li>>1); a colour pass then reads each
+    // neighbour from another colour's array at consecutive addresses across
+    // the lanes (no bank conflicts), with compile-time offsets per colour
+    __shared__ vec2s<T> zs[4 * NB];
     __shared__ T ss[RY * 40];
     const int tid = threadIdx.x;
-    const int m = L.m, n = L.n;
+    const int m = L.m, n = L.n, nb = L.nb;
     const int i0 = blockIdx.x * TX - H, j0 = blockIdx.y * TY - H;
 
-    // per-row stencils (rows outside the level are never used)
-    for (int t = tid; t < RY * 40; t += NT) {
-        const int r = t / 40, k = t - r * 40, j = j0 + r;
-        ss[t] = (j >= 0 && j < n) ? L.S[(size_t)j * L.rs * 40 + k] : (T)0;
-    }
+    // interior stencil: wave-uniform, held in registers
+    T si[40];
+#pragma unroll
+    for (int k = 0; k < 40; k++) si[k] = L.Si[k];
+    // wall-adjacent rows' stencils, only if this tile reaches them
+    const bool edge_tile = j0 < nb || j0 + RY > n - nb;
+    if (edge_tile)
+        for (int t = tid; t < RY * 40; t += NT) {
+            const int r = t / 40, k = t - r * 40, j = j0 + r;
+            ss[t] = (j >= 0 && j < n) ? L.S[(size_t)j * 40 + k] : (T)0;
+        }
     // z into LDS (zero outside the level), b into registers
     T bx[BPT][4], by[BPT][4];
 #pragma unroll
@@ -96,8 +128,8 @@ __global__ void __launch_bounds__(256) smooth_tile_kernel(LevelArgs<T> L, T *__r
                           wy1 * (wx0 * z1[2 * cx0 + 1] + wx1 * z1[2 * cx1 + 1]);
                 }
             }
-            zs[lj * RX + li].x = zx;
-            zs[lj * RX + li].y = zy;
+            zs[c * NB + blk].x = zx;
+            zs[c * NB + blk].y = zy;
         }
     }
     __syncthreads();
@@ -115,25 +147,34 @@ __global__ void __launch_bounds__(256) smooth_tile_kernel(LevelArgs<T> L, T *__r
                 const int li = 2 * bxi + (c & 1), lj = 2 * byi + (c >> 1);
                 const int gj = j0 + lj;
                 if (li == 0 || li == RX - 1 || lj == 0 || lj == RY - 1 || gj < 0 || gj >= n) continue;
-                const T *s = ss + lj * 40;
-                const vec2s<T> *zd = zs + (lj - 1) * RX + li, *zc = zs + lj * RX + li,
-                               *zu = zs + (lj + 1) * RX + li;
-                T ax = 0, ay = 0;
-#define ACC(K, P)                                           \
-    {                                                       \
-        const T px = (P).x, py = (P).y;                     \
-        ax += s[4 * K] * px + s[4 * K + 1] * py;            \
-        ay += s[4 * K + 2] * px + s[4 * K + 3] * py;        \
-    }
-                ACC(0, zd[-1]) ACC(1, zd[0]) ACC(2, zd[1])
-                ACC(3, zc[-1]) ACC(5, zc[1])
-                ACC(6, zu[-1]) ACC(7, zu[0]) ACC(8, zu[1])
-#undef ACC
-                const T rx = bx[q][c] - ax, ry = by[q][c] - ay;
+                // neighbours: x-neighbours have colour cX, y-neighbours cY,
+                // diagonal ones cD; block offsets depend only on (cx, cy)
+                const int cx = c & 1, cy = c >> 1;
+                const int cX = (1 - cx) + 2 * cy, cY = cx + 2 * (1 - cy), cD = (1 - cx) + 2 * (1 - cy);
+                const int xl = cx ? 0 : -1, xr = cx ? 1 : 0, yd = cy ? 0 : -BX, yu = cy ? BX : 0;
+                const vec2s<T> *zX = zs + cX * NB + blk, *zY = zs + cY * NB + blk, *zD = zs + cD * NB + blk;
+                const vec2s<T> dl = zD[yd + xl], dc = zY[yd], dr = zD[yd + xr];
+                const vec2s<T> cl = zX[xl], cr = zX[xr];
+                const vec2s<T> ul = zD[yu + xl], uc = zY[yu], ur = zD[yu + xr];
+                T ax, ay;
                 vec2s<T> out;
-                out.x = s[36] * rx + s[37] * ry;
-                out.y = s[38] * rx + s[39] * ry;
-                zs[lj * RX + li] = out;
+                if (gj < nb || gj >= n - nb) {
+                    const T *s = ss + lj * 40;
+#define SK_LDS(k) s[k]
+                    STZ_OFFDIAG8(SK_LDS, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay)
+                    const T rx = bx[q][c] - ax, ry = by[q][c] - ay;
+                    out.x = s[36] * rx + s[37] * ry;
+                    out.y = s[38] * rx + s[39] * ry;
+#undef SK_LDS
+                } else {
+#define SK_REG(k) si[k]
+                    STZ_OFFDIAG8(SK_REG, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay)
+                    const T rx = bx[q][c] - ax, ry = by[q][c] - ay;
+                    out.x = si[36] * rx + si[37] * ry;
+                    out.y = si[38] * rx + si[39] * ry;
+#undef SK_REG
+                }
+                zs[c * NB + blk] = out;
             }
             __syncthreads();
         }
@@ -145,8 +186,9 @@ __global__ void __launch_bounds__(256) smooth_tile_kernel(LevelArgs<T> L, T *__r
         const int gj = j0 + lj, gi = i0 + li;
         if (gj >= n) continue;
         const size_t k = 2 * ((size_t)gi + (size_t)m * gj);
-        zout[k] = zs[lj * RX + li].x;
-        zout[k + 1] = zs[lj * RX + li].y;
+        const vec2s<T> v = zs[((li & 1) + 2 * (lj & 1)) * NB + (lj >> 1) * BX + (li >> 1)];
+        zout[k] = v.x;
+        zout[k + 1] = v.y;
     }
 }
 
@@ -160,14 +202,17 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
     __shared__ vec2s<T> zs[ZY * ZX];
     __shared__ vec2s<T> rs[RYr * RXr];
     __shared__ T ss[RYr * 40];
-    const int tid = threadIdx.x, m = F.m, n = F.n;
+    const int tid = threadIdx.x, m = F.m, n = F.n, nb = F.nb;
     const int I0 = blockIdx.x * CX, J0 = blockIdx.y * CY;
     const int zi0 = 2 * I0 - 2, zj0 = 2 * J0 - 3;
+    T si[40];
+#pragma unroll
+    for (int k = 0; k < 40; k++) si[k] = F.Si[k];
     for (int t = tid; t < ZX * ZY; t += NT) {
         const int lj = t / ZX, li = t - lj * ZX;
         const int gj = zj0 + lj;
         int gi = zi0 + li;
-        gi = gi < 0 ? gi + m : (gi >= m ? gi - m : gi);
+        gi = ((gi % m) + m) % m;
         vec2s<T> v;
         v.x = 0;
         v.y = 0;
@@ -178,10 +223,11 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
         }
         zs[t] = v;
     }
-    for (int t = tid; t < RYr * 40; t += NT) {
-        const int r = t / 40, k = t - r * 40, j = zj0 + 1 + r;
-        ss[t] = (j >= 0 && j < n) ? F.S[(size_t)j * F.rs * 40 + k] : (T)0;
-    }
+    if (zj0 + 1 < nb || zj0 + 1 + RYr > n - nb)
+        for (int t = tid; t < RYr * 40; t += NT) {
+            const int r = t / 40, k = t - r * 40, j = zj0 + 1 + r;
+            ss[t] = (j >= 0 && j < n) ? F.S[(size_t)j * 40 + k] : (T)0;
+        }
     __syncthreads();
     for (int t = tid; t < RXr * RYr; t += NT) {
         const int lj = t / RXr, li = t - lj * RXr;
@@ -191,23 +237,27 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
         r.y = 0;
         if (gj >= 0 && gj < n) {
             int gi = zi0 + 1 + li;
-            gi = gi < 0 ? gi + m : (gi >= m ? gi - m : gi);
-            const T *s = ss + lj * 40;
+            gi = ((gi % m) + m) % m;
             const vec2s<T> *zd = zs + lj * ZX + li + 1, *zc = zd + ZX, *zu = zc + ZX;
-            T ax = 0, ay = 0;
-#define ACC(K, P)                                           \
-    {                                                       \
-        const T px = (P).x, py = (P).y;                     \
-        ax += s[4 * K] * px + s[4 * K + 1] * py;            \
-        ay += s[4 * K + 2] * px + s[4 * K + 3] * py;        \
-    }
-            ACC(0, zd[-1]) ACC(1, zd[0]) ACC(2, zd[1])
-            ACC(3, zc[-1]) ACC(4, zc[0]) ACC(5, zc[1])
-            ACC(6, zu[-1]) ACC(7, zu[0]) ACC(8, zu[1])
-#undef ACC
+            T ax, ay, cx, cy;
+            const vec2s<T> z0 = zc[0];
+            if (gj < nb || gj >= n - nb) {
+                const T *s = ss + lj * 40;
+#define SK_LDS(k) s[k]
+                STZ_OFFDIAG(SK_LDS, zd, zc, zu, ax, ay)
+                cx = s[16] * z0.x + s[17] * z0.y;
+                cy = s[18] * z0.x + s[19] * z0.y;
+#undef SK_LDS
+            } else {
+#define SK_REG(k) si[k]
+                STZ_OFFDIAG(SK_REG, zd, zc, zu, ax, ay)
+                cx = si[16] * z0.x + si[17] * z0.y;
+                cy = si[18] * z0.x + si[19] * z0.y;
+#undef SK_REG
+            }
             const size_t k = 2 * ((size_t)gi + (size_t)m * gj);
-            r.x = F.b[k] - ax;
-            r.y = F.b[k + 1] - ay;
+            r.x = F.b[k] - ax - cx;
+            r.y = F.b[k + 1] - ay - cy;
         }
         rs[t] = r;
     }
@@ -243,14 +293,17 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
     }
 }
 
-// one FP64 refinement step of the mixed form on the top level (64x16 tiles)
+// FP64 residual of the top level (64x16 tiles) -> FP32 source, |r|^2 partials
 __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float *b32, double *partial) {
     constexpr int TX = 64, TY = 16, ZX = TX + 2, ZY = TY + 2, NT = 256;
     __shared__ vec2s<double> zs[ZY * ZX];
     __shared__ double ss[TY * 40];
     __shared__ double ws[4];
-    const int tid = threadIdx.x, m = L.m, n = L.n;
+    const int tid = threadIdx.x, m = L.m, n = L.n, nb = L.nb;
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    double si[40];
+#pragma unroll
+    for (int k = 0; k < 40; k++) si[k] = L.Si[k];
     for (int t = tid; t < ZX * ZY; t += NT) {
         const int lj = t / ZX, li = t - lj * ZX;
         const int gj = j0 - 1 + lj;
@@ -266,31 +319,36 @@ __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float 
         }
         zs[t] = v;
     }
-    for (int t = tid; t < TY * 40; t += NT) {
-        const int r = t / 40, k = t - r * 40, j = j0 + r;
-        ss[t] = j < n ? L.S[(size_t)j * L.rs * 40 + k] : 0.0;
-    }
+    if (j0 < nb || j0 + TY > n - nb)
+        for (int t = tid; t < TY * 40; t += NT) {
+            const int r = t / 40, k = t - r * 40, j = j0 + r;
+            ss[t] = j < n ? L.S[(size_t)j * 40 + k] : 0.0;
+        }
     __syncthreads();
     double acc = 0;
     for (int t = tid; t < TX * TY; t += NT) {
         const int li = t % TX, lj = t / TX;
         const int gi = i0 + li, gj = j0 + lj;
         if (gi >= m || gj >= n) continue;
-        const double *s = ss + lj * 40;
         const vec2s<double> *zd = zs + lj * ZX + li + 1, *zc = zd + ZX, *zu = zc + ZX;
-        double ax = 0, ay = 0;
-#define ACC(K, P)                                           \
-    {                                                       \
-        const double px = (P).x, py = (P).y;                \
-        ax += s[4 * K] * px + s[4 * K + 1] * py;            \
-        ay += s[4 * K + 2] * px + s[4 * K + 3] * py;        \
-    }
-        ACC(0, zd[-1]) ACC(1, zd[0]) ACC(2, zd[1])
-        ACC(3, zc[-1]) ACC(4, zc[0]) ACC(5, zc[1])
-        ACC(6, zu[-1]) ACC(7, zu[0]) ACC(8, zu[1])
-#undef ACC
+        const vec2s<double> z0 = zc[0];
+        double ax, ay, cx, cy;
+        if (gj < nb || gj >= n - nb) {
+            const double *s = ss + lj * 40;
+#define SK_LDS(k) s[k]
+            STZ_OFFDIAG(SK_LDS, zd, zc, zu, ax, ay)
+            cx = s[16] * z0.x + s[17] * z0.y;
+            cy = s[18] * z0.x + s[19] * z0.y;
+#undef SK_LDS
+        } else {
+#define SK_REG(k) si[k]
+            STZ_OFFDIAG(SK_REG, zd, zc, zu, ax, ay)
+            cx = si[16] * z0.x + si[17] * z0.y;
+            cy = si[18] * z0.x + si[19] * z0.y;
+#undef SK_REG
+        }
         const size_t k = 2 * ((size_t)gi + (size_t)m * gj);
-        const double rx = L.b[k] - ax, ry = L.b[k + 1] - ay;
+        const double rx = L.b[k] - ax - cx, ry = L.b[k + 1] - ay - cy;
         b32[k] = (float)rx;
         b32[k + 1] = (float)ry;
         acc += rx * rx + ry * ry;
@@ -305,19 +363,20 @@ __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename T> struct TileShape;
-template <> struct TileShape<float> { static constexpr int TX = 64, TY = 32; };
-template <> struct TileShape<double> { static constexpr int TX = 64, TY = 32; };
+template <typename T> bool tiled_ok(const LevelArgs<T> &L) { return L.cs == 0 && L.m % 32 == 0 && L.m >= 32; }
 
-template <typename T> bool tiled_ok(const LevelArgs<T> &L) {
-    return L.cs == 0 && L.m % TileShape<T>::TX == 0 && L.m >= TileShape<T>::TX;
-}
-
+// large tiles (64x32) when they give at least one workgroup per CU, else
+// 32x16 tiles: on the small levels the kernel is latency-bound and more,
+// lighter workgroups finish sooner
 template <typename T, int S, int MODE>
 static void launch_tile(const LevelArgs<T> &L, T *zout, const LevelArgs<T> &C, hipStream_t st) {
-    constexpr int TX = TileShape<T>::TX, TY = TileShape<T>::TY;
-    const dim3 g(L.m / TX, (L.n + TY - 1) / TY);
-    hipLaunchKernelGGL((smooth_tile_kernel<T, TX, TY, S, MODE>), g, dim3(256), 0, st, L, zout, C);
+    if (L.m % 64 == 0 && (L.m / 64) * ((L.n + 31) / 32) >= 256) {
+        const dim3 g(L.m / 64, (L.n + 31) / 32);
+        hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE>), g, dim3(256), 0, st, L, zout, C);
+    } else {
+        const dim3 g(L.m / 32, (L.n + 15) / 16);
+        hipLaunchKernelGGL((smooth_tile_kernel<T, 32, 16, S, MODE>), g, dim3(256), 0, st, L, zout, C);
+    }
 }
 
 // `sweeps` sweeps on level L; the first kernel starts from zero (TILE_ZERO),
@@ -354,11 +413,16 @@ template <typename T> static int smooth_level(LevelArgs<T> &L, const LevelArgs<T
 
 template <typename T> static int restrict_level(const LevelArgs<T> &F, const LevelArgs<T> &C, hipStream_t st) {
     if (F.cs != 0) return mg_restrict(F, C, st);  // per-node stencils: simple kernel (zeroes C.z)
-    constexpr int CX = 32, CY = sizeof(T) == 8 ? 8 : 16;
-    const dim3 g((C.m + CX - 1) / CX, (C.n + CY - 1) / CY);
     // the coarse z must start at zero unless its first smoothing is tiled
     const int zero_z = tiled_ok(C) ? 0 : 1;
-    hipLaunchKernelGGL((restrict_tile_kernel<T, CX, CY>), g, dim3(256), 0, st, F, C, zero_z);
+    constexpr int CY = sizeof(T) == 8 ? 8 : 16;
+    if (((C.m + 31) / 32) * ((C.n + CY - 1) / CY) >= 256) {
+        const dim3 g((C.m + 31) / 32, (C.n + CY - 1) / CY);
+        hipLaunchKernelGGL((restrict_tile_kernel<T, 32, CY>), g, dim3(256), 0, st, F, C, zero_z);
+    } else {
+        const dim3 g((C.m + 15) / 16, (C.n + 7) / 8);
+        hipLaunchKernelGGL((restrict_tile_kernel<T, 16, 8>), g, dim3(256), 0, st, F, C, zero_z);
+    }
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
@@ -381,8 +445,6 @@ template <typename T> int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipSt
 template <typename T> int mg_smooth_level(LevelArgs<T> &L, int sweeps, hipStream_t st) {
     return smooth_level(L, (const LevelArgs<T> *)nullptr, sweeps, TILE_LOAD, st);
 }
-template int mg_smooth_level<double>(LevelArgs<double> &, int, hipStream_t);
-template int mg_smooth_level<float>(LevelArgs<float> &, int, hipStream_t);
 
 int mg_refine_blocks(const LevelArgs<double> &L) { return ((L.m + 63) / 64) * ((L.n + 15) / 16); }
 
@@ -397,5 +459,7 @@ template bool tiled_ok<double>(const LevelArgs<double> &);
 template bool tiled_ok<float>(const LevelArgs<float> &);
 template int mg_vcycle_tiled<double>(CoarseArgs<double> &, bool, hipStream_t);
 template int mg_vcycle_tiled<float>(CoarseArgs<float> &, bool, hipStream_t);
+template int mg_smooth_level<double>(LevelArgs<double> &, int, hipStream_t);
+template int mg_smooth_level<float>(LevelArgs<float> &, int, hipStream_t);
 
 }  // namespace stz

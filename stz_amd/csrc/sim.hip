@@ -37,6 +37,7 @@ void set_error(const char *fmt, ...) {
 // ---------------------------------------------------------------------------
 struct HostLevel {
     int m, n;
+    int nb = 0;              // rows at each wall whose stencil is not the interior one
     bool xinv;               // stencil independent of i
     std::vector<double> S;   // 40 per row (xinv) or per node
     const double *at(int i, int j) const { return &S[(size_t)(xinv ? j : (i + m * j)) * 40]; }
@@ -265,7 +266,7 @@ struct stz_sim {
     stz_solve_info last{};
 
     LevelArgs<double> top64() const {
-        LevelArgs<double> L;
+        LevelArgs<double> L = mg64.args.lv[0];
         L.m = c.m;
         L.n = c.n + 1;
         L.z = vel;
@@ -336,20 +337,27 @@ template <typename T> static int alloc_levels(stz_sim *s, DevMG<T> &mg, bool own
         }
     }
     // first level run by the workgroup solver: the largest suffix of levels
-    // whose z and b fit the LDS budget, never the top level
+    // whose z, b, stencils and bottom inverse fit the LDS budget, never the
+    // top level (stencil sizes assume one per row; mg_build re-checks)
     const size_t budget = 96 * 1024;
     int lc = mg.L - 1;
-    size_t bytes = 4 * (size_t)mg.args.lv[lc].m * mg.args.lv[lc].n * sizeof(T);
     while (lc > 1) {
-        const size_t add = 4 * (size_t)mg.args.lv[lc - 1].m * mg.args.lv[lc - 1].n * sizeof(T);
-        if (bytes + add > budget) break;
-        bytes += add;
+        mg.args.lc = lc - 1;
+        if ((size_t)coarse_layout(mg.args) * sizeof(T) > budget) break;
         lc--;
     }
     if (mg.L == 1) lc = 0;
     mg.args.lc = lc;
-    mg.lds = bytes;
+    mg.lds = (size_t)coarse_layout(mg.args) * sizeof(T);
     return STZ_OK;
+}
+
+template <typename T> static int set_lds_attr(const DevMG<T> &mg) {
+    if (mg.lds > 160 * 1024) {
+        set_error("coarse-level working set (%zu bytes) exceeds LDS", mg.lds);
+        return STZ_ERR_STATE;
+    }
+    return mg_set_lds_limit<T>(mg.lds);
 }
 
 // (re)build operators for viscosity/dt = viscdt (qs_multi::init, qs_multi.cc:31-54)
@@ -365,29 +373,57 @@ static int mg_build(stz_sim *s, double viscdt) {
     }
     int e;
     for (size_t l = 0; l < s->hl.size(); l++) {
-        if ((e = upload(&s->mg64.S[l], s->hl[l].S))) return e;
+        HostLevel &H = s->hl[l];
+        // rows at each wall whose stencil differs from the middle row
+        H.nb = H.n;
+        if (H.xinv && H.n >= 3) {
+            const double *mid = &H.S[(size_t)(H.n / 2) * 40];
+            int nb = 0;
+            for (int j = 0; j < H.n / 2; j++) {
+                const bool lo = std::memcmp(&H.S[(size_t)j * 40], mid, 40 * sizeof(double)) != 0;
+                const bool hi = std::memcmp(&H.S[(size_t)(H.n - 1 - j) * 40], mid, 40 * sizeof(double)) != 0;
+                if (lo || hi) nb = j + 1;
+            }
+            H.nb = nb;
+        }
+        // S holds the row stencils followed by one copy of the interior stencil
+        std::vector<double> Sx(H.S);
+        Sx.insert(Sx.end(), H.S.begin() + (size_t)(H.xinv ? H.n / 2 : 0) * 40,
+                  H.S.begin() + (size_t)(H.xinv ? H.n / 2 : 0) * 40 + 40);
+        if ((e = upload(&s->mg64.S[l], Sx))) return e;
         LevelArgs<double> &A = s->mg64.args.lv[l];
         A.S = s->mg64.S[l];
-        A.rs = s->hl[l].xinv ? 1 : A.m;
-        A.cs = s->hl[l].xinv ? 0 : 1;
+        A.rs = H.xinv ? 1 : A.m;
+        A.cs = H.xinv ? 0 : 1;
+        A.Si = s->mg64.S[l] + H.S.size();
+        A.nb = H.nb;
     }
     if ((e = upload(&s->mg64.binv, binv))) return e;
     s->mg64.args.binv = s->mg64.binv;
+    s->mg64.lds = (size_t)coarse_layout(s->mg64.args) * sizeof(double);
+    if ((e = set_lds_attr(s->mg64))) return e;
     if (s->precision == STZ_PREC_MIXED) {
         for (size_t l = 0; l < s->hl.size(); l++) {
-            if ((e = upload(&s->mg32.S[l], s->hl[l].S))) return e;
+            HostLevel &H = s->hl[l];
+            std::vector<double> Sx(H.S);
+            Sx.insert(Sx.end(), H.S.begin() + (size_t)(H.xinv ? H.n / 2 : 0) * 40,
+                      H.S.begin() + (size_t)(H.xinv ? H.n / 2 : 0) * 40 + 40);
+            if ((e = upload(&s->mg32.S[l], Sx))) return e;
             LevelArgs<float> &A = s->mg32.args.lv[l];
             A.S = s->mg32.S[l];
-            A.rs = s->hl[l].xinv ? 1 : A.m;
-            A.cs = s->hl[l].xinv ? 0 : 1;
+            A.rs = H.xinv ? 1 : A.m;
+            A.cs = H.xinv ? 0 : 1;
+            A.Si = s->mg32.S[l] + H.S.size();
+            A.nb = H.nb;
         }
         if ((e = upload(&s->mg32.binv, binv))) return e;
         s->mg32.args.binv = s->mg32.binv;
+        s->mg32.lds = (size_t)coarse_layout(s->mg32.args) * sizeof(float);
+        if ((e = set_lds_attr(s->mg32))) return e;
     }
     return STZ_OK;
 }
 
-template <typename T> static int set_lds_attr(const DevMG<T> &mg) { return mg_set_lds_limit<T>(mg.lds); }
 
 // mixed form: FP64 residual of the top level -> FP32 source of the next
 // correction equation (and zero correction for the per-colour path)
