@@ -204,7 +204,10 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("colour_kernel_float_bytes_per_launch")
+            tj = json.load(f)
+        # HBM bytes per launch from FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel
+        if tj.get("kernel") == "smooth_tile_kernel<float,64,32,2,0>" and tj.get("level", "").startswith("%dx" % N):
+            traffic = tj.get("traffic_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu and elastic_at in res["fp64"]["kept"] and plastic_at in res["fp64"]["kept"]:

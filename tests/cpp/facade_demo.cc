@@ -3,36 +3,40 @@
 // model and the simulation, initialise the fields, edit fm on the host,
 // upload, step, download, and dump the state for comparison.
 //
-// usage: facade_demo OUT m n steps dt chi_case u_bdry
+// usage: facade_demo OUT m n steps dt chi_case ay by u_bdry
 //                    mu K visc adapt_fac tmult
 //                    TZ c0 tau0 kappa Delta Omegaeps0 chi_inf theta
 #include <cstdio>
+#include <cmath>
 #include <cstdlib>
 
 #include "shear_sim.hh"
 
 int main(int argc, char **argv) {
-    if (argc != 21) {
-        std::fprintf(stderr, "facade_demo: expected 20 arguments, got %d\n", argc - 1);
+    if (argc != 23) {
+        std::fprintf(stderr, "facade_demo: expected 22 arguments, got %d\n", argc - 1);
         return 2;
     }
     const char *out = argv[1];
     int m = std::atoi(argv[2]), n = std::atoi(argv[3]), steps = std::atoi(argv[4]);
     double dt = std::atof(argv[5]);
     int chi_case = std::atoi(argv[6]);
+    double ay = std::atof(argv[7]), by = std::atof(argv[8]);
     double a[14];
-    for (int k = 0; k < 14; k++) a[k] = std::atof(argv[7 + k]);
+    for (int k = 0; k < 14; k++) a[k] = std::atof(argv[9 + k]);
     double u_bdry = a[0], mu = a[1], K = a[2], visc = a[3], adapt_fac = a[4], tmult = a[5];
 
     stz_dynamics_linear_athermal stz(a[6], a[7], a[8], a[9], a[10], a[11], a[12], a[13], 0);
-    shear_sim ss(m, n, -1, 1, -1, 1, mu, K, visc, 0, 0, 1.0, tmult, adapt_fac, u_bdry, &stz, "unused");
+    shear_sim ss(m, n, -1, 1, ay, by, mu, K, visc, 0, 0, 1.0, tmult, adapt_fac, u_bdry, &stz, "unused");
     stz_mg_config c = ss.mg_config();
     c.tol_mult = 1e-12;
     ss.set_mg_config(c);
 
     ss.init_fields(chi_case, 600, 200);
+    // a host-side edit of fm; square roots of distinct integers, so that no
+    // exact ENO ties arise (at a tie, rounding noise picks the branch)
     for (int j = 0; j < n; j++)
-        for (int i = 0; i < m; i++) ss.fm[j * m + i].chi += 1e-3 * ((i * 7 + j * 13) % 17) / 17.0;
+        for (int i = 0; i < m; i++) ss.fm[j * m + i].chi += 1e-4 * std::sqrt(1.0 + (i * 37 + j * 101) % 97);
     ss.time = 0.25;
     ss.upload();
     for (int k = 0; k < steps; k++) ss.step_forward_quasistatic(dt);

@@ -32,7 +32,8 @@ def demo_args(p, out, steps, dt, chi_case):
     s = p["stz"]
     vals = [p["u_bdry"], p["mu"], p["K"], p["viscosity"], p["adapt_fac"], p["tmult"],
             s["TZ"], s["c0"], s["tau0"], s["kappa"], s["Delta"], s["Omegaeps0"], s["chi_inf"], s["theta"]]
-    return [out, str(p["m"]), str(p["n"]), str(steps), repr(dt), str(chi_case)] + [repr(float(v)) for v in vals]
+    return [out, str(p["m"]), str(p["n"]), str(steps), repr(dt), str(chi_case), repr(p["ay"]), repr(p["by"])] + \
+        [repr(float(v)) for v in vals]
 
 
 def test_facade_fails_loudly_without_device(demo, tmp_path):
@@ -51,6 +52,9 @@ def test_facade_fails_loudly_without_device(demo, tmp_path):
 def test_facade_driver_matches_checker(demo, tmp_path, m, n, chi_case):
     print("case facade %dx%d chi_case %d" % (m, n, chi_case), flush=True)
     p = simple_shear(m, n, u_bdry=2e-3)
+    # equal spacing in x and y: the reference's smoother operator
+    # (qs_multi.cc regh/regv) matches its residual operator only then
+    p["ay"], p["by"] = -float(n) / m, float(n) / m
     steps, dt = 3, 0.5
     out = str(tmp_path / "state.bin")
     r = subprocess.run([demo] + demo_args(p, out, steps, dt, chi_case),
@@ -68,7 +72,7 @@ def test_facade_driver_matches_checker(demo, tmp_path, m, n, chi_case):
     o.init_fields(chi_case, 600.0, 200.0)
     f = o.get_fields()
     jj, ii = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")
-    f[6][1:n + 1] += 1e-3 * ((ii * 7 + jj * 13) % 17) / 17.0
+    f[6][1:n + 1] += 1e-4 * np.sqrt(1.0 + (ii * 37 + jj * 101) % 97)
     o.set_fields(f)
     o.time = 0.25
     for _ in range(steps):
