@@ -34,7 +34,7 @@ template <typename T> struct CoarseArgs {
     const T *binv;  // dense inverse of the coarsest operator, (2mn)^2
     LevelArgs<T> lv[kMaxLevels];
     // LDS layout of the workgroup solver, in elements of T
-    int zoff[kMaxLevels], boff[kMaxLevels], soff[kMaxLevels], binv_off;
+    int zoff[kMaxLevels], boff[kMaxLevels], soff[kMaxLevels], roff, binv_off;
     int lds_elems;
 };
 

@@ -129,13 +129,17 @@ __global__ void __launch_bounds__(256) correct64_kernel(double *z, const float *
 }
 
 // ---------------------------------------------------------------------------
-// workgroup coarse solver: levels lc..L-1 entirely in LDS (z, b, stencils and
-// the coarsest level's dense inverse), 256 threads, one barrier per pass.
-// z and b of a level are stored colour-split: node (i,j) in parity array
-// (i&1) + 2(j&1) at ((j>>1), (i>>1)), so the lanes of a colour pass, and the
-// neighbours they read, are at consecutive addresses (no bank conflicts).
-// All addressing is relative to the shared array, so accesses are LDS ones.
+// workgroup coarse solver: levels lc..L-1 entirely in LDS (z, b, stencils, a
+// residual scratch and the coarsest level's dense inverse), kCoarseThreads
+// threads, one barrier per pass.  z and b of a level are stored colour-split:
+// node (i,j) in parity array (i&1) + 2(j&1) at ((j>>1), (i>>1)), so the lanes
+// of a colour pass, and the neighbours they read, are at consecutive
+// addresses.  Each node update issues all of its LDS loads before any
+// arithmetic (no branch between them: rows beyond a wall are read from a
+// clamped row and zeroed), which keeps one pass to one LDS round trip.
 // ---------------------------------------------------------------------------
+constexpr int kCoarseThreads = 512;
+
 template <typename T> struct WGLevel {
     int m, n, rs, cs, xodd, hm, hn;
     int zo, bo, so;
@@ -161,37 +165,37 @@ template <typename T> __device__ __forceinline__ static int cidx(const WGLevel<T
     return 2 * ((((i & 1) + 2 * (j & 1)) * W.hn + (j >> 1)) * W.hm + (i >> 1));
 }
 
+template <typename T> __device__ __forceinline__ static vec2s<T> lds2(const T *sm, int k) {
+    return *reinterpret_cast<const vec2s<T> *>(sm + k);
+}
+
+// off-diagonal sum at node (i,j) of an LDS-resident level
 template <typename T>
 __device__ __forceinline__ static void wg_offdiag(const T *sm, const WGLevel<T> &W, int i, int j, const T *s, T &ax,
                                                   T &ay) {
     const int m = W.m;
     const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
+    const int jd = j > 0 ? j - 1 : j, ju = j < W.n - 1 ? j + 1 : j;
     const T *z = sm + W.zo;
-    ax = 0;
-    ay = 0;
-    if (j > 0) {
-        acc2(ax, ay, s + 0, z + cidx(W, il, j - 1));
-        acc2(ax, ay, s + 4, z + cidx(W, i, j - 1));
-        acc2(ax, ay, s + 8, z + cidx(W, ir, j - 1));
-    }
-    acc2(ax, ay, s + 12, z + cidx(W, il, j));
-    acc2(ax, ay, s + 20, z + cidx(W, ir, j));
-    if (j < W.n - 1) {
-        acc2(ax, ay, s + 24, z + cidx(W, il, j + 1));
-        acc2(ax, ay, s + 28, z + cidx(W, i, j + 1));
-        acc2(ax, ay, s + 32, z + cidx(W, ir, j + 1));
-    }
+    vec2s<T> dl = lds2(z, cidx(W, il, jd)), dc = lds2(z, cidx(W, i, jd)), dr = lds2(z, cidx(W, ir, jd));
+    const vec2s<T> cl = lds2(z, cidx(W, il, j)), cr = lds2(z, cidx(W, ir, j));
+    vec2s<T> ul = lds2(z, cidx(W, il, ju)), uc = lds2(z, cidx(W, i, ju)), ur = lds2(z, cidx(W, ir, ju));
+    const vec2s<T> zero = {0, 0};
+    if (j == 0) dl = dc = dr = zero;
+    if (j == W.n - 1) ul = uc = ur = zero;
+    offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
 }
 
-template <typename T> __device__ __forceinline__ static void wg_resid(const T *sm, const WGLevel<T> &W, int i, int j,
-                                                                      T &rx, T &ry) {
+// residual at (i,j) written in natural order to r
+template <typename T> __device__ __forceinline__ static void wg_resid(T *sm, const WGLevel<T> &W, int ro, int i,
+                                                                      int j) {
     const T *s = sm + W.so + (j * W.rs + i * W.cs) * 40;
-    T ax, ay;
+    T ax, ay, rx, ry;
     wg_offdiag(sm, W, i, j, s, ax, ay);
     const int k = cidx(W, i, j);
-    const T zx = sm[W.zo + k], zy = sm[W.zo + k + 1];
-    rx = sm[W.bo + k] - ax - (s[16] * zx + s[17] * zy);
-    ry = sm[W.bo + k + 1] - ay - (s[18] * zx + s[19] * zy);
+    resid_value<T>(s, sm[W.bo + k], sm[W.bo + k + 1], ax, ay, lds2(sm + W.zo, k), rx, ry);
+    sm[ro + 2 * (i + W.m * j)] = rx;
+    sm[ro + 2 * (i + W.m * j) + 1] = ry;
 }
 
 template <typename T> __device__ static void wg_smooth(T *sm, const WGLevel<T> &W, int sweeps) {
@@ -200,29 +204,31 @@ template <typename T> __device__ static void wg_smooth(T *sm, const WGLevel<T> &
         for (int cy = 0; cy < 2; cy++)
             for (int cx = 0; cx < 2 + W.xodd; cx++) {
                 const int cnt = cx == 2 ? hn : hm * hn;
-                for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+                for (int t = threadIdx.x; t < cnt; t += kCoarseThreads) {
                     int i, j;
                     if (cx == 2) { i = W.m - 1; j = 2 * t + cy; }
                     else { const int ty = t / hm; i = 2 * (t - ty * hm) + cx; j = 2 * ty + cy; }
-                    if (i >= W.m || j >= W.n) continue;
-                    if (cx < 2 && W.xodd && i == W.m - 1) continue;
+                    if (i >= W.m || j >= W.n || (cx < 2 && W.xodd && i == W.m - 1)) continue;
                     const T *s = sm + W.so + (j * W.rs + i * W.cs) * 40;
                     T ax, ay;
                     wg_offdiag(sm, W, i, j, s, ax, ay);
                     const int k = cidx(W, i, j);
-                    const T rx = sm[W.bo + k] - ax, ry = sm[W.bo + k + 1] - ay;
-                    sm[W.zo + k] = s[36] * rx + s[37] * ry;
-                    sm[W.zo + k + 1] = s[38] * rx + s[39] * ry;
+                    const vec2s<T> o = gs_value<T>(s, sm[W.bo + k], sm[W.bo + k + 1], ax, ay);
+                    *reinterpret_cast<vec2s<T> *>(sm + W.zo + k) = o;
                 }
                 __syncthreads();
             }
 }
 
-template <typename T> __global__ void __launch_bounds__(256) coarse_solve_kernel(CoarseArgs<T> A) {
+template <typename T>
+__global__ void __launch_bounds__(kCoarseThreads) coarse_solve_kernel(CoarseArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *sm = reinterpret_cast<T *>(smem_raw);
-    const int tid = threadIdx.x, NT = blockDim.x;
+    const int tid = threadIdx.x, NT = kCoarseThreads;
+    const int ro = A.roff;
     // prologue: stencils of every level, source of level lc, bottom inverse
+    // (stored transposed, so that the threads of the bottom solve, one per
+    // row, read consecutive addresses)
     for (int l = A.lc; l < A.L; l++) {
         const int cnt = (A.lv[l].cs ? A.lv[l].m * A.lv[l].n : A.lv[l].n) * 40;
         const T *src = A.lv[l].S;
@@ -239,31 +245,39 @@ template <typename T> __global__ void __launch_bounds__(256) coarse_solve_kernel
             sm[W.zo + k + 1] = 0;
         }
         const int N = 2 * A.lv[A.L - 1].m * A.lv[A.L - 1].n;
-        for (int t = tid; t < N * N; t += NT) sm[A.binv_off + t] = A.binv[t];
+        for (int t = tid; t < N * N; t += NT) {
+            const int r = t / N, c = t - r * N;
+            sm[A.binv_off + c * N + r] = A.binv[t];
+        }
     }
     __syncthreads();
-    // down
+    // down: smooth, fine residuals into r (one pass), P^T-weighted sums
     for (int l = A.lc; l < A.L - 1; l++) {
         const WGLevel<T> F = wg_level(A, l), C = wg_level(A, l + 1);
         wg_smooth(sm, F, A.nu1);
+        for (int t = tid; t < F.m * F.n; t += NT) {
+            const int j = t / F.m, i = t - j * F.m;
+            wg_resid(sm, F, ro, i, j);
+        }
+        __syncthreads();
         for (int t = tid; t < C.m * C.n; t += NT) {
             const int J = t / C.m, I = t - J * C.m;
             const int fx0 = 2 * I, fy0 = ((F.n & 1) == 0 && J == C.n - 1) ? F.n - 1 : 2 * J;
             T bx = 0, by = 0;
+#pragma unroll
             for (int b = -1; b <= 1; b++) {
                 const int fy = fy0 + b;
                 const float wy = pw_bounded(fy, F.n, C.n, J);
-                if (wy == 0.f) continue;
+                const int fyc = fy < 0 ? 0 : (fy >= F.n ? F.n - 1 : fy);
+#pragma unroll
                 for (int a = -1; a <= 1; a++) {
                     const int fx = fx0 + a;
                     const float wx = pw_periodic(fx, F.m, C.m, I);
-                    if (wx == 0.f) continue;
                     const int fi = fx < 0 ? fx + F.m : (fx >= F.m ? fx - F.m : fx);
-                    T rx, ry;
-                    wg_resid(sm, F, fi, fy, rx, ry);
+                    const vec2s<T> r = lds2(sm + ro, 2 * (fi + F.m * fyc));
                     const T w = (T)(wx * wy);
-                    bx += w * rx;
-                    by += w * ry;
+                    bx = fmav(w, r.x, bx);
+                    by = fmav(w, r.y, by);
                 }
             }
             const int k = cidx(C, I, J);
@@ -278,19 +292,21 @@ template <typename T> __global__ void __launch_bounds__(256) coarse_solve_kernel
     {
         const WGLevel<T> B = wg_level(A, A.L - 1);
         const int N = 2 * B.m * B.n;
+        for (int t = tid; t < B.m * B.n; t += NT) {
+            const int j = t / B.m, i = t - j * B.m, k = cidx(B, i, j);
+            sm[ro + 2 * t] = sm[B.bo + k];
+            sm[ro + 2 * t + 1] = sm[B.bo + k + 1];
+        }
+        __syncthreads();
         for (int r = tid; r < N; r += NT) {
-            const T *row = sm + A.binv_off + r * N;
             T acc = 0;
-            for (int k = 0; k < N; k++) {
-                const int node = k >> 1, j = node / B.m, i = node - j * B.m;
-                acc += row[k] * sm[B.bo + cidx(B, i, j) + (k & 1)];
-            }
+            for (int c = 0; c < N; c++) acc = fmav(sm[A.binv_off + c * N + r], sm[ro + c], acc);
             const int node = r >> 1, j = node / B.m, i = node - j * B.m;
             sm[B.zo + cidx(B, i, j) + (r & 1)] = acc;
         }
         __syncthreads();
     }
-    // up
+    // up: z += P zc, then smooth
     for (int l = A.L - 2; l >= A.lc; l--) {
         const WGLevel<T> F = wg_level(A, l), C = wg_level(A, l + 1);
         for (int t = tid; t < F.m * F.n; t += NT) {
@@ -304,12 +320,11 @@ template <typename T> __global__ void __launch_bounds__(256) coarse_solve_kernel
             else if ((j & 1) == 0) { cy0 = j >> 1; wy0 = 1; cy1 = cy0; wy1 = 0; }
             else { cy0 = j >> 1; cy1 = cy0 + 1; wy0 = wy1 = (T)0.5; }
             const T *zc = sm + C.zo;
-            const int a00 = cidx(C, cx0, cy0), a10 = cidx(C, cx1, cy0), a01 = cidx(C, cx0, cy1),
-                      a11 = cidx(C, cx1, cy1);
+            const vec2s<T> a00 = lds2(zc, cidx(C, cx0, cy0)), a10 = lds2(zc, cidx(C, cx1, cy0)),
+                           a01 = lds2(zc, cidx(C, cx0, cy1)), a11 = lds2(zc, cidx(C, cx1, cy1));
             const int k = cidx(F, i, j);
-            sm[F.zo + k] += wy0 * (wx0 * zc[a00] + wx1 * zc[a10]) + wy1 * (wx0 * zc[a01] + wx1 * zc[a11]);
-            sm[F.zo + k + 1] +=
-                wy0 * (wx0 * zc[a00 + 1] + wx1 * zc[a10 + 1]) + wy1 * (wx0 * zc[a01 + 1] + wx1 * zc[a11 + 1]);
+            sm[F.zo + k] += wy0 * (wx0 * a00.x + wx1 * a10.x) + wy1 * (wx0 * a01.x + wx1 * a11.x);
+            sm[F.zo + k + 1] += wy0 * (wx0 * a00.y + wx1 * a10.y) + wy1 * (wx0 * a01.y + wx1 * a11.y);
         }
         __syncthreads();
         wg_smooth(sm, F, A.nu2);
@@ -339,6 +354,11 @@ template <typename T> int coarse_layout(CoarseArgs<T> &A) {
         off += (A.lv[l].cs ? A.lv[l].m * A.lv[l].n : A.lv[l].n) * 40;
         off = (off + 3) & ~3;
     }
+    // residual scratch: natural order, the largest level (also holds the
+    // coarsest source for the dense solve)
+    A.roff = off;
+    off += 2 * A.lv[A.lc].m * A.lv[A.lc].n;
+    off = (off + 3) & ~3;
     const int N = 2 * A.lv[A.L - 1].m * A.lv[A.L - 1].n;
     A.binv_off = off;
     off += N * N;
@@ -381,7 +401,7 @@ template <typename T> int mg_prolong(const LevelArgs<T> &C, const LevelArgs<T> &
 template <typename T> size_t coarse_lds_bytes(const CoarseArgs<T> &A) { return (size_t)A.lds_elems * sizeof(T); }
 
 template <typename T> int mg_coarse(const CoarseArgs<T> &A, hipStream_t st) {
-    hipLaunchKernelGGL(coarse_solve_kernel<T>, dim3(1), dim3(256), coarse_lds_bytes(A), st, A);
+    hipLaunchKernelGGL(coarse_solve_kernel<T>, dim3(1), dim3(kCoarseThreads), coarse_lds_bytes(A), st, A);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }

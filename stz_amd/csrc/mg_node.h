@@ -6,42 +6,98 @@
 
 namespace stz {
 
+template <typename T> struct alignas(2 * sizeof(T)) vec2s { T x, y; };
+
+__device__ __forceinline__ static float fmav(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ static double fmav(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // ---------------------------------------------------------------------------
-// node-level device helpers (shared by the grid kernels and the workgroup
-// coarse solver)
+// (ax, ay) = A'z, the off-diagonal part of the 9-point 2x2-block stencil, from
+// the eight neighbours in stencil order dl,dc,dr,cl,cr,ul,uc,ur (block k has
+// entries s[4k..4k+3] = (a, b; c, d)).  The sum is formed as three row sums
+// (below, centre, above) of fused multiply-adds, so that the six chains of
+// each node are independent and the dependent depth is 8 instead of 16.
+// Every kernel (per-colour, tiled, workgroup coarse solver, residuals) uses
+// this one routine, so all paths round identically.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ static void acc2(T &ax, T &ay, const T *s, const T *zz) {
-    const T zx = zz[0], zy = zz[1];
-    ax += s[0] * zx + s[1] * zy;
-    ay += s[2] * zx + s[3] * zy;
+template <typename T, typename SP>
+__device__ __forceinline__ static void offdiag9(const SP &s, const vec2s<T> &dl, const vec2s<T> &dc,
+                                                const vec2s<T> &dr, const vec2s<T> &cl, const vec2s<T> &cr,
+                                                const vec2s<T> &ul, const vec2s<T> &uc, const vec2s<T> &ur, T &ax,
+                                                T &ay) {
+    T xd = s[0] * dl.x, yd = s[2] * dl.x;
+    xd = fmav(s[1], dl.y, xd);
+    yd = fmav(s[3], dl.y, yd);
+    xd = fmav(s[4], dc.x, xd);
+    yd = fmav(s[6], dc.x, yd);
+    xd = fmav(s[5], dc.y, xd);
+    yd = fmav(s[7], dc.y, yd);
+    xd = fmav(s[8], dr.x, xd);
+    yd = fmav(s[10], dr.x, yd);
+    xd = fmav(s[9], dr.y, xd);
+    yd = fmav(s[11], dr.y, yd);
+    T xc = s[12] * cl.x, yc = s[14] * cl.x;
+    xc = fmav(s[13], cl.y, xc);
+    yc = fmav(s[15], cl.y, yc);
+    xc = fmav(s[20], cr.x, xc);
+    yc = fmav(s[22], cr.x, yc);
+    xc = fmav(s[21], cr.y, xc);
+    yc = fmav(s[23], cr.y, yc);
+    T xu = s[24] * ul.x, yu = s[26] * ul.x;
+    xu = fmav(s[25], ul.y, xu);
+    yu = fmav(s[27], ul.y, yu);
+    xu = fmav(s[28], uc.x, xu);
+    yu = fmav(s[30], uc.x, yu);
+    xu = fmav(s[29], uc.y, xu);
+    yu = fmav(s[31], uc.y, yu);
+    xu = fmav(s[32], ur.x, xu);
+    yu = fmav(s[34], ur.x, yu);
+    xu = fmav(s[33], ur.y, xu);
+    yu = fmav(s[35], ur.y, yu);
+    ax = (xd + xu) + xc;
+    ay = (yd + yu) + yc;
 }
 
-// sum of the off-diagonal blocks times z (the reference's mul_a)
+// Gauss-Seidel value from the off-diagonal sum: inv(cc) (b - A'z)
+template <typename T, typename SP>
+__device__ __forceinline__ static vec2s<T> gs_value(const SP &s, T bx, T by, T ax, T ay) {
+    const T rx = bx - ax, ry = by - ay;
+    vec2s<T> o;
+    o.x = fmav(s[37], ry, s[36] * rx);
+    o.y = fmav(s[39], ry, s[38] * rx);
+    return o;
+}
+
+// residual b - A'z - cc z
+template <typename T, typename SP>
+__device__ __forceinline__ static void resid_value(const SP &s, T bx, T by, T ax, T ay, const vec2s<T> &z, T &rx,
+                                                   T &ry) {
+    rx = bx - ax - fmav(s[17], z.y, s[16] * z.x);
+    ry = by - ay - fmav(s[19], z.y, s[18] * z.x);
+}
+
+// the eight neighbours of node (i,j) of a natural-order level (zero outside
+// the walls; x periodic)
 template <typename T>
-__device__ __forceinline__ static void offdiag(const LevelArgs<T> &L, int i, int j, const T *s, T &ax, T &ay) {
+__device__ __forceinline__ static void neighbours(const LevelArgs<T> &L, int i, int j, vec2s<T> &dl, vec2s<T> &dc,
+                                                  vec2s<T> &dr, vec2s<T> &cl, vec2s<T> &cr, vec2s<T> &ul,
+                                                  vec2s<T> &uc, vec2s<T> &ur) {
     const int m = L.m;
     const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
-    const T *z = L.z;
-    ax = 0;
-    ay = 0;
-    if (j > 0) {
-        const T *zr = z + 2 * m * (j - 1);
-        acc2(ax, ay, s + 0, zr + 2 * il);
-        acc2(ax, ay, s + 4, zr + 2 * i);
-        acc2(ax, ay, s + 8, zr + 2 * ir);
-    }
-    {
-        const T *zr = z + 2 * m * j;
-        acc2(ax, ay, s + 12, zr + 2 * il);
-        acc2(ax, ay, s + 20, zr + 2 * ir);
-    }
-    if (j < L.n - 1) {
-        const T *zr = z + 2 * m * (j + 1);
-        acc2(ax, ay, s + 24, zr + 2 * il);
-        acc2(ax, ay, s + 28, zr + 2 * i);
-        acc2(ax, ay, s + 32, zr + 2 * ir);
-    }
+    const int jd = j > 0 ? j - 1 : j, ju = j < L.n - 1 ? j + 1 : j;
+    const vec2s<T> *z = reinterpret_cast<const vec2s<T> *>(L.z);
+    const vec2s<T> *zd = z + (size_t)m * jd, *zc = z + (size_t)m * j, *zu = z + (size_t)m * ju;
+    dl = zd[il];
+    dc = zd[i];
+    dr = zd[ir];
+    cl = zc[il];
+    cr = zc[ir];
+    ul = zu[il];
+    uc = zu[i];
+    ur = zu[ir];
+    const vec2s<T> zero = {0, 0};
+    if (j == 0) dl = dc = dr = zero;
+    if (j == L.n - 1) ul = uc = ur = zero;
 }
 
 template <typename T> __device__ __forceinline__ static const T *stencil_at(const LevelArgs<T> &L, int i, int j) {
@@ -51,24 +107,27 @@ template <typename T> __device__ __forceinline__ static const T *stencil_at(cons
 // Gauss-Seidel update of node (i,j): z = inv(cc) (b - A'z)
 template <typename T> __device__ __forceinline__ static void gs_node(const LevelArgs<T> &L, int i, int j) {
     const T *s = stencil_at(L, i, j);
+    vec2s<T> dl, dc, dr, cl, cr, ul, uc, ur;
+    neighbours(L, i, j, dl, dc, dr, cl, cr, ul, uc, ur);
     T ax, ay;
-    offdiag(L, i, j, s, ax, ay);
+    offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
     const int k = 2 * (i + L.m * j);
-    const T rx = L.b[k] - ax, ry = L.b[k + 1] - ay;
-    L.z[k] = s[36] * rx + s[37] * ry;
-    L.z[k + 1] = s[38] * rx + s[39] * ry;
+    const vec2s<T> o = gs_value<T>(s, L.b[k], L.b[k + 1], ax, ay);
+    L.z[k] = o.x;
+    L.z[k + 1] = o.y;
 }
 
 // residual b - A z at node (i,j)
 template <typename T> __device__ __forceinline__ static void resid_node(const LevelArgs<T> &L, int i, int j, T &rx,
                                                                         T &ry) {
     const T *s = stencil_at(L, i, j);
+    vec2s<T> dl, dc, dr, cl, cr, ul, uc, ur;
+    neighbours(L, i, j, dl, dc, dr, cl, cr, ul, uc, ur);
     T ax, ay;
-    offdiag(L, i, j, s, ax, ay);
+    offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
     const int k = 2 * (i + L.m * j);
-    const T zx = L.z[k], zy = L.z[k + 1];
-    rx = L.b[k] - ax - (s[16] * zx + s[17] * zy);
-    ry = L.b[k + 1] - ay - (s[18] * zx + s[19] * zy);
+    const vec2s<T> z0 = {L.z[k], L.z[k + 1]};
+    resid_value<T>(s, L.b[k], L.b[k + 1], ax, ay, z0, rx, ry);
 }
 
 // 1D prolongation weight of coarse c in fine f.  Periodic axis: f unwrapped,

@@ -34,33 +34,16 @@ namespace stz {
 
 enum { TILE_LOAD = 0, TILE_ZERO = 1, TILE_PROLONG = 2 };
 
-template <typename T> struct alignas(2 * sizeof(T)) vec2s { T x, y; };
-
-// (ax,ay) = A' z, the off-diagonal part, from the eight neighbours in
-// stencil order (dl,dc,dr,cl,cr,ul,uc,ur); SK(k) yields stencil value k
-#define STZ_ACC(SK, K, P, ax, ay)                                                  \
-    {                                                                              \
-        const auto p_ = (P);                                                       \
-        ax += SK(4 * K) * p_.x + SK(4 * K + 1) * p_.y;                             \
-        ay += SK(4 * K + 2) * p_.x + SK(4 * K + 3) * p_.y;                         \
-    }
-#define STZ_OFFDIAG8(SK, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay)                   \
-    {                                                                              \
-        ax = 0;                                                                    \
-        ay = 0;                                                                    \
-        STZ_ACC(SK, 0, dl, ax, ay) STZ_ACC(SK, 1, dc, ax, ay)                      \
-        STZ_ACC(SK, 2, dr, ax, ay) STZ_ACC(SK, 3, cl, ax, ay)                      \
-        STZ_ACC(SK, 5, cr, ax, ay) STZ_ACC(SK, 6, ul, ax, ay)                      \
-        STZ_ACC(SK, 7, uc, ax, ay) STZ_ACC(SK, 8, ur, ax, ay)                      \
-    }
-// the same for a row-major neighbourhood: rows zd, zc, zu around column 0
-#define STZ_OFFDIAG(SK, zd, zc, zu, ax, ay)                                        \
-    STZ_OFFDIAG8(SK, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay)
+// one thread per 2x2 block of the tile-plus-halo region
+__host__ __device__ constexpr int tile_threads(int TX, int TY, int S) {
+    return (((TX + 8 * S) / 2) * ((TY + 8 * S) / 2) + 63) / 64 * 64;
+}
 
 template <typename T, int TX, int TY, int S, int MODE>
-__global__ void __launch_bounds__(256) smooth_tile_kernel(LevelArgs<T> L, T *__restrict__ zout, LevelArgs<T> Cz) {
+__global__ void __launch_bounds__(tile_threads(TX, TY, S))
+    smooth_tile_kernel(LevelArgs<T> L, T *__restrict__ zout, LevelArgs<T> Cz) {
     constexpr int H = 4 * S, RX = TX + 2 * H, RY = TY + 2 * H;
-    constexpr int BX = RX / 2, BY = RY / 2, NB = BX * BY, NT = 256;
+    constexpr int BX = RX / 2, BY = RY / 2, NB = BX * BY, NT = tile_threads(TX, TY, S);
     constexpr int BPT = (NB + NT - 1) / NT;
     static_assert(RX % 2 == 0 && RY % 2 == 0, "even region");
     // colour-split LDS: node (li,lj) lives in array c = (li&1) + 2(lj&1) at
@@ -160,19 +143,11 @@ __global__ void __launch_bounds__(256) smooth_tile_kernel(LevelArgs<T> L, T *__r
                 vec2s<T> out;
                 if (gj < nb || gj >= n - nb) {
                     const T *s = ss + lj * 40;
-#define SK_LDS(k) s[k]
-                    STZ_OFFDIAG8(SK_LDS, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay)
-                    const T rx = bx[q][c] - ax, ry = by[q][c] - ay;
-                    out.x = s[36] * rx + s[37] * ry;
-                    out.y = s[38] * rx + s[39] * ry;
-#undef SK_LDS
+                    offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+                    out = gs_value<T>(s, bx[q][c], by[q][c], ax, ay);
                 } else {
-#define SK_REG(k) si[k]
-                    STZ_OFFDIAG8(SK_REG, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay)
-                    const T rx = bx[q][c] - ax, ry = by[q][c] - ay;
-                    out.x = si[36] * rx + si[37] * ry;
-                    out.y = si[38] * rx + si[39] * ry;
-#undef SK_REG
+                    offdiag9<T>(si, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+                    out = gs_value<T>(si, bx[q][c], by[q][c], ax, ay);
                 }
                 zs[c * NB + blk] = out;
             }
@@ -239,25 +214,17 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
             int gi = zi0 + 1 + li;
             gi = ((gi % m) + m) % m;
             const vec2s<T> *zd = zs + lj * ZX + li + 1, *zc = zd + ZX, *zu = zc + ZX;
-            T ax, ay, cx, cy;
+            T ax, ay;
             const vec2s<T> z0 = zc[0];
+            const size_t k = 2 * ((size_t)gi + (size_t)m * gj);
             if (gj < nb || gj >= n - nb) {
                 const T *s = ss + lj * 40;
-#define SK_LDS(k) s[k]
-                STZ_OFFDIAG(SK_LDS, zd, zc, zu, ax, ay)
-                cx = s[16] * z0.x + s[17] * z0.y;
-                cy = s[18] * z0.x + s[19] * z0.y;
-#undef SK_LDS
+                offdiag9<T>(s, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
+                resid_value<T>(s, F.b[k], F.b[k + 1], ax, ay, z0, r.x, r.y);
             } else {
-#define SK_REG(k) si[k]
-                STZ_OFFDIAG(SK_REG, zd, zc, zu, ax, ay)
-                cx = si[16] * z0.x + si[17] * z0.y;
-                cy = si[18] * z0.x + si[19] * z0.y;
-#undef SK_REG
+                offdiag9<T>(si, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
+                resid_value<T>(si, F.b[k], F.b[k + 1], ax, ay, z0, r.x, r.y);
             }
-            const size_t k = 2 * ((size_t)gi + (size_t)m * gj);
-            r.x = F.b[k] - ax - cx;
-            r.y = F.b[k + 1] - ay - cy;
         }
         rs[t] = r;
     }
@@ -332,23 +299,16 @@ __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float 
         if (gi >= m || gj >= n) continue;
         const vec2s<double> *zd = zs + lj * ZX + li + 1, *zc = zd + ZX, *zu = zc + ZX;
         const vec2s<double> z0 = zc[0];
-        double ax, ay, cx, cy;
+        const size_t k = 2 * ((size_t)gi + (size_t)m * gj);
+        double ax, ay, rx, ry;
         if (gj < nb || gj >= n - nb) {
             const double *s = ss + lj * 40;
-#define SK_LDS(k) s[k]
-            STZ_OFFDIAG(SK_LDS, zd, zc, zu, ax, ay)
-            cx = s[16] * z0.x + s[17] * z0.y;
-            cy = s[18] * z0.x + s[19] * z0.y;
-#undef SK_LDS
+            offdiag9<double>(s, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
+            resid_value<double>(s, L.b[k], L.b[k + 1], ax, ay, z0, rx, ry);
         } else {
-#define SK_REG(k) si[k]
-            STZ_OFFDIAG(SK_REG, zd, zc, zu, ax, ay)
-            cx = si[16] * z0.x + si[17] * z0.y;
-            cy = si[18] * z0.x + si[19] * z0.y;
-#undef SK_REG
+            offdiag9<double>(si, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
+            resid_value<double>(si, L.b[k], L.b[k + 1], ax, ay, z0, rx, ry);
         }
-        const size_t k = 2 * ((size_t)gi + (size_t)m * gj);
-        const double rx = L.b[k] - ax - cx, ry = L.b[k + 1] - ay - cy;
         b32[k] = (float)rx;
         b32[k + 1] = (float)ry;
         acc += rx * rx + ry * ry;
@@ -372,10 +332,12 @@ template <typename T, int S, int MODE>
 static void launch_tile(const LevelArgs<T> &L, T *zout, const LevelArgs<T> &C, hipStream_t st) {
     if (L.m % 64 == 0 && (L.m / 64) * ((L.n + 31) / 32) >= 256) {
         const dim3 g(L.m / 64, (L.n + 31) / 32);
-        hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE>), g, dim3(256), 0, st, L, zout, C);
+        hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE>), g, dim3(tile_threads(64, 32, S)), 0, st, L, zout,
+                           C);
     } else {
         const dim3 g(L.m / 32, (L.n + 15) / 16);
-        hipLaunchKernelGGL((smooth_tile_kernel<T, 32, 16, S, MODE>), g, dim3(256), 0, st, L, zout, C);
+        hipLaunchKernelGGL((smooth_tile_kernel<T, 32, 16, S, MODE>), g, dim3(tile_threads(32, 16, S)), 0, st, L, zout,
+                           C);
     }
 }
 

@@ -339,7 +339,7 @@ template <typename T> static int alloc_levels(stz_sim *s, DevMG<T> &mg, bool own
     // first level run by the workgroup solver: the largest suffix of levels
     // whose z, b, stencils and bottom inverse fit the LDS budget, never the
     // top level (stencil sizes assume one per row; mg_build re-checks)
-    const size_t budget = 96 * 1024;
+    const size_t budget = 128 * 1024;
     int lc = mg.L - 1;
     while (lc > 1) {
         mg.args.lc = lc - 1;
