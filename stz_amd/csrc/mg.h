@@ -1,6 +1,8 @@
 // mg.h -- device-side multigrid structures and launchers (mg.hip).
 #pragma once
 
+#include <vector>
+
 #include "device.h"
 
 namespace stz {
@@ -31,16 +33,25 @@ template <typename T> struct LevelArgs {
 template <typename T> struct CoarseArgs {
     int lc, L;
     int nu1, nu2;
-    const T *binv;  // dense inverse of the coarsest operator, (2mn)^2
     LevelArgs<T> lv[kMaxLevels];
-    // LDS layout of the workgroup solver, in elements of T
+    // LDS layout of the workgroup solver, in elements of T: first the part
+    // that is the same for every solve (row stencils of levels lc..L-1, the
+    // coarsest level's dense inverse, transposed), copied from `pack`; then
+    // z and b of every level and a residual scratch
+    const T *pack;
+    int pack_elems;
     int zoff[kMaxLevels], boff[kMaxLevels], soff[kMaxLevels], roff, binv_off;
     int lds_elems;
 };
 
-// LDS layout for levels lc..L-1 (z, b, stencils, bottom inverse); returns
-// the number of elements of T
+// LDS layout for levels lc..L-1; returns the number of elements of T
 template <typename T> int coarse_layout(CoarseArgs<T> &A);
+// host: the constant part of the layout, to be uploaded as A.pack.  S[l] are
+// the stencils of level l as stored on the device, binv the coarsest
+// level's dense inverse (row-major)
+template <typename T>
+void coarse_pack(const CoarseArgs<T> &A, const std::vector<double> *S, const std::vector<double> &binv,
+                 std::vector<T> &out);
 
 template <typename T> int mg_smooth(const LevelArgs<T> &L, int sweeps, hipStream_t st);
 template <typename T> int mg_restrict(const LevelArgs<T> &F, const LevelArgs<T> &C, hipStream_t st);

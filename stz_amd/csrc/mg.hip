@@ -140,8 +140,13 @@ __global__ void __launch_bounds__(256) correct64_kernel(double *z, const float *
 // ---------------------------------------------------------------------------
 constexpr int kCoarseThreads = 512;
 
+// stencil values stored for a level: one 40-value stencil per row, or per node
+template <typename T> __host__ __device__ __forceinline__ static int stencil_count(const LevelArgs<T> &L) {
+    return (L.cs ? L.m * L.n : L.n) * 40;
+}
+
 template <typename T> struct WGLevel {
-    int m, n, rs, cs, xodd, hm, hn;
+    int m, n, rs, cs, xodd, hm, hn, nb;
     int zo, bo, so;
 };
 
@@ -152,6 +157,7 @@ template <typename T> __device__ __forceinline__ static WGLevel<T> wg_level(cons
     W.rs = A.lv[l].rs;
     W.cs = A.lv[l].cs;
     W.xodd = A.lv[l].xodd;
+    W.nb = A.lv[l].cs ? W.n : A.lv[l].nb;
     W.hm = (W.m + 1) >> 1;
     W.hn = (W.n + 1) >> 1;
     W.zo = A.zoff[l];
@@ -170,8 +176,8 @@ template <typename T> __device__ __forceinline__ static vec2s<T> lds2(const T *s
 }
 
 // off-diagonal sum at node (i,j) of an LDS-resident level
-template <typename T>
-__device__ __forceinline__ static void wg_offdiag(const T *sm, const WGLevel<T> &W, int i, int j, const T *s, T &ax,
+template <typename T, typename SP>
+__device__ __forceinline__ static void wg_offdiag(const T *sm, const WGLevel<T> &W, int i, int j, const SP &s, T &ax,
                                                   T &ay) {
     const int m = W.m;
     const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
@@ -186,19 +192,36 @@ __device__ __forceinline__ static void wg_offdiag(const T *sm, const WGLevel<T> 
     offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
 }
 
-// residual at (i,j) written in natural order to r
-template <typename T> __device__ __forceinline__ static void wg_resid(T *sm, const WGLevel<T> &W, int ro, int i,
-                                                                      int j) {
-    const T *s = sm + W.so + (j * W.rs + i * W.cs) * 40;
+// residuals of every node of level W into r (natural order): rows nb..n-nb-1
+// with the interior stencil si held in registers, the wall rows with their
+// own stencils from LDS (two loops, so that the register path never goes
+// through a pointer)
+template <typename T, typename SP>
+__device__ __forceinline__ static void wg_resid_node(T *sm, const WGLevel<T> &W, const SP &s, int ro, int i, int j) {
     T ax, ay, rx, ry;
-    wg_offdiag(sm, W, i, j, s, ax, ay);
     const int k = cidx(W, i, j);
+    wg_offdiag(sm, W, i, j, s, ax, ay);
     resid_value<T>(s, sm[W.bo + k], sm[W.bo + k + 1], ax, ay, lds2(sm + W.zo, k), rx, ry);
     sm[ro + 2 * (i + W.m * j)] = rx;
     sm[ro + 2 * (i + W.m * j) + 1] = ry;
 }
 
-template <typename T> __device__ static void wg_smooth(T *sm, const WGLevel<T> &W, int sweeps) {
+template <typename T>
+__device__ __forceinline__ static void wg_resid_all(T *sm, const WGLevel<T> &W, const T (&si)[40], int ro) {
+    const int lo = W.nb < W.n ? W.nb : W.n, hi = W.n - W.nb > lo ? W.n - W.nb : lo, m = W.m;
+    for (int t = threadIdx.x; t < (hi - lo) * m; t += kCoarseThreads) {
+        const int r = t / m, i = t - r * m;
+        wg_resid_node(sm, W, si, ro, i, lo + r);
+    }
+    for (int t = threadIdx.x; t < (lo + W.n - hi) * m; t += kCoarseThreads) {
+        const int r = t / m, i = t - r * m, j = r < lo ? r : hi + (r - lo);
+        const T *s = sm + W.so + (j * W.rs + i * W.cs) * 40;
+        wg_resid_node(sm, W, s, ro, i, j);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ static void wg_smooth(T *sm, const WGLevel<T> &W, const T (&si)[40], int sweeps) {
     const int hm = W.hm, hn = W.hn;
     for (int sw = 0; sw < sweeps; sw++)
         for (int cy = 0; cy < 2; cy++)
@@ -209,15 +232,284 @@ template <typename T> __device__ static void wg_smooth(T *sm, const WGLevel<T> &
                     if (cx == 2) { i = W.m - 1; j = 2 * t + cy; }
                     else { const int ty = t / hm; i = 2 * (t - ty * hm) + cx; j = 2 * ty + cy; }
                     if (i >= W.m || j >= W.n || (cx < 2 && W.xodd && i == W.m - 1)) continue;
-                    const T *s = sm + W.so + (j * W.rs + i * W.cs) * 40;
                     T ax, ay;
-                    wg_offdiag(sm, W, i, j, s, ax, ay);
                     const int k = cidx(W, i, j);
-                    const vec2s<T> o = gs_value<T>(s, sm[W.bo + k], sm[W.bo + k + 1], ax, ay);
+                    vec2s<T> o;
+                    if (j < W.nb || j >= W.n - W.nb) {
+                        const T *s = sm + W.so + (j * W.rs + i * W.cs) * 40;
+                        wg_offdiag(sm, W, i, j, s, ax, ay);
+                        o = gs_value<T>(s, sm[W.bo + k], sm[W.bo + k + 1], ax, ay);
+                    } else {
+                        wg_offdiag(sm, W, i, j, si, ax, ay);
+                        o = gs_value<T>(si, sm[W.bo + k], sm[W.bo + k + 1], ax, ay);
+                    }
                     *reinterpret_cast<vec2s<T> *>(sm + W.zo + k) = o;
                 }
                 __syncthreads();
             }
+}
+
+// Fast path of wg_smooth for even m when every colour has at most one node
+// per thread: the LDS offsets of a thread's four nodes (one per colour), of
+// their neighbours and stencils, and their sources b, do not change between
+// sweeps, so they are set up once per visit of the level and each colour
+// pass is loads, the FMA tree and one store.
+template <typename T>
+__device__ __forceinline__ static void wg_smooth_fast(T *sm, const WGLevel<T> &W, const T (&si)[40], int sweeps) {
+    const int t = threadIdx.x, hm = W.hm;
+    const int ty = t / hm, tx = t - ty * hm;
+    int kz[4], nbo[4][8], so[4];
+    unsigned fl[4];  // 1 active, 2 wall-row stencil, 4 no row below, 8 no row above
+    T bx[4], by[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int cx = c & 1, cy = c >> 1;
+        const int i = 2 * tx + cx, j = 2 * ty + cy;
+        const bool act = t < hm * W.hn && j < W.n;
+        const int ii = act ? i : 0, jj = act ? j : 0;
+        const int il = ii == 0 ? W.m - 1 : ii - 1, ir = ii == W.m - 1 ? 0 : ii + 1;
+        const int jd = jj > 0 ? jj - 1 : jj, ju = jj < W.n - 1 ? jj + 1 : jj;
+        kz[c] = cidx(W, ii, jj);
+        nbo[c][0] = cidx(W, il, jd);
+        nbo[c][1] = cidx(W, ii, jd);
+        nbo[c][2] = cidx(W, ir, jd);
+        nbo[c][3] = cidx(W, il, jj);
+        nbo[c][4] = cidx(W, ir, jj);
+        nbo[c][5] = cidx(W, il, ju);
+        nbo[c][6] = cidx(W, ii, ju);
+        nbo[c][7] = cidx(W, ir, ju);
+        so[c] = W.so + (jj * W.rs + ii * W.cs) * 40;
+        fl[c] = (act ? 1u : 0u) | ((jj < W.nb || jj >= W.n - W.nb) ? 2u : 0u) | (jj == 0 ? 4u : 0u) |
+                (jj == W.n - 1 ? 8u : 0u);
+        bx[c] = sm[W.bo + kz[c]];
+        by[c] = sm[W.bo + kz[c] + 1];
+    }
+    const T *z = sm + W.zo;
+    const vec2s<T> zero = {0, 0};
+    for (int sw = 0; sw < sweeps; sw++) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (fl[c] & 1u) {
+                vec2s<T> dl = lds2(z, nbo[c][0]), dc = lds2(z, nbo[c][1]), dr = lds2(z, nbo[c][2]);
+                const vec2s<T> cl = lds2(z, nbo[c][3]), cr = lds2(z, nbo[c][4]);
+                vec2s<T> ul = lds2(z, nbo[c][5]), uc = lds2(z, nbo[c][6]), ur = lds2(z, nbo[c][7]);
+                if (fl[c] & 4u) dl = dc = dr = zero;
+                if (fl[c] & 8u) ul = uc = ur = zero;
+                T ax, ay;
+                vec2s<T> o;
+                if (fl[c] & 2u) {
+                    const T *s = sm + so[c];
+                    offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+                    o = gs_value<T>(s, bx[c], by[c], ax, ay);
+                } else {
+                    offdiag9<T>(si, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+                    o = gs_value<T>(si, bx[c], by[c], ax, ay);
+                }
+                *reinterpret_cast<vec2s<T> *>(sm + W.zo + kz[c]) = o;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ static void wg_smooth_level(T *sm, const WGLevel<T> &W, const T (&si)[40], int sweeps) {
+    if (!W.xodd && W.hm * W.hn <= kCoarseThreads) wg_smooth_fast(sm, W, si, sweeps);
+    else wg_smooth(sm, W, si, sweeps);
+}
+
+// ---------------------------------------------------------------------------
+// Block path of the workgroup solver: m even, one stencil per row (cs == 0)
+// and at most kCoarseThreads 2x2 blocks.  Thread t owns block (tx, ty), the
+// nodes (2tx..2tx+1, 2ty..2ty+1), one of each colour.  For the whole visit of
+// a level it keeps the stencils of its two rows and its four sources in
+// registers, so a colour pass is eight LDS loads, the FMA tree and a store.
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ static bool block_ok(const WGLevel<T> &W) {
+    return !W.xodd && W.cs == 0 && W.hm * W.hn <= kCoarseThreads;
+}
+
+// element offset of block (row, col) in colour array c
+template <typename T> __device__ __forceinline__ static int boff(const WGLevel<T> &W, int c, int row, int col) {
+    return 2 * ((c * W.hn + row) * W.hm + col);
+}
+
+struct BlockPos {
+    bool act;
+    int tx, ty, txl, txr, tyd, tyu;
+};
+
+template <typename T> __device__ __forceinline__ static BlockPos block_pos(const WGLevel<T> &W) {
+    BlockPos b;
+    const int t = threadIdx.x, hm = W.hm, hn = W.hn;
+    b.act = t < hm * hn;
+    b.ty = b.act ? t / hm : 0;
+    b.tx = b.act ? t - b.ty * hm : 0;
+    b.txl = b.tx == 0 ? hm - 1 : b.tx - 1;
+    b.txr = b.tx == hm - 1 ? 0 : b.tx + 1;
+    b.tyd = b.ty > 0 ? b.ty - 1 : 0;
+    b.tyu = b.ty < hn - 1 ? b.ty + 1 : hn - 1;
+    return b;
+}
+
+// the eight neighbours of this thread's node of colour (CX, CY)
+template <typename T, int CX, int CY>
+__device__ __forceinline__ static void block_neighbours(const T *z, const WGLevel<T> &W, const BlockPos &b, vec2s<T> &dl,
+                                                        vec2s<T> &dc, vec2s<T> &dr, vec2s<T> &cl, vec2s<T> &cr,
+                                                        vec2s<T> &ul, vec2s<T> &uc, vec2s<T> &ur) {
+    constexpr int cX = (1 - CX) + 2 * CY, cY = CX + 2 * (1 - CY), cD = (1 - CX) + 2 * (1 - CY);
+    const int colL = CX ? b.tx : b.txl, colR = CX ? b.txr : b.tx;
+    const int rowD = CY ? b.ty : b.tyd, rowU = CY ? b.tyu : b.ty;
+    dl = lds2(z, boff(W, cD, rowD, colL));
+    dc = lds2(z, boff(W, cY, rowD, b.tx));
+    dr = lds2(z, boff(W, cD, rowD, colR));
+    cl = lds2(z, boff(W, cX, b.ty, colL));
+    cr = lds2(z, boff(W, cX, b.ty, colR));
+    ul = lds2(z, boff(W, cD, rowU, colL));
+    uc = lds2(z, boff(W, cY, rowU, b.tx));
+    ur = lds2(z, boff(W, cD, rowU, colR));
+    const int j = 2 * b.ty + CY;
+    const vec2s<T> zero = {0, 0};
+    if (j == 0) dl = dc = dr = zero;
+    if (j == W.n - 1) ul = uc = ur = zero;
+}
+
+template <typename T, int CX, int CY>
+__device__ __forceinline__ static void block_gs(T *sm, const WGLevel<T> &W, const BlockPos &b, const T (&s)[40], T bx,
+                                                T by) {
+    if (b.act && 2 * b.ty + CY < W.n) {
+        vec2s<T> dl, dc, dr, cl, cr, ul, uc, ur;
+        block_neighbours<T, CX, CY>(sm + W.zo, W, b, dl, dc, dr, cl, cr, ul, uc, ur);
+        T ax, ay;
+        offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+        *reinterpret_cast<vec2s<T> *>(sm + W.zo + boff(W, CX + 2 * CY, b.ty, b.tx)) = gs_value<T>(s, bx, by, ax, ay);
+    }
+    __syncthreads();
+}
+
+// residuals of this thread's four nodes from one load of their 4x4
+// neighbourhood (rows 2ty-1..2ty+2, columns 2tx-1..2tx+2; zero beyond a wall)
+template <typename T>
+__device__ __forceinline__ static void block_resid4(T *sm, const WGLevel<T> &W, const BlockPos &b, const T (&s0)[40],
+                                                    const T (&s1)[40], const T (&bx)[4], const T (&by)[4], int ro) {
+    if (!b.act) return;
+    const T *z = sm + W.zo;
+    const int rows[4] = {b.tyd, b.ty, b.ty, b.tyu}, cols[4] = {b.txl, b.tx, b.tx, b.txr};
+    vec2s<T> Z[4][4];
+    const vec2s<T> zero = {0, 0};
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const int j = 2 * b.ty - 1 + a, cy = (a + 1) & 1;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int cx = (c + 1) & 1;
+            Z[a][c] = lds2(z, boff(W, cx + 2 * cy, rows[a], cols[c]));
+            if (j < 0 || j >= W.n) Z[a][c] = zero;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int cx = c & 1, cy = c >> 1, i = 2 * b.tx + cx, j = 2 * b.ty + cy;
+        if (j >= W.n) continue;
+        T ax, ay, rx, ry;
+        if (cy == 0) {
+            offdiag9<T>(s0, Z[0][cx], Z[0][cx + 1], Z[0][cx + 2], Z[1][cx], Z[1][cx + 2], Z[2][cx], Z[2][cx + 1],
+                        Z[2][cx + 2], ax, ay);
+            resid_value<T>(s0, bx[c], by[c], ax, ay, Z[1][cx + 1], rx, ry);
+        } else {
+            offdiag9<T>(s1, Z[1][cx], Z[1][cx + 1], Z[1][cx + 2], Z[2][cx], Z[2][cx + 2], Z[3][cx], Z[3][cx + 1],
+                        Z[3][cx + 2], ax, ay);
+            resid_value<T>(s1, bx[c], by[c], ax, ay, Z[2][cx + 1], rx, ry);
+        }
+        sm[ro + 2 * (i + W.m * j)] = rx;
+        sm[ro + 2 * (i + W.m * j) + 1] = ry;
+    }
+}
+
+// z(i,j) += (P zc)(i,j) for one fine node
+template <typename T>
+__device__ __forceinline__ static void wg_prolong_node(T *sm, const WGLevel<T> &F, const WGLevel<T> &C, int i, int j) {
+    const int sc = C.m;
+    int cx0, cx1, cy0, cy1;
+    T wx0, wx1, wy0, wy1;
+    if ((i & 1) == 0) { cx0 = i >> 1; wx0 = 1; cx1 = cx0; wx1 = 0; }
+    else { cx0 = i >> 1; cx1 = cx0 + 1 == sc ? 0 : cx0 + 1; wx0 = wx1 = (T)0.5; }
+    if ((F.n & 1) == 0 && j == F.n - 1) { cy0 = C.n - 1; wy0 = 1; cy1 = cy0; wy1 = 0; }
+    else if ((j & 1) == 0) { cy0 = j >> 1; wy0 = 1; cy1 = cy0; wy1 = 0; }
+    else { cy0 = j >> 1; cy1 = cy0 + 1; wy0 = wy1 = (T)0.5; }
+    const T *zc = sm + C.zo;
+    const vec2s<T> a00 = lds2(zc, cidx(C, cx0, cy0)), a10 = lds2(zc, cidx(C, cx1, cy0)),
+                   a01 = lds2(zc, cidx(C, cx0, cy1)), a11 = lds2(zc, cidx(C, cx1, cy1));
+    const int k = cidx(F, i, j);
+    sm[F.zo + k] += wy0 * (wx0 * a00.x + wx1 * a10.x) + wy1 * (wx0 * a01.x + wx1 * a11.x);
+    sm[F.zo + k + 1] += wy0 * (wx0 * a00.y + wx1 * a10.y) + wy1 * (wx0 * a01.y + wx1 * a11.y);
+}
+
+// One visit of a block-path level: [z += P zc], `sweeps` four-colour sweeps,
+// [residuals into r].  Ends with a barrier unless residuals were written
+// (the caller's next barrier covers them).
+template <typename T>
+__device__ __forceinline__ static void wg_blocks(T *sm, const WGLevel<T> &W, int sweeps, const WGLevel<T> *Cp, int ro) {
+    const BlockPos b = block_pos(W);
+    const int j0 = 2 * b.ty, j1 = j0 + 1 < W.n ? j0 + 1 : j0;
+    // a wall row reads its own stencil; every other row reads the interior
+    // one (row n/2), so most lanes of a wave load the same LDS address
+    const int r0 = (j0 < W.nb || j0 >= W.n - W.nb) ? j0 : W.n / 2;
+    const int r1 = (j1 < W.nb || j1 >= W.n - W.nb) ? j1 : W.n / 2;
+    T s0[40], s1[40];
+#pragma unroll
+    for (int k = 0; k < 40; k++) {
+        s0[k] = sm[W.so + r0 * 40 + k];
+        s1[k] = sm[W.so + r1 * 40 + k];
+    }
+    if (Cp) {
+        if (b.act)
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int i = 2 * b.tx + (c & 1), j = 2 * b.ty + (c >> 1);
+                if (j < W.n) wg_prolong_node(sm, W, *Cp, i, j);
+            }
+        __syncthreads();
+    }
+    T bx[4], by[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int k = boff(W, c, b.ty, b.tx);
+        bx[c] = sm[W.bo + k];
+        by[c] = sm[W.bo + k + 1];
+    }
+    for (int sw = 0; sw < sweeps; sw++) {
+        block_gs<T, 0, 0>(sm, W, b, s0, bx[0], by[0]);
+        block_gs<T, 1, 0>(sm, W, b, s0, bx[1], by[1]);
+        block_gs<T, 0, 1>(sm, W, b, s1, bx[2], by[2]);
+        block_gs<T, 1, 1>(sm, W, b, s1, bx[3], by[3]);
+    }
+    if (ro >= 0) block_resid4(sm, W, b, s0, s1, bx, by, ro);
+}
+
+// copy cnt values from global memory into LDS through dst(t, value), eight
+// loads in flight per thread before the first store
+template <typename T, typename D> __device__ __forceinline__ static void g2lds(const T *src, int cnt, D dst) {
+    constexpr int K = 8;
+    for (int base = threadIdx.x; base < cnt; base += K * kCoarseThreads) {
+        T v[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int t = base + k * kCoarseThreads;
+            v[k] = t < cnt ? src[t] : (T)0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int t = base + k * kCoarseThreads;
+            if (t < cnt) dst(t, v[k]);
+        }
+    }
+}
+
+// the level's interior stencil into registers (uniform across the workgroup)
+template <typename T> __device__ __forceinline__ static void load_si(const LevelArgs<T> &L, T *si) {
+#pragma unroll
+    for (int k = 0; k < 40; k++) si[k] = L.Si[k];
 }
 
 template <typename T>
@@ -226,58 +518,84 @@ __global__ void __launch_bounds__(kCoarseThreads) coarse_solve_kernel(CoarseArgs
     T *sm = reinterpret_cast<T *>(smem_raw);
     const int tid = threadIdx.x, NT = kCoarseThreads;
     const int ro = A.roff;
-    // prologue: stencils of every level, source of level lc, bottom inverse
-    // (stored transposed, so that the threads of the bottom solve, one per
-    // row, read consecutive addresses)
-    for (int l = A.lc; l < A.L; l++) {
-        const int cnt = (A.lv[l].cs ? A.lv[l].m * A.lv[l].n : A.lv[l].n) * 40;
-        const T *src = A.lv[l].S;
-        for (int t = tid; t < cnt; t += NT) sm[A.soff[l] + t] = src[t];
-    }
+    // prologue: the constant part (stencils, transposed bottom inverse) is
+    // one contiguous copy of A.pack; the source of level lc goes colour-split
+    // with z = 0.  Every load of a batch is issued before its stores.
     {
         const WGLevel<T> W = wg_level(A, A.lc);
-        const T *gb = A.lv[A.lc].b;
-        for (int t = tid; t < W.m * W.n; t += NT) {
-            const int j = t / W.m, i = t - j * W.m, k = cidx(W, i, j);
-            sm[W.bo + k] = gb[2 * t];
-            sm[W.bo + k + 1] = gb[2 * t + 1];
-            sm[W.zo + k] = 0;
-            sm[W.zo + k + 1] = 0;
-        }
-        const int N = 2 * A.lv[A.L - 1].m * A.lv[A.L - 1].n;
-        for (int t = tid; t < N * N; t += NT) {
-            const int r = t / N, c = t - r * N;
-            sm[A.binv_off + c * N + r] = A.binv[t];
+        using V2 = vec2s<T>;
+        const int np = A.pack_elems / 2, nb = W.m * W.n;  // pack_elems is a multiple of 4
+        const V2 *pk = reinterpret_cast<const V2 *>(A.pack);
+        const V2 *gb = reinterpret_cast<const V2 *>(A.lv[A.lc].b);
+        constexpr int K = 8;
+        for (int base = tid; base < np + nb; base += K * NT) {
+            V2 v[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int e = base + k * NT;
+                v[k] = e < np ? pk[e] : (e < np + nb ? gb[e - np] : V2{0, 0});
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int e = base + k * NT;
+                if (e < np) {
+                    reinterpret_cast<V2 *>(sm)[e] = v[k];
+                } else if (e < np + nb) {
+                    const int node = e - np, j = node / W.m, i = node - j * W.m, c = cidx(W, i, j);
+                    *reinterpret_cast<V2 *>(sm + W.bo + c) = v[k];
+                    *reinterpret_cast<V2 *>(sm + W.zo + c) = V2{0, 0};
+                }
+            }
         }
     }
     __syncthreads();
     // down: smooth, fine residuals into r (one pass), P^T-weighted sums
     for (int l = A.lc; l < A.L - 1; l++) {
         const WGLevel<T> F = wg_level(A, l), C = wg_level(A, l + 1);
-        wg_smooth(sm, F, A.nu1);
-        for (int t = tid; t < F.m * F.n; t += NT) {
-            const int j = t / F.m, i = t - j * F.m;
-            wg_resid(sm, F, ro, i, j);
+        {
+            if (block_ok(F)) {
+                wg_blocks(sm, F, A.nu1, (const WGLevel<T> *)nullptr, ro);
+            } else {
+                T si[40];
+                load_si(A.lv[l], si);
+                wg_smooth_level(sm, F, si, A.nu1);
+                wg_resid_all(sm, F, si, ro);
+            }
         }
         __syncthreads();
         for (int t = tid; t < C.m * C.n; t += NT) {
             const int J = t / C.m, I = t - J * C.m;
             const int fx0 = 2 * I, fy0 = ((F.n & 1) == 0 && J == C.n - 1) ? F.n - 1 : 2 * J;
             T bx = 0, by = 0;
+            if (!F.xodd) {
+                // even fine width: x weights 1/2, 1, 1/2 at 2I-1 (periodic), 2I, 2I+1
+                const int fl = fx0 == 0 ? F.m - 1 : fx0 - 1;
 #pragma unroll
-            for (int b = -1; b <= 1; b++) {
-                const int fy = fy0 + b;
-                const float wy = pw_bounded(fy, F.n, C.n, J);
-                const int fyc = fy < 0 ? 0 : (fy >= F.n ? F.n - 1 : fy);
+                for (int b = -1; b <= 1; b++) {
+                    const int fy = fy0 + b;
+                    const T wy = (T)pw_bounded(fy, F.n, C.n, J);
+                    const int row = ro + 2 * F.m * (fy < 0 ? 0 : (fy >= F.n ? F.n - 1 : fy));
+                    const vec2s<T> rl = lds2(sm, row + 2 * fl), rc = lds2(sm, row + 2 * fx0),
+                                   rr = lds2(sm, row + 2 * fx0 + 2);
+                    bx = fmav(wy, fmav((T)0.5, rl.x + rr.x, rc.x), bx);
+                    by = fmav(wy, fmav((T)0.5, rl.y + rr.y, rc.y), by);
+                }
+            } else {
 #pragma unroll
-                for (int a = -1; a <= 1; a++) {
-                    const int fx = fx0 + a;
-                    const float wx = pw_periodic(fx, F.m, C.m, I);
-                    const int fi = fx < 0 ? fx + F.m : (fx >= F.m ? fx - F.m : fx);
-                    const vec2s<T> r = lds2(sm + ro, 2 * (fi + F.m * fyc));
-                    const T w = (T)(wx * wy);
-                    bx = fmav(w, r.x, bx);
-                    by = fmav(w, r.y, by);
+                for (int b = -1; b <= 1; b++) {
+                    const int fy = fy0 + b;
+                    const float wy = pw_bounded(fy, F.n, C.n, J);
+                    const int fyc = fy < 0 ? 0 : (fy >= F.n ? F.n - 1 : fy);
+#pragma unroll
+                    for (int a = -1; a <= 1; a++) {
+                        const int fx = fx0 + a;
+                        const float wx = pw_periodic(fx, F.m, C.m, I);
+                        const int fi = fx < 0 ? fx + F.m : (fx >= F.m ? fx - F.m : fx);
+                        const vec2s<T> r = lds2(sm + ro, 2 * (fi + F.m * fyc));
+                        const T w = (T)(wx * wy);
+                        bx = fmav(w, r.x, bx);
+                        by = fmav(w, r.y, by);
+                    }
                 }
             }
             const int k = cidx(C, I, J);
@@ -309,25 +627,18 @@ __global__ void __launch_bounds__(kCoarseThreads) coarse_solve_kernel(CoarseArgs
     // up: z += P zc, then smooth
     for (int l = A.L - 2; l >= A.lc; l--) {
         const WGLevel<T> F = wg_level(A, l), C = wg_level(A, l + 1);
-        for (int t = tid; t < F.m * F.n; t += NT) {
-            const int j = t / F.m, i = t - j * F.m;
-            const int sc = C.m;
-            int cx0, cx1, cy0, cy1;
-            T wx0, wx1, wy0, wy1;
-            if ((i & 1) == 0) { cx0 = i >> 1; wx0 = 1; cx1 = cx0; wx1 = 0; }
-            else { cx0 = i >> 1; cx1 = cx0 + 1 == sc ? 0 : cx0 + 1; wx0 = wx1 = (T)0.5; }
-            if ((F.n & 1) == 0 && j == F.n - 1) { cy0 = C.n - 1; wy0 = 1; cy1 = cy0; wy1 = 0; }
-            else if ((j & 1) == 0) { cy0 = j >> 1; wy0 = 1; cy1 = cy0; wy1 = 0; }
-            else { cy0 = j >> 1; cy1 = cy0 + 1; wy0 = wy1 = (T)0.5; }
-            const T *zc = sm + C.zo;
-            const vec2s<T> a00 = lds2(zc, cidx(C, cx0, cy0)), a10 = lds2(zc, cidx(C, cx1, cy0)),
-                           a01 = lds2(zc, cidx(C, cx0, cy1)), a11 = lds2(zc, cidx(C, cx1, cy1));
-            const int k = cidx(F, i, j);
-            sm[F.zo + k] += wy0 * (wx0 * a00.x + wx1 * a10.x) + wy1 * (wx0 * a01.x + wx1 * a11.x);
-            sm[F.zo + k + 1] += wy0 * (wx0 * a00.y + wx1 * a10.y) + wy1 * (wx0 * a01.y + wx1 * a11.y);
+        if (block_ok(F)) {
+            wg_blocks(sm, F, A.nu2, &C, -1);
+        } else {
+            T si[40];
+            load_si(A.lv[l], si);
+            for (int t = tid; t < F.m * F.n; t += NT) {
+                const int j = t / F.m, i = t - j * F.m;
+                wg_prolong_node(sm, F, C, i, j);
+            }
+            __syncthreads();
+            wg_smooth_level(sm, F, si, A.nu2);
         }
-        __syncthreads();
-        wg_smooth(sm, F, A.nu2);
     }
     // store the level-lc solution in natural order
     {
@@ -344,27 +655,48 @@ __global__ void __launch_bounds__(kCoarseThreads) coarse_solve_kernel(CoarseArgs
 template <typename T> int coarse_layout(CoarseArgs<T> &A) {
     int off = 0;
     for (int l = A.lc; l < A.L; l++) {
+        A.soff[l] = off;
+        off += stencil_count(A.lv[l]);
+        off = (off + 3) & ~3;
+    }
+    const int N = 2 * A.lv[A.L - 1].m * A.lv[A.L - 1].n;
+    A.binv_off = off;
+    off += N * N;
+    off = (off + 3) & ~3;
+    A.pack_elems = off;
+    for (int l = A.lc; l < A.L; l++) {
         // colour-split storage: four parity arrays of ceil(m/2) x ceil(n/2)
         const int cnt = 8 * ((A.lv[l].m + 1) >> 1) * ((A.lv[l].n + 1) >> 1);
         A.zoff[l] = off;
         off += cnt;
         A.boff[l] = off;
         off += cnt;
-        A.soff[l] = off;
-        off += (A.lv[l].cs ? A.lv[l].m * A.lv[l].n : A.lv[l].n) * 40;
-        off = (off + 3) & ~3;
     }
     // residual scratch: natural order, the largest level (also holds the
     // coarsest source for the dense solve)
     A.roff = off;
     off += 2 * A.lv[A.lc].m * A.lv[A.lc].n;
     off = (off + 3) & ~3;
-    const int N = 2 * A.lv[A.L - 1].m * A.lv[A.L - 1].n;
-    A.binv_off = off;
-    off += N * N;
     A.lds_elems = off;
     return off;
 }
+
+template <typename T>
+void coarse_pack(const CoarseArgs<T> &A, const std::vector<double> *S, const std::vector<double> &binv,
+                 std::vector<T> &out) {
+    out.assign((size_t)A.pack_elems, (T)0);
+    for (int l = A.lc; l < A.L; l++) {
+        const int cnt = stencil_count(A.lv[l]);
+        for (int k = 0; k < cnt; k++) out[(size_t)A.soff[l] + k] = (T)S[l][(size_t)k];
+    }
+    const int N = 2 * A.lv[A.L - 1].m * A.lv[A.L - 1].n;
+    for (int r = 0; r < N; r++)
+        for (int c = 0; c < N; c++) out[(size_t)A.binv_off + (size_t)c * N + r] = (T)binv[(size_t)r * N + c];
+}
+template void coarse_pack<double>(const CoarseArgs<double> &, const std::vector<double> *, const std::vector<double> &,
+                                  std::vector<double> &);
+template void coarse_pack<float>(const CoarseArgs<float> &, const std::vector<double> *, const std::vector<double> &,
+                                 std::vector<float> &);
 template int coarse_layout<double>(CoarseArgs<double> &);
 template int coarse_layout<float>(CoarseArgs<float> &);
 

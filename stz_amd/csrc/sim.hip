@@ -211,7 +211,8 @@ template <typename T> struct DevMG {
     CoarseArgs<T> args{};
     T *S[kMaxLevels] = {};
     T *zb[kMaxLevels] = {};  // z and b of levels (own allocations)
-    T *binv = nullptr;
+    T *pack = nullptr;       // constant part of the workgroup solver's LDS image
+    size_t pack_elems = 0;
     size_t lds = 0;
 
     void free_all() {
@@ -220,8 +221,9 @@ template <typename T> struct DevMG {
             if (zb[l]) (void)hipFree(zb[l]);
             S[l] = zb[l] = nullptr;
         }
-        if (binv) (void)hipFree(binv);
-        binv = nullptr;
+        if (pack) (void)hipFree(pack);
+        pack = nullptr;
+        pack_elems = 0;
     }
 };
 
@@ -360,6 +362,37 @@ template <typename T> static int set_lds_attr(const DevMG<T> &mg) {
     return mg_set_lds_limit<T>(mg.lds);
 }
 
+// upload the stencils of every level in precision T, recompute the
+// workgroup solver's LDS layout and upload its constant part
+template <typename T>
+static int set_level_stencils(stz_sim *s, DevMG<T> &mg, const std::vector<std::vector<double>> &Sx,
+                              const std::vector<double> &binv) {
+    int e;
+    for (size_t l = 0; l < s->hl.size(); l++) {
+        const HostLevel &H = s->hl[l];
+        if ((e = upload(&mg.S[l], Sx[l]))) return e;
+        LevelArgs<T> &A = mg.args.lv[l];
+        A.S = mg.S[l];
+        A.rs = H.xinv ? 1 : A.m;
+        A.cs = H.xinv ? 0 : 1;
+        A.Si = mg.S[l] + H.S.size();
+        A.nb = H.nb;
+    }
+    mg.lds = (size_t)coarse_layout(mg.args) * sizeof(T);
+    if ((e = set_lds_attr(mg))) return e;
+    std::vector<T> pk;
+    coarse_pack(mg.args, Sx.data(), binv, pk);
+    if (mg.pack && mg.pack_elems != pk.size()) {
+        (void)hipFree(mg.pack);
+        mg.pack = nullptr;
+    }
+    if (!mg.pack) STZ_HIP(hipMalloc(&mg.pack, pk.size() * sizeof(T)));
+    mg.pack_elems = pk.size();
+    STZ_HIP(hipMemcpy(mg.pack, pk.data(), pk.size() * sizeof(T), hipMemcpyHostToDevice));
+    mg.args.pack = mg.pack;
+    return STZ_OK;
+}
+
 // (re)build operators for viscosity/dt = viscdt (qs_multi::init, qs_multi.cc:31-54)
 static int mg_build(stz_sim *s, double viscdt) {
     if (viscdt == s->viscdt_prev) return STZ_OK;
@@ -372,6 +405,7 @@ static int mg_build(stz_sim *s, double viscdt) {
         return STZ_ERR_STATE;
     }
     int e;
+    std::vector<std::vector<double>> Sx(s->hl.size());
     for (size_t l = 0; l < s->hl.size(); l++) {
         HostLevel &H = s->hl[l];
         // rows at each wall whose stencil differs from the middle row
@@ -387,43 +421,14 @@ static int mg_build(stz_sim *s, double viscdt) {
             H.nb = nb;
         }
         // S holds the row stencils followed by one copy of the interior stencil
-        std::vector<double> Sx(H.S);
-        Sx.insert(Sx.end(), H.S.begin() + (size_t)(H.xinv ? H.n / 2 : 0) * 40,
-                  H.S.begin() + (size_t)(H.xinv ? H.n / 2 : 0) * 40 + 40);
-        if ((e = upload(&s->mg64.S[l], Sx))) return e;
-        LevelArgs<double> &A = s->mg64.args.lv[l];
-        A.S = s->mg64.S[l];
-        A.rs = H.xinv ? 1 : A.m;
-        A.cs = H.xinv ? 0 : 1;
-        A.Si = s->mg64.S[l] + H.S.size();
-        A.nb = H.nb;
+        Sx[l] = H.S;
+        Sx[l].insert(Sx[l].end(), H.S.begin() + (size_t)(H.xinv ? H.n / 2 : 0) * 40,
+                     H.S.begin() + (size_t)(H.xinv ? H.n / 2 : 0) * 40 + 40);
     }
-    if ((e = upload(&s->mg64.binv, binv))) return e;
-    s->mg64.args.binv = s->mg64.binv;
-    s->mg64.lds = (size_t)coarse_layout(s->mg64.args) * sizeof(double);
-    if ((e = set_lds_attr(s->mg64))) return e;
-    if (s->precision == STZ_PREC_MIXED) {
-        for (size_t l = 0; l < s->hl.size(); l++) {
-            HostLevel &H = s->hl[l];
-            std::vector<double> Sx(H.S);
-            Sx.insert(Sx.end(), H.S.begin() + (size_t)(H.xinv ? H.n / 2 : 0) * 40,
-                      H.S.begin() + (size_t)(H.xinv ? H.n / 2 : 0) * 40 + 40);
-            if ((e = upload(&s->mg32.S[l], Sx))) return e;
-            LevelArgs<float> &A = s->mg32.args.lv[l];
-            A.S = s->mg32.S[l];
-            A.rs = H.xinv ? 1 : A.m;
-            A.cs = H.xinv ? 0 : 1;
-            A.Si = s->mg32.S[l] + H.S.size();
-            A.nb = H.nb;
-        }
-        if ((e = upload(&s->mg32.binv, binv))) return e;
-        s->mg32.args.binv = s->mg32.binv;
-        s->mg32.lds = (size_t)coarse_layout(s->mg32.args) * sizeof(float);
-        if ((e = set_lds_attr(s->mg32))) return e;
-    }
+    if ((e = set_level_stencils(s, s->mg64, Sx, binv))) return e;
+    if (s->precision == STZ_PREC_MIXED && (e = set_level_stencils(s, s->mg32, Sx, binv))) return e;
     return STZ_OK;
 }
-
 
 // mixed form: FP64 residual of the top level -> FP32 source of the next
 // correction equation (and zero correction for the per-colour path)
