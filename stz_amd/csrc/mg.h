@@ -69,6 +69,9 @@ int mg_sum_partials(const double *partial, int np, double *out, hipStream_t st);
 template <typename T> int mg_fill_zero(T *p, size_t count, hipStream_t st);
 
 // tiled path (mg_tiled.hip)
+// row stencils with at most two distinct wall rows at each wall: the LDS
+// tiles of the tiled kernels and the refinement kernel apply
+template <typename T> bool wall_ok(const LevelArgs<T> &L);
 template <typename T> bool tiled_ok(const LevelArgs<T> &L);
 // one V-cycle with the tiled kernels where a level allows them; swaps the
 // z/z2 pointers of the levels it smooths; top_zero starts the top level

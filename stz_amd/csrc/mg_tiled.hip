@@ -34,16 +34,69 @@ namespace stz {
 
 enum { TILE_LOAD = 0, TILE_ZERO = 1, TILE_PROLONG = 2 };
 
-// one thread per 2x2 block of the tile-plus-halo region
-__host__ __device__ constexpr int tile_threads(int TX, int TY, int S) {
-    return (((TX + 8 * S) / 2) * ((TY + 8 * S) / 2) + 63) / 64 * 64;
+// Tile geometry.  The halo is 4S nodes for S fused sweeps, two more when the
+// kernel also restricts (the residual ring around the tile needs z valid one
+// node further out).  One thread per 2x2 block of the tile-plus-halo region,
+// or two when that would exceed 1024 threads.
+__host__ __device__ constexpr int tile_halo(int S, bool fuse) { return 4 * S + (fuse ? 2 : 0); }
+__host__ __device__ constexpr int tile_blocks(int TX, int TY, int H) { return ((TX + 2 * H) / 2) * ((TY + 2 * H) / 2); }
+__host__ __device__ constexpr int tile_bpt(int TX, int TY, int H) { return (tile_blocks(TX, TY, H) + 1023) / 1024; }
+__host__ __device__ constexpr int tile_threads(int TX, int TY, int H) {
+    return ((tile_blocks(TX, TY, H) + tile_bpt(TX, TY, H) - 1) / tile_bpt(TX, TY, H) + 63) / 64 * 64;
 }
 
-template <typename T, int TX, int TY, int S, int MODE>
-__global__ void __launch_bounds__(tile_threads(TX, TY, S))
-    smooth_tile_kernel(LevelArgs<T> L, T *__restrict__ zout, LevelArgs<T> Cz) {
-    constexpr int H = 4 * S, RX = TX + 2 * H, RY = TY + 2 * H;
-    constexpr int BX = RX / 2, BY = RY / 2, NB = BX * BY, NT = tile_threads(TX, TY, S);
+// the eight neighbours of block blk's node of colour C in colour-split LDS:
+// x-neighbours have colour cX, y-neighbours cY, diagonal ones cD, at block
+// offsets that depend only on the colour
+template <typename T, int BX, int NB, int C>
+__device__ __forceinline__ static void tile_neighbours(const vec2s<T> *zs, int blk, vec2s<T> &dl, vec2s<T> &dc,
+                                                       vec2s<T> &dr, vec2s<T> &cl, vec2s<T> &cr, vec2s<T> &ul,
+                                                       vec2s<T> &uc, vec2s<T> &ur) {
+    constexpr int cx = C & 1, cy = C >> 1;
+    constexpr int cX = (1 - cx) + 2 * cy, cY = cx + 2 * (1 - cy), cD = (1 - cx) + 2 * (1 - cy);
+    constexpr int xl = cx ? 0 : -1, xr = cx ? 1 : 0, yd = cy ? 0 : -BX, yu = cy ? BX : 0;
+    const vec2s<T> *zX = zs + cX * NB + blk, *zY = zs + cY * NB + blk, *zD = zs + cD * NB + blk;
+    dl = zD[yd + xl];
+    dc = zY[yd];
+    dr = zD[yd + xr];
+    cl = zX[xl];
+    cr = zX[xr];
+    ul = zD[yu + xl];
+    uc = zY[yu];
+    ur = zD[yu + xr];
+}
+
+// Wall rows (j < nb or j >= n - nb, nb <= kMaxWall) have stencils of their
+// own; every other row has the interior one.  A tile keeps the 2 nb wall
+// stencils in LDS, wherever it lies.
+constexpr int kMaxWall = 2;
+
+template <typename T>
+__device__ __forceinline__ static void load_wall_stencils(T *ss, const LevelArgs<T> &L, int tid, int nt) {
+    // the host launches tiled kernels only when wall_ok(L); the bound keeps
+    // the stores inside ss regardless
+    const int cnt = 2 * (L.nb < kMaxWall ? L.nb : kMaxWall) * 40;
+    for (int t = tid; t < cnt; t += nt) {
+        const int r = t / 40, k = t - r * 40, j = r < L.nb ? r : L.n - 2 * L.nb + r;
+        ss[t] = L.S[(size_t)j * 40 + k];
+    }
+}
+
+// the wall stencil of row gj (gj < nb or gj >= n - nb)
+template <typename T> __device__ __forceinline__ static const T *wall_stencil(const T *ss, int n, int nb, int gj) {
+    return ss + (gj < nb ? gj : nb + gj - (n - nb)) * 40;
+}
+
+// S four-colour Gauss-Seidel sweeps of one tile.  With FUSE the kernel then
+// also forms the residual on the tile and its first ring and restricts it to
+// the coarse nodes whose centre lies in the tile (R = P^T, full weighting in
+// x, tgmg.in.cc:626-647), writing Cr's source (and zeroing its z when
+// zero_z), so the V-cycle needs no separate restriction launch.
+template <typename T, int TX, int TY, int S, int MODE, bool FUSE>
+__global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
+    smooth_tile_kernel(LevelArgs<T> L, T *__restrict__ zout, LevelArgs<T> Cz, LevelArgs<T> Cr, int zero_z) {
+    constexpr int H = tile_halo(S, FUSE), RX = TX + 2 * H, RY = TY + 2 * H;
+    constexpr int BX = RX / 2, BY = RY / 2, NB = BX * BY, NT = tile_threads(TX, TY, H);
     constexpr int BPT = (NB + NT - 1) / NT;
     static_assert(RX % 2 == 0 && RY % 2 == 0, "even region");
     // colour-split LDS: node (li,lj) lives in array c = (li&1) + 2(lj&1) at
@@ -51,7 +104,7 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, S))
     // neighbour from another colour's array at consecutive addresses across
     // the lanes (no bank conflicts), with compile-time offsets per colour
     __shared__ vec2s<T> zs[4 * NB];
-    __shared__ T ss[RY * 40];
+    __shared__ T ss[2 * kMaxWall * 40];
     const int tid = threadIdx.x;
     const int m = L.m, n = L.n, nb = L.nb;
     const int i0 = blockIdx.x * TX - H, j0 = blockIdx.y * TY - H;
@@ -60,13 +113,8 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, S))
     T si[40];
 #pragma unroll
     for (int k = 0; k < 40; k++) si[k] = L.Si[k];
-    // wall-adjacent rows' stencils, only if this tile reaches them
-    const bool edge_tile = j0 < nb || j0 + RY > n - nb;
-    if (edge_tile)
-        for (int t = tid; t < RY * 40; t += NT) {
-            const int r = t / 40, k = t - r * 40, j = j0 + r;
-            ss[t] = (j >= 0 && j < n) ? L.S[(size_t)j * 40 + k] : (T)0;
-        }
+    // wall rows' stencils, only if this tile reaches them
+    if (j0 < nb || j0 + RY > n - nb) load_wall_stencils(ss, L, tid, NT);
     // z into LDS (zero outside the level), b into registers
     T bx[BPT][4], by[BPT][4];
 #pragma unroll
@@ -130,19 +178,15 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, S))
                 const int li = 2 * bxi + (c & 1), lj = 2 * byi + (c >> 1);
                 const int gj = j0 + lj;
                 if (li == 0 || li == RX - 1 || lj == 0 || lj == RY - 1 || gj < 0 || gj >= n) continue;
-                // neighbours: x-neighbours have colour cX, y-neighbours cY,
-                // diagonal ones cD; block offsets depend only on (cx, cy)
-                const int cx = c & 1, cy = c >> 1;
-                const int cX = (1 - cx) + 2 * cy, cY = cx + 2 * (1 - cy), cD = (1 - cx) + 2 * (1 - cy);
-                const int xl = cx ? 0 : -1, xr = cx ? 1 : 0, yd = cy ? 0 : -BX, yu = cy ? BX : 0;
-                const vec2s<T> *zX = zs + cX * NB + blk, *zY = zs + cY * NB + blk, *zD = zs + cD * NB + blk;
-                const vec2s<T> dl = zD[yd + xl], dc = zY[yd], dr = zD[yd + xr];
-                const vec2s<T> cl = zX[xl], cr = zX[xr];
-                const vec2s<T> ul = zD[yu + xl], uc = zY[yu], ur = zD[yu + xr];
+                vec2s<T> dl, dc, dr, cl, cr, ul, uc, ur;
+                if (c == 0) tile_neighbours<T, BX, NB, 0>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
+                else if (c == 1) tile_neighbours<T, BX, NB, 1>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
+                else if (c == 2) tile_neighbours<T, BX, NB, 2>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
+                else tile_neighbours<T, BX, NB, 3>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
                 T ax, ay;
                 vec2s<T> out;
                 if (gj < nb || gj >= n - nb) {
-                    const T *s = ss + lj * 40;
+                    const T *s = wall_stencil(ss, n, nb, gj);
                     offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
                     out = gs_value<T>(s, bx[q][c], by[q][c], ax, ay);
                 } else {
@@ -152,6 +196,40 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, S))
                 zs[c * NB + blk] = out;
             }
             __syncthreads();
+        }
+    }
+
+    // residuals of the tile and its first ring (z is valid two rings out)
+    T rxs[FUSE ? BPT : 1][4], rys[FUSE ? BPT : 1][4];
+    if (FUSE) {
+#pragma unroll
+        for (int q = 0; q < BPT; q++) {
+            const int blk = tid + q * NT;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                rxs[q][c] = 0;
+                rys[q][c] = 0;
+                if (blk >= NB) continue;
+                const int bxi = blk % BX, byi = blk / BX;
+                const int li = 2 * bxi + (c & 1), lj = 2 * byi + (c >> 1);
+                const int gj = j0 + lj;
+                if (li < H - 1 || li > H + TX || lj < H - 1 || lj > H + TY || gj < 0 || gj >= n) continue;
+                vec2s<T> dl, dc, dr, cl, cr, ul, uc, ur;
+                if (c == 0) tile_neighbours<T, BX, NB, 0>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
+                else if (c == 1) tile_neighbours<T, BX, NB, 1>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
+                else if (c == 2) tile_neighbours<T, BX, NB, 2>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
+                else tile_neighbours<T, BX, NB, 3>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
+                const vec2s<T> z0 = zs[c * NB + blk];
+                T ax, ay;
+                if (gj < nb || gj >= n - nb) {
+                    const T *s = wall_stencil(ss, n, nb, gj);
+                    offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+                    resid_value<T>(s, bx[q][c], by[q][c], ax, ay, z0, rxs[q][c], rys[q][c]);
+                } else {
+                    offdiag9<T>(si, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+                    resid_value<T>(si, bx[q][c], by[q][c], ax, ay, z0, rxs[q][c], rys[q][c]);
+                }
+            }
         }
     }
 
@@ -165,6 +243,61 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, S))
         zout[k] = v.x;
         zout[k + 1] = v.y;
     }
+    if (!FUSE) return;
+
+    // the residual ring region, natural order, in the z storage
+    constexpr int RW = TX + 2;
+    static_assert((TX + 2) * (TY + 2) <= 4 * NB, "residual region fits the tile storage");
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+        const int blk = tid + q * NT;
+        if (blk >= NB) continue;
+        const int bxi = blk % BX, byi = blk / BX;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int li = 2 * bxi + (c & 1), lj = 2 * byi + (c >> 1);
+            if (li < H - 1 || li > H + TX || lj < H - 1 || lj > H + TY) continue;
+            zs[(lj - (H - 1)) * RW + (li - (H - 1))] = vec2s<T>{rxs[q][c], rys[q][c]};
+        }
+    }
+    __syncthreads();
+    // coarse nodes of this tile: I in [fi0/2, fi0/2 + TX/2), J with its fine
+    // centre row fy0(J) in [fj0, fj0 + TY) (for an even fine height the last
+    // coarse row is centred on the last fine row)
+    const int fi0 = blockIdx.x * TX, fj0 = blockIdx.y * TY;
+    for (int t = tid; t < (TX / 2) * (TY / 2 + 1); t += NT) {
+        const int tI = t % (TX / 2), lr = t / (TX / 2);
+        int J;
+        if (lr < TY / 2) {
+            J = fj0 / 2 + lr;
+        } else {
+            J = Cr.n - 1;
+            if ((n & 1) || (J >= fj0 / 2 && J < fj0 / 2 + TY / 2)) continue;
+        }
+        if (J >= Cr.n) continue;
+        const int fy0 = ((n & 1) == 0 && J == Cr.n - 1) ? n - 1 : 2 * J;
+        if (fy0 < fj0 || fy0 >= fj0 + TY) continue;
+        const int I = fi0 / 2 + tI;
+        T bxs = 0, bys = 0;
+#pragma unroll
+        for (int b = -1; b <= 1; b++) {
+            const int fy = fy0 + b;
+            const T wy = (T)pw_bounded(fy, n, Cr.n, J);
+            const vec2s<T> *rp = zs + (fy - (fj0 - 1)) * RW + 2 * tI;  // fine columns 2I-1 .. 2I+1
+            if (wy != (T)0) {
+                bxs = fmav(wy, fmav((T)0.5, rp[0].x + rp[2].x, rp[1].x), bxs);
+                bys = fmav(wy, fmav((T)0.5, rp[0].y + rp[2].y, rp[1].y), bys);
+            }
+        }
+        const size_t k = 2 * ((size_t)I + (size_t)Cr.m * J);
+        Cr.b[k] = bxs;
+        Cr.b[k + 1] = bys;
+        if (zero_z) {
+            Cr.z[k] = 0;
+            Cr.z[k + 1] = 0;
+        }
+    }
 }
 
 // restriction: coarse tile CX x CY of C from fine level F
@@ -176,7 +309,7 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
     constexpr int ZX = RXr + 2, ZY = RYr + 2, NT = 256;
     __shared__ vec2s<T> zs[ZY * ZX];
     __shared__ vec2s<T> rs[RYr * RXr];
-    __shared__ T ss[RYr * 40];
+    __shared__ T ss[2 * kMaxWall * 40];
     const int tid = threadIdx.x, m = F.m, n = F.n, nb = F.nb;
     const int I0 = blockIdx.x * CX, J0 = blockIdx.y * CY;
     const int zi0 = 2 * I0 - 2, zj0 = 2 * J0 - 3;
@@ -198,11 +331,7 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
         }
         zs[t] = v;
     }
-    if (zj0 + 1 < nb || zj0 + 1 + RYr > n - nb)
-        for (int t = tid; t < RYr * 40; t += NT) {
-            const int r = t / 40, k = t - r * 40, j = zj0 + 1 + r;
-            ss[t] = (j >= 0 && j < n) ? F.S[(size_t)j * 40 + k] : (T)0;
-        }
+    if (zj0 + 1 < nb || zj0 + 1 + RYr > n - nb) load_wall_stencils(ss, F, tid, NT);
     __syncthreads();
     for (int t = tid; t < RXr * RYr; t += NT) {
         const int lj = t / RXr, li = t - lj * RXr;
@@ -218,7 +347,7 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
             const vec2s<T> z0 = zc[0];
             const size_t k = 2 * ((size_t)gi + (size_t)m * gj);
             if (gj < nb || gj >= n - nb) {
-                const T *s = ss + lj * 40;
+                const T *s = wall_stencil(ss, n, nb, gj);
                 offdiag9<T>(s, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
                 resid_value<T>(s, F.b[k], F.b[k + 1], ax, ay, z0, r.x, r.y);
             } else {
@@ -264,7 +393,7 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
 __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float *b32, double *partial) {
     constexpr int TX = 64, TY = 16, ZX = TX + 2, ZY = TY + 2, NT = 256;
     __shared__ vec2s<double> zs[ZY * ZX];
-    __shared__ double ss[TY * 40];
+    __shared__ double ss[2 * kMaxWall * 40];
     __shared__ double ws[4];
     const int tid = threadIdx.x, m = L.m, n = L.n, nb = L.nb;
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
@@ -286,11 +415,7 @@ __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float 
         }
         zs[t] = v;
     }
-    if (j0 < nb || j0 + TY > n - nb)
-        for (int t = tid; t < TY * 40; t += NT) {
-            const int r = t / 40, k = t - r * 40, j = j0 + r;
-            ss[t] = j < n ? L.S[(size_t)j * 40 + k] : 0.0;
-        }
+    if (j0 < nb || j0 + TY > n - nb) load_wall_stencils(ss, L, tid, NT);
     __syncthreads();
     double acc = 0;
     for (int t = tid; t < TX * TY; t += NT) {
@@ -302,7 +427,7 @@ __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float 
         const size_t k = 2 * ((size_t)gi + (size_t)m * gj);
         double ax, ay, rx, ry;
         if (gj < nb || gj >= n - nb) {
-            const double *s = ss + lj * 40;
+            const double *s = wall_stencil(ss, n, nb, gj);
             offdiag9<double>(s, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
             resid_value<double>(s, L.b[k], L.b[k + 1], ax, ay, z0, rx, ry);
         } else {
@@ -323,48 +448,77 @@ __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename T> bool tiled_ok(const LevelArgs<T> &L) { return L.cs == 0 && L.m % 32 == 0 && L.m >= 32; }
+template <typename T> bool wall_ok(const LevelArgs<T> &L) {
+    return L.cs == 0 && L.nb <= kMaxWall && L.n >= 2 * L.nb;
+}
+template <typename T> bool tiled_ok(const LevelArgs<T> &L) { return wall_ok(L) && L.m % 32 == 0 && L.m >= 32; }
 
 // large tiles (64x32) when they give at least one workgroup per CU, else
 // 32x16 tiles: on the small levels the kernel is latency-bound and more,
 // lighter workgroups finish sooner
-template <typename T, int S, int MODE>
-static void launch_tile(const LevelArgs<T> &L, T *zout, const LevelArgs<T> &C, hipStream_t st) {
-    if (L.m % 64 == 0 && (L.m / 64) * ((L.n + 31) / 32) >= 256) {
-        const dim3 g(L.m / 64, (L.n + 31) / 32);
-        hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE>), g, dim3(tile_threads(64, 32, S)), 0, st, L, zout,
-                           C);
+template <typename T> static bool big_tiles(const LevelArgs<T> &L) {
+    return L.m % 64 == 0 && (L.m / 64) * ((L.n + 31) / 32) >= 256;
+}
+
+// Fusing the restriction into the last pre-smoothing launch pays except for
+// FP64 64x32 tiles, whose widened halo needs two blocks per thread and
+// leaves one workgroup per CU
+template <typename T> static bool fuse_ok(const LevelArgs<T> &L) { return sizeof(T) == 4 || !big_tiles(L); }
+
+template <typename T, int S, int MODE, bool FUSE>
+static void launch_tile(const LevelArgs<T> &L, T *zout, const LevelArgs<T> &Cz, const LevelArgs<T> &Cr, int zero_z,
+                        hipStream_t st) {
+    if (big_tiles(L)) {
+        const dim3 g(L.m / 64, (L.n + 31) / 32), b(tile_threads(64, 32, tile_halo(S, FUSE)));
+        hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE, FUSE>), g, b, 0, st, L, zout, Cz, Cr, zero_z);
     } else {
-        const dim3 g(L.m / 32, (L.n + 15) / 16);
-        hipLaunchKernelGGL((smooth_tile_kernel<T, 32, 16, S, MODE>), g, dim3(tile_threads(32, 16, S)), 0, st, L, zout,
-                           C);
+        const dim3 g(L.m / 32, (L.n + 15) / 16), b(tile_threads(32, 16, tile_halo(S, FUSE)));
+        hipLaunchKernelGGL((smooth_tile_kernel<T, 32, 16, S, MODE, FUSE>), g, b, 0, st, L, zout, Cz, Cr, zero_z);
     }
 }
 
+template <typename T, int S, bool FUSE>
+static void launch_tile_mode(int mode, const LevelArgs<T> &L, T *zout, const LevelArgs<T> &Cz, const LevelArgs<T> &Cr,
+                             int zero_z, hipStream_t st) {
+    if (mode == TILE_ZERO) launch_tile<T, S, TILE_ZERO, FUSE>(L, zout, Cz, Cr, zero_z, st);
+    else if (mode == TILE_PROLONG) launch_tile<T, S, TILE_PROLONG, FUSE>(L, zout, Cz, Cr, zero_z, st);
+    else launch_tile<T, S, TILE_LOAD, FUSE>(L, zout, Cz, Cr, zero_z, st);
+}
+
+// the coarse z must start at zero unless its first smoothing is tiled
+// (which starts from zero by itself)
+template <typename T> static int coarse_needs_zero(const LevelArgs<T> &C) { return tiled_ok(C) ? 0 : 1; }
+
 // `sweeps` sweeps on level L; the first kernel starts from zero (TILE_ZERO),
 // loads z (TILE_LOAD) or adds the prolongation of C (TILE_PROLONG).  Tiled
-// kernels read L.z and write L.z2, then the two are swapped.  Levels the
-// tiled kernel does not cover use the per-colour kernels in place.
-template <typename T> static int smooth_level(LevelArgs<T> &L, const LevelArgs<T> *C, int sweeps, int mode,
-                                              hipStream_t st) {
+// kernels read L.z and write L.z2, then the two are swapped.  With R, the
+// last tiled launch also restricts the residual into R (returns 1 in
+// *restricted).  Levels the tiled kernel does not cover use the per-colour
+// kernels in place.
+template <typename T>
+static int smooth_level(LevelArgs<T> &L, const LevelArgs<T> *C, int sweeps, int mode, hipStream_t st,
+                        const LevelArgs<T> *R = nullptr, int *restricted = nullptr) {
     int e;
+    if (restricted) *restricted = 0;
     if (!tiled_ok(L)) {
         if (mode == TILE_PROLONG && (e = mg_prolong(*C, L, st))) return e;
         if (mode == TILE_ZERO && (e = mg_fill_zero(L.z, 2 * (size_t)L.m * L.n, st))) return e;
         return mg_smooth(L, sweeps, st);
     }
     const LevelArgs<T> Cv = C ? *C : L;
+    const LevelArgs<T> Rv = R ? *R : L;
+    const int zz = R ? coarse_needs_zero(*R) : 0;
     while (sweeps > 0) {
         const int s = sweeps >= 2 ? 2 : 1;
+        const bool fuse = R && s == sweeps;
         if (s == 2) {
-            if (mode == TILE_ZERO) launch_tile<T, 2, TILE_ZERO>(L, L.z2, Cv, st);
-            else if (mode == TILE_PROLONG) launch_tile<T, 2, TILE_PROLONG>(L, L.z2, Cv, st);
-            else launch_tile<T, 2, TILE_LOAD>(L, L.z2, Cv, st);
+            if (fuse) launch_tile_mode<T, 2, true>(mode, L, L.z2, Cv, Rv, zz, st);
+            else launch_tile_mode<T, 2, false>(mode, L, L.z2, Cv, Rv, zz, st);
         } else {
-            if (mode == TILE_ZERO) launch_tile<T, 1, TILE_ZERO>(L, L.z2, Cv, st);
-            else if (mode == TILE_PROLONG) launch_tile<T, 1, TILE_PROLONG>(L, L.z2, Cv, st);
-            else launch_tile<T, 1, TILE_LOAD>(L, L.z2, Cv, st);
+            if (fuse) launch_tile_mode<T, 1, true>(mode, L, L.z2, Cv, Rv, zz, st);
+            else launch_tile_mode<T, 1, false>(mode, L, L.z2, Cv, Rv, zz, st);
         }
+        if (fuse && restricted) *restricted = 1;
         std::swap(L.z, L.z2);
         sweeps -= s;
         mode = TILE_LOAD;
@@ -374,9 +528,8 @@ template <typename T> static int smooth_level(LevelArgs<T> &L, const LevelArgs<T
 }
 
 template <typename T> static int restrict_level(const LevelArgs<T> &F, const LevelArgs<T> &C, hipStream_t st) {
-    if (F.cs != 0) return mg_restrict(F, C, st);  // per-node stencils: simple kernel (zeroes C.z)
-    // the coarse z must start at zero unless its first smoothing is tiled
-    const int zero_z = tiled_ok(C) ? 0 : 1;
+    if (!wall_ok(F)) return mg_restrict(F, C, st);  // per-node stencils: simple kernel (zeroes C.z)
+    const int zero_z = coarse_needs_zero(C);
     constexpr int CY = sizeof(T) == 8 ? 8 : 16;
     if (((C.m + 31) / 32) * ((C.n + CY - 1) / CY) >= 256) {
         const dim3 g((C.m + 31) / 32, (C.n + CY - 1) / CY);
@@ -391,13 +544,18 @@ template <typename T> static int restrict_level(const LevelArgs<T> &F, const Lev
 
 template <typename T> int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipStream_t st) {
     int e;
-    if ((e = smooth_level(A.lv[0], (const LevelArgs<T> *)nullptr, A.nu1, top_zero ? TILE_ZERO : TILE_LOAD, st)))
-        return e;
     for (int l = 0; l < A.lc; l++) {
-        if ((e = restrict_level(A.lv[l], A.lv[l + 1], st))) return e;
-        if (l + 1 < A.lc && (e = smooth_level(A.lv[l + 1], (const LevelArgs<T> *)nullptr, A.nu1, TILE_ZERO, st)))
+        // pre-smoothing, fused with the restriction into level l+1 when tiled
+        int done = 0;
+        if ((e = smooth_level(A.lv[l], (const LevelArgs<T> *)nullptr, A.nu1,
+                              l == 0 ? (top_zero ? TILE_ZERO : TILE_LOAD) : TILE_ZERO, st,
+                              fuse_ok(A.lv[l]) ? &A.lv[l + 1] : nullptr, &done)))
             return e;
+        if (!done && (e = restrict_level(A.lv[l], A.lv[l + 1], st))) return e;
     }
+    if (A.lc == 0 && (e = smooth_level(A.lv[0], (const LevelArgs<T> *)nullptr, A.nu1,
+                                       top_zero ? TILE_ZERO : TILE_LOAD, st)))
+        return e;
     if ((e = mg_coarse(A, st))) return e;
     for (int l = A.lc; l >= 1; l--)
         if ((e = smooth_level(A.lv[l - 1], &A.lv[l], A.nu2, TILE_PROLONG, st))) return e;
@@ -417,6 +575,8 @@ int mg_refine(const LevelArgs<double> &L, float *b32, double *partial, double *o
     return mg_sum_partials(partial, (int)(g.x * g.y), out, st);
 }
 
+template bool wall_ok<double>(const LevelArgs<double> &);
+template bool wall_ok<float>(const LevelArgs<float> &);
 template bool tiled_ok<double>(const LevelArgs<double> &);
 template bool tiled_ok<float>(const LevelArgs<float> &);
 template int mg_vcycle_tiled<double>(CoarseArgs<double> &, bool, hipStream_t);

@@ -435,7 +435,7 @@ static int mg_build(stz_sim *s, double viscdt) {
 static int refine_resid(stz_sim *s) {
     LevelArgs<double> t = s->top64();
     LevelArgs<float> &f = s->mg32.args.lv[0];
-    if (s->cfg.kernels == 0) return mg_refine(t, f.b, s->partial, s->dnorm, s->stream);
+    if (s->cfg.kernels == 0 && wall_ok(t)) return mg_refine(t, f.b, s->partial, s->dnorm, s->stream);
     return mg_resid64_to32(t, f.b, f.z, s->partial, s->dnorm, s->stream);
 }
 
@@ -447,7 +447,6 @@ static int top_l2(stz_sim *s, double *out) {
     } else {
         if ((e = mg_resid_norm(s->top64(), s->partial, s->dnorm, s->stream))) return e;
     }
-    STZ_HIP(hipMemcpyAsync(s->hnorm, s->dnorm, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     STZ_HIP(hipStreamSynchronize(s->stream));
     *out = *s->hnorm / ((double)s->c.m * (s->c.n + 1));
     return STZ_OK;
@@ -517,7 +516,6 @@ static int mg_solve_impl(stz_sim *s) {
     int n = s->lim / g.predict_mult;
     if ((e = run_cycles(s, n, true))) return e;
     if (s->precision == STZ_PREC_MIXED) {
-        STZ_HIP(hipMemcpyAsync(s->hnorm, s->dnorm, sizeof(double), hipMemcpyDeviceToHost, s->stream));
         STZ_HIP(hipStreamSynchronize(s->stream));
         acc = *s->hnorm / ((double)s->c.m * (s->c.n + 1));
     } else if ((e = top_l2(s, &acc))) return e;
@@ -537,7 +535,6 @@ static int mg_solve_impl(stz_sim *s) {
             n += k;
             if ((e = run_cycles(s, k, true))) return e;
             if (s->precision == STZ_PREC_MIXED) {
-                STZ_HIP(hipMemcpyAsync(s->hnorm, s->dnorm, sizeof(double), hipMemcpyDeviceToHost, s->stream));
                 STZ_HIP(hipStreamSynchronize(s->stream));
                 acc = *s->hnorm / ((double)s->c.m * (s->c.n + 1));
             } else if ((e = top_l2(s, &acc))) return e;
@@ -682,8 +679,10 @@ int stz_sim_create(const stz_sim_params *p, int precision, stz_sim **out) {
     s->nblocks_norm = (int)std::max<size_t>((nodes + 255) / 256,
                                             (size_t)((c.m + 63) / 64) * ((c.n + 1 + 15) / 16));
     if (hipMalloc(&s->partial, (s->nblocks_norm + 1) * sizeof(double)) != hipSuccess) return fail(STZ_ERR_HIP);
-    s->dnorm = s->partial + s->nblocks_norm;
-    if (hipHostMalloc(&s->hnorm, sizeof(double), hipHostMallocDefault) != hipSuccess) return fail(STZ_ERR_HIP);
+    // the residual norm is written by the summing kernel straight into
+    // coherent pinned host memory: no copy launch before each convergence test
+    if (hipHostMalloc(&s->hnorm, sizeof(double), hipHostMallocCoherent) != hipSuccess) return fail(STZ_ERR_HIP);
+    s->dnorm = s->hnorm;
 
     build_dims(s);
     if ((e = alloc_levels(s, s->mg64, false))) return fail(e);
@@ -938,7 +937,6 @@ int stz_mg_l2_error(stz_sim *s, double *err) {
     if (s->viscdt_prev < 0) { set_error("stz_mg_setup has not been called"); return STZ_ERR_STATE; }
     int e = mg_resid_norm(s->top64(), s->partial, s->dnorm, s->stream);
     if (e) return e;
-    STZ_HIP(hipMemcpyAsync(s->hnorm, s->dnorm, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     STZ_HIP(hipStreamSynchronize(s->stream));
     *err = *s->hnorm / ((double)s->c.m * (s->c.n + 1));
     return STZ_OK;
