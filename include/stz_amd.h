@@ -80,6 +80,10 @@ enum {
     STZ_F_U = 0, STZ_F_V, STZ_F_P, STZ_F_Q, STZ_F_S, STZ_F_TAU, STZ_F_CHI,
     STZ_F_X, STZ_F_Y, STZ_F_DDEV, STZ_NFIELDS
 };
+/* The c_field::cu slot (fields.hh:28), which the reference's diagnostics
+ * use as their output plane (qs_measure, compute_strain).  Same plane
+ * layout; not part of the state a step reads. */
+enum { STZ_F_CU = STZ_NFIELDS };
 
 /* ---- precision of the projection solve --------------------------------- */
 enum {
@@ -134,7 +138,11 @@ int stz_sim_destroy(stz_sim *sim);
 int stz_sim_set_stream(stz_sim *sim, void *hip_stream);
 int stz_sim_synchronize(stz_sim *sim);
 
-/* Field access (public `fm`, shear_sim.hh:79).  Plane = m*(n+2) doubles. */
+/* Field access (public `fm`, shear_sim.hh:79).  Plane = m*(n+2) doubles;
+ * field is one of STZ_F_U .. STZ_F_DDEV, or STZ_F_CU.  The device pointer is
+ * the field's current plane: p, q, s, tau, chi, X and Y alternate between two
+ * planes at every step, so a caller that keeps device pointers queries them
+ * again after stepping. */
 int stz_sim_upload_field(stz_sim *sim, int field, const double *host_plane);
 int stz_sim_download_field(stz_sim *sim, int field, double *host_plane);
 int stz_sim_field_device_ptr(stz_sim *sim, int field, double **dev_plane);
@@ -190,6 +198,38 @@ int stz_mg_l2_error(stz_sim *sim, double *err);
 int stz_mg_smooth(stz_sim *sim, int level, int sweeps);
 /* Upload a source array (m*(n+1) vec2) for a standalone solve. */
 int stz_mg_upload_src(stz_sim *sim, const double *host_vec2);
+
+/* ---- direct step, diagnostics, initial conditions ------------------------ */
+
+/* shear_sim::step_forward (shear_sim.cc:185-276), the direct (dynamic)
+ * explicit step, nsteps times. */
+int stz_sim_step_forward(stz_sim *sim, double dt, int nsteps);
+/* shear_sim::tractions (shear_sim.cc:1003-1028): out4 = (bx, by, tx, ty). */
+int stz_sim_tractions(stz_sim *sim, double *out4);
+/* shear_sim::qs_measure (shear_sim.cc:733-775); the integrand goes to the
+ * STZ_F_CU plane (rows 1..n-2). */
+int stz_sim_qs_measure(stz_sim *sim, int staggered, double *Q, double *max_qs);
+/* shear_sim::compute_strain (shear_sim.cc:778-824): deviatoric modulus of the
+ * Hencky strain into the STZ_F_CU plane (rows 0..n-1).  corrected = 0 gives
+ * the reference's result, with its two slips: the y difference on the
+ * interior rows takes both ends from row j+1 (shear_sim.cc:804), which yields
+ * 0 there, and the seam column i = m-1 subtracts the periodic shift of its
+ * right neighbour (:793) instead of adding it.  corrected = 1 uses rows j+1
+ * and j-1 on the interior rows and adds the shift. */
+int stz_sim_compute_strain(stz_sim *sim, int corrected);
+/* shear_sim::l2_comparison (shear_sim.cc:460-502) of a against b (same size,
+ * same device): out4 = velocity, stress, chi, reference-map squared norms. */
+int stz_sim_l2_comparison(stz_sim *a, stz_sim *b, double *out4);
+/* shear_sim::initialize_random (shear_sim.cc:932-985): the noise is drawn
+ * from the C library's rand() in the reference's order (seed it with srand
+ * first, as the reference's drivers do); the Gaussian smoothing runs on the
+ * device.  Writes u, v, X, Y on rows 0..n and chi, p, q, s, tau on rows
+ * 0..n-1. */
+int stz_sim_initialize_random(stz_sim *sim, double l, double tem_avg, double tem_stdev);
+/* shear_sim::initialize_chi_bicubic (shear_sim.cc:832-913): chi on rows
+ * 0..n-1 from the mm x nn row-major array arr (lo + sca * interpolant); the
+ * other fields reset on rows -1..n. */
+int stz_sim_initialize_chi_bicubic(stz_sim *sim, int mm, int nn, const double *arr, double lo, double sca);
 
 /* Per-cell decision bits of the last advection step (shear_sim.cc:296-349,
  * 513-548), one uint32 per cell (i,j), j in [0,n), i fastest.  "ENO" bits are

@@ -84,6 +84,11 @@ def _bind(lib, prefix):
         "avg_iters": (d, [vp]),
         "tractions": (None, [vp, _dp]),
         "qs_measure": (None, [vp, i, C.POINTER(d), C.POINTER(d)]),
+        "get_cu": (None, [vp, _dp]),
+        "step_forward": (None, [vp, d]),
+        "compute_strain": (None, [vp, i]),
+        "l2_comparison": (None, [vp, vp, _dp]),
+        "initialize_chi_bicubic": (None, [vp, i, i, _dp, d, d]),
     }
     out = {}
     for name, (res, args) in sig.items():
@@ -205,6 +210,27 @@ class _Sim:
         self._f["tractions"](self._h, o)
         return o
 
+    def get_cu(self):
+        a = np.empty(self.plane_shape, dtype=np.float64)
+        self._f["get_cu"](self._h, a)
+        return a
+
+    def step_forward(self, dt):
+        self._f["step_forward"](self._h, float(dt))
+
+    def compute_strain(self, corrected=False):
+        self._f["compute_strain"](self._h, 1 if corrected else 0)
+
+    def l2_comparison(self, other):
+        out = np.zeros(4)
+        self._f["l2_comparison"](self._h, other._h, out)
+        return out
+
+    def initialize_chi_bicubic(self, arr, lo=0.0, sca=1.0):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        nn, mm = arr.shape
+        self._f["initialize_chi_bicubic"](self._h, int(mm), int(nn), arr, float(lo), float(sca))
+
     def qs_measure(self, staggered=False):
         q, mq = C.c_double(), C.c_double()
         self._f["qs_measure"](self._h, 1 if staggered else 0, C.byref(q), C.byref(mq))
@@ -298,6 +324,7 @@ class Reference(_Sim):
             lib = C.CDLL(REF_SO)
             f = _bind(lib, "ref_")
             for name, res, args in (
+                ("output", None, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int]),
                 ("step", None, [C.c_void_p, C.c_double, C.c_int]),
                 ("projection_step", None, [C.c_void_p, C.c_double]),
                 ("mg_solve", C.c_int, [C.c_void_p]),
@@ -325,6 +352,10 @@ class Reference(_Sim):
 
     def mg_lim(self):
         return self._f["mg_lim"](self._h)
+
+    def output(self, directory, prefix, mode, sn):
+        """shear_sim::output: Gnuplot binary matrix of one field"""
+        self._f["output"](self._h, directory.encode(), prefix.encode(), int(mode), int(sn))
 
     @staticmethod
     def set_threads(n):

@@ -168,5 +168,29 @@ void ref_qs_measure(void *h, int staggered, double *Q, double *mq) {
     static_cast<ref_sim *>(h)->sim->qs_measure(*Q, *mq, staggered != 0);
 }
 
+// the c_field::cu slot (rows -1..n), the diagnostic output of qs_measure,
+// compute_strain and the direct step
+void ref_get_cu(void *h, double *plane) { ref_get_field(h, 10, plane); }
+void ref_step_forward(void *h, double dt) { static_cast<ref_sim *>(h)->sim->step_forward(dt); }
+// the reference has only its own form (corrected is not available)
+void ref_compute_strain(void *h, int corrected) {
+    (void)corrected;
+    static_cast<ref_sim *>(h)->sim->compute_strain();
+}
+void ref_l2_comparison(void *a, void *b, double *l2) {
+    static_cast<ref_sim *>(a)->sim->l2_comparison(*static_cast<ref_sim *>(b)->sim, l2);
+}
+void ref_initialize_chi_bicubic(void *h, int mm, int nn, const double *arr, double lo, double sca) {
+    static_cast<ref_sim *>(h)->sim->initialize_chi_bicubic(mm, nn, const_cast<double *>(arr), lo, sca);
+}
+// shear_sim::output (shear_sim.cc:688-726) of one field into directory dir
+void ref_output(void *h, const char *dir, const char *prefix, int mode, int sn) {
+    shear_sim &s = *static_cast<ref_sim *>(h)->sim;
+    const char *keep = s.filename;
+    s.filename = dir;
+    s.output(prefix, mode, sn);
+    s.filename = keep;
+}
+
 void ref_set_threads(int n) { omp_set_num_threads(n); }
 }

@@ -50,6 +50,7 @@ struct orc_sim {
     double *buf;                  /* all planes */
     double *f[STZ_NFIELDS];       /* offset so f[k][i+m*j], j in [-1,n] */
     double *cX, *cY, *cp, *cq, *cs, *ctau, *cchi;
+    double *cu, *cv;              /* cu doubles as the diagnostic output slot */
     v2 *vel, *src;                /* m*(n+1) nodes */
     uint32_t *dec;
     int rec_dec;
@@ -215,7 +216,7 @@ orc_sim *orc_create(const stz_sim_params *p) {
     s->time = 0;
     s->chi_inf = p->stz.chi_inf;
     size_t plane = (size_t)m * (n + 2);
-    s->buf = (double *)calloc(plane * (STZ_NFIELDS + 7), sizeof(double));
+    s->buf = (double *)calloc(plane * (STZ_NFIELDS + 9), sizeof(double));
     for (k = 0; k < STZ_NFIELDS; k++) s->f[k] = s->buf + plane * k + m;
     s->cX = s->buf + plane * (STZ_NFIELDS + 0) + m;
     s->cY = s->buf + plane * (STZ_NFIELDS + 1) + m;
@@ -224,6 +225,8 @@ orc_sim *orc_create(const stz_sim_params *p) {
     s->cs = s->buf + plane * (STZ_NFIELDS + 4) + m;
     s->ctau = s->buf + plane * (STZ_NFIELDS + 5) + m;
     s->cchi = s->buf + plane * (STZ_NFIELDS + 6) + m;
+    s->cu = s->buf + plane * (STZ_NFIELDS + 7) + m;
+    s->cv = s->buf + plane * (STZ_NFIELDS + 8) + m;
     s->vel = (v2 *)calloc((size_t)m * (n + 1), sizeof(v2));
     s->src = (v2 *)calloc((size_t)m * (n + 1), sizeof(v2));
     s->rtmp = (v2 *)calloc((size_t)m * (n + 1), sizeof(v2));
@@ -274,6 +277,7 @@ void orc_destroy(orc_sim *s) {
 
 void orc_set_field(orc_sim *s, int k, const double *pl) { memcpy(s->f[k] - s->m, pl, sizeof(double) * s->m * (s->n + 2)); }
 void orc_get_field(orc_sim *s, int k, double *pl) { memcpy(pl, s->f[k] - s->m, sizeof(double) * s->m * (s->n + 2)); }
+void orc_get_cu(orc_sim *s, double *pl) { memcpy(pl, s->cu - s->m, sizeof(double) * s->m * (s->n + 2)); }
 void orc_set_vel(orc_sim *s, const double *v) { memcpy(s->vel, v, sizeof(v2) * s->qmn); }
 void orc_get_vel(orc_sim *s, double *v) { memcpy(v, s->vel, sizeof(v2) * s->qmn); }
 void orc_get_src(orc_sim *s, double *v) { memcpy(v, s->src, sizeof(v2) * s->qmn); }
@@ -1060,6 +1064,7 @@ void orc_qs_measure(orc_sim *s, int staggered, double *Qo, double *max_qs) {
             }
             l1 = ux + vy; l2 = ux - vy; l3 = uy + vx;
             cu = qsf1 * l1 * l1 + qsf2 * (l2 * l2 + l3 * l3);
+            s->cu[ij] = cu;
             l += cu;
             if (max_l < cu) max_l = cu;
         }
@@ -1068,4 +1073,309 @@ void orc_qs_measure(orc_sim *s, int staggered, double *Qo, double *max_qs) {
     }
     *Qo = sqrt(Qs * s->dx * s->dy);
     *max_qs = mq;
+}
+
+/* ------------------------------------------------------------------------ */
+/* direct (dynamic) step: shear_sim.cc:185-276                               */
+/* ------------------------------------------------------------------------ */
+
+void orc_step_forward(orc_sim *s, double dt) {
+    const int m = s->m, n = s->n;
+    const double third = 1. / 3;
+    const double hx = 0.5 * s->xsp * dt, hy = 0.5 * s->ysp * dt, lx = s->lx;
+    const double hxxv = 2 * s->xsp * hx * s->p.viscosity, hyyv = 2 * s->ysp * hy * s->p.viscosity,
+                 hcv = 2 * (hxxv + hyyv);
+    const double hxxcd = 2 * s->xsp * hx * s->p.chi_diff, hyycd = 2 * s->ysp * hy * s->p.chi_diff,
+                 hccd = 2 * (hxxcd + hyycd);
+    const double K = s->p.K, mu = s->p.mu, mu_inv = s->mu_inv;
+    double *U = s->f[STZ_F_U], *V = s->f[STZ_F_V], *P = s->f[STZ_F_P], *Q = s->f[STZ_F_Q],
+           *S = s->f[STZ_F_S], *T = s->f[STZ_F_TAU], *C = s->f[STZ_F_CHI], *X = s->f[STZ_F_X],
+           *Y = s->f[STZ_F_Y], *D = s->f[STZ_F_DDEV];
+    int i, j, k, br;
+    s->time += dt;
+    for (j = 0; j < n; j++) for (i = 0; i < m; i++) {
+        const int ij = i + m * j;
+        const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
+        const int ill = i <= 1 ? i + m - 2 : i - 2, irr = i >= m - 2 ? i + 2 - m : i + 2;
+        const int l = il + m * j, r = ir + m * j, d = ij - m, u = ij + m;
+        const int ll = ill + m * j, rr = irr + m * j, ru = ir + m * (j + 1);
+        double uc, vc, ux, vx, uy, vy, uvisc, vvisc, two_omega, dtmp;
+        if (j > 0) {
+            double Xx, Xy, Yx, Yy, fx, fy;
+            uc = U[ij]; vc = V[ij];
+            /* :203-219, rmv_eno2 :611-615, rm_eno2 :622-625, rmv_one_sided :588-592 */
+            if (uc > 0) {
+                ux = hx * eno2b(U[r], U[ij], U[l], U[ll], &br);
+                vx = hx * eno2b(V[r], V[ij], V[l], V[ll], &br);
+                Xx = hx * eno2b(X[r] + (i == m - 1 ? lx : 0), X[ij], X[l] + (i == 0 ? -lx : 0),
+                                X[ll] + (i <= 1 ? -lx : 0), &br);
+                Yx = hx * eno2b(Y[r], Y[ij], Y[l], Y[ll], &br);
+            } else {
+                ux = -hx * eno2b(U[l], U[ij], U[r], U[rr], &br);
+                vx = -hx * eno2b(V[l], V[ij], V[r], V[rr], &br);
+                Xx = -hx * eno2b(X[l] + (i == 0 ? -lx : 0), X[ij], X[r] + (i == m - 1 ? lx : 0),
+                                 X[rr] + (i >= m - 2 ? lx : 0), &br);
+                Yx = -hx * eno2b(Y[l], Y[ij], Y[r], Y[rr], &br);
+            }
+            if (vc > 0) {
+                if (j > 1) {
+                    uy = hy * eno2b(U[u], U[ij], U[d], U[ij - 2 * m], &br);
+                    vy = hy * eno2b(V[u], V[ij], V[d], V[ij - 2 * m], &br);
+                    Xy = hy * eno2b(X[u], X[ij], X[d], X[ij - 2 * m], &br);
+                    Yy = hy * eno2b(Y[u], Y[ij], Y[d], Y[ij - 2 * m], &br);
+                } else {
+                    uy = 2 * hy * (U[ij] - U[d]);
+                    vy = 2 * hy * (V[ij] - V[d]);
+                    Xy = (hy * 2) * (X[ij] - X[d]);
+                    Yy = (hy * 2) * (Y[ij] - Y[d]);
+                }
+            } else {
+                if (j < n - 1) {
+                    uy = -hy * eno2b(U[d], U[ij], U[u], U[ij + 2 * m], &br);
+                    vy = -hy * eno2b(V[d], V[ij], V[u], V[ij + 2 * m], &br);
+                    Xy = -hy * eno2b(X[d], X[ij], X[u], X[ij + 2 * m], &br);
+                    Yy = -hy * eno2b(Y[d], Y[ij], Y[u], Y[ij + 2 * m], &br);
+                } else {
+                    uy = 2 * -hy * (U[ij] - U[u]);
+                    vy = 2 * -hy * (V[ij] - V[u]);
+                    Xy = (-hy * 2) * (X[ij] - X[u]);
+                    Yy = (-hy * 2) * (Y[ij] - Y[u]);
+                }
+            }
+            /* net_force(fx,fy,-hx,-hy,flp,fp), :447-453 */
+            {
+                const double xf = -hx, yf = -hy;
+                const int c0 = l - m, c1 = ij - m, c2 = l, c3 = ij;
+                fx = (-P[c0] + P[c1] - P[c2] + P[c3] - Q[c0] + Q[c1] - Q[c2] + Q[c3] + S[c0] - S[c1] + S[c2] - S[c3]) * xf
+                   + (T[c0] + T[c1] - T[c2] - T[c3]) * yf;
+                fy = (T[c0] - T[c1] + T[c2] - T[c3]) * xf
+                   + (-P[c0] - P[c1] + P[c2] + P[c3] - Q[c0] - Q[c1] + Q[c2] + Q[c3] - S[c0] - S[c1] + S[c2] + S[c3]) * yf;
+            }
+            uvisc = hxxv * (U[r] + U[l]) + hyyv * (U[u] + U[d]) - hcv * uc;
+            vvisc = hxxv * (V[r] + V[l]) + hyyv * (V[u] + V[d]) - hcv * vc;
+            s->cu[ij] = -uc * ux - vc * uy + mu_inv * (fx + uvisc);
+            s->cv[ij] = -uc * vx - vc * vy + mu_inv * (fy + vvisc);
+            s->cX[ij] = -uc * Xx - vc * Xy;
+            s->cY[ij] = -uc * Yx - vc * Yy;
+        }
+        /* :240-247 */
+        uc = 0.25 * (U[ij] + U[r] + U[u] + U[ru]);
+        vc = 0.25 * (V[ij] + V[r] + V[u] + V[ru]);
+        ux = hx * (-U[ij] + U[r] - U[u] + U[ru]);
+        uy = hy * (-U[ij] - U[r] + U[u] + U[ru]);
+        vx = hx * (-V[ij] + V[r] - V[u] + V[ru]);
+        vy = hy * (-V[ij] - V[r] + V[u] + V[ru]);
+        two_omega = vx - uy;
+        /* :250-252 */
+        double f1[5], f2[5];
+        double *fl5[5] = {P, Q, S, T, C};
+        if (uc > 0) {
+            for (k = 0; k < 5; k++) { double *F = fl5[k]; f1[k] = hx * eno2b(F[r], F[ij], F[l], F[ll], &br); }
+        } else {
+            for (k = 0; k < 5; k++) { double *F = fl5[k]; f1[k] = -hx * eno2b(F[l], F[ij], F[r], F[rr], &br); }
+        }
+        if (vc > 0) {
+            if (j > 0) for (k = 0; k < 5; k++) { double *F = fl5[k]; f2[k] = hy * eno2b(F[u], F[ij], F[d], F[ij - 2 * m], &br); }
+            else for (k = 0; k < 5; k++) { double *F = fl5[k]; f2[k] = (hy * 2) * (F[ij] - F[d]); }
+        } else {
+            if (j < n - 1) for (k = 0; k < 5; k++) { double *F = fl5[k]; f2[k] = -hy * eno2b(F[d], F[ij], F[u], F[ij + 2 * m], &br); }
+            else for (k = 0; k < 5; k++) { double *F = fl5[k]; f2[k] = (-hy * 2) * (F[ij] - F[u]); }
+        }
+        /* :256-265 */
+        {
+            double cchi = C[ij] - uc * f1[4] - vc * f2[4] + hxxcd * (C[r] + C[l]) + hyycd * (C[u] + C[d]) - hccd * C[ij];
+            double sbar = sqrt(3 * Q[ij] * Q[ij] + S[ij] * S[ij] + T[ij] * T[ij]);
+            int trips;
+            dtmp = plastic_term(s, sbar, &cchi, &D[ij], dt, &trips);
+            s->cchi[ij] = cchi;
+        }
+        s->cp[ij] = -uc * f1[0] - vc * f2[0] - K * (ux + vy);
+        s->cq[ij] = -uc * f1[1] - vc * f2[1] - mu * third * (ux + vy) - dtmp * Q[ij];
+        s->cs[ij] = -uc * f1[2] - vc * f2[2] + two_omega * T[ij] + mu * (ux - vy) - dtmp * S[ij];
+        s->ctau[ij] = -uc * f1[3] - vc * f2[3] - two_omega * S[ij] + mu * (vx + uy) - dtmp * T[ij];
+        D[ij] *= sqrt(3 * Q[ij] * Q[ij] + S[ij] * S[ij] + T[ij] * T[ij]);
+    }
+    /* chi_diffusion (:269), shear_sim.cc:364-386 */
+    {
+        double dfac = 0.25 * s->p.chi_len * s->p.chi_len * s->mu_inv, dfacx = dfac * s->xsp * s->xsp,
+               dfacy = dfac * s->ysp * s->ysp;
+        for (j = 0; j < n; j++) for (i = 0; i < m; i++) {
+            const int ij = i + m * j;
+            const int l = (i == 0 ? m - 1 : i - 1) + m * j, r = (i == m - 1 ? 0 : i + 1) + m * j;
+            const int d = ij - m, u = ij + m;
+            double Dd = D[ij] + D[d], Dl = D[ij] + D[l], Dr = D[ij] + D[r], Du = D[ij] + D[u];
+            s->cchi[ij] += hxxcd * (C[r] + C[l]) + hyycd * (C[u] + C[d]) - hccd * C[ij]
+                         + dfacx * (Dr * C[r] - (Dr + Dl) * C[ij] + Dl * C[l])
+                         + dfacy * (Du * C[u] - (Du + Dd) * C[ij] + Dd * C[d]);
+        }
+    }
+    /* update (:272-275, fields.hh:56-80), then set_boundaries (:275) */
+    for (j = 0; j < n; j++) for (i = 0; i < m; i++) {
+        const int ij = i + m * j;
+        if (j > 0) { U[ij] += s->cu[ij]; V[ij] += s->cv[ij]; X[ij] += s->cX[ij]; Y[ij] += s->cY[ij]; }
+        P[ij] += s->cp[ij]; Q[ij] += s->cq[ij]; S[ij] += s->cs[ij]; T[ij] += s->ctau[ij];
+        C[ij] = s->cchi[ij];
+    }
+    orc_set_boundaries(s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* diagnostics and initial conditions                                        */
+/* ------------------------------------------------------------------------ */
+
+/* mat::sym_eigenvectors, shared/mat.cc:8-26: l1 >= l2, Lam = (a,b,c,d) */
+static void sym_eig(const double *F, double *l1, double *l2, double *L) {
+    const double a = F[0], b = F[1], d = F[3];
+    if (fabs(b) < 1e-12 * (fabs(a) + fabs(d))) {
+        if (a > d) { *l1 = a; *l2 = d; L[0] = 1; L[1] = 0; L[2] = 0; L[3] = 1; }
+        else { *l1 = d; *l2 = a; L[0] = 0; L[1] = 1; L[2] = 1; L[3] = 0; }
+        return;
+    }
+    double det = sqrt((a - d) * (a - d) + 4 * b * b);
+    *l1 = 0.5 * (a + d + det);
+    *l2 = 0.5 * (a + d - det);
+    double r = b, s = a - *l1;
+    double nor1 = 1 / sqrt(r * r + s * s);
+    L[0] = r * nor1; L[1] = -s * nor1; L[2] = s * nor1; L[3] = r * nor1;
+}
+
+static void mat_mul(const double *A, const double *B, double *o) {
+    /* mat.hh operator* (a*e.a+b*e.c, a*e.b+b*e.d, c*e.a+d*e.c, c*e.b+d*e.d) */
+    o[0] = A[0] * B[0] + A[1] * B[2];
+    o[1] = A[0] * B[1] + A[1] * B[3];
+    o[2] = A[2] * B[0] + A[3] * B[2];
+    o[3] = A[2] * B[1] + A[3] * B[3];
+}
+
+/* shear_sim.cc:778-824.  corrected = 0 follows the reference exactly, with
+ * its two slips: on the interior rows the y difference takes both ends from
+ * row j+1 (fu = fd = fp+m, :804), so Jinv = 0 there and the strain comes out
+ * 0; and in the seam column i = m-1 the periodic shift of the right
+ * neighbour is subtracted (:793) where it should be added.  corrected = 1
+ * uses (row j+1) - (row j-1) on the interior rows and +(bx-ax) at the seam. */
+void orc_compute_strain(orc_sim *s, int corrected) {
+    const int m = s->m, n = s->n;
+    const double *X = s->f[STZ_F_X], *Y = s->f[STZ_F_Y];
+    const double lxs = s->p.bx - s->p.ax;
+    int i, j;
+    for (j = 0; j < n; j++) for (i = 0; i < m; i++) {
+        const int ij = i + m * j;
+        double l1, l2, J, Jinv, Xx, Xy, Yx, Yy, F[4], FT[4], FTF[4], Lam[4], LT[4], D[4], t1[4], sig[4];
+        if (i == 0) { Xx = X[ij + 1] - X[ij + (m - 1)] + lxs; Yx = Y[ij + 1] - Y[ij + (m - 1)]; }
+        else if (i == m - 1) { Xx = X[ij + (1 - m)] - X[ij - 1] + (corrected ? lxs : -lxs); Yx = Y[ij + (1 - m)] - Y[ij - 1]; }
+        else { Xx = X[ij + 1] - X[ij - 1]; Yx = Y[ij + 1] - Y[ij - 1]; }
+        Xx *= 0.5 * s->xsp; Yx *= 0.5 * s->xsp;
+        int fu, fd;
+        double yfac;
+        if (j == 0) { yfac = s->ysp; fu = ij + m; fd = ij; }
+        else if (j == n - 1) { yfac = s->ysp; fu = ij + m; fd = ij; }
+        else { yfac = 0.5 * s->ysp; fu = ij + m; fd = corrected ? ij - m : ij + m; }
+        Xy = yfac * (X[fu] - X[fd]);
+        Yy = yfac * (Y[fu] - Y[fd]);
+        J = 1 / (Jinv = Xx * Yy - Xy * Yx);
+        F[0] = Yy * J; F[1] = -Xy * J; F[2] = -Yx * J; F[3] = Xx * J;
+        FT[0] = F[0]; FT[1] = F[2]; FT[2] = F[1]; FT[3] = F[3];
+        mat_mul(FT, F, FTF);
+        sym_eig(FTF, &l1, &l2, Lam);
+        if (l1 > 0 || l2 > 0) {
+            LT[0] = Lam[0]; LT[1] = Lam[2]; LT[2] = Lam[1]; LT[3] = Lam[3];
+            D[0] = 0.5 * log(l1); D[1] = 0; D[2] = 0; D[3] = 0.5 * log(l2);
+            mat_mul(LT, D, t1);
+            mat_mul(t1, Lam, sig);
+            /* mat::devmod, mat.hh:51-53 */
+            s->cu[ij] = sqrt(0.5 * (sig[0] - sig[3]) * (sig[0] - sig[3]) + sig[1] * sig[1] + sig[2] * sig[2]);
+        } else s->cu[ij] = 0;
+        (void)Jinv;
+    }
+}
+
+/* shear_sim.cc:460-502 */
+void orc_l2_comparison(orc_sim *a, orc_sim *b, double *l2) {
+    const int m = a->m, n = a->n, mn = a->mn;
+    int i, j;
+    l2[0] = l2[1] = l2[2] = l2[3] = 0;
+    for (j = 0; j < n; j++) {
+        double c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        for (i = 0; i < m; i++) {
+            const int ij = i + m * j;
+#define DIF(k) (a->f[k][ij] - b->f[k][ij])
+            double du = DIF(STZ_F_U), dv = DIF(STZ_F_V), dp = DIF(STZ_F_P), dq = DIF(STZ_F_Q), ds = DIF(STZ_F_S),
+                   dtau = DIF(STZ_F_TAU), dchi = DIF(STZ_F_CHI), dX = DIF(STZ_F_X), dY = DIF(STZ_F_Y);
+#undef DIF
+            c0 += du * du + dv * dv;
+            c1 += 2 * (dtau * dtau + ds * ds) + 3 * dp * dp + 6 * dq * dq;
+            c2 += dchi * dchi;
+            c3 += dX * dX + dY * dY;
+        }
+        if (j == 0) { c0 *= 0.5; c3 *= 0.5; }
+        l2[0] += c0; l2[1] += c1; l2[2] += c2; l2[3] += c3;
+    }
+    for (i = mn; i < mn + m; i++) {
+        double du = a->f[STZ_F_U][i] - b->f[STZ_F_U][i], dv = a->f[STZ_F_V][i] - b->f[STZ_F_V][i];
+        double dX = a->f[STZ_F_X][i] - b->f[STZ_F_X][i], dY = a->f[STZ_F_Y][i] - b->f[STZ_F_Y][i];
+        l2[0] += 0.5 * (du * du + dv * dv);
+        l2[3] += 0.5 * (dX * dX + dY * dY);
+    }
+    {
+        double fac = a->dx * a->dy;
+        l2[0] *= fac; l2[1] *= fac; l2[2] *= fac; l2[3] *= fac;
+    }
+}
+
+/* shear_sim.cc:832-913 */
+void orc_initialize_chi_bicubic(orc_sim *s, int mm, int nn, const double *arr, double lo, double sca) {
+    const int m = s->m, n = s->n, mmnn = mm * nn;
+    double *xd = (double *)malloc(sizeof(double) * 3 * mmnn), *yd = xd + mmnn, *xyd = yd + mmnn;
+    int i, j;
+    for (j = 0; j < nn; j++) {
+        int ij = j * mm;
+        for (i = 0; i < mm; i++, ij++) xd[ij] = 0.5 * (arr[i == mm - 1 ? ij - mm + 1 : ij + 1] - arr[i == 0 ? ij + mm - 1 : ij - 1]);
+    }
+    for (j = 0; j < nn; j++) {
+        int ij = j * mm;
+        if (j == 0) {
+            for (i = 0; i < mm; i++, ij++) { yd[ij] = arr[ij + mm] - arr[ij]; xyd[ij] = xd[ij + mm] - xd[ij]; }
+        } else if (j == nn - 1) {
+            for (i = 0; i < mm; i++, ij++) { yd[ij] = arr[ij] - arr[ij - mm]; xyd[ij] = xd[ij] - xd[ij - mm]; }
+        } else {
+            for (i = 0; i < mm; i++, ij++) {
+                yd[ij] = 0.5 * (arr[ij + mm] - arr[ij - mm]);
+                xyd[ij] = 0.5 * (xd[ij + mm] - xd[ij - mm]);
+            }
+        }
+    }
+    {
+        double xsca = s->dx * mm / (s->p.bx - s->p.ax), ysca = s->dy * nn / (s->p.by - s->p.ay);
+        for (j = 0; j < n; j++) {
+            double Yv = (j + 0.5) * ysca - 0.5, mY;
+            int jj = (int)Yv;
+            if (jj < 0) jj = 0; else if (jj > nn - 2) jj = nn - 2;
+            Yv -= jj; mY = 1 - Yv;
+            double b0 = Yv * mY * mY, b1 = mY * mY * (1 + 2 * Yv), b2 = Yv * Yv * (3 - 2 * Yv), b3 = -Yv * Yv * mY;
+            for (i = 0; i < m; i++) {
+                double Xv = (i + 0.5) * xsca - 0.5, mX;
+                if (Xv < 0) Xv += mm;
+                int ii = (int)Xv, ij, ijp;
+                if (ii >= mm) ii -= mm;
+                ij = ii + mm * jj; ijp = (ii == mm - 1 ? ii - mm : ii) + 1 + mm * jj;
+                Xv -= ii; mX = 1 - Xv;
+                double c0 = Xv * mX * mX, c1 = mX * mX * (1 + 2 * Xv), c2 = Xv * Xv * (3 - 2 * Xv), c3 = -Xv * Xv * mX;
+                s->f[STZ_F_CHI][i + m * j] = lo + sca * (b0 * (xyd[ij] * c0 + yd[ij] * c1 + yd[ijp] * c2 + xyd[ijp] * c3)
+                    + b1 * (xd[ij] * c0 + arr[ij] * c1 + arr[ijp] * c2 + xd[ijp] * c3)
+                    + b2 * (xd[ij + mm] * c0 + arr[ij + mm] * c1 + arr[ijp + mm] * c2 + xd[ijp + mm] * c3)
+                    + b3 * (xyd[ij + mm] * c0 + yd[ij + mm] * c1 + yd[ijp + mm] * c2 + xyd[ijp + mm] * c3));
+            }
+        }
+    }
+    for (j = -1; j < n + 1; j++) {
+        double Yv = s->p.ay + j * s->dy;
+        for (i = 0; i < m; i++) {
+            const int ij = i + m * j;
+            s->f[STZ_F_U][ij] = s->f[STZ_F_V][ij] = s->f[STZ_F_DDEV][ij] = 0;
+            s->f[STZ_F_P][ij] = s->f[STZ_F_Q][ij] = s->f[STZ_F_S][ij] = s->f[STZ_F_TAU][ij] = 0;
+            s->f[STZ_F_X][ij] = s->p.ax + i * s->dx;
+            s->f[STZ_F_Y][ij] = Yv;
+        }
+    }
+    free(xd);
 }

@@ -75,6 +75,12 @@ double orc_plastic_term(orc_sim *s, double sbar, double *chiv, double *ddev,
 /* diagnostics (shear_sim.cc:733-775, 778-824, 1003-1028) */
 void orc_tractions(orc_sim *s, double *out4);
 void orc_qs_measure(orc_sim *s, int staggered, double *Q, double *max_qs);
+/* the c_field::cu slot (rows -1..n) that qs_measure and compute_strain fill */
+void orc_get_cu(orc_sim *s, double *plane);
+void orc_step_forward(orc_sim *s, double dt);
+void orc_compute_strain(orc_sim *s, int corrected);
+void orc_l2_comparison(orc_sim *a, orc_sim *b, double *l2);
+void orc_initialize_chi_bicubic(orc_sim *s, int mm, int nn, const double *arr, double lo, double sca);
 
 #ifdef __cplusplus
 }

@@ -17,7 +17,8 @@ OK, ERR_ARG, ERR_HIP, ERR_NOT_CONVERGED, ERR_NAN, ERR_NO_DEVICE, ERR_STATE = ran
 PREC_FP64, PREC_MIXED = 0, 1
 NFIELDS = 10
 FIELD_NAMES = ("u", "v", "p", "q", "s", "tau", "chi", "X", "Y", "ddev")
-FIELD_INDEX = {k: i for i, k in enumerate(FIELD_NAMES)}
+F_CU = 10  # the c_field::cu slot: output plane of the diagnostics
+FIELD_INDEX = dict({k: i for i, k in enumerate(FIELD_NAMES)}, cu=F_CU)
 
 _MODEL_DOUBLES = ("TZ", "c0", "tau0", "kappa", "Delta", "Omegaeps0", "chi_inf",
                   "theta", "rho", "t_icpt", "t_grad")
@@ -106,6 +107,13 @@ _SIGS = {
     "stz_mg_l2_error": [C.c_void_p, C.POINTER(C.c_double)],
     "stz_mg_upload_src": [C.c_void_p, _dp],
     "stz_mg_smooth": [C.c_void_p, C.c_int, C.c_int],
+    "stz_sim_step_forward": [C.c_void_p, C.c_double, C.c_int],
+    "stz_sim_tractions": [C.c_void_p, _dp],
+    "stz_sim_qs_measure": [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    "stz_sim_compute_strain": [C.c_void_p, C.c_int],
+    "stz_sim_l2_comparison": [C.c_void_p, C.c_void_p, _dp],
+    "stz_sim_initialize_random": [C.c_void_p, C.c_double, C.c_double, C.c_double],
+    "stz_sim_initialize_chi_bicubic": [C.c_void_p, C.c_int, C.c_int, _dp, C.c_double, C.c_double],
     "stz_sim_record_decisions": [C.c_void_p, C.c_int],
     "stz_sim_download_decisions": [C.c_void_p, _up],
 }

@@ -45,6 +45,8 @@ struct SimConst {
 // launchers (explicit.hip)
 int launch_advect(const SimConst &c, const FieldPtrs &old_f, const FieldPtrs &new_f, double dt,
                   uint32_t *dec, hipStream_t st);
+int launch_direct(const SimConst &c, const FieldPtrs &old_f, const FieldPtrs &new_f, double *un, double *vn,
+                  double dt, hipStream_t st);
 int launch_chi_diffusion(const SimConst &c, const FieldPtrs &old_f, const FieldPtrs &new_f, double dt,
                          hipStream_t st);
 int launch_source(const SimConst &c, const FieldPtrs &f, double dt, double xbc, double bvel, double *src,

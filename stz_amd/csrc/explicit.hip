@@ -304,6 +304,183 @@ __global__ void __launch_bounds__(256) advect_kernel(SimConst c, FieldPtrs o, Fi
     }
 }
 
+// ---------------------------------------------------------------------------
+// direct (dynamic) step, shear_sim.cc:185-276: the advection kernel's work
+// plus the velocity update (ENO2 advection of u,v, the stress-divergence
+// force net_force(..,-hx,-hy,..) and viscosity), the compressive terms of
+// the stress update, and constant chi diffusion inside the cell update.
+// New u,v go to un,vn (rows 1..n-1; the caller copies them back), all other
+// new fields to the plane set w.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) direct_kernel(SimConst c, FieldPtrs o, FieldPtrs w, double *un, double *vn,
+                                                     double dt) {
+    const int m = c.m, n = c.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= m || j >= n) return;
+    const double third = 1. / 3;
+    const double hx = 0.5 * c.xsp * dt, hy = 0.5 * c.ysp * dt, lx = c.lx;
+    const double hxxv = 2 * c.xsp * hx * c.viscosity, hyyv = 2 * c.ysp * hy * c.viscosity, hcv = 2 * (hxxv + hyyv);
+    const double hxxcd = 2 * c.xsp * hx * c.chi_diff, hyycd = 2 * c.ysp * hy * c.chi_diff,
+                 hccd = 2 * (hxxcd + hyycd);
+    const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
+    const int ill = i <= 1 ? i + m - 2 : i - 2, irr = i >= m - 2 ? i + 2 - m : i + 2;
+    uint32_t br;
+    double uc, vc, ux, vx, uy, vy, two_omega, dtmp;
+
+    if (j > 0) {
+        double Xx, Xy, Yx, Yy, fx, fy;
+        uc = PL(o.u, i, j);
+        vc = PL(o.v, i, j);
+        const double X0 = PL(o.X, i, j), Y0 = PL(o.Y, i, j);
+        // rmv_eno2 / rmv_one_sided, shear_sim.cc:203-219, 588-625
+        if (uc > 0) {
+            ux = hx * eno2(PL(o.u, ir, j), uc, PL(o.u, il, j), PL(o.u, ill, j), br);
+            vx = hx * eno2(PL(o.v, ir, j), vc, PL(o.v, il, j), PL(o.v, ill, j), br);
+            Xx = hx * eno2(PL(o.X, ir, j) + (i == m - 1 ? lx : 0), X0, PL(o.X, il, j) + (i == 0 ? -lx : 0),
+                           PL(o.X, ill, j) + (i <= 1 ? -lx : 0), br);
+            Yx = hx * eno2(PL(o.Y, ir, j), Y0, PL(o.Y, il, j), PL(o.Y, ill, j), br);
+        } else {
+            ux = -hx * eno2(PL(o.u, il, j), uc, PL(o.u, ir, j), PL(o.u, irr, j), br);
+            vx = -hx * eno2(PL(o.v, il, j), vc, PL(o.v, ir, j), PL(o.v, irr, j), br);
+            Xx = -hx * eno2(PL(o.X, il, j) + (i == 0 ? -lx : 0), X0, PL(o.X, ir, j) + (i == m - 1 ? lx : 0),
+                            PL(o.X, irr, j) + (i >= m - 2 ? lx : 0), br);
+            Yx = -hx * eno2(PL(o.Y, il, j), Y0, PL(o.Y, ir, j), PL(o.Y, irr, j), br);
+        }
+        if (vc > 0) {
+            if (j > 1) {
+                uy = hy * eno2(PL(o.u, i, j + 1), uc, PL(o.u, i, j - 1), PL(o.u, i, j - 2), br);
+                vy = hy * eno2(PL(o.v, i, j + 1), vc, PL(o.v, i, j - 1), PL(o.v, i, j - 2), br);
+                Xy = hy * eno2(PL(o.X, i, j + 1), X0, PL(o.X, i, j - 1), PL(o.X, i, j - 2), br);
+                Yy = hy * eno2(PL(o.Y, i, j + 1), Y0, PL(o.Y, i, j - 1), PL(o.Y, i, j - 2), br);
+            } else {
+                uy = 2 * hy * (uc - PL(o.u, i, j - 1));
+                vy = 2 * hy * (vc - PL(o.v, i, j - 1));
+                const double hs = hy * 2;
+                Xy = hs * (X0 - PL(o.X, i, j - 1));
+                Yy = hs * (Y0 - PL(o.Y, i, j - 1));
+            }
+        } else {
+            if (j < n - 1) {
+                uy = -hy * eno2(PL(o.u, i, j - 1), uc, PL(o.u, i, j + 1), PL(o.u, i, j + 2), br);
+                vy = -hy * eno2(PL(o.v, i, j - 1), vc, PL(o.v, i, j + 1), PL(o.v, i, j + 2), br);
+                Xy = -hy * eno2(PL(o.X, i, j - 1), X0, PL(o.X, i, j + 1), PL(o.X, i, j + 2), br);
+                Yy = -hy * eno2(PL(o.Y, i, j - 1), Y0, PL(o.Y, i, j + 1), PL(o.Y, i, j + 2), br);
+            } else {
+                uy = 2 * -hy * (uc - PL(o.u, i, j + 1));
+                vy = 2 * -hy * (vc - PL(o.v, i, j + 1));
+                const double hs = -hy * 2;
+                Xy = hs * (X0 - PL(o.X, i, j + 1));
+                Yy = hs * (Y0 - PL(o.Y, i, j + 1));
+            }
+        }
+        // net_force(fx,fy,-hx,-hy,flp,fp), shear_sim.cc:222, 447-453
+        {
+            const double xf = -hx, yf = -hy;
+            const double p0 = PL(o.p, il, j - 1), p1 = PL(o.p, i, j - 1), p2 = PL(o.p, il, j), p3 = PL(o.p, i, j);
+            const double q0 = PL(o.q, il, j - 1), q1 = PL(o.q, i, j - 1), q2 = PL(o.q, il, j), q3 = PL(o.q, i, j);
+            const double s0 = PL(o.s, il, j - 1), s1 = PL(o.s, i, j - 1), s2 = PL(o.s, il, j), s3 = PL(o.s, i, j);
+            const double t0 = PL(o.tau, il, j - 1), t1 = PL(o.tau, i, j - 1), t2 = PL(o.tau, il, j),
+                         t3 = PL(o.tau, i, j);
+            fx = (-p0 + p1 - p2 + p3 - q0 + q1 - q2 + q3 + s0 - s1 + s2 - s3) * xf + (t0 + t1 - t2 - t3) * yf;
+            fy = (t0 - t1 + t2 - t3) * xf + (-p0 - p1 + p2 + p3 - q0 - q1 + q2 + q3 - s0 - s1 + s2 + s3) * yf;
+        }
+        const double uvisc = hxxv * (PL(o.u, ir, j) + PL(o.u, il, j)) + hyyv * (PL(o.u, i, j + 1) + PL(o.u, i, j - 1)) -
+                             hcv * uc;
+        const double vvisc = hxxv * (PL(o.v, ir, j) + PL(o.v, il, j)) + hyyv * (PL(o.v, i, j + 1) + PL(o.v, i, j - 1)) -
+                             hcv * vc;
+        // update() (fields.hh:56-60): u += cu, v += cv, X += cX, Y += cY
+        PL(un, i, j) = uc + (-uc * ux - vc * uy + c.mu_inv * (fx + uvisc));
+        PL(vn, i, j) = vc + (-uc * vx - vc * vy + c.mu_inv * (fy + vvisc));
+        PL(w.X, i, j) = X0 + (-uc * Xx - vc * Xy);
+        PL(w.Y, i, j) = Y0 + (-uc * Yx - vc * Yy);
+    } else {
+        // row 0 keeps u, v, X, Y until set_boundaries; ghost row -1 carried over
+        PL(w.X, i, 0) = PL(o.X, i, 0);
+        PL(w.Y, i, 0) = PL(o.Y, i, 0);
+        PL(w.X, i, -1) = PL(o.X, i, -1);
+        PL(w.Y, i, -1) = PL(o.Y, i, -1);
+        PL(w.p, i, -1) = PL(o.p, i, -1);
+        PL(w.q, i, -1) = PL(o.q, i, -1);
+        PL(w.s, i, -1) = PL(o.s, i, -1);
+        PL(w.tau, i, -1) = PL(o.tau, i, -1);
+        PL(w.chi, i, -1) = PL(o.chi, i, -1);
+    }
+    if (j == n - 1) {
+        PL(w.X, i, n) = PL(o.X, i, n);
+        PL(w.Y, i, n) = PL(o.Y, i, n);
+        PL(w.p, i, n) = PL(o.p, i, n);
+        PL(w.q, i, n) = PL(o.q, i, n);
+        PL(w.s, i, n) = PL(o.s, i, n);
+        PL(w.tau, i, n) = PL(o.tau, i, n);
+        PL(w.chi, i, n) = PL(o.chi, i, n);
+    }
+
+    // deformation rate, staggered velocity and spin, shear_sim.cc:240-247
+    {
+        const double u00 = PL(o.u, i, j), u10 = PL(o.u, ir, j), u01 = PL(o.u, i, j + 1), u11 = PL(o.u, ir, j + 1);
+        const double v00 = PL(o.v, i, j), v10 = PL(o.v, ir, j), v01 = PL(o.v, i, j + 1), v11 = PL(o.v, ir, j + 1);
+        uc = 0.25 * (u00 + u10 + u01 + u11);
+        vc = 0.25 * (v00 + v10 + v01 + v11);
+        ux = hx * (-u00 + u10 - u01 + u11);
+        uy = hy * (-u00 - u10 + u01 + u11);
+        vx = hx * (-v00 + v10 - v01 + v11);
+        vy = hy * (-v00 - v10 + v01 + v11);
+        two_omega = vx - uy;
+    }
+
+    // ENO derivatives of p,q,s,tau,chi, shear_sim.cc:250-252
+    const double *F[5] = {o.p, o.q, o.s, o.tau, o.chi};
+    double f0[5], f1[5], f2[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) f0[k] = PL(F[k], i, j);
+    if (uc > 0) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) f1[k] = hx * eno2(PL(F[k], ir, j), f0[k], PL(F[k], il, j), PL(F[k], ill, j), br);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; k++) f1[k] = -hx * eno2(PL(F[k], il, j), f0[k], PL(F[k], ir, j), PL(F[k], irr, j), br);
+    }
+    if (vc > 0) {
+        if (j > 0) {
+#pragma unroll
+            for (int k = 0; k < 5; k++)
+                f2[k] = hy * eno2(PL(F[k], i, j + 1), f0[k], PL(F[k], i, j - 1), PL(F[k], i, j - 2), br);
+        } else {
+            const double hs = hy * 2;
+#pragma unroll
+            for (int k = 0; k < 5; k++) f2[k] = hs * (f0[k] - PL(F[k], i, j - 1));
+        }
+    } else {
+        if (j < n - 1) {
+#pragma unroll
+            for (int k = 0; k < 5; k++)
+                f2[k] = -hy * eno2(PL(F[k], i, j - 1), f0[k], PL(F[k], i, j + 1), PL(F[k], i, j + 2), br);
+        } else {
+            const double hs = -hy * 2;
+#pragma unroll
+            for (int k = 0; k < 5; k++) f2[k] = hs * (f0[k] - PL(F[k], i, j + 1));
+        }
+    }
+
+    // chi with its constant diffusion, then plasticity, shear_sim.cc:256-257
+    double cchi = f0[4] - uc * f1[4] - vc * f2[4] + hxxcd * (PL(o.chi, ir, j) + PL(o.chi, il, j)) +
+                  hyycd * (PL(o.chi, i, j + 1) + PL(o.chi, i, j - 1)) - hccd * f0[4];
+    const double sbar = sqrt(3 * f0[1] * f0[1] + f0[2] * f0[2] + f0[3] * f0[3]);
+    double ddev;
+    int trips;
+    dtmp = plastic_term(c, sbar, cchi, ddev, dt, trips);
+    PL(w.chi, i, j) = cchi;
+    // f.ddev *= f.dev() (:265), with the stresses not yet updated
+    PL(w.ddev, i, j) = ddev * sbar;
+
+    // stress updates, shear_sim.cc:260-263, applied as in fields.hh:61-67
+    PL(w.p, i, j) = f0[0] + (-uc * f1[0] - vc * f2[0] - c.K * (ux + vy));
+    PL(w.q, i, j) = f0[1] + (-uc * f1[1] - vc * f2[1] - c.mu * third * (ux + vy) - dtmp * f0[1]);
+    PL(w.s, i, j) = f0[2] + (-uc * f1[2] - vc * f2[2] + two_omega * f0[3] + c.mu * (ux - vy) - dtmp * f0[2]);
+    PL(w.tau, i, j) = f0[3] + (-uc * f1[3] - vc * f2[3] - two_omega * f0[2] + c.mu * (vx + uy) - dtmp * f0[3]);
+}
+
 // chi_diffusion, shear_sim.cc:364-386: w.chi holds the post-plastic chi
 // (cchi) and receives the diffused value; o.chi is the previous chi.
 __global__ void __launch_bounds__(256) chi_diffusion_kernel(SimConst c, const double *__restrict__ chi_old,
@@ -491,6 +668,14 @@ int launch_advect(const SimConst &c, const FieldPtrs &o, const FieldPtrs &w, dou
                   hipStream_t st) {
     dim3 b(64, 4);
     hipLaunchKernelGGL(advect_kernel, grid2(c.m, c.n, b), b, 0, st, c, o, w, dt, dec);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int launch_direct(const SimConst &c, const FieldPtrs &o, const FieldPtrs &w, double *un, double *vn, double dt,
+                  hipStream_t st) {
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(direct_kernel, grid2(c.m, c.n, b), b, 0, st, c, o, w, un, vn, dt);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }

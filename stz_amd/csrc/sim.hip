@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "device.h"
+#include "diag.h"
 #include "mg.h"
 
 namespace stz {
@@ -252,6 +253,8 @@ struct stz_sim {
     // fields
     double *planes = nullptr;
     FieldPtrs cur{}, alt{};
+    double *cu = nullptr;              // c_field::cu: diagnostic output plane
+    double *un = nullptr, *vn = nullptr;  // new u, v of the direct step
     double *vel = nullptr, *src = nullptr;
     uint32_t *dec = nullptr;
     bool rec_dec = false;
@@ -655,11 +658,11 @@ int stz_sim_create(const stz_sim_params *p, int precision, stz_sim **out) {
     }
     s->own_stream = true;
     const size_t pe = plane_elems(s);
-    if (hipMalloc(&s->planes, pe * 17 * sizeof(double)) != hipSuccess) {
-        set_error("out of device memory for %zu planes", pe * 17);
+    if (hipMalloc(&s->planes, pe * 20 * sizeof(double)) != hipSuccess) {
+        set_error("out of device memory for %zu planes", pe * 20);
         return fail(STZ_ERR_HIP);
     }
-    (void)hipMemset(s->planes, 0, pe * 17 * sizeof(double));
+    (void)hipMemset(s->planes, 0, pe * 20 * sizeof(double));
     double *P = s->planes;
     s->cur.u = s->alt.u = P + 0 * pe;
     s->cur.v = s->alt.v = P + 1 * pe;
@@ -670,6 +673,9 @@ int stz_sim_create(const stz_sim_params *p, int precision, stz_sim **out) {
         *cf[k] = P + (3 + k) * pe;
         *af[k] = P + (10 + k) * pe;
     }
+    s->cu = P + 17 * pe;
+    s->un = P + 18 * pe;
+    s->vn = P + 19 * pe;
     const size_t nodes = (size_t)c.m * (c.n + 1);
     if (hipMalloc(&s->vel, 4 * nodes * sizeof(double)) != hipSuccess) return fail(STZ_ERR_HIP);
     (void)hipMemset(s->vel, 0, 4 * nodes * sizeof(double));
@@ -678,10 +684,14 @@ int stz_sim_create(const stz_sim_params *p, int precision, stz_sim **out) {
     (void)hipMemset(s->dec, 0, (size_t)c.m * c.n * sizeof(uint32_t));
     s->nblocks_norm = (int)std::max<size_t>((nodes + 255) / 256,
                                             (size_t)((c.m + 63) / 64) * ((c.n + 1 + 15) / 16));
-    if (hipMalloc(&s->partial, (s->nblocks_norm + 1) * sizeof(double)) != hipSuccess) return fail(STZ_ERR_HIP);
-    // the residual norm is written by the summing kernel straight into
-    // coherent pinned host memory: no copy launch before each convergence test
-    if (hipHostMalloc(&s->hnorm, sizeof(double), hipHostMallocCoherent) != hipSuccess) return fail(STZ_ERR_HIP);
+    {
+        const size_t np = std::max<size_t>(s->nblocks_norm + 1, 4 * (size_t)diag_partials_needed(c));
+        if (hipMalloc(&s->partial, np * sizeof(double)) != hipSuccess) return fail(STZ_ERR_HIP);
+    }
+    // the residual norm (and the diagnostics' sums, from element 1 on) are
+    // written by the summing kernels straight into coherent pinned host
+    // memory: no copy launch before each convergence test
+    if (hipHostMalloc(&s->hnorm, 8 * sizeof(double), hipHostMallocCoherent) != hipSuccess) return fail(STZ_ERR_HIP);
     s->dnorm = s->hnorm;
 
     build_dims(s);
@@ -734,8 +744,8 @@ int stz_sim_synchronize(stz_sim *s) {
 
 static double *field_plane(stz_sim *s, int k) {
     FieldPtrs &f = s->cur;
-    double *tab[STZ_NFIELDS] = {f.u, f.v, f.p, f.q, f.s, f.tau, f.chi, f.X, f.Y, f.ddev};
-    return (k >= 0 && k < STZ_NFIELDS) ? tab[k] : nullptr;
+    double *tab[STZ_NFIELDS + 1] = {f.u, f.v, f.p, f.q, f.s, f.tau, f.chi, f.X, f.Y, f.ddev, s->cu};
+    return (k >= 0 && k <= STZ_F_CU) ? tab[k] : nullptr;
 }
 
 int stz_sim_upload_field(stz_sim *s, int k, const double *h) {
@@ -966,6 +976,179 @@ int stz_mg_upload_src(stz_sim *s, const double *h) {
     STZ_HIP(hipMemcpyAsync(s->src, h, 2 * (size_t)s->c.m * (s->c.n + 1) * sizeof(double), hipMemcpyHostToDevice,
                            s->stream));
     STZ_HIP(hipStreamSynchronize(s->stream));
+    return STZ_OK;
+}
+
+// ---- direct step, diagnostics, initial conditions --------------------------
+
+int stz_sim_step_forward(stz_sim *s, double dt, int nsteps) {
+    CHECK_SIM(s);
+    if (nsteps < 0) { set_error("nsteps < 0"); return STZ_ERR_ARG; }
+    const SimConst &c = s->c;
+    const size_t row = (size_t)c.m;
+    int e;
+    for (int k = 0; k < nsteps; k++) {
+        s->time += dt;  // shear_sim.cc:192
+        if ((e = launch_direct(c, s->cur, s->alt, s->un, s->vn, dt, s->stream))) return e;
+        if (c.chi_diff != 0 || c.chi_len != 0)
+            if ((e = launch_chi_diffusion(c, s->cur, s->alt, dt, s->stream))) return e;
+        std::swap(s->cur, s->alt);
+        // new u, v of rows 1..n-1 (plane rows 2..n)
+        STZ_HIP(hipMemcpyAsync(s->cur.u + 2 * row, s->un + 2 * row, (c.n - 1) * row * sizeof(double),
+                               hipMemcpyDeviceToDevice, s->stream));
+        STZ_HIP(hipMemcpyAsync(s->cur.v + 2 * row, s->vn + 2 * row, (c.n - 1) * row * sizeof(double),
+                               hipMemcpyDeviceToDevice, s->stream));
+        if ((e = launch_boundaries(c, s->cur, bdry_vel(s), bdry_pos(s), s->stream))) return e;
+    }
+    STZ_HIP(hipStreamSynchronize(s->stream));
+    return STZ_OK;
+}
+
+int stz_sim_tractions(stz_sim *s, double *out4) {
+    CHECK_SIM(s);
+    if (!out4) { set_error("null pointer"); return STZ_ERR_ARG; }
+    int e = diag_tractions(s->c, s->cur, s->hnorm + 1, s->stream);
+    if (e) return e;
+    STZ_HIP(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < 4; k++) out4[k] = s->hnorm[1 + k];
+    return STZ_OK;
+}
+
+int stz_sim_qs_measure(stz_sim *s, int staggered, double *Q, double *max_qs) {
+    CHECK_SIM(s);
+    if (!Q || !max_qs) { set_error("null pointer"); return STZ_ERR_ARG; }
+    int e = diag_qs(s->c, s->cur, s->vel, staggered != 0, s->cu, s->partial, s->hnorm + 1, s->stream);
+    if (e) return e;
+    STZ_HIP(hipStreamSynchronize(s->stream));
+    *Q = std::sqrt(s->hnorm[1] * s->c.dx * s->c.dy);  // shear_sim.cc:775
+    *max_qs = s->hnorm[2];
+    return STZ_OK;
+}
+
+int stz_sim_compute_strain(stz_sim *s, int corrected) {
+    CHECK_SIM(s);
+    int e = diag_strain(s->c, s->cur, corrected != 0, s->cu, s->stream);
+    if (e) return e;
+    STZ_HIP(hipStreamSynchronize(s->stream));
+    return STZ_OK;
+}
+
+int stz_sim_l2_comparison(stz_sim *a, stz_sim *b, double *out4) {
+    CHECK_SIM(a);
+    if (!b || !out4) { set_error("null argument"); return STZ_ERR_ARG; }
+    if (a->c.m != b->c.m || a->c.n != b->c.n || a->device != b->device) {
+        set_error("l2_comparison needs two simulations of the same size on one device");
+        return STZ_ERR_ARG;
+    }
+    STZ_HIP(hipStreamSynchronize(b->stream));
+    int e = diag_l2(a->c, a->cur, b->cur, a->partial, a->hnorm + 1, a->stream);
+    if (e) return e;
+    STZ_HIP(hipStreamSynchronize(a->stream));
+    const double fac = a->c.dx * a->c.dy;  // shear_sim.cc:500-501
+    for (int k = 0; k < 4; k++) out4[k] = a->hnorm[1 + k] * fac;
+    return STZ_OK;
+}
+
+// box_muller, shear_sim.cc:987-998, on the C library's rand() stream
+static void box_muller(double &r0, double &r1) {
+    double x, y, r;
+    do {
+        x = 2. / RAND_MAX * rand() - 1;
+        y = 2. / RAND_MAX * rand() - 1;
+        r = x * x + y * y;
+    } while (r == 0.0 || r > 1.0);
+    const double d = std::sqrt(-2.0 * std::log(r) / r);
+    r0 = x * d;
+    r1 = y * d;
+}
+
+int stz_sim_initialize_random(stz_sim *s, double l, double tem_avg, double tem_stdev) {
+    CHECK_SIM(s);
+    if (!(l > 0)) { set_error("smoothing length must be positive"); return STZ_ERR_ARG; }
+    const SimConst &c = s->c;
+    const int m = c.m, n = c.n, cut = int(l * 5.), nx = n + (cut << 1);
+    const double llinv = 0.5 / (l * l), TZ = s->p.stz.TZ;
+    // gaussian_normal_factor, shear_sim.cc:918-929
+    double vfac = 1;
+    for (int j = 1; j < cut; j++) {
+        const double q = std::exp(-j * j * llinv);
+        for (int k = 0; k <= j; k++) {
+            const double q2 = q * std::exp(-k * k * llinv);
+            vfac += (k == 0 || k == j ? 4 : 8) * q2 * q2;
+        }
+    }
+    const double chi0 = tem_avg / TZ, chi1 = tem_stdev / (std::sqrt(vfac) * TZ);
+    // the noise grid, drawn in the reference's order (shear_sim.cc:944-955):
+    // the primary rows first, then the padding, outward from them
+    std::vector<double> no((size_t)m * nx);
+    double *np = no.data() + (size_t)cut * m, *ne = np + (size_t)m * n, *np2 = np;
+    for (; np < ne - 1; np += 2) box_muller(*np, np[1]);
+    if (np == ne - 1) {
+        double r1;
+        box_muller(*np, r1);
+    }
+    while (np < no.data() + (size_t)m * nx) {
+        np2--;
+        box_muller(*np, *np2);
+        np++;
+    }
+    std::vector<double> w(cut + 1);
+    for (int k = 0; k <= cut; k++) w[k] = std::exp(-llinv * (k * k));
+    double *d = nullptr;
+    const size_t tot = 2 * (size_t)m * nx + w.size();
+    STZ_HIP(hipMalloc(&d, tot * sizeof(double)));
+    int e = STZ_OK;
+    if (hipMemcpyAsync(d, no.data(), no.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemcpyAsync(d + 2 * (size_t)m * nx, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice,
+                       s->stream) != hipSuccess) {
+        set_error("initialize_random: upload failed");
+        e = STZ_ERR_HIP;
+    }
+    if (!e) e = diag_random_fields(c, s->cur, cut, d + 2 * (size_t)m * nx, d, d + (size_t)m * nx, chi0, chi1,
+                                   s->stream);
+    const hipError_t se = hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (e) return e;
+    STZ_HIP(se);
+    return STZ_OK;
+}
+
+int stz_sim_initialize_chi_bicubic(stz_sim *s, int mm, int nn, const double *arr, double lo, double sca) {
+    CHECK_SIM(s);
+    if (!arr || mm < 1 || nn < 2) { set_error("bad array %d x %d", mm, nn); return STZ_ERR_ARG; }
+    const int mmnn = mm * nn;
+    // x, y and xy differences, shear_sim.cc:836-865
+    std::vector<double> h(4 * (size_t)mmnn);
+    double *a = h.data(), *xd = a + mmnn, *yd = xd + mmnn, *xyd = yd + mmnn;
+    std::memcpy(a, arr, mmnn * sizeof(double));
+    for (int j = 0; j < nn; j++)
+        for (int i = 0, ij = j * mm; i < mm; i++, ij++)
+            xd[ij] = 0.5 * (arr[i == mm - 1 ? ij - mm + 1 : ij + 1] - arr[i == 0 ? ij + mm - 1 : ij - 1]);
+    for (int j = 0; j < nn; j++)
+        for (int i = 0, ij = j * mm; i < mm; i++, ij++) {
+            if (j == 0) {
+                yd[ij] = arr[ij + mm] - arr[ij];
+                xyd[ij] = xd[ij + mm] - xd[ij];
+            } else if (j == nn - 1) {
+                yd[ij] = arr[ij] - arr[ij - mm];
+                xyd[ij] = xd[ij] - xd[ij - mm];
+            } else {
+                yd[ij] = 0.5 * (arr[ij + mm] - arr[ij - mm]);
+                xyd[ij] = 0.5 * (xd[ij + mm] - xd[ij - mm]);
+            }
+        }
+    double *d = nullptr;
+    STZ_HIP(hipMalloc(&d, h.size() * sizeof(double)));
+    int e = STZ_OK;
+    if (hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+        set_error("initialize_chi_bicubic: upload failed");
+        e = STZ_ERR_HIP;
+    }
+    if (!e) e = diag_bicubic(s->c, s->cur, mm, nn, d, lo, sca, s->stream);
+    const hipError_t se = hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (e) return e;
+    STZ_HIP(se);
     return STZ_OK;
 }
 

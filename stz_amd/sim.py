@@ -198,6 +198,62 @@ class ShearSim:
         a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.n + 1, self.m, 2)
         check(self._lib.stz_mg_upload_src(self._h, a))
 
+    # ---- direct step, diagnostics, initial conditions
+    def step_forward(self, dt, nsteps=1):
+        """shear_sim::step_forward: direct (dynamic) explicit step"""
+        check(self._lib.stz_sim_step_forward(self._h, float(dt), int(nsteps)))
+
+    def tractions(self):
+        """(bx, by, tx, ty) of shear_sim::tractions"""
+        out = np.zeros(4)
+        check(self._lib.stz_sim_tractions(self._h, out))
+        return tuple(out)
+
+    def qs_measure(self, staggered=False):
+        """(Q, max_qs) of shear_sim::qs_measure; the integrand is left in
+        get_field("cu")"""
+        q, mq = C.c_double(), C.c_double()
+        check(self._lib.stz_sim_qs_measure(self._h, 1 if staggered else 0, C.byref(q), C.byref(mq)))
+        return q.value, mq.value
+
+    def compute_strain(self, corrected=False):
+        """shear_sim::compute_strain into get_field("cu").  corrected=True
+        fixes the reference formula's two slips: a centred y difference on the
+        interior rows (where the reference gets 0) and the periodic shift's
+        sign in the seam column"""
+        check(self._lib.stz_sim_compute_strain(self._h, 1 if corrected else 0))
+
+    def l2_comparison(self, other):
+        out = np.zeros(4)
+        check(self._lib.stz_sim_l2_comparison(self._h, other._h, out))
+        return out
+
+    def initialize_random(self, l, tem_avg, tem_stdev, seed=None):
+        """shear_sim::initialize_random; with seed, the C library's rand() is
+        seeded first (srand), as the reference's drivers do"""
+        if seed is not None:
+            C.CDLL(None).srand(C.c_uint(int(seed)))
+        check(self._lib.stz_sim_initialize_random(self._h, float(l), float(tem_avg), float(tem_stdev)))
+
+    def initialize_chi_bicubic(self, arr, lo=0.0, sca=1.0):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        nn, mm = arr.shape
+        check(self._lib.stz_sim_initialize_chi_bicubic(self._h, int(mm), int(nn), arr, float(lo), float(sca)))
+
+    def output(self, directory, prefix, mode, sn):
+        """shear_sim::output: one field as a Gnuplot binary matrix"""
+        from . import frames
+        cu = self.get_field("cu") if mode == 11 else None
+        return frames.write_field(directory, prefix, self.params, self.get_fields(), mode, sn,
+                                  self.params["stz"]["TZ"], cu)
+
+    def write_files(self, directory, k, flags=None):
+        """shear_sim::write_files: the reference's default field selection"""
+        from . import frames
+        flags = frames.DEFAULT_FLAGS if flags is None else flags
+        cu = self.get_field("cu") if flags & (1 << 11) else None
+        return frames.write_files(directory, self.params, self.get_fields(), k, self.params["stz"]["TZ"], flags, cu)
+
     # ---- decisions
     def record_decisions(self, on=True):
         check(self._lib.stz_sim_record_decisions(self._h, 1 if on else 0))

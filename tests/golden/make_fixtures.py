@@ -82,6 +82,60 @@ def rand_init_case(name, N, l, avg, sd, seed):
     print(name, "done")
 
 
+def direct_case(name, params, seed, dt, t0, steps):
+    """shear_sim::step_forward (shear_sim.cc:185-276) from a seeded state"""
+    planes, vel = random_state(params, seed, vel=3e-3)
+    r = Reference(params)
+    r.set_fields(planes)
+    r.time = t0
+    out = {"params": json.dumps(params), "seed": seed, "dt": dt, "t0": t0, "steps": np.array(steps),
+           "in_planes": planes}
+    done = 0
+    for k in steps:
+        for _ in range(k - done):
+            r.step_forward(dt)
+        done = k
+        out["planes_%d" % k] = r.get_fields()
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "done")
+
+
+def diag_case(name, params, seed):
+    """diagnostics of one seeded state (and a perturbed copy): tractions,
+    qs_measure (both velocity forms, with the cu slot), compute_strain,
+    l2_comparison, initialize_chi_bicubic, and the Gnuplot matrix writer"""
+    planes, vel = random_state(params, seed, vel=3e-3)
+    r = Reference(params)
+    r.set_fields(planes)
+    r.set_vel(vel)
+    out = {"params": json.dumps(params), "seed": seed, "in_planes": planes, "in_vel": vel}
+    out["tractions"] = r.tractions()
+    for st in (0, 1):
+        out["qs_%d" % st] = np.array(r.qs_measure(bool(st)))
+        out["qs_cu_%d" % st] = r.get_cu()
+    r.compute_strain()
+    out["strain_cu"] = r.get_cu()
+    other = Reference(params)
+    pl2 = planes * (1 + 0.01 * np.random.default_rng(seed + 100).standard_normal(planes.shape))
+    other.set_fields(pl2)
+    out["other_planes"] = pl2
+    out["l2"] = r.l2_comparison(other)
+    arr = 0.03 + 0.005 * np.random.default_rng(seed + 200).standard_normal((7, 9))
+    b = Reference(params)
+    b.set_fields(planes)
+    b.initialize_chi_bicubic(arr, 0.001, 2.0)
+    out["bicubic_arr"] = arr
+    out["bicubic_planes"] = b.get_fields()
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        for mode, prefix in ((0, "u"), (2, "p"), (7, "tem"), (8, "dev"), (9, "X"), (10, "Y")):
+            r.output(d, prefix, mode, 3)
+            with open(os.path.join(d, "%s.3" % prefix), "rb") as f:
+                out["gnuplot_%d" % mode] = np.frombuffer(f.read(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "done")
+
+
 def main():
     shear = dict(u_bdry=2e-3)
     run_case("rand32", simple_shear(32, **shear), 1, 0.5, 0.8, [1, 2, 10])
@@ -97,6 +151,10 @@ def main():
     run_case("diff32", p, 6, 0.5, 1.5, [1, 2])
     traj_case("traj64", 64, 0.5, 40, 11)
     rand_init_case("rand_init48", 48, 2.0, 600.0, 20.0, 7)
+    p = simple_shear(32, 24, **shear)
+    p.update(box(32, 24, 2.0 / 32))
+    direct_case("direct32x24", p, 8, 2e-4, 1.0, [1, 4])
+    diag_case("diag40x24", dict(simple_shear(40, 24, **shear), **box(40, 24, 2.0 / 40)), 9)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,123 @@
+"""GPU parity of the frame diagnostics, initial conditions and the direct
+step (through the C ABI) against the reference's fixtures and the FP64
+checker.  Bars: per-node values without transcendentals bit-identical;
+reductions (different summation order) within 1e-12 relative; values built
+on exp/log/sqrt within 1e-12 relative."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import field_errors, load_fixture
+from oracle.pyoracle import Oracle
+from stz_amd import ShearSim
+from stz_amd.problems import simple_shear
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def diag():
+    return load_fixture("diag40x24")
+
+
+def gpu_state(fx, planes=None):
+    g = ShearSim(fx["params"])
+    g.set_fields(fx["in_planes"] if planes is None else planes)
+    g.set_vel(fx["in_vel"])
+    return g
+
+
+def test_direct_step_matches_reference():
+    fx = load_fixture("direct32x24")
+    print("case direct32x24", flush=True)
+    g = ShearSim(fx["params"])
+    g.set_fields(fx["in_planes"])
+    g.time = float(fx["t0"])
+    done = 0
+    for k in [int(s) for s in fx["steps"]]:
+        g.step_forward(float(fx["dt"]), k - done)
+        done = k
+        f, ref = g.get_fields(), fx["planes_%d" % k]
+        for q in (0, 1, 7, 8):  # u, v, X, Y: no transcendental on their path
+            assert rel(f[q], ref[q]) < 1e-14, (k, q)
+        err = field_errors(f, ref)
+        print("direct step", k, ["%.1e" % e for e in err])
+        assert max(err) < 1e-12, (k, err)
+    assert g.time == pytest.approx(float(fx["t0"]) + done * float(fx["dt"]), rel=1e-15)
+
+
+def test_tractions_qs_measure(diag):
+    print("case diag tractions/qs", flush=True)
+    g = gpu_state(diag)
+    t = np.array(g.tractions())
+    assert rel(t, diag["tractions"]) < 1e-12
+    for st in (0, 1):
+        q = np.array(g.qs_measure(bool(st)))
+        assert rel(q, diag["qs_%d" % st]) < 1e-12, st
+        cu = g.get_field("cu")
+        assert np.array_equal(cu[2:-2], diag["qs_cu_%d" % st][2:-2]), st
+
+
+def test_compute_strain(diag):
+    print("case diag strain", flush=True)
+    g = gpu_state(diag)
+    g.compute_strain(False)
+    cu = g.get_field("cu")
+    assert rel(cu[1:-1], diag["strain_cu"][1:-1]) < 1e-12
+    assert not np.any(cu[2:-2])
+    o = Oracle(diag["params"])
+    o.set_fields(diag["in_planes"])
+    o.compute_strain(True)
+    g.compute_strain(True)
+    assert rel(g.get_field("cu")[1:-1], o.get_cu()[1:-1]) < 1e-12
+
+
+def test_l2_comparison(diag):
+    print("case diag l2", flush=True)
+    a = gpu_state(diag)
+    b = gpu_state(diag, diag["other_planes"])
+    assert rel(a.l2_comparison(b), diag["l2"]) < 1e-12
+    assert np.array_equal(a.l2_comparison(a), np.zeros(4))
+
+
+def test_initialize_chi_bicubic(diag):
+    print("case diag bicubic", flush=True)
+    g = gpu_state(diag)
+    g.initialize_chi_bicubic(diag["bicubic_arr"], 0.001, 2.0)
+    assert np.array_equal(g.get_fields(), diag["bicubic_planes"])
+
+
+def test_initialize_random_matches_reference():
+    fx = load_fixture("rand_init48")
+    print("case rand_init48", flush=True)
+    g = ShearSim(fx["params"])
+    g.init_fields(0, 600.0, 200.0)
+    g.initialize_random(float(fx["l"]), float(fx["avg"]), float(fx["sd"]), seed=int(fx["seed"]))
+    chi = g.get_field("chi")
+    # rows 0..n-1 from the smoothing; the ghost rows keep init_fields
+    assert rel(chi, fx["chi"]) < 1e-12
+    o = Oracle(fx["params"])
+    o.init_fields(0, 600.0, 200.0)
+    o.initialize_random(float(fx["l"]), float(fx["avg"]), float(fx["sd"]), int(fx["seed"]))
+    fo, fg = o.get_fields(), g.get_fields()
+    for k in (0, 1, 2, 3, 4, 5, 7, 8, 9):
+        assert np.array_equal(fo[k], fg[k]), k
+
+
+def test_writer_from_device_state(diag):
+    print("case diag writer", flush=True)
+    g = gpu_state(diag)
+    with tempfile.TemporaryDirectory() as d:
+        for mode, prefix in ((0, "u"), (2, "p"), (7, "tem"), (8, "dev"), (9, "X"), (10, "Y")):
+            path = g.output(d, prefix, mode, 3)
+            with open(path, "rb") as f:
+                assert f.read() == diag["gnuplot_%d" % mode].tobytes(), mode
+        names = sorted(os.path.basename(p) for p in g.write_files(d, 5))
+        assert names == ["X.5", "Y.5", "dev.5", "p.5", "tem.5"]
