@@ -14,8 +14,10 @@
 #ifndef STZ_AMD_SHEAR_SIM_HH
 #define STZ_AMD_SHEAR_SIM_HH
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "../../include/stz_amd.h"
@@ -156,26 +158,148 @@ class shear_sim {
         sync_scalars();
         stz_amd_detail::check(stz_sim_projection_step(h, dt), "projection_step");
     }
-    /** shear_sim::solve_quasistatic (shear_sim.cc:159-181) without file
-     * output: frames of `steps` device steps each, with the reference's
-     * per-frame diagnostic line; the host mirrors hold the last frame. */
+    /** shear_sim::solve_quasistatic (shear_sim.cc:159-181): frames of
+     * `steps` device steps each, the frame files written to `filename` after
+     * each, and the reference's per-frame diagnostic line */
     void solve_quasistatic(double duration, int frames, int steps) {
         double dt = duration / frames * (1.0 / steps);
+        if (f_num == 0) {
+            write_files(0);
+            std::puts("# Output frame 0");
+        }
         for (int k = 1; k <= frames; k++) {
             step_forward_quasistatic(dt, steps);
-            download();
+            write_files(k + f_num);
             std::printf("# Output frame %d [%d] {%.2f}\n", k + f_num, steps, qsm.tp.avg_iters());
         }
         f_num += frames;
+    }
+    /** shear_sim::step_forward (shear_sim.cc:185-276): the direct step */
+    void step_forward(double dt) {
+        sync_scalars();
+        stz_amd_detail::check(stz_sim_step_forward(h, dt, 1), "step_forward");
+        pull_time();
+    }
+    /** shear_sim::solve (shear_sim.cc:120-152): direct steps of dt =
+     * tmult dx^2 up to each frame time, the last one shortened to land on it;
+     * frames written as they fall due */
+    void solve(double duration, int frames) {
+        const double time_interval = duration / frames, t_start = time, dt = dx * dx * tmult;
+        if (f_num == 0) {
+            write_files(0);
+            std::puts("# Output frame 0");
+        }
+        for (int k = 1; k <= frames; k++) {
+            const double target_time = t_start + time_interval * k;
+            int l = 1;
+            while (time + dt * (1 + 1e-8) < target_time) {
+                l++;
+                step_forward(dt);
+            }
+            step_forward(target_time - time);
+            write_files(k + f_num);
+            std::printf("# Output frame %d [%d]\n", k + f_num, l);
+        }
+        f_num += frames;
+    }
+    /** shear_sim::write_files (shear_sim.cc:659-676): the reference's field
+     * selection (p, tem, dev, X, Y) as Gnuplot binary matrices */
+    void write_files(int k) {
+        const int fflags = 4 | 128 | 256 | 512 | 1024;
+        download();
+        static const char *names[12] = {"u", "v", "p", "q", "s", "tau", "chi", "tem", "dev", "X", "Y", "proj"};
+        for (int mode = 0; mode < 11; mode++)
+            if (fflags & (1 << mode)) output(names[mode], mode, k);
+    }
+    /** shear_sim::output (shear_sim.cc:688-726) of the host mirror fm
+     * (call download() first) */
+    void output(const char *prefix, const int mode, const int sn) {
+        const bool st = mode >= 2 && mode <= 8;
+        const int l = st ? m : m + 1;
+        std::vector<char> name(std::strlen(filename) + std::strlen(prefix) + 32);
+        std::snprintf(name.data(), name.size(), "%s/%s.%d", filename, prefix, sn);
+        FILE *outf = std::fopen(name.data(), "wb");
+        if (!outf) {
+            std::fprintf(stderr, "stz_amd: can't open %s\n", name.data());
+            std::exit(1);
+        }
+        std::vector<float> buf(l + 1);
+        buf[0] = l;
+        for (int i = 0; i < l; i++) buf[i + 1] = ax + (st ? (i + 0.5) * dx : i * dx);
+        std::fwrite(buf.data(), sizeof(float), l + 1, outf);
+        c_field *fp = fm;
+        for (int j = 0; j < (st ? n : n + 1); j++) {
+            buf[0] = ay + (st ? (j + 0.5) * dy : j * dy);
+            for (int i = 0; i < m; i++, fp++) {
+                double v = 0;
+                switch (mode) {
+                case 0: v = fp->u; break;
+                case 1: v = fp->v; break;
+                case 2: v = fp->p; break;
+                case 3: v = fp->q; break;
+                case 4: v = fp->s; break;
+                case 5: v = fp->tau; break;
+                case 6: v = fp->chi; break;
+                case 7: v = fp->chi * TZ; break;
+                case 8: v = fp->dev(); break;
+                case 9: v = fp->X; break;
+                case 10: v = fp->Y; break;
+                default: v = fp->cu;
+                }
+                buf[i + 1] = v;
+            }
+            if (!st) buf[m + 1] = mode == 9 ? buf[1] + bx - ax : buf[1];
+            std::fwrite(buf.data(), sizeof(float), l + 1, outf);
+        }
+        std::fclose(outf);
+    }
+    /** shear_sim::tractions (shear_sim.cc:1003-1028) */
+    void tractions(double &bx_, double &by_, double &tx, double &ty) {
+        double o[4];
+        stz_amd_detail::check(stz_sim_tractions(h, o), "tractions");
+        bx_ = o[0];
+        by_ = o[1];
+        tx = o[2];
+        ty = o[3];
+    }
+    /** shear_sim::qs_measure (shear_sim.cc:733-775); the integrand is in
+     * fm[].cu after download() */
+    void qs_measure(double &Q, double &max_qs, bool staggered = false) {
+        stz_amd_detail::check(stz_sim_qs_measure(h, staggered ? 1 : 0, &Q, &max_qs), "qs_measure");
+    }
+    /** shear_sim::compute_strain (shear_sim.cc:778-824) into fm[].cu (after
+     * download()); corrected fixes the reference formula's interior-row y
+     * difference and seam-column sign */
+    void compute_strain(bool corrected = false) {
+        stz_amd_detail::check(stz_sim_compute_strain(h, corrected ? 1 : 0), "compute_strain");
+    }
+    /** shear_sim::l2_comparison (shear_sim.cc:460-502) */
+    void l2_comparison(shear_sim &ss, double *l2) {
+        stz_amd_detail::check(stz_sim_l2_comparison(h, ss.h, l2), "l2_comparison");
+    }
+    /** shear_sim::initialize_chi_bicubic (shear_sim.cc:832-913) */
+    void initialize_chi_bicubic(int mm, int nn, double *arr, double lo = 0, double sca = 1) {
+        sync_scalars();
+        stz_amd_detail::check(stz_sim_initialize_chi_bicubic(h, mm, nn, arr, lo, sca), "initialize_chi_bicubic");
+        download();
+    }
+    /** shear_sim::initialize_random (shear_sim.cc:932-985), on the C
+     * library's rand() stream as the reference */
+    void initialize_random(double l, double tem_avg, double tem_stdev) {
+        std::printf("# Setting up random field (convolution window size: %d g.p.)\n", (int(l * 5.) << 1) + 1);
+        sync_scalars();
+        stz_amd_detail::check(stz_sim_initialize_random(h, l, tem_avg, tem_stdev), "initialize_random");
+        download();
     }
     inline double avg_iters() { return qsm.tp.avg_iters(); }
     inline double bdry_vel() { return time < 1 ? time * u_bdry : u_bdry; }
     inline double bdry_pos() { return time < 1 ? u_bdry * time * time * 0.5 : u_bdry * (time - 0.5); }
 
-    /** copy the device state into fm (rows -1..n), vel and src */
+    /** copy the device state into fm (rows -1..n; with the cu slot), vel
+     * and src */
     void download() {
         std::vector<double> plane((size_t)m * (n + 2));
-        for (int f = 0; f < STZ_NFIELDS; f++) {
+        for (int f = 0; f <= STZ_F_CU; f++) {
             stz_amd_detail::check(stz_sim_download_field(h, f, plane.data()), "download_field");
             for (int k = 0; k < m * (n + 2); k++) (&fm[k - m].u)[f] = plane[k];
         }

@@ -83,3 +83,43 @@ def test_facade_driver_matches_checker(demo, tmp_path, m, n, chi_case):
     assert max(errs.values()) < 1e-10, errs
     ov = o.get_vel()
     assert np.max(np.abs(vel - ov)) <= 1e-10 * np.max(np.abs(ov))
+
+
+@pytest.fixture(scope="module")
+def frames_demo(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("facade") / "facade_frames")
+    subprocess.check_call(["g++", "-O2", "-std=c++11", "-Wall", "-Wextra", "-Werror",
+                           "-I", os.path.join(ROOT, "stz_amd", "host"),
+                           os.path.join(ROOT, "tests", "cpp", "facade_frames.cc"),
+                           "-o", exe, "-L", LIBDIR, "-lstz_amd", "-Wl,-rpath," + LIBDIR])
+    return exe
+
+
+@pytest.mark.gpu
+def test_facade_diagnostics_and_frames(frames_demo, tmp_path):
+    """write_files/output, tractions, qs_measure and compute_strain of the C++
+    drop-in against the reference's own outputs (tests/golden/diag40x24)"""
+    from conftest import load_fixture
+    fx = load_fixture("diag40x24")
+    print("case facade frames diag40x24", flush=True)
+    p = fx["params"]
+    m, n = p["m"], p["n"]
+    state = tmp_path / "state.bin"
+    np.concatenate([fx["in_planes"].ravel(), fx["in_vel"].ravel()]).tofile(state)
+    cufile = tmp_path / "cu.bin"
+    s = p["stz"]
+    args = [str(state), str(tmp_path), str(cufile), str(m), str(n)] + [repr(float(v)) for v in (
+        p["ax"], p["bx"], p["ay"], p["by"], p["u_bdry"], p["mu"], p["K"], p["viscosity"], p["adapt_fac"],
+        p["tmult"], s["TZ"], s["c0"], s["tau0"], s["kappa"], s["Delta"], s["Omegaeps0"], s["chi_inf"],
+        s["theta"])]
+    r = subprocess.run([frames_demo] + args, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    vals = {l.split()[0]: np.array([float(x) for x in l.split()[1:]]) for l in r.stdout.splitlines()}
+    assert np.allclose(vals["tractions"], fx["tractions"], rtol=1e-12, atol=0)
+    assert np.allclose(vals["qs"], np.concatenate([fx["qs_0"], fx["qs_1"]]), rtol=1e-12, atol=0)
+    cu = np.fromfile(cufile, dtype=np.float64).reshape(n + 2, m)
+    assert np.allclose(cu[1:-1], fx["strain_cu"][1:-1], rtol=1e-12, atol=1e-300)
+    for mode, prefix in ((0, "u"), (2, "p"), (7, "tem"), (8, "dev"), (9, "X"), (10, "Y")):
+        with open(tmp_path / ("%s.3" % prefix), "rb") as f:
+            assert f.read() == fx["gnuplot_%d" % mode].tobytes(), prefix
