@@ -257,6 +257,112 @@ enum {
 int stz_sim_record_decisions(stz_sim *sim, int enable);
 int stz_sim_download_decisions(stz_sim *sim, uint32_t *host_cells);
 
+/* ======================================================================== *
+ * Free-boundary simulation: a body bounded by the zero contour of a level
+ * set inside a rectangular box, the reference's `bd_sim` (sb/main/
+ * bd_sim.hh:27-190) with its problem class `qs_multi_bd`
+ * (sb/main/qs_multi_bd.hh:11-72) and the narrow-band level set `levelset`
+ * (levelset/levelset.hh:19-114).
+ *
+ * Grids: m x n cells (cell (i,j) centred at (ax+(i+1/2)dx, ay+(j+1/2)dy))
+ * and (m+1) x (n+1) nodes ((ax+i dx, ay+j dy)).  Cell planes are m*n FP64
+ * row-major (p, q, s, tau, chi, ddev and the level set phi), node planes
+ * (m+1)*(n+1) (u, v, X, Y); the multigrid velocity/source are interleaved
+ * vec2 over the nodes.  phi < 0 inside the body; the band is |phi| < 5 dx
+ * and cells outside it hold phi = +-5 dx, as in the reference
+ * (bd_sim.cc:40, levelset.cc:412-418).
+ *
+ * Cell status (levelset.hh s[], one int per cell): 0 far inside, 1 inside
+ * and next to the band, 2 inside band, 3 inside and next to the interface,
+ * 4 outside next to the interface, 5 outside band, 6 outside next to the
+ * band, 7 far outside (the codes of levelset.cc:458-520).  Node status
+ * (bd_sim::cc, bd_sim.cc:125-172): 0 inside, 7 outside.
+ * ======================================================================== */
+typedef struct stz_bd stz_bd;       /* opaque */
+
+/* Constructor arguments of bd_sim (bd_sim.hh:147-151, bd_sim.cc:25-53). */
+typedef struct stz_bd_params {
+    int32_t m, n;                   /* cells in x and y (both bounded) */
+    double ax, bx, ay, by;          /* box */
+    double mu, K, viscosity;        /* shear/bulk modulus, viscous damping */
+    double chi_len;                 /* Dpl-mediated chi diffusion length (0: off) */
+    double filter_stress;           /* stress filtering factor (advection_step) */
+    double t_scale, tmult, adapt_fac;
+    stz_model_params stz;
+} stz_bd_params;
+
+enum { STZ_BD_P = 0, STZ_BD_Q, STZ_BD_S, STZ_BD_TAU, STZ_BD_CHI, STZ_BD_DDEV, STZ_BD_PHI, STZ_BD_NCELL };
+enum { STZ_BD_U = 0, STZ_BD_V, STZ_BD_X, STZ_BD_Y, STZ_BD_NNODE };
+
+int stz_bd_create(const stz_bd_params *p, stz_bd **out);
+int stz_bd_destroy(stz_bd *b);
+int stz_bd_set_stream(stz_bd *b, void *hip_stream);
+int stz_bd_synchronize(stz_bd *b);
+
+/* Level set from the signed function phi (m*n cell values, < 0 inside):
+ * levelset::build_band (levelset.cc:209-345) -- the interface cells get their
+ * distance to the zero contour by the Newton closest-point iteration, the
+ * rest of the band by the eikonal update of the reference's fast march --
+ * then bd_sim::set_staggered_status (bd_sim.cc:125-172). */
+int stz_bd_set_level_set(stz_bd *b, const double *phi);
+/* A level set that is already a narrow band, with its cell statuses (codes
+ * above), taken as they are -- a saved state or another implementation's;
+ * node statuses are derived (bd_sim.cc:125-172). */
+int stz_bd_upload_level_set(stz_bd *b, const double *phi, const int32_t *cell_status);
+/* bd_sim::init_bar (bd_sim.cc:63-123): geometry case ca (1 bar, 2 four
+ * rounded notches, other: ellipse), fields inside the body, extrapolation. */
+int stz_bd_init_bar(stz_bd *b, int ca);
+
+int stz_bd_upload_cells(stz_bd *b, int k, const double *host);    /* k < STZ_BD_PHI */
+int stz_bd_download_cells(stz_bd *b, int k, double *host);        /* k <= STZ_BD_PHI */
+int stz_bd_upload_nodes(stz_bd *b, int k, const double *host);
+int stz_bd_download_nodes(stz_bd *b, int k, double *host);
+int stz_bd_download_vel(stz_bd *b, double *host);                 /* multigrid solution */
+int stz_bd_download_src(stz_bd *b, double *host);                 /* multigrid source */
+/* status arrays (int32, either pointer may be NULL) */
+int stz_bd_download_status(stz_bd *b, int32_t *cell_status, int32_t *node_status);
+int stz_bd_set_time(stz_bd *b, double t);
+int stz_bd_get_time(stz_bd *b, double *t);
+
+/* bd_sim::advection_step (bd_sim.cc:374-469): level set moved
+ * (levelset::move_and_smooth, levelset.hh:159-205) then the explicit
+ * reference-map / stress / chi update. */
+int stz_bd_advection_step(stz_bd *b, double dt);
+/* bd_sim::projection_step (bd_sim.cc:510-545): qs_multi_bd::init
+ * (qs_multi_bd.cc:26-156), the multigrid solve to the mean-square residual
+ * 1e-20 (qs_multi_bd.cc:16, tgmg.in.cc:242-255), velocity and stress
+ * update. */
+int stz_bd_projection_step(stz_bd *b, double dt);
+/* bd_sim::post_process (bd_sim.cc:256-259): extrapolate the cell fields
+ * (all_fields: also the node fields) across the interface, then the box
+ * boundary conditions. */
+int stz_bd_post_process(stz_bd *b, int all_fields);
+/* bd_sim::step_forward_quasistatic (bd_sim.cc:240-254), nsteps times. */
+int stz_bd_step_quasistatic(stz_bd *b, double dt, int nsteps);
+int stz_bd_last_solve(stz_bd *b, stz_solve_info *info);
+
+/* Diagnostics of single pieces of the step, on the current state:
+ * the interface speed of levelset::move_and_smooth (levelset.hh:159-205) at
+ * every interface cell (0 elsewhere), without moving anything; and the
+ * operator of qs_multi_bd::init (qs_multi_bd.cc:26-156) for time step dt:
+ * row type per node (-1 interior, -2 box edge, -3 outside, 0 free surface),
+ * the nine 2x2 blocks a_dl..a_ur per node (36 doubles, block order dl dc dr
+ * cl cc cr ul uc ur, each a b c d row-major) and the source.  Any pointer
+ * may be NULL. */
+int stz_bd_interface_velocity(stz_bd *b, double *vp);
+int stz_bd_operator(stz_bd *b, double dt, int32_t *row_type, double *blocks36, double *src);
+
+/* Per-node body force (interleaved vec2, (m+1)*(n+1) nodes; NULL removes
+ * it): the value that extra_force::force(i,j) returns (extra_force.hh:5-28),
+ * added as idt*force to the interior rows' source (qs_multi_bd.cc:71). */
+int stz_bd_set_body_force(stz_bd *b, const double *force);
+/* Edge velocity: u = u_left on the nodes of column 0 and u_right on column
+ * m, the reference maps there following the edge (X = ax + integral of
+ * u_left, likewise at bx).  Default 0, which is the reference's
+ * set_boundaries (bd_sim.cc:732-761); no counterpart in the reference, it
+ * loads a body held between two moving grips. */
+int stz_bd_set_edge_velocity(stz_bd *b, double u_left, double u_right);
+
 #ifdef __cplusplus
 }
 #endif

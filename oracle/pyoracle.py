@@ -360,3 +360,172 @@ class Reference(_Sim):
     @staticmethod
     def set_threads(n):
         Reference._fns["set_threads"](int(n))
+
+
+# ---------------------------------------------------------------------------
+# free-boundary reference (bd_sim) through oracle/ref_bd_harness.cc
+# ---------------------------------------------------------------------------
+_BD_DOUBLES = ("ax", "bx", "ay", "by", "mu", "K", "viscosity", "chi_len",
+               "filter_stress", "t_scale", "tmult", "adapt_fac")
+
+
+class BdParams(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32)] + \
+        [(k, C.c_double) for k in _BD_DOUBLES] + [("stz", ModelParams)]
+
+
+def make_bd_params(d):
+    p = BdParams()
+    p.m, p.n = int(d["m"]), int(d["n"])
+    for k in _BD_DOUBLES:
+        setattr(p, k, float(d[k]))
+    mp = d["stz"]
+    p.stz.kind = int(mp.get("kind", 0))
+    p.stz.t_model = int(mp.get("t_model", -1))
+    for k in _MODEL_DOUBLES:
+        setattr(p.stz, k, float(mp[k]))
+    return p
+
+
+class RefBd:
+    """The reference's bd_sim compiled from its own sources
+    (oracle/_ref/libstzref.so); same method names as stz_amd.BdSim."""
+    _f = None
+    CELL = ("p", "q", "s", "tau", "chi", "ddev")
+    NODE = ("u", "v", "X", "Y")
+
+    def __init__(self, params, outdir):
+        if RefBd._f is None:
+            lib = C.CDLL(REF_SO)
+            vp, d, i = C.c_void_p, C.c_double, C.c_int
+            sig = {
+                "create": (vp, [C.POINTER(BdParams), C.c_char_p]),
+                "destroy": (None, [vp]),
+                "init_bar": (None, [vp, i]),
+                "set_phi_build": (None, [vp, _dp]),
+                "get_phi": (None, [vp, _dp]),
+                "get_status": (None, [vp, _ip, _ip]),
+                "set_cells": (None, [vp, i, _dp]),
+                "get_cells": (None, [vp, i, _dp]),
+                "set_nodes": (None, [vp, i, _dp]),
+                "get_nodes": (None, [vp, i, _dp]),
+                "get_vel": (None, [vp, _dp]),
+                "get_src": (None, [vp, _dp]),
+                "set_time": (None, [vp, d]),
+                "set_force": (None, [vp, _dp]),
+                "use_builtin_force": (None, [vp, i]),
+                "advection_step": (None, [vp, d]),
+                "projection_step": (None, [vp, d]),
+                "post_process": (None, [vp, i]),
+                "step": (None, [vp, d, i]),
+                "operator": (None, [vp, d, _ip, _dp, _dp]),
+                "interface_velocity": (None, [vp, _dp]),
+                "set_bc_smooth": (None, [vp, d]),
+            }
+            f = {}
+            for name, (res, args) in sig.items():
+                fn = getattr(lib, "ref_bd_" + name)
+                fn.restype = res
+                fn.argtypes = args
+                f[name] = fn
+            RefBd._f = f
+        self.params = params
+        self._p = make_bd_params(params)
+        self.m, self.n = self._p.m, self._p.n
+        # bd_sim keeps the directory name pointer (bd_sim.hh:83)
+        self._outdir = str(outdir).encode()
+        self._h = self._f["create"](C.byref(self._p), self._outdir)
+
+    def close(self):
+        if self._h:
+            self._f["destroy"](self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def cell_shape(self):
+        return (self.n, self.m)
+
+    @property
+    def node_shape(self):
+        return (self.n + 1, self.m + 1)
+
+    def set_level_set(self, phi):
+        self._f["set_phi_build"](self._h, np.ascontiguousarray(phi, dtype=np.float64).ravel())
+
+    def init_bar(self, ca):
+        self._f["init_bar"](self._h, int(ca))
+
+    def get_cell(self, k):
+        a = np.zeros(self.m * self.n)
+        if k == "phi":
+            self._f["get_phi"](self._h, a)
+        else:
+            self._f["get_cells"](self._h, self.CELL.index(k), a)
+        return a.reshape(self.cell_shape)
+
+    def set_cell(self, k, v):
+        a = np.ascontiguousarray(np.broadcast_to(v, self.cell_shape), dtype=np.float64).ravel()
+        self._f["set_cells"](self._h, self.CELL.index(k), a)
+
+    def get_node(self, k):
+        a = np.zeros((self.m + 1) * (self.n + 1))
+        self._f["get_nodes"](self._h, self.NODE.index(k), a)
+        return a.reshape(self.node_shape)
+
+    def set_node(self, k, v):
+        a = np.ascontiguousarray(np.broadcast_to(v, self.node_shape), dtype=np.float64).ravel()
+        self._f["set_nodes"](self._h, self.NODE.index(k), a)
+
+    def status(self):
+        c = np.zeros(self.m * self.n, np.int32)
+        cc = np.zeros((self.m + 1) * (self.n + 1), np.int32)
+        self._f["get_status"](self._h, c, cc)
+        return c.reshape(self.cell_shape), cc.reshape(self.node_shape)
+
+    def get_vel(self):
+        a = np.zeros(2 * (self.m + 1) * (self.n + 1))
+        self._f["get_vel"](self._h, a)
+        return a.reshape(self.node_shape + (2,))
+
+    def get_src(self):
+        a = np.zeros(2 * (self.m + 1) * (self.n + 1))
+        self._f["get_src"](self._h, a)
+        return a.reshape(self.node_shape + (2,))
+
+    def set_time(self, t):
+        self._f["set_time"](self._h, float(t))
+
+    def set_body_force(self, F):
+        self._f["set_force"](self._h, np.ascontiguousarray(F, dtype=np.float64).ravel())
+
+    def advection_step(self, dt):
+        self._f["advection_step"](self._h, float(dt))
+
+    def projection_step(self, dt):
+        self._f["projection_step"](self._h, float(dt))
+
+    def post_process(self, all_fields=True):
+        self._f["post_process"](self._h, 1 if all_fields else 0)
+
+    def step_forward_quasistatic(self, dt, nsteps=1):
+        self._f["step"](self._h, float(dt), int(nsteps))
+
+    def operator(self, dt):
+        nn = (self.m + 1) * (self.n + 1)
+        d = np.zeros(nn, np.int32)
+        mats = np.zeros(36 * nn)
+        b = np.zeros(2 * nn)
+        self._f["operator"](self._h, float(dt), d, mats, b)
+        return d.reshape(self.node_shape), mats.reshape(self.node_shape + (9, 2, 2)), \
+            b.reshape(self.node_shape + (2,))
+
+    def interface_velocity(self):
+        a = np.zeros(self.m * self.n)
+        self._f["interface_velocity"](self._h, a)
+        return a.reshape(self.cell_shape)

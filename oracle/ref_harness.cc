@@ -37,7 +37,7 @@ struct ref_sim {
     double acc_base;
 };
 
-static stz_dynamics *make_stz(const stz_model_params &p) {
+stz_dynamics *ref_make_stz(const stz_model_params &p) {
     switch (p.kind) {
     case STZ_MODEL_LINEAR:
         return new stz_dynamics_linear(p.TZ, p.c0, p.tau0, p.kappa, p.Delta, p.Omegaeps0, p.chi_inf, p.theta, p.rho,
@@ -61,7 +61,7 @@ extern "C" {
 
 void *ref_create(const stz_sim_params *p) {
     ref_sim *r = new ref_sim;
-    r->stz = make_stz(p->stz);
+    r->stz = ref_make_stz(p->stz);
     r->sim = new shear_sim(p->m, p->n, p->ax, p->bx, p->ay, p->by, p->mu, p->K, p->viscosity, p->chi_diff,
                            p->chi_len, p->t_scale, p->tmult, p->adapt_fac, p->u_bdry, r->stz, ".");
     shear_sim &s = *r->sim;

@@ -68,6 +68,29 @@ def make_params(d):
     return p
 
 
+_BD_DOUBLES = ("ax", "bx", "ay", "by", "mu", "K", "viscosity", "chi_len",
+               "filter_stress", "t_scale", "tmult", "adapt_fac")
+
+
+class BdParams(C.Structure):
+    """stz_bd_params: bd_sim constructor arguments (bd_sim.hh:147-151)."""
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32)] + \
+        [(k, C.c_double) for k in _BD_DOUBLES] + [("stz", ModelParams)]
+
+
+def make_bd_params(d):
+    p = BdParams()
+    p.m, p.n = int(d["m"]), int(d["n"])
+    for k in _BD_DOUBLES:
+        setattr(p, k, float(d[k]))
+    mp = d["stz"]
+    p.stz.kind = int(mp.get("kind", 0))
+    p.stz.t_model = int(mp.get("t_model", -1))
+    for k in _MODEL_DOUBLES:
+        setattr(p.stz, k, float(mp[k]))
+    return p
+
+
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -116,6 +139,31 @@ _SIGS = {
     "stz_sim_initialize_chi_bicubic": [C.c_void_p, C.c_int, C.c_int, _dp, C.c_double, C.c_double],
     "stz_sim_record_decisions": [C.c_void_p, C.c_int],
     "stz_sim_download_decisions": [C.c_void_p, _up],
+    "stz_bd_create": [C.POINTER(BdParams), C.POINTER(C.c_void_p)],
+    "stz_bd_destroy": [C.c_void_p],
+    "stz_bd_set_stream": [C.c_void_p, C.c_void_p],
+    "stz_bd_synchronize": [C.c_void_p],
+    "stz_bd_set_level_set": [C.c_void_p, _dp],
+    "stz_bd_upload_level_set": [C.c_void_p, _dp, _ip],
+    "stz_bd_init_bar": [C.c_void_p, C.c_int],
+    "stz_bd_upload_cells": [C.c_void_p, C.c_int, _dp],
+    "stz_bd_download_cells": [C.c_void_p, C.c_int, _dp],
+    "stz_bd_upload_nodes": [C.c_void_p, C.c_int, _dp],
+    "stz_bd_download_nodes": [C.c_void_p, C.c_int, _dp],
+    "stz_bd_download_vel": [C.c_void_p, _dp],
+    "stz_bd_download_src": [C.c_void_p, _dp],
+    "stz_bd_download_status": [C.c_void_p, _ip, _ip],
+    "stz_bd_set_time": [C.c_void_p, C.c_double],
+    "stz_bd_get_time": [C.c_void_p, C.POINTER(C.c_double)],
+    "stz_bd_advection_step": [C.c_void_p, C.c_double],
+    "stz_bd_projection_step": [C.c_void_p, C.c_double],
+    "stz_bd_post_process": [C.c_void_p, C.c_int],
+    "stz_bd_step_quasistatic": [C.c_void_p, C.c_double, C.c_int],
+    "stz_bd_last_solve": [C.c_void_p, C.POINTER(SolveInfo)],
+    "stz_bd_interface_velocity": [C.c_void_p, _dp],
+    "stz_bd_operator": [C.c_void_p, C.c_double, _ip, _dp, _dp],
+    "stz_bd_set_body_force": [C.c_void_p, C.c_void_p],
+    "stz_bd_set_edge_velocity": [C.c_void_p, C.c_double, C.c_double],
 }
 _RESTYPE = {"stz_last_error": C.c_char_p}
 

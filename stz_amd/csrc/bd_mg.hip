@@ -1,0 +1,647 @@
+// bd_mg.hip -- gfx950 solver for the free-boundary projection: the
+// operator of qs_multi_bd (sb/main/qs_multi_bd.hh:31-57) on the nodes, a
+// Galerkin hierarchy of tgmg (tgmg.in.cc:10-30, 667-779; coarse sizes
+// (m+2)>>1 on both bounded axes, tgmg.hh:109-110, and two levels fewer than
+// tgmg would build, bd_sim.cc:51) rebuilt on the device at every solve, and
+// V-cycles of four-colour Gauss-Seidel sweeps.
+//
+// The reference iterates bare V-cycles to a mean-square residual of 1e-20
+// (qs_multi_bd.cc:16, tgmg.in.cc:242-255).  Its free-surface rows are not
+// diagonally dominant, and on some geometries (a plate with a hole at 64^2
+// or 256^2 cells, for one) those V-cycles diverge although the linear system
+// is well posed.  Here one V-cycle is the preconditioner of BiCGStab
+// (van der Vorst 1992, right preconditioning), run to the same residual
+// threshold: where the reference converges both reach the same solution,
+// and where it diverges this one still converges.
+//
+// Top level: matrix-free from the row types (-1 interior with the constant
+// stencil of qs_multi_bd::mul_a, -2 box edge, -3 outside, >= 0 a
+// free-surface row whose eight blocks sit in a compact table).  Coarse
+// levels: 40 doubles per node, nine 2x2 blocks (dl,dc,dr,cl,cc,cr,ul,uc,ur)
+// and the inverse of cc.  Vectors are interleaved (x,y) pairs.
+#include "bd_mg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace stz {
+
+namespace {
+
+constexpr int kThreads = 256;
+inline dim3 blocks(long n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
+
+struct V2 {
+    double x, y;
+};
+
+// one entry (2x2 block k of row (i,j)) of the top operator, as the
+// reference's a_xx accessors give it (qs_multi_bd.hh:31-39)
+__device__ __forceinline__ void top_entry(const BdTop &t, int ij, int k, double *o) {
+    const int d = t.rowt[ij];
+    o[0] = o[1] = o[2] = o[3] = 0;
+    if (d == -1) {
+        switch (k) {
+        case 0: case 8: o[1] = o[2] = t.qK; break;            // rc
+        case 2: case 6: o[1] = o[2] = -t.qK; break;           // mrc
+        case 1: case 7: o[0] = t.mx; o[3] = t.Kmy; break;     // rv
+        case 3: case 5: o[0] = t.Kmx; o[3] = t.my; break;     // rh
+        default: o[0] = t.cxx; o[3] = t.cyy;                  // ru
+        }
+        return;
+    }
+    if (k == 4) {
+        const double s = d == -2 ? t.bcs : (d == -3 ? t.ocs : t.ncs);
+        o[0] = s * t.cxx;
+        o[3] = s * t.cyy;
+        return;
+    }
+    if (d >= 0) {
+        const double *a = t.fmats + 32 * (size_t)d + 4 * (k < 4 ? k : k - 1);
+        o[0] = a[0];
+        o[1] = a[1];
+        o[2] = a[2];
+        o[3] = a[3];
+    }
+}
+
+// off-diagonal part of row ij applied to z (qs_multi_bd::mul_a,
+// qs_multi_bd.hh:41-56) and the diagonal block
+__device__ __forceinline__ V2 top_offdiag(const BdTop &t, const double *z, int ij, int d) {
+    const int m = t.m;
+    if (d == -1) {
+        const double *zp = z + 2 * (size_t)ij;
+        const double *zl = zp - 2, *zr = zp + 2, *zd = zp - 2 * m, *zu = zp + 2 * m;
+        const double *zdl = zd - 2, *zdr = zd + 2, *zul = zu - 2, *zur = zu + 2;
+        return V2{t.Kmx * (zl[0] + zr[0]) + t.my * (zd[0] + zu[0]) + t.qK * (zdl[1] + zur[1] - zdr[1] - zul[1]),
+                  t.mx * (zl[1] + zr[1]) + t.Kmy * (zd[1] + zu[1]) + t.qK * (zdl[0] + zur[0] - zdr[0] - zul[0])};
+    }
+    if (d < 0) return V2{0, 0};
+    const double *a = t.fmats + 32 * (size_t)d;
+    const int off[8] = {-m - 1, -m, -m + 1, -1, 1, m - 1, m, m + 1};
+    double rx = 0, ry = 0;
+    for (int q = 0; q < 8; q++) {
+        const double *zz = z + 2 * (size_t)(ij + off[q]);
+        rx += a[4 * q] * zz[0] + a[4 * q + 1] * zz[1];
+        ry += a[4 * q + 2] * zz[0] + a[4 * q + 3] * zz[1];
+    }
+    return V2{rx, ry};
+}
+__device__ __forceinline__ double top_diag_scale(const BdTop &t, int d) {
+    return d == -1 ? 1.0 : (d == -2 ? t.bcs : (d == -3 ? t.ocs : t.ncs));
+}
+
+__device__ __forceinline__ V2 lvl_offdiag(const BdLevel &L, const double *z, int i, int j) {
+    const double *s = L.S + 40 * (size_t)(i + L.m * j);
+    double rx = 0, ry = 0;
+    for (int k = 0; k < 9; k++) {
+        if (k == 4) continue;
+        const int ii = i + k % 3 - 1, jj = j + k / 3 - 1;
+        if (ii < 0 || ii >= L.m || jj < 0 || jj >= L.n) continue;
+        const double *zz = z + 2 * (size_t)(ii + L.m * jj);
+        rx += s[4 * k] * zz[0] + s[4 * k + 1] * zz[1];
+        ry += s[4 * k + 2] * zz[0] + s[4 * k + 3] * zz[1];
+    }
+    return V2{rx, ry};
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Seidel sweeps, one colour (i&1, j&1) per launch
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) gs_top_kernel(BdTop t, double *__restrict__ z,
+                                                          const double *__restrict__ b, int colour) {
+    const int hm = (t.m + 1 - (colour & 1)) >> 1, hn = (t.n + 1 - (colour >> 1)) >> 1;
+    const int q = blockIdx.x * kThreads + threadIdx.x;
+    if (q >= hm * hn) return;
+    const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + t.m * j;
+    const int d = t.rowt[ij];
+    const V2 o = top_offdiag(t, z, ij, d);
+    const double sc = top_diag_scale(t, d);
+    z[2 * (size_t)ij] = (b[2 * (size_t)ij] - o.x) / (sc * t.cxx);
+    z[2 * (size_t)ij + 1] = (b[2 * (size_t)ij + 1] - o.y) / (sc * t.cyy);
+}
+
+__global__ void __launch_bounds__(kThreads) gs_level_kernel(BdLevel L, int colour) {
+    const int hm = (L.m + 1 - (colour & 1)) >> 1, hn = (L.n + 1 - (colour >> 1)) >> 1;
+    const int q = blockIdx.x * kThreads + threadIdx.x;
+    if (q >= hm * hn) return;
+    const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + L.m * j;
+    const V2 o = lvl_offdiag(L, L.z, i, j);
+    const double rx = L.b[2 * (size_t)ij] - o.x, ry = L.b[2 * (size_t)ij + 1] - o.y;
+    const double *iv = L.S + 40 * (size_t)ij + 36;
+    L.z[2 * (size_t)ij] = iv[0] * rx + iv[1] * ry;
+    L.z[2 * (size_t)ij + 1] = iv[2] * rx + iv[3] * ry;
+}
+
+// the bottom level in one workgroup: z in LDS, `sweeps` four-colour sweeps
+__global__ void __launch_bounds__(1024) gs_bottom_kernel(BdLevel L, int sweeps) {
+    extern __shared__ double zs[];
+    const int N = L.m * L.n;
+    for (int q = threadIdx.x; q < 2 * N; q += blockDim.x) zs[q] = L.z[q];
+    __syncthreads();
+    for (int sw = 0; sw < sweeps; sw++)
+        for (int colour = 0; colour < 4; colour++) {
+            const int hm = (L.m + 1 - (colour & 1)) >> 1, hn = (L.n + 1 - (colour >> 1)) >> 1;
+            for (int q = threadIdx.x; q < hm * hn; q += blockDim.x) {
+                const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + L.m * j;
+                const V2 o = lvl_offdiag(L, zs, i, j);
+                const double rx = L.b[2 * ij] - o.x, ry = L.b[2 * ij + 1] - o.y;
+                const double *iv = L.S + 40 * (size_t)ij + 36;
+                zs[2 * ij] = iv[0] * rx + iv[1] * ry;
+                zs[2 * ij + 1] = iv[2] * rx + iv[3] * ry;
+            }
+            __syncthreads();
+        }
+    for (int q = threadIdx.x; q < 2 * N; q += blockDim.x) L.z[q] = zs[q];
+}
+
+// r = b - A z on the top level; optional per-block |r|^2 partial sums
+__global__ void __launch_bounds__(kThreads) resid_top_kernel(BdTop t, const double *__restrict__ z,
+                                                             const double *__restrict__ b, double *__restrict__ r,
+                                                             double *__restrict__ partial) {
+    __shared__ double red[kThreads / 64];
+    const int ij = blockIdx.x * kThreads + threadIdx.x;
+    double acc = 0;
+    if (ij < t.m * t.n) {
+        const int d = t.rowt[ij];
+        const V2 o = top_offdiag(t, z, ij, d);
+        const double sc = top_diag_scale(t, d);
+        const double rx = b[2 * (size_t)ij] - o.x - sc * t.cxx * z[2 * (size_t)ij];
+        const double ry = b[2 * (size_t)ij + 1] - o.y - sc * t.cyy * z[2 * (size_t)ij + 1];
+        if (r) {
+            r[2 * (size_t)ij] = rx;
+            r[2 * (size_t)ij + 1] = ry;
+        }
+        acc = rx * rx + ry * ry;
+    }
+    if (!partial) return;
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < kThreads / 64; w++) s += red[w];
+        partial[blockIdx.x] = s;
+    }
+}
+
+// y = A x on the top level
+__global__ void __launch_bounds__(kThreads) apply_top_kernel(BdTop t, const double *__restrict__ x,
+                                                             double *__restrict__ y) {
+    const int ij = blockIdx.x * kThreads + threadIdx.x;
+    if (ij >= t.m * t.n) return;
+    const int d = t.rowt[ij];
+    const V2 o = top_offdiag(t, x, ij, d);
+    const double sc = top_diag_scale(t, d);
+    y[2 * (size_t)ij] = o.x + sc * t.cxx * x[2 * (size_t)ij];
+    y[2 * (size_t)ij + 1] = o.y + sc * t.cyy * x[2 * (size_t)ij + 1];
+}
+
+__global__ void __launch_bounds__(kThreads) resid_level_kernel(BdLevel L) {
+    const int ij = blockIdx.x * kThreads + threadIdx.x;
+    if (ij >= L.m * L.n) return;
+    const int i = ij % L.m, j = ij / L.m;
+    const V2 o = lvl_offdiag(L, L.z, i, j);
+    const double *s = L.S + 40 * (size_t)ij + 16;
+    const double zx = L.z[2 * (size_t)ij], zy = L.z[2 * (size_t)ij + 1];
+    L.r[2 * (size_t)ij] = L.b[2 * (size_t)ij] - o.x - (s[0] * zx + s[1] * zy);
+    L.r[2 * (size_t)ij + 1] = L.b[2 * (size_t)ij + 1] - o.y - (s[2] * zx + s[3] * zy);
+}
+
+// 1D prolongation parents on a bounded axis (tgmg.in.cc:552-621)
+__device__ __forceinline__ int parents(int f, int uf, int sc, int *c, double *w) {
+    if (f < 0 || f >= uf) return 0;
+    if ((uf & 1) == 0 && f == uf - 1) { c[0] = sc - 1; w[0] = 1; return 1; }
+    if ((f & 1) == 0) { c[0] = f / 2; w[0] = 1; return 1; }
+    c[0] = f / 2; w[0] = 0.5; c[1] = f / 2 + 1; w[1] = 0.5;
+    return 2;
+}
+__device__ __forceinline__ double weight_of(int np, const int *c, const double *w, int target) {
+    for (int k = 0; k < np; k++)
+        if (c[k] == target) return w[k];
+    return 0;
+}
+
+// b_c = P^T r_f (restriction with the unnormalised weights 1, 1/2, 1/4)
+__global__ void __launch_bounds__(kThreads) restrict_kernel(const double *__restrict__ rf, int fm, int fn,
+                                                            double *__restrict__ bc, int cm, int cn) {
+    const int IJ = blockIdx.x * kThreads + threadIdx.x;
+    if (IJ >= cm * cn) return;
+    const int I = IJ % cm, J = IJ / cm;
+    double sx = 0, sy = 0;
+    int c[2];
+    double w[2];
+    for (int fy = 2 * J - 1; fy <= 2 * J + 1; fy++) {
+        const double wy = weight_of(parents(fy, fn, cn, c, w), c, w, J);
+        if (wy == 0) continue;
+        for (int fx = 2 * I - 1; fx <= 2 * I + 1; fx++) {
+            const double wx = weight_of(parents(fx, fm, cm, c, w), c, w, I);
+            if (wx == 0) continue;
+            const double *r = rf + 2 * (size_t)(fx + fm * fy);
+            sx += wx * wy * r[0];
+            sy += wx * wy * r[1];
+        }
+    }
+    bc[2 * (size_t)IJ] = sx;
+    bc[2 * (size_t)IJ + 1] = sy;
+}
+
+// z_f += P z_c
+__global__ void __launch_bounds__(kThreads) prolong_kernel(const double *__restrict__ zc, int cm, int cn,
+                                                           double *__restrict__ zf, int fm, int fn) {
+    const int ij = blockIdx.x * kThreads + threadIdx.x;
+    if (ij >= fm * fn) return;
+    const int i = ij % fm, j = ij / fm;
+    int cx[2], cy[2];
+    double wx[2], wy[2];
+    const int nx = parents(i, fm, cm, cx, wx), ny = parents(j, fn, cn, cy, wy);
+    double sx = 0, sy = 0;
+    for (int b = 0; b < ny; b++)
+        for (int a = 0; a < nx; a++) {
+            const double w = wx[a] * wy[b];
+            const double *z = zc + 2 * (size_t)(cx[a] + cm * cy[b]);
+            sx += w * z[0];
+            sy += w * z[1];
+        }
+    zf[2 * (size_t)ij] += sx;
+    zf[2 * (size_t)ij + 1] += sy;
+}
+
+// Galerkin coarse block row at coarse node (I,J): sum over fine nodes f
+// with weight P(f,(I,J)) of P(f,.)^T A(f,g) P(g,.), then the inverse of cc
+template <bool TOP>
+__global__ void __launch_bounds__(kThreads) rap_kernel(BdTop t, BdLevel F, BdLevel C) {
+    const int IJ = blockIdx.x * kThreads + threadIdx.x;
+    if (IJ >= C.m * C.n) return;
+    const int I = IJ % C.m, J = IJ / C.m;
+    const int fm = TOP ? t.m : F.m, fn = TOP ? t.n : F.n;
+    double acc[36];
+    for (int q = 0; q < 36; q++) acc[q] = 0;
+    int c[2], cx[2], cy[2];
+    double w[2], wx[2], wy[2];
+    for (int fy = 2 * J - 1; fy <= 2 * J + 1; fy++) {
+        const double wfy = weight_of(parents(fy, fn, C.n, c, w), c, w, J);
+        if (wfy == 0) continue;
+        for (int fx = 2 * I - 1; fx <= 2 * I + 1; fx++) {
+            const double wfx = weight_of(parents(fx, fm, C.m, c, w), c, w, I);
+            if (wfx == 0) continue;
+            const int fij = fx + fm * fy;
+            for (int k = 0; k < 9; k++) {
+                const int gx = fx + k % 3 - 1, gy = fy + k / 3 - 1;
+                if (gx < 0 || gx >= fm || gy < 0 || gy >= fn) continue;
+                double A[4];
+                if (TOP) top_entry(t, fij, k, A);
+                else {
+                    const double *s = F.S + 40 * (size_t)fij + 4 * k;
+                    A[0] = s[0]; A[1] = s[1]; A[2] = s[2]; A[3] = s[3];
+                }
+                if (A[0] == 0 && A[1] == 0 && A[2] == 0 && A[3] == 0) continue;
+                const int ny = parents(gy, fn, C.n, cy, wy), nx = parents(gx, fm, C.m, cx, wx);
+                for (int q = 0; q < ny; q++)
+                    for (int p = 0; p < nx; p++) {
+                        const int dI = cx[p] - I, dJ = cy[q] - J;
+                        if (dI < -1 || dI > 1 || dJ < -1 || dJ > 1) continue;
+                        const double ww = wfx * wfy * wx[p] * wy[q];
+                        double *e = acc + 4 * ((dI + 1) + 3 * (dJ + 1));
+                        for (int r = 0; r < 4; r++) e[r] += ww * A[r];
+                    }
+            }
+        }
+    }
+    double *o = C.S + 40 * (size_t)IJ;
+    for (int q = 0; q < 36; q++) o[q] = acc[q];
+    const double a = acc[16], b = acc[17], cc = acc[18], d = acc[19];
+    const double idet = 1. / (a * d - b * cc);
+    o[36] = d * idet;
+    o[37] = -b * idet;
+    o[38] = -cc * idet;
+    o[39] = a * idet;
+}
+
+__global__ void __launch_bounds__(kThreads) fill_kernel(double *p, long n, double v) {
+    const long q = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (q < n) p[q] = v;
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab vector kernels; scalars live on the device (sc[]):
+// 0 rho, 1 rho_prev, 2 alpha, 3 omega, 4 (rhat,v), 5 (t,s), 6 (t,t), 7 (r,r)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) dot_kernel(const double *__restrict__ a, const double *__restrict__ b,
+                                                       long n, double *__restrict__ partial) {
+    __shared__ double red[kThreads / 64];
+    double acc = 0;
+    for (long q = (long)blockIdx.x * kThreads + threadIdx.x; q < n; q += (long)gridDim.x * kThreads)
+        acc += a[q] * b[q];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < kThreads / 64; w++) s += red[w];
+        partial[blockIdx.x] = s;
+    }
+}
+
+// sum partials into sc[slot]; then the step's scalar update:
+// op 2: alpha = rho/(rhat,v); op 3: omega = (t,s)/(t,t); op 0: none
+__global__ void __launch_bounds__(kThreads) finish_dot_kernel(const double *__restrict__ partial, int np,
+                                                              double *sc, int slot, int op,
+                                                              volatile double *host) {
+    __shared__ double red[kThreads / 64];
+    double acc = 0;
+    for (int q = threadIdx.x; q < np; q += kThreads) acc += partial[q];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double s = 0;
+    for (int w = 0; w < kThreads / 64; w++) s += red[w];
+    sc[slot] = s;
+    if (op == 2) sc[2] = sc[0] / sc[4];
+    else if (op == 3) sc[3] = sc[5] / sc[6];
+    if (host) *host = s;
+}
+
+// p = r + beta (p - omega v), beta = (rho/rho_prev)(alpha/omega)
+__global__ void __launch_bounds__(kThreads) bicg_p_kernel(double *p, const double *__restrict__ r,
+                                                          const double *__restrict__ v, const double *sc, long n,
+                                                          int first) {
+    const long q = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= n) return;
+    const double beta = (sc[0] / sc[1]) * (sc[2] / sc[3]);
+    p[q] = first ? r[q] : r[q] + beta * (p[q] - sc[3] * v[q]);
+}
+// s = r - alpha v
+__global__ void __launch_bounds__(kThreads) bicg_s_kernel(double *s, const double *__restrict__ r,
+                                                          const double *__restrict__ v, const double *sc, long n) {
+    const long q = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (q < n) s[q] = r[q] - sc[2] * v[q];
+}
+// x += alpha phat + omega shat; r = s - omega t; rho_prev = rho
+__global__ void __launch_bounds__(kThreads) bicg_x_kernel(double *x, double *r, const double *__restrict__ ph,
+                                                          const double *__restrict__ sh,
+                                                          const double *__restrict__ s,
+                                                          const double *__restrict__ t, double *sc, long n) {
+    const long q = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (q == 0) sc[1] = sc[0];
+    if (q >= n) return;
+    const double a = sc[2], w = sc[3];
+    x[q] += a * ph[q] + w * sh[q];
+    r[q] = s[q] - w * t[q];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+int BdSolver::dims_for(int tm, int tn, std::vector<std::pair<int, int>> &out) {
+    // tgmg.in.cc:13-29 with tgmg_grid_min = 10, then two levels fewer
+    out.clear();
+    int am = (tm + 2) >> 1, an = (tn + 2) >> 1;
+    std::vector<std::pair<int, int>> all;
+    while ((long)am * an > 10) {
+        all.push_back({am, an});
+        const int nm = (am + 2) >> 1, nn = (an + 2) >> 1;
+        am = nm;
+        an = nn;
+    }
+    const int keep = (int)all.size() - 2;
+    for (int l = 0; l < keep; l++) out.push_back(all[l]);
+    return (int)out.size();
+}
+
+int BdSolver::init(int tm, int tn) {
+    free_all();
+    top_m = tm;
+    top_n = tn;
+    std::vector<std::pair<int, int>> d;
+    nl = dims_for(tm, tn, d);
+    const size_t nv = 2 * (size_t)tm * tn;
+    for (int k = 0; k < kBdVecs; k++) STZ_HIP(hipMalloc(&vec[k], nv * sizeof(double)));
+    STZ_HIP(hipMalloc(&partial, (size_t)kMaxPartial * sizeof(double)));
+    STZ_HIP(hipMalloc(&sc, 16 * sizeof(double)));
+    STZ_HIP(hipHostMalloc((void **)&hscal, 4 * sizeof(double), hipHostMallocCoherent));
+    for (int l = 0; l < nl; l++) {
+        BdLevel &L = lv[l];
+        L.m = d[l].first;
+        L.n = d[l].second;
+        const size_t n2 = 2 * (size_t)L.m * L.n;
+        STZ_HIP(hipMalloc(&L.z, n2 * sizeof(double)));
+        STZ_HIP(hipMalloc(&L.b, n2 * sizeof(double)));
+        STZ_HIP(hipMalloc(&L.r, n2 * sizeof(double)));
+        STZ_HIP(hipMalloc(&L.S, 20 * n2 * sizeof(double)));
+    }
+    return STZ_OK;
+}
+
+void BdSolver::free_all() {
+    for (int k = 0; k < kBdVecs; k++) {
+        if (vec[k]) (void)hipFree(vec[k]);
+        vec[k] = nullptr;
+    }
+    for (int l = 0; l < kMaxLevels; l++) {
+        BdLevel &L = lv[l];
+        if (L.z) (void)hipFree(L.z);
+        if (L.b) (void)hipFree(L.b);
+        if (L.r) (void)hipFree(L.r);
+        if (L.S) (void)hipFree(L.S);
+        L = BdLevel{};
+    }
+    if (partial) (void)hipFree(partial);
+    if (sc) (void)hipFree(sc);
+    if (hscal) (void)hipHostFree(hscal);
+    partial = sc = nullptr;
+    hscal = nullptr;
+    nl = 0;
+}
+
+int BdSolver::setup(const BdTop &t, hipStream_t s) {
+    top = t;
+    for (int l = 0; l < nl; l++) {
+        const dim3 g = blocks((long)lv[l].m * lv[l].n);
+        if (l == 0) hipLaunchKernelGGL(rap_kernel<true>, g, dim3(kThreads), 0, s, top, BdLevel{}, lv[0]);
+        else hipLaunchKernelGGL(rap_kernel<false>, g, dim3(kThreads), 0, s, top, lv[l - 1], lv[l]);
+        STZ_HIP(hipGetLastError());
+    }
+    return STZ_OK;
+}
+
+int BdSolver::smooth_top(double *z, const double *b, int sweeps, hipStream_t s) {
+    for (int k = 0; k < sweeps; k++)
+        for (int c = 0; c < 4; c++) {
+            const long cnt = (long)((top_m + 1 - (c & 1)) >> 1) * ((top_n + 1 - (c >> 1)) >> 1);
+            hipLaunchKernelGGL(gs_top_kernel, blocks(cnt), dim3(kThreads), 0, s, top, z, b, c);
+        }
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int BdSolver::smooth_level(int l, int sweeps, hipStream_t s) {
+    const BdLevel &L = lv[l];
+    const size_t lds = 2 * (size_t)L.m * L.n * sizeof(double);
+    if (l == nl - 1 && lds <= 64 * 1024) {
+        hipLaunchKernelGGL(gs_bottom_kernel, dim3(1), dim3(1024), lds, s, L, sweeps);
+        STZ_HIP(hipGetLastError());
+        return STZ_OK;
+    }
+    for (int k = 0; k < sweeps; k++)
+        for (int c = 0; c < 4; c++) {
+            const long cnt = (long)((L.m + 1 - (c & 1)) >> 1) * ((L.n + 1 - (c >> 1)) >> 1);
+            hipLaunchKernelGGL(gs_level_kernel, blocks(cnt), dim3(kThreads), 0, s, L, c);
+        }
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+// one V-cycle with zero start on the top level: z ~= A^-1 rhs
+int BdSolver::vcycle(const double *rhs, double *z, hipStream_t s) {
+    const long n2 = 2L * top_m * top_n;
+    hipLaunchKernelGGL(fill_kernel, blocks(n2), dim3(kThreads), 0, s, z, n2, 0.0);
+    int e = smooth_top(z, rhs, nu1, s);
+    if (e) return e;
+    if (nl > 0) {
+        double *r0 = vec[kTmp];
+        hipLaunchKernelGGL(resid_top_kernel, blocks((long)top_m * top_n), dim3(kThreads), 0, s, top, z, rhs, r0,
+                           (double *)nullptr);
+        const double *rf = r0;
+        int fm = top_m, fn = top_n;
+        for (int l = 0; l < nl; l++) {
+            BdLevel &L = lv[l];
+            hipLaunchKernelGGL(restrict_kernel, blocks((long)L.m * L.n), dim3(kThreads), 0, s, rf, fm, fn, L.b, L.m,
+                               L.n);
+            const long c2 = 2L * L.m * L.n;
+            hipLaunchKernelGGL(fill_kernel, blocks(c2), dim3(kThreads), 0, s, L.z, c2, 0.0);
+            if (l == nl - 1) break;
+            if ((e = smooth_level(l, nu1, s))) return e;
+            hipLaunchKernelGGL(resid_level_kernel, blocks((long)L.m * L.n), dim3(kThreads), 0, s, L);
+            rf = L.r;
+            fm = L.m;
+            fn = L.n;
+        }
+        if ((e = smooth_level(nl - 1, 2 * bottom_half, s))) return e;
+        for (int l = nl - 1; l >= 0; l--) {
+            const BdLevel &C = lv[l];
+            double *zf = l == 0 ? z : lv[l - 1].z;
+            const int fm2 = l == 0 ? top_m : lv[l - 1].m, fn2 = l == 0 ? top_n : lv[l - 1].n;
+            hipLaunchKernelGGL(prolong_kernel, blocks((long)fm2 * fn2), dim3(kThreads), 0, s, C.z, C.m, C.n, zf,
+                               fm2, fn2);
+            if (l > 0 && (e = smooth_level(l - 1, nu2, s))) return e;
+        }
+        STZ_HIP(hipGetLastError());
+    }
+    return smooth_top(z, rhs, nu2, s);
+}
+
+int BdSolver::dot(const double *a, const double *b, int slot, int op, int host_slot, hipStream_t s) {
+    const long n2 = 2L * top_m * top_n;
+    const int nb = (int)std::min<long>((n2 + kThreads - 1) / kThreads, kMaxPartial);
+    hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(kThreads), 0, s, a, b, n2, partial);
+    hipLaunchKernelGGL(finish_dot_kernel, dim3(1), dim3(kThreads), 0, s, partial, nb, sc, slot, op,
+                       host_slot >= 0 ? hscal + host_slot : (volatile double *)nullptr);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+// BiCGStab (van der Vorst 1992) with one V-cycle as right preconditioner.
+// When the shadow residual becomes orthogonal to the residual (the box-edge
+// rows are solved exactly by the first sweep, so a residual that starts
+// there loses all overlap with it) the iteration restarts from the current
+// residual.
+int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_solve_info &info, hipStream_t s) {
+    const long mn = (long)top_m * top_n, n2 = 2 * mn;
+    const dim3 g = blocks(n2), b1(kThreads);
+    double *r = vec[kR], *rh = vec[kRhat], *p = vec[kP], *v = vec[kV], *sv = vec[kS], *t = vec[kT],
+           *ph = vec[kPhat], *sh = vec[kShat];
+    info = stz_solve_info{};
+    info.tolerance = acc;
+    int e;
+    // restart: r = b - A x, r_hat = r, rho = (r,r)
+    auto restart = [&](bool recompute, double &rr) -> int {
+        if (recompute)
+            hipLaunchKernelGGL(resid_top_kernel, blocks(mn), b1, 0, s, top, x, b, r, (double *)nullptr);
+        STZ_HIP(hipMemcpyAsync(rh, r, n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+        int e2 = dot(r, r, 0, 0, 0, s);
+        if (e2) return e2;
+        STZ_HIP(hipStreamSynchronize(s));
+        rr = *hscal;
+        return STZ_OK;
+    };
+    double rr = 0, rhr = 0, rh2 = 0;
+    if ((e = restart(true, rr))) return e;
+    rh2 = rr;
+    double res = rr / mn;
+    info.residual = res;
+    if (std::isnan(res)) {
+        info.status = STZ_ERR_NAN;
+        return STZ_OK;
+    }
+    if (res <= acc) {
+        info.status = STZ_OK;
+        return STZ_OK;
+    }
+    bool first = true;
+    for (int total = 1; total <= max_iter; total++) {
+        hipLaunchKernelGGL(bicg_p_kernel, g, b1, 0, s, p, r, v, sc, n2, first ? 1 : 0);
+        first = false;
+        if ((e = vcycle(p, ph, s))) return e;
+        hipLaunchKernelGGL(apply_top_kernel, blocks(mn), b1, 0, s, top, ph, v);
+        if ((e = dot(rh, v, 4, 2, -1, s))) return e;
+        hipLaunchKernelGGL(bicg_s_kernel, g, b1, 0, s, sv, r, v, sc, n2);
+        if ((e = vcycle(sv, sh, s))) return e;
+        hipLaunchKernelGGL(apply_top_kernel, blocks(mn), b1, 0, s, top, sh, t);
+        if ((e = dot(t, sv, 5, 0, -1, s))) return e;
+        if ((e = dot(t, t, 6, 3, -1, s))) return e;
+        hipLaunchKernelGGL(bicg_x_kernel, g, b1, 0, s, x, r, ph, sh, sv, t, sc, n2);
+        STZ_HIP(hipGetLastError());
+        info.cycles = info.total_cycles = 2 * total;
+        if ((e = dot(r, r, 7, 0, 0, s))) return e;      // (r,r) -> host[0]
+        if ((e = dot(rh, r, 0, 0, 1, s))) return e;     // next rho -> sc[0], host[1]
+        STZ_HIP(hipStreamSynchronize(s));
+        rr = hscal[0];
+        rhr = hscal[1];
+        res = rr / mn;
+        info.residual = res;
+        if (std::isnan(res)) {
+            info.status = STZ_ERR_NAN;
+            return STZ_OK;
+        }
+        bool again = false;
+        if (res <= acc) {
+            // confirm with the true residual
+            if ((e = restart(true, rr))) return e;
+            res = rr / mn;
+            info.residual = res;
+            info.refinements++;
+            if (res <= acc) {
+                info.status = STZ_OK;
+                return STZ_OK;
+            }
+            again = true;
+        } else if (!(std::fabs(rhr) > 1e-12 * std::sqrt(rh2 * rr))) {
+            if ((e = restart(false, rr))) return e;
+            again = true;
+        }
+        if (again) {
+            rh2 = rr;
+            first = true;
+        }
+    }
+    info.status = STZ_ERR_NOT_CONVERGED;
+    return STZ_OK;
+}
+
+int BdSolver::l2_error(const double *x, const double *b, double *out, hipStream_t s) {
+    const long mn = (long)top_m * top_n;
+    double *r = vec[kTmp];
+    hipLaunchKernelGGL(resid_top_kernel, blocks(mn), dim3(kThreads), 0, s, top, x, b, r, (double *)nullptr);
+    int e = dot(r, r, 7, 0, 0, s);
+    if (e) return e;
+    STZ_HIP(hipStreamSynchronize(s));
+    *out = *hscal / mn;
+    return STZ_OK;
+}
+
+}  // namespace stz

@@ -1,0 +1,120 @@
+// stz_law.h -- the STZ plasticity law (sb/main/stz_model.hh:9-207) and the
+// adaptive plastic substepping (shear_sim.cc:513-548, bd_sim.cc:693-730),
+// shared by the periodic-shear and free-boundary kernels.  Expressions keep
+// the reference's evaluation order; FMA contraction is off.
+#pragma once
+
+#include "device.h"
+
+#pragma clang fp contract(off)
+
+namespace stz {
+
+// ---------------------------------------------------------------------------
+// STZ law (stz_model.hh:9-207)
+// ---------------------------------------------------------------------------
+
+__device__ static double dpl_common(const stz_model_params &mp, double sbar, double chi, double &dchi1,
+                                    double &dchi2, double sC1, double sC2, double sC, double sT) {
+    // stz_model.hh:30-47
+    const double s0 = 1;
+    double sbart = sbar > 1e-12 ? sbar : 1e-12, bfac = exp(-1 / chi),
+           pM = 1 + sbart * sT / s0 + mp.rho / (2 * sC2),
+           M = s0 / (2 * sbart) * (pM - sqrt(pM * pM - 4 * sbart * sT / s0)),
+           Dpl = bfac * sC * (sT - M) / mp.tau0,
+           Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
+    bfac /= mp.tau0 * mp.c0;
+    dchi1 = bfac * Gamma;
+    dchi2 = bfac * mp.kappa * sC1 * mp.rho;
+    return Dpl;
+}
+
+__device__ static double dplastic(const stz_model_params &mp, double thetainv, double sbar, double chi,
+                                  double &dchi1, double &dchi2) {
+    const double s0 = 1;
+    switch (mp.kind) {
+    case STZ_MODEL_LINEAR: {  // stz_model.hh:58-67
+        double sC1 = exp(-mp.Delta * thetainv);
+        double sC2 = cosh(mp.Omegaeps0 * sbar * thetainv), sC = sC1 * sC2,
+               sT = tanh(mp.Omegaeps0 * sbar * thetainv);
+        return dpl_common(mp, sbar, chi, dchi1, dchi2, sC1, sC2, sC, sT);
+    }
+    case STZ_MODEL_NONLINEAR: {  // stz_model.hh:76-87
+        double x = mp.Omegaeps0 * sbar / mp.Delta, sC1 = exp(-mp.Delta * thetainv),
+               sC = exp(-mp.Delta * cosh(x) * thetainv) * cosh(mp.Delta * sinh(x) * thetainv), sC2 = sC / sC1,
+               sT = tanh(mp.Delta * sinh(x) * thetainv);
+        return dpl_common(mp, sbar, chi, dchi1, dchi2, sC1, sC2, sC, sT);
+    }
+    case STZ_MODEL_NONLINEAR_ATHERMAL: {  // stz_model.hh:179-206
+        dchi2 = 0;
+        if (sbar < 1) { dchi1 = 0; return 0; }
+        double x = mp.Omegaeps0 * sbar / mp.Delta,
+               sC = 0.5 * (exp(-mp.Delta * thetainv * exp(x)) + exp(-mp.Delta * thetainv * exp(-x))),
+               sT = tanh(mp.Delta * sinh(x) * thetainv);
+        double bfac = exp(-1 / chi), Dpl = bfac * sC * (sT - s0 / sbar) / mp.tau0,
+               Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
+        dchi1 = bfac * Gamma / (mp.tau0 * mp.c0);
+        return Dpl;
+    }
+    default: {  // linear athermal, stz_model.hh:103-151
+        double tmp, sC, sT;
+        dchi2 = 0;
+        if (sbar < 1) { dchi1 = 0; return 0; }
+        if (mp.t_model == -1 || mp.Delta >= mp.Omegaeps0 * sbar)
+            sC = 0.5 * (exp((mp.Omegaeps0 * sbar - mp.Delta) * thetainv) +
+                        exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv));
+        else switch (mp.t_model) {
+            case 0: sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) + 1); break;
+            case 1:
+                tmp = mp.Omegaeps0 * sbar / mp.Delta;
+                sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) + sqrt(0.5 * (1 + tmp * tmp)));
+                break;
+            case 2:
+                tmp = mp.Omegaeps0 * sbar / mp.Delta;
+                sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) + (0.5 * (1 + tmp * tmp)));
+                break;
+            case 3: sC = mp.t_icpt + sbar * mp.t_grad; break;
+            case 4:
+                sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) +
+                            (mp.Omegaeps0 * sbar - mp.Delta) * thetainv + 1);
+                break;
+            default: sC = 0;
+        }
+        sT = mp.t_model != 3 ? tanh(mp.Omegaeps0 * sbar * thetainv) : 1;
+        double bfac = exp(-1 / chi), Dpl = bfac * sC * (sT - s0 / sbar) / mp.tau0,
+               Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
+        dchi1 = bfac * Gamma / (mp.tau0 * mp.c0);
+        return Dpl;
+    }
+    }
+}
+
+// shear_sim.cc:513-548 (adaptive_plastic_term); trips counts loop iterations
+template <class C>
+__device__ static double plastic_term(const C &c, double sbar, double &chiv, double &ddev, double dt,
+                                      int &trips) {
+    const double small_dev_cutoff = 1e-12;
+    const double theta = 0;
+    bool adapt = true;
+    int t = 0;
+    double dtl = dt, osbar = sbar, dplas, dchi1, dchi2, adt;
+    if (sbar < small_dev_cutoff) { ddev = 0; trips = 0; return 0; }
+    do {
+        dplas = 2 * c.t_scale * c.mu * dplastic(c.stz, c.thetainv, sbar, chiv, dchi1, dchi2);
+        if (fabs(dtl * dplas) > c.adapt_fac) {
+            adt = c.adapt_fac / fabs(dplas);
+            dtl -= adt;
+        } else {
+            adapt = false;
+            adt = dtl;
+        }
+        sbar -= adt * dplas;
+        chiv += (dchi1 * (c.chi_inf - chiv) + dchi2 * (theta - chiv)) * c.t_scale * adt;
+        t++;
+    } while (adapt);
+    trips = t;
+    ddev = osbar - sbar;
+    return ddev / osbar;
+}
+
+}  // namespace stz

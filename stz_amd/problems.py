@@ -19,6 +19,8 @@ Choices the reference leaves to the driver, recorded here:
 """
 import math
 
+import numpy as np
+
 TZ = 21000.0
 KB = 1.3806503e-23          # shear_small.cc:14
 S_Y = 0.85e9                # Pa
@@ -78,3 +80,66 @@ def describe(params):
 
 
 assert math.isclose(simple_shear(8)["stz"]["Omegaeps0"], 0.2638, rel_tol=1e-3)
+
+
+# ---------------------------------------------------------------------------
+# Free-boundary workloads (bd_sim).  The body is where phi < 0.
+# ---------------------------------------------------------------------------
+
+def stz_bd_test():
+    """The STZ law of the reference's free-boundary driver bd_test.cc:26-38
+    (Omega*eps0 with 1.1e9 in place of s_y, bd_test.cc:38)."""
+    d = stz_linear_athermal()
+    d["Omegaeps0"] = OMEGA * 1e-30 * EPS0 * 1.1e9 / (TZ * KB)
+    d["t_grad"] = 0.5 * d["Omegaeps0"] / d["Delta"]
+    return d
+
+
+# bd_test.cc:47-51: Vitreloy 1, E = 101 GPa, nu = 0.35, s_y = 0.85 GPa
+E_BD, NU_BD, SY_BD = 101.0, 0.35, 0.85
+K_BD = E_BD / (3 * (1 - 2 * NU_BD) * SY_BD)
+MU_BD = E_BD / (2 * (1 + NU_BD) * SY_BD)
+
+
+def bd_params(m, n=None, box=(-1.0, 1.0, -1.0, 1.0), viscosity=0.02,
+              filter_stress=0.001, t_scale=1.0, adapt_fac=0.002, chi_len=0.0):
+    """bd_sim constructor arguments with the material of bd_test.cc.
+
+    The reference's free-boundary runs all use these moduli (K/mu = 3).  With
+    the stiffer simple-shear set (K/mu = 7.2) the free-surface rows of
+    qs_multi_bd make Gauss-Seidel V-cycles diverge on plate geometries, the
+    reference's as well as this build's, so the free-boundary workloads keep
+    the reference's material."""
+    n = m if n is None else n
+    ax, bx, ay, by = box
+    return {
+        "m": m, "n": n, "ax": ax, "bx": bx, "ay": ay, "by": by,
+        "mu": MU_BD, "K": K_BD, "viscosity": viscosity, "chi_len": chi_len,
+        "filter_stress": filter_stress, "t_scale": t_scale, "tmult": 0.5,
+        "adapt_fac": adapt_fac, "stz": stz_bd_test(),
+    }
+
+
+def plate_hole_phi(m, n=None, box=(-1.0, 1.0, -1.0, 1.0), half_height=0.9,
+                   radius=None):
+    """Level set of a plate |y| < half_height with a circular hole of radius
+    0.2 L (L = box width) at the centre: phi = max(|y| - h, R - r) on the
+    cell centres, negative inside the material."""
+    n = m if n is None else n
+    ax, bx, ay, by = box
+    L = bx - ax
+    R = 0.2 * L if radius is None else radius
+    dx, dy = (bx - ax) / m, (by - ay) / n
+    x = ax + (np.arange(m) + 0.5) * dx - 0.5 * (ax + bx)
+    y = ay + (np.arange(n) + 0.5) * dy - 0.5 * (ay + by)
+    X, Y = np.meshgrid(x, y)
+    return np.maximum(np.abs(Y) - half_height, R - np.sqrt(X * X + Y * Y))
+
+
+# C3: plate with a hole pulled apart by grips on the left and right edges at
+# nominal strain rate 1e-4 (grip speed 1e-4 * L/2), chi = 580 K + 30 K
+# Gaussian field with correlation length 4 cells
+BD_WORKLOADS = {
+    "C3": dict(N=4096, steps=2000, chi_mean=580.0, chi_std=30.0, corr=4.0,
+               strain_rate=1e-4),
+}
