@@ -38,6 +38,7 @@ def test_struct_layout_matches_header(tmp_path):
         "stz_sim_params": _lib.SimParams,
         "stz_mg_config": _lib.MGConfig,
         "stz_solve_info": _lib.SolveInfo,
+        "stz_bd_params": _lib.BdParams,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void) {"]
     for cname, py in structs.items():
@@ -57,6 +58,9 @@ def test_struct_layout_matches_header(tmp_path):
     assert C.sizeof(pyoracle.SimParams) == C.sizeof(_lib.SimParams)
     for fname, _ in _lib.SimParams._fields_:
         assert getattr(pyoracle.SimParams, fname).offset == getattr(_lib.SimParams, fname).offset
+    assert C.sizeof(pyoracle.BdParams) == C.sizeof(_lib.BdParams)
+    for fname, _ in _lib.BdParams._fields_:
+        assert getattr(pyoracle.BdParams, fname).offset == getattr(_lib.BdParams, fname).offset
 
 
 def test_no_device_fails_loudly():
@@ -64,6 +68,15 @@ def test_no_device_fails_loudly():
         pytest.skip("a device is visible")
     with pytest.raises(stz_amd.StzError) as e:
         stz_amd.ShearSim(stz_amd.problems.simple_shear(16))
+    assert e.value.code == _lib.ERR_NO_DEVICE
+
+
+def test_no_device_fails_loudly_free_boundary():
+    if stz_amd.device_count() > 0:
+        pytest.skip("a device is visible")
+    from stz_amd.bd import BdSim
+    with pytest.raises(stz_amd.StzError) as e:
+        BdSim(stz_amd.problems.bd_params(16))
     assert e.value.code == _lib.ERR_NO_DEVICE
 
 
@@ -77,3 +90,8 @@ def test_bad_arguments_rejected_before_device_use():
     assert lib.stz_sim_create(C.byref(p), 7, C.byref(h)) == _lib.ERR_ARG
     assert lib.stz_sim_destroy(None) == _lib.OK
     assert lib.stz_sim_set_time(None, 1.0) == _lib.ERR_ARG
+    pb = _lib.make_bd_params(stz_amd.problems.bd_params(4))
+    assert lib.stz_bd_create(C.byref(pb), C.byref(h)) == _lib.ERR_ARG
+    assert b"m >= 8" in lib.stz_last_error()
+    assert lib.stz_bd_destroy(None) == _lib.OK
+    assert lib.stz_bd_step_quasistatic(None, 1.0, 1) == _lib.ERR_ARG
