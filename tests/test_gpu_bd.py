@@ -82,10 +82,14 @@ def test_extrapolation(plate):
         a, b = g.get_cell(k), plate["post_" + k]
         assert rel_err(a, b) < 1e-13, k
         assert (a != b).sum() <= 0.01 * outside, k
+    # nodes: the visiting order comes from the cells' phi, and the plate's
+    # grid-aligned flat surfaces make whole rows of exact ties, so a few
+    # per cent of the extrapolated nodes differ in their last bits
+    outside_nodes = int((plate["build_nstatus"] == 7).sum())
     for k in NODES:
         a, b = g.get_node(k), plate["post_" + k]
         assert rel_err(a, b) < 1e-14, k
-        assert (a != b).sum() <= 4, k
+        assert (a != b).sum() <= 0.05 * outside_nodes, k
 
 
 def test_operator_rows(plate):
