@@ -350,6 +350,15 @@ int stz_bd_last_solve(stz_bd *b, stz_solve_info *info);
  * cl cc cr ul uc ur, each a b c d row-major) and the source.  Any pointer
  * may be NULL. */
 int stz_bd_interface_velocity(stz_bd *b, double *vp);
+/* Solver settings: pre/post Gauss-Seidel sweeps per level of the V-cycle
+ * preconditioner (default 1, 1 as tgmg.hh:439), sweeps on its coarsest
+ * level (20), the
+ * BiCGStab iteration cap (500) and the mean-square residual threshold
+ * (1e-20, qs_multi_bd.cc:16). */
+int stz_bd_set_solver(stz_bd *b, int nu1, int nu2, int bottom_sweeps, int max_iter, double tol);
+/* Work counts of the last step: out4 = {march rounds (both sides), cell
+ * extrapolation waves, node extrapolation waves, BiCGStab iterations}. */
+int stz_bd_step_stats(stz_bd *b, int32_t *out4);
 int stz_bd_operator(stz_bd *b, double dt, int32_t *row_type, double *blocks36, double *src);
 
 /* Per-node body force (interleaved vec2, (m+1)*(n+1) nodes; NULL removes

@@ -160,6 +160,21 @@ class BdSim:
                 "vcycles": info.total_cycles, "residual": info.residual,
                 "tolerance": info.tolerance}
 
+    def set_solver(self, nu1=1, nu2=1, bottom_sweeps=20, max_iter=500, tol=1e-20):
+        check(self._lib.stz_bd_set_solver(self._h, int(nu1), int(nu2), int(bottom_sweeps), int(max_iter),
+                                          float(tol)))
+
+    def step_stats(self):
+        a = np.zeros(4, np.int32)
+        check(self._lib.stz_bd_step_stats(self._h, a))
+        return dict(zip(("march_rounds", "cell_waves", "node_waves", "bicgstab_iterations"), a.tolist()))
+
+    def synchronize(self):
+        check(self._lib.stz_bd_synchronize(self._h))
+
+    def set_stream(self, stream):
+        check(self._lib.stz_bd_set_stream(self._h, C.c_void_p(stream)))
+
     # ---- diagnostics of single pieces
     def interface_velocity(self):
         a = np.zeros(self.m * self.n)

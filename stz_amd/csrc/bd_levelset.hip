@@ -29,6 +29,7 @@
 // the same.
 #include "bd.h"
 
+#include <algorithm>
 #include <climits>
 
 #pragma clang fp contract(off)
@@ -871,22 +872,44 @@ __global__ void __launch_bounds__(kThreads) node_status_kernel(BdConst c, const 
 // done[]: 0 source (inside band), -1 pending (outside band), INT_MAX not in
 // the band; a finished cell holds the wave that computed it
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) xc_prepare_kernel(BdConst c, const int8_t *__restrict__ st, int *done,
-                                                              int *counter) {
-    const int ij = blockIdx.x * kThreads + threadIdx.x;
-    if (ij == 0) counter[0] = counter[1] = counter[2] = 0;
-    if (ij >= c.m * c.n) return;
-    const int s = code(st[ij]);
-    done[ij] = (s >= 2 && s <= 3) ? 0 : ((s >= 4 && s <= 5) ? -1 : INT_MAX);
+__global__ void zero_counters_kernel(int *counter) {
+    if (threadIdx.x < 4) counter[threadIdx.x] = 0;
 }
 
-// counter[w&1] counts the cells wave w leaves pending; the wave clears the
-// other slot for the next one; counter[2] counts failed fits
+// keep the still pending entries of a list (counter[3] = new length)
+__global__ void __launch_bounds__(kThreads) compact_kernel(const int *__restrict__ done,
+                                                           const int *__restrict__ list, int count,
+                                                           int *__restrict__ out, int *counter) {
+    const int q = blockIdx.x * kThreads + threadIdx.x;
+    if (q >= count) return;
+    const int k = list[q];
+    if (done[k] == -1) out[atomicAdd(&counter[3], 1)] = k;
+}
+
+__global__ void zero_one_kernel(int *p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+// the pending cells are also listed (counter[3] entries of list) so that the
+// waves run over them only
+__global__ void __launch_bounds__(kThreads) xc_prepare_kernel(BdConst c, const int8_t *__restrict__ st, int *done,
+                                                              int *counter, int *list) {
+    const int ij = blockIdx.x * kThreads + threadIdx.x;
+    if (ij >= c.m * c.n) return;
+    const int s = code(st[ij]);
+    const int d = (s >= 2 && s <= 3) ? 0 : ((s >= 4 && s <= 5) ? -1 : INT_MAX);
+    done[ij] = d;
+    if (d == -1) list[atomicAdd(&counter[3], 1)] = ij;
+}
+
+// one wave over the listed pending cells; counter[2] counts failed fits
 __global__ void __launch_bounds__(kThreads) xc_wave_kernel(BdConst c, const double *__restrict__ phi, BdCells f,
-                                                           int *done, int wave, int *counter) {
-    const int ij = blockIdx.x * kThreads + threadIdx.x, m = c.m, n = c.n;
-    if (ij == 0) counter[(wave + 1) & 1] = 0;
-    if (ij >= m * n || done[ij] != -1) return;
+                                                           int *done, int wave, int *counter,
+                                                           const int *__restrict__ list, int count) {
+    const int q = blockIdx.x * kThreads + threadIdx.x, m = c.m, n = c.n;
+    if (q >= count) return;
+    const int ij = list[q];
+    if (done[ij] != -1) return;
     const int i = ij % m, j = ij / m;
     const double ph = phi[ij];
     double s, sx, sy, sxx, sxy, syy, det = 0, p[15];
@@ -898,10 +921,7 @@ __global__ void __launch_bounds__(kThreads) xc_wave_kernel(BdConst c, const doub
             for (int di = ilo; di <= ihi; di++) {
                 const int k = ij + dj * m + di, d = done[k];
                 if (d == INT_MAX || !(phi[k] < ph)) continue;
-                if (d < 0 || d >= wave) {
-                    atomicAdd(&counter[wave & 1], 1);
-                    return;
-                }
+                if (d < 0 || d >= wave) return;
             }
         s = sx = sy = sxx = sxy = syy = 0;
         for (int q = 0; q < 15; q++) p[q] = 0;
@@ -975,9 +995,8 @@ struct NodeKey {
 __global__ void __launch_bounds__(kThreads) xn_prepare_kernel(BdConst c, const double *__restrict__ phi,
                                                               const int8_t *__restrict__ st,
                                                               const int8_t *__restrict__ nst, int *done,
-                                                              double *keyd, int *keyi, int *counter) {
+                                                              double *keyd, int *keyi, int *counter, int *list) {
     const int k = blockIdx.x * kThreads + threadIdx.x, mne = c.me * c.ne;
-    if (k == 0) counter[0] = counter[1] = counter[2] = 0;
     if (k >= mne) return;
     const int ie = k % c.me, je = k / c.me, m = c.m, n = c.n;
     const double spt = -1.2 * sqrt(c.dx * c.dx + c.dy * c.dy);
@@ -1009,6 +1028,7 @@ __global__ void __launch_bounds__(kThreads) xn_prepare_kernel(BdConst c, const d
         return;
     }
     done[k] = -1;
+    list[atomicAdd(&counter[3], 1)] = k;
     keyd[2 * (size_t)k] = best;
     keyi[2 * (size_t)k] = bidx;
     keyi[2 * (size_t)k + 1] = corner;
@@ -1026,10 +1046,12 @@ __device__ __forceinline__ bool key_less(const double *keyd, const int *keyi, in
 
 __global__ void __launch_bounds__(kThreads) xn_wave_kernel(BdConst c, BdNodes f, int *done,
                                                            const double *__restrict__ keyd,
-                                                           const int *__restrict__ keyi, int wave, int *counter) {
-    const int k = blockIdx.x * kThreads + threadIdx.x, me = c.me, ne = c.ne;
-    if (k == 0) counter[(wave + 1) & 1] = 0;
-    if (k >= me * ne || done[k] != -1) return;
+                                                           const int *__restrict__ keyi, int wave, int *counter,
+                                                           const int *__restrict__ list, int count) {
+    const int q0 = blockIdx.x * kThreads + threadIdx.x, me = c.me, ne = c.ne;
+    if (q0 >= count) return;
+    const int k = list[q0];
+    if (done[k] != -1) return;
     const int ie = k % me, je = k / me;
     const double sp = keyd[2 * (size_t)k + 1];
     const int r = 2;
@@ -1047,10 +1069,7 @@ __global__ void __launch_bounds__(kThreads) xn_wave_kernel(BdConst c, BdNodes f,
     };
     for (int dj = jlo; dj <= jhi; dj++)
         for (int di = ilo; di <= ihi; di++)
-            if (used(k + dj * me + di) < 0) {
-                atomicAdd(&counter[wave & 1], 1);
-                return;
-            }
+            if (used(k + dj * me + di) < 0) return;
     double s = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0, p[12];
     for (int q = 0; q < 12; q++) p[q] = 0;
     for (int dj = jlo; dj <= jhi; dj++)
@@ -1176,47 +1195,79 @@ int bd_ls_node_status(const BdConst &c, const double *phi, const int8_t *st, int
     return STZ_OK;
 }
 
+// Waves in batches that double from 4 to 64 launches; after each batch the
+// list keeps only the cells still pending (so late waves, when a few long
+// dependency chains remain, touch a few cells).  list has room for twice
+// the longest list.
 template <class Launch>
-static int run_waves(Launch launch, const int *counter, volatile int *hflag, int *waves, const char *what,
-                     hipStream_t s) {
-    const int batch = 4, max_waves = 8192;
-    int w = 1;
+static int run_waves(Launch launch, const int *done, int *counter, int *list, int count, volatile int *hflag,
+                     int *waves, const char *what, hipStream_t s) {
+    const int max_waves = 1 << 16;
+    int w = 1, batch = 4;
+    int *cur = list, *nxt = list + count;
     for (;;) {
-        for (int q = 0; q < batch; q++, w++) launch(w);
+        for (int q = 0; q < batch; q++, w++) launch(w, cur, count);
+        hipLaunchKernelGGL(zero_one_kernel, dim3(1), dim3(64), 0, s, counter + 3);
+        hipLaunchKernelGGL(compact_kernel, blocks(count), dim3(kThreads), 0, s, done, cur, count, nxt, counter);
+        hipLaunchKernelGGL(flag_kernel, dim3(1), dim3(64), 0, s, counter + 3, hflag);
         STZ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(flag_kernel, dim3(1), dim3(64), 0, s, counter + ((w - 1) & 1), hflag);
         STZ_HIP(hipStreamSynchronize(s));
-        if (*hflag == 0) break;
+        count = *hflag;
+        if (count == 0) break;
         if (w >= max_waves) {
             set_error("%s extrapolation did not finish in %d waves", what, max_waves);
             return STZ_ERR_NOT_CONVERGED;
         }
+        std::swap(cur, nxt);
+        batch = batch < 64 ? 2 * batch : 64;
     }
     *waves = w - 1;
     return STZ_OK;
 }
 
-// counter: [0],[1] pending counts by wave parity, [2] failed fits
+static int pending_count(const int *counter, volatile int *hflag, hipStream_t s, int *count) {
+    hipLaunchKernelGGL(flag_kernel, dim3(1), dim3(64), 0, s, counter + 3, hflag);
+    STZ_HIP(hipStreamSynchronize(s));
+    *count = *hflag;
+    return STZ_OK;
+}
+
+// counter: [0],[1] pending counts by wave parity, [2] failed fits, [3] list length
 int bd_extrap_cells(const BdConst &c, const double *phi, const int8_t *st, const BdCells &f, int *done,
-                    int *counter, volatile int *hflag, int *waves, hipStream_t s) {
+                    int *counter, int *list, volatile int *hflag, int *waves, hipStream_t s) {
     const dim3 g = blocks((long)c.m * c.n), b(kThreads);
-    hipLaunchKernelGGL(xc_prepare_kernel, g, b, 0, s, c, st, done, counter);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counter);
+    hipLaunchKernelGGL(xc_prepare_kernel, g, b, 0, s, c, st, done, counter, list);
     STZ_HIP(hipGetLastError());
+    int count = 0, e = pending_count(counter, hflag, s, &count);
+    if (e) return e;
+    *waves = 0;
+    if (count == 0) return STZ_OK;
     return run_waves(
-        [&](int w) { hipLaunchKernelGGL(xc_wave_kernel, g, b, 0, s, c, phi, f, done, w, counter); }, counter,
-        hflag, waves, "cell", s);
+        [&](int w, const int *l, int n) {
+            hipLaunchKernelGGL(xc_wave_kernel, blocks(n), b, 0, s, c, phi, f, done, w, counter, l, n);
+        },
+        done, counter, list, count, hflag, waves, "cell", s);
 }
 
 int bd_extrap_nodes(const BdConst &c, const double *phi, const int8_t *st, const int8_t *nst, const BdNodes &f,
-                    int *done, double *key, int *counter, volatile int *hflag, int *waves, hipStream_t s) {
+                    int *done, double *key, int *counter, int *list, volatile int *hflag, int *waves,
+                    hipStream_t s) {
     const long mne = (long)c.me * c.ne;
     const dim3 g = blocks(mne), b(kThreads);
     int *keyi = reinterpret_cast<int *>(key + 2 * mne);
-    hipLaunchKernelGGL(xn_prepare_kernel, g, b, 0, s, c, phi, st, nst, done, key, keyi, counter);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counter);
+    hipLaunchKernelGGL(xn_prepare_kernel, g, b, 0, s, c, phi, st, nst, done, key, keyi, counter, list);
     STZ_HIP(hipGetLastError());
+    int count = 0, e = pending_count(counter, hflag, s, &count);
+    if (e) return e;
+    *waves = 0;
+    if (count == 0) return STZ_OK;
     return run_waves(
-        [&](int w) { hipLaunchKernelGGL(xn_wave_kernel, g, b, 0, s, c, f, done, key, keyi, w, counter); },
-        counter, hflag, waves, "node", s);
+        [&](int w, const int *l, int n) {
+            hipLaunchKernelGGL(xn_wave_kernel, blocks(n), b, 0, s, c, f, done, key, keyi, w, counter, l, n);
+        },
+        done, counter, list, count, hflag, waves, "node", s);
 }
 
 }  // namespace stz

@@ -17,10 +17,14 @@ struct BdTop {
     double Kmx, Kmy, mx, my, qK, cxx, cyy, bcs, ocs, ncs;
 };
 
+// Coarse level.  Rows whose fine rows were all interior rows share one
+// stencil Sc (type -1); the others keep their own 40 doubles in S (type 0).
 struct BdLevel {
     int m = 0, n = 0;
     double *z = nullptr, *b = nullptr, *r = nullptr;
-    double *S = nullptr;   // 40 per node
+    double *S = nullptr;   // 40 per node, read only where type >= 0
+    int *type = nullptr;
+    double *Sc = nullptr;  // 40
 };
 
 enum { kR = 0, kRhat, kP, kV, kS, kT, kPhat, kShat, kTmp, kBdVecs };
@@ -28,7 +32,7 @@ constexpr int kMaxPartial = 4096;
 
 struct BdSolver {
     int top_m = 0, top_n = 0, nl = 0;
-    int nu1 = 2, nu2 = 2, bottom_half = 10;
+    int nu1 = 1, nu2 = 1, bottom_half = 10;
     BdTop top{};
     BdLevel lv[kMaxLevels];
     double *vec[kBdVecs] = {};

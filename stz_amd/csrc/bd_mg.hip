@@ -92,8 +92,12 @@ __device__ __forceinline__ double top_diag_scale(const BdTop &t, int d) {
     return d == -1 ? 1.0 : (d == -2 ? t.bcs : (d == -3 ? t.ocs : t.ncs));
 }
 
+__device__ __forceinline__ const double *lvl_stencil(const BdLevel &L, int ij) {
+    return L.type[ij] < 0 ? L.Sc : L.S + 40 * (size_t)ij;
+}
+
 __device__ __forceinline__ V2 lvl_offdiag(const BdLevel &L, const double *z, int i, int j) {
-    const double *s = L.S + 40 * (size_t)(i + L.m * j);
+    const double *s = lvl_stencil(L, i + L.m * j);
     double rx = 0, ry = 0;
     for (int k = 0; k < 9; k++) {
         if (k == 4) continue;
@@ -129,7 +133,7 @@ __global__ void __launch_bounds__(kThreads) gs_level_kernel(BdLevel L, int colou
     const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + L.m * j;
     const V2 o = lvl_offdiag(L, L.z, i, j);
     const double rx = L.b[2 * (size_t)ij] - o.x, ry = L.b[2 * (size_t)ij + 1] - o.y;
-    const double *iv = L.S + 40 * (size_t)ij + 36;
+    const double *iv = lvl_stencil(L, ij) + 36;
     L.z[2 * (size_t)ij] = iv[0] * rx + iv[1] * ry;
     L.z[2 * (size_t)ij + 1] = iv[2] * rx + iv[3] * ry;
 }
@@ -147,7 +151,7 @@ __global__ void __launch_bounds__(1024) gs_bottom_kernel(BdLevel L, int sweeps) 
                 const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + L.m * j;
                 const V2 o = lvl_offdiag(L, zs, i, j);
                 const double rx = L.b[2 * ij] - o.x, ry = L.b[2 * ij + 1] - o.y;
-                const double *iv = L.S + 40 * (size_t)ij + 36;
+                const double *iv = lvl_stencil(L, ij) + 36;
                 zs[2 * ij] = iv[0] * rx + iv[1] * ry;
                 zs[2 * ij + 1] = iv[2] * rx + iv[3] * ry;
             }
@@ -203,7 +207,7 @@ __global__ void __launch_bounds__(kThreads) resid_level_kernel(BdLevel L) {
     if (ij >= L.m * L.n) return;
     const int i = ij % L.m, j = ij / L.m;
     const V2 o = lvl_offdiag(L, L.z, i, j);
-    const double *s = L.S + 40 * (size_t)ij + 16;
+    const double *s = lvl_stencil(L, ij) + 16;
     const double zx = L.z[2 * (size_t)ij], zy = L.z[2 * (size_t)ij + 1];
     L.r[2 * (size_t)ij] = L.b[2 * (size_t)ij] - o.x - (s[0] * zx + s[1] * zy);
     L.r[2 * (size_t)ij + 1] = L.b[2 * (size_t)ij + 1] - o.y - (s[2] * zx + s[3] * zy);
@@ -268,36 +272,76 @@ __global__ void __launch_bounds__(kThreads) prolong_kernel(const double *__restr
     zf[2 * (size_t)ij + 1] += sy;
 }
 
+// Sources of fine-level entries for the Galerkin product: the top operator,
+// a stored coarse level, or (for the shared interior stencil) a level whose
+// every row is its interior row
+struct TopSrc {
+    BdTop t;
+    __device__ int m() const { return t.m; }
+    __device__ int n() const { return t.n; }
+    __device__ bool interior(int f) const { return t.rowt[f] == -1; }
+    __device__ void entry(int f, int k, double *A) const { top_entry(t, f, k, A); }
+};
+struct LvlSrc {
+    BdLevel F;
+    __device__ int m() const { return F.m; }
+    __device__ int n() const { return F.n; }
+    __device__ bool interior(int f) const { return F.type[f] < 0; }
+    __device__ void entry(int f, int k, double *A) const {
+        const double *s = lvl_stencil(F, f) + 4 * k;
+        A[0] = s[0]; A[1] = s[1]; A[2] = s[2]; A[3] = s[3];
+    }
+};
+struct TopConstSrc {
+    BdTop t;
+    __device__ int m() const { return t.m; }
+    __device__ int n() const { return t.n; }
+    __device__ bool interior(int) const { return true; }
+    __device__ void entry(int, int k, double *A) const {
+        // rows of type -1 do not depend on the row index
+        BdTop u = t;
+        const int d = -1;
+        u.rowt = &d;
+        top_entry(u, 0, k, A);
+    }
+};
+struct LvlConstSrc {
+    BdLevel F;
+    __device__ int m() const { return F.m; }
+    __device__ int n() const { return F.n; }
+    __device__ bool interior(int) const { return true; }
+    __device__ void entry(int, int k, double *A) const {
+        const double *s = F.Sc + 4 * k;
+        A[0] = s[0]; A[1] = s[1]; A[2] = s[2]; A[3] = s[3];
+    }
+};
+
 // Galerkin coarse block row at coarse node (I,J): sum over fine nodes f
-// with weight P(f,(I,J)) of P(f,.)^T A(f,g) P(g,.), then the inverse of cc
-template <bool TOP>
-__global__ void __launch_bounds__(kThreads) rap_kernel(BdTop t, BdLevel F, BdLevel C) {
-    const int IJ = blockIdx.x * kThreads + threadIdx.x;
-    if (IJ >= C.m * C.n) return;
-    const int I = IJ % C.m, J = IJ / C.m;
-    const int fm = TOP ? t.m : F.m, fn = TOP ? t.n : F.n;
+// with weight P(f,(I,J)) of P(f,.)^T A(f,g) P(g,.), then the inverse of cc.
+// Returns whether every fine row used was an interior row.
+template <class Src>
+__device__ bool rap_node(const Src &src, int cm, int cn, int I, int J, double *o) {
+    const int fm = src.m(), fn = src.n();
     double acc[36];
     for (int q = 0; q < 36; q++) acc[q] = 0;
+    bool all_interior = true;
     int c[2], cx[2], cy[2];
     double w[2], wx[2], wy[2];
     for (int fy = 2 * J - 1; fy <= 2 * J + 1; fy++) {
-        const double wfy = weight_of(parents(fy, fn, C.n, c, w), c, w, J);
+        const double wfy = weight_of(parents(fy, fn, cn, c, w), c, w, J);
         if (wfy == 0) continue;
         for (int fx = 2 * I - 1; fx <= 2 * I + 1; fx++) {
-            const double wfx = weight_of(parents(fx, fm, C.m, c, w), c, w, I);
+            const double wfx = weight_of(parents(fx, fm, cm, c, w), c, w, I);
             if (wfx == 0) continue;
             const int fij = fx + fm * fy;
+            all_interior = all_interior && src.interior(fij);
             for (int k = 0; k < 9; k++) {
                 const int gx = fx + k % 3 - 1, gy = fy + k / 3 - 1;
                 if (gx < 0 || gx >= fm || gy < 0 || gy >= fn) continue;
                 double A[4];
-                if (TOP) top_entry(t, fij, k, A);
-                else {
-                    const double *s = F.S + 40 * (size_t)fij + 4 * k;
-                    A[0] = s[0]; A[1] = s[1]; A[2] = s[2]; A[3] = s[3];
-                }
+                src.entry(fij, k, A);
                 if (A[0] == 0 && A[1] == 0 && A[2] == 0 && A[3] == 0) continue;
-                const int ny = parents(gy, fn, C.n, cy, wy), nx = parents(gx, fm, C.m, cx, wx);
+                const int ny = parents(gy, fn, cn, cy, wy), nx = parents(gx, fm, cm, cx, wx);
                 for (int q = 0; q < ny; q++)
                     for (int p = 0; p < nx; p++) {
                         const int dI = cx[p] - I, dJ = cy[q] - J;
@@ -309,7 +353,6 @@ __global__ void __launch_bounds__(kThreads) rap_kernel(BdTop t, BdLevel F, BdLev
             }
         }
     }
-    double *o = C.S + 40 * (size_t)IJ;
     for (int q = 0; q < 36; q++) o[q] = acc[q];
     const double a = acc[16], b = acc[17], cc = acc[18], d = acc[19];
     const double idet = 1. / (a * d - b * cc);
@@ -317,6 +360,29 @@ __global__ void __launch_bounds__(kThreads) rap_kernel(BdTop t, BdLevel F, BdLev
     o[37] = -b * idet;
     o[38] = -cc * idet;
     o[39] = a * idet;
+    return all_interior;
+}
+
+template <class Src>
+__global__ void __launch_bounds__(kThreads) rap_kernel(Src src, BdLevel C, int shared) {
+    const int IJ = blockIdx.x * kThreads + threadIdx.x;
+    if (IJ >= C.m * C.n) return;
+    double o[40];
+    const bool interior = rap_node(src, C.m, C.n, IJ % C.m, IJ / C.m, o) && shared;
+    C.type[IJ] = interior ? -1 : 0;
+    if (interior) return;
+    double *S = C.S + 40 * (size_t)IJ;
+    for (int q = 0; q < 40; q++) S[q] = o[q];
+}
+
+// the shared interior stencil of a level: the product at a node whose
+// footprint lies in the interior (same arithmetic as rap_kernel)
+template <class Src>
+__global__ void rap_const_kernel(Src src, BdLevel C) {
+    if (threadIdx.x != 0) return;
+    double o[40];
+    (void)rap_node(src, C.m, C.n, 2, 2, o);
+    for (int q = 0; q < 40; q++) C.Sc[q] = o[q];
 }
 
 __global__ void __launch_bounds__(kThreads) fill_kernel(double *p, long n, double v) {
@@ -433,6 +499,8 @@ int BdSolver::init(int tm, int tn) {
         STZ_HIP(hipMalloc(&L.b, n2 * sizeof(double)));
         STZ_HIP(hipMalloc(&L.r, n2 * sizeof(double)));
         STZ_HIP(hipMalloc(&L.S, 20 * n2 * sizeof(double)));
+        STZ_HIP(hipMalloc(&L.type, (size_t)L.m * L.n * sizeof(int)));
+        STZ_HIP(hipMalloc(&L.Sc, 40 * sizeof(double)));
     }
     return STZ_OK;
 }
@@ -448,6 +516,8 @@ void BdSolver::free_all() {
         if (L.b) (void)hipFree(L.b);
         if (L.r) (void)hipFree(L.r);
         if (L.S) (void)hipFree(L.S);
+        if (L.type) (void)hipFree(L.type);
+        if (L.Sc) (void)hipFree(L.Sc);
         L = BdLevel{};
     }
     if (partial) (void)hipFree(partial);
@@ -462,8 +532,17 @@ int BdSolver::setup(const BdTop &t, hipStream_t s) {
     top = t;
     for (int l = 0; l < nl; l++) {
         const dim3 g = blocks((long)lv[l].m * lv[l].n);
-        if (l == 0) hipLaunchKernelGGL(rap_kernel<true>, g, dim3(kThreads), 0, s, top, BdLevel{}, lv[0]);
-        else hipLaunchKernelGGL(rap_kernel<false>, g, dim3(kThreads), 0, s, top, lv[l - 1], lv[l]);
+        // the shared stencil needs a fine footprint of 7x7 nodes
+        const bool cst = (l == 0 ? top_m : lv[l - 1].m) >= 7 && (l == 0 ? top_n : lv[l - 1].n) >= 7;
+        if (l == 0) {
+            hipLaunchKernelGGL(rap_kernel<TopSrc>, g, dim3(kThreads), 0, s, TopSrc{top}, lv[0], cst ? 1 : 0);
+            if (cst) hipLaunchKernelGGL(rap_const_kernel<TopConstSrc>, dim3(1), dim3(64), 0, s, TopConstSrc{top}, lv[0]);
+        } else {
+            hipLaunchKernelGGL(rap_kernel<LvlSrc>, g, dim3(kThreads), 0, s, LvlSrc{lv[l - 1]}, lv[l], cst ? 1 : 0);
+            if (cst)
+                hipLaunchKernelGGL(rap_const_kernel<LvlConstSrc>, dim3(1), dim3(64), 0, s, LvlConstSrc{lv[l - 1]},
+                                   lv[l]);
+        }
         STZ_HIP(hipGetLastError());
     }
     return STZ_OK;

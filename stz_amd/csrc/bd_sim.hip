@@ -25,7 +25,7 @@ struct stz_bd {
     double *vel = nullptr, *src = nullptr, *force = nullptr;
     double *nkey = nullptr;         // node extrapolation keys (3 doubles per node)
     int8_t *st = nullptr, *nst = nullptr;
-    int *rowt = nullptr, *done_c = nullptr, *done_n = nullptr, *counters = nullptr;
+    int *rowt = nullptr, *done_c = nullptr, *done_n = nullptr, *counters = nullptr, *xlist = nullptr;
     double *fmats = nullptr;
     int fcap = 0;
     unsigned long long *dmin = nullptr;
@@ -113,7 +113,7 @@ int advection(stz_bd *b, double dt) {
 // bd_sim::post_process (bd_sim.cc:256-259)
 int post_process(stz_bd *b, bool all) {
     int e, fails = 0;
-    if ((e = bd_extrap_cells(b->c, b->phi, b->st, b->cells(b->cc), b->done_c, b->counters,
+    if ((e = bd_extrap_cells(b->c, b->phi, b->st, b->cells(b->cc), b->done_c, b->counters, b->xlist,
                              (volatile int *)b->hint, &b->waves_c, b->stream)))
         return e;
     if ((e = read_int(b, b->counters + 2, &fails))) return e;
@@ -122,7 +122,7 @@ int post_process(stz_bd *b, bool all) {
         return STZ_ERR_STATE;
     }
     if (all) {
-        if ((e = bd_extrap_nodes(b->c, b->phi, b->st, b->nst, b->nodes(b->nc), b->done_n, b->nkey, b->counters,
+        if ((e = bd_extrap_nodes(b->c, b->phi, b->st, b->nst, b->nodes(b->nc), b->done_n, b->nkey, b->counters, b->xlist,
                                  (volatile int *)b->hint, &b->waves_n, b->stream)))
             return e;
         if ((e = read_int(b, b->counters + 2, &fails))) return e;
@@ -265,6 +265,7 @@ int stz_bd_create(const stz_bd_params *p, stz_bd **out) {
     if (hipMalloc(&b->st, mn) != hipSuccess || hipMalloc(&b->nst, mne) != hipSuccess ||
         hipMalloc(&b->rowt, mne * sizeof(int)) != hipSuccess || hipMalloc(&b->done_c, mn * sizeof(int)) != hipSuccess ||
         hipMalloc(&b->done_n, mne * sizeof(int)) != hipSuccess ||
+        hipMalloc(&b->xlist, 2 * std::max(mn, mne) * sizeof(int)) != hipSuccess ||
         hipMalloc(&b->counters, 16 * sizeof(int)) != hipSuccess ||
         hipMalloc(&b->dmin, sizeof(unsigned long long)) != hipSuccess) {
         set_error("out of device memory");
@@ -297,6 +298,7 @@ int stz_bd_destroy(stz_bd *b) {
     if (b->rowt) (void)hipFree(b->rowt);
     if (b->done_c) (void)hipFree(b->done_c);
     if (b->done_n) (void)hipFree(b->done_n);
+    if (b->xlist) (void)hipFree(b->xlist);
     if (b->counters) (void)hipFree(b->counters);
     if (b->dmin) (void)hipFree(b->dmin);
     if (b->fmats) (void)hipFree(b->fmats);
@@ -367,6 +369,30 @@ int stz_bd_interface_velocity(stz_bd *b, double *vp) {
     return STZ_OK;
 }
 
+int stz_bd_set_solver(stz_bd *b, int nu1, int nu2, int bottom_sweeps, int max_iter, double tol) {
+    CHECK_BD(b);
+    if (nu1 < 0 || nu2 < 0 || nu1 + nu2 < 1 || bottom_sweeps < 1 || max_iter < 1 || !(tol > 0)) {
+        set_error("bad solver settings");
+        return STZ_ERR_ARG;
+    }
+    b->mg.nu1 = nu1;
+    b->mg.nu2 = nu2;
+    b->mg.bottom_half = (bottom_sweeps + 1) / 2;
+    b->max_iter = max_iter;
+    b->acc = tol;
+    return STZ_OK;
+}
+
+int stz_bd_step_stats(stz_bd *b, int32_t *out4) {
+    CHECK_BD(b);
+    if (!out4) { set_error("null argument"); return STZ_ERR_ARG; }
+    out4[0] = b->march_rounds;
+    out4[1] = b->waves_c;
+    out4[2] = b->waves_n;
+    out4[3] = b->last.total_cycles / 2;
+    return STZ_OK;
+}
+
 int stz_bd_operator(stz_bd *b, double dt, int32_t *rowt, double *blocks36, double *src) {
     CHECK_BD(b);
     const BdConst &c = b->c;
@@ -418,10 +444,10 @@ int stz_bd_init_bar(stz_bd *b, int ca) {
     if ((e = bd_init_bar_fields(b->c, ca, b->st, b->nst, b->cells(b->cc), b->nodes(b->nc), b->stream))) return e;
     // bd_sim::extrapolate (bd_sim.hh:162-165), no boundary conditions
     int fails = 0;
-    if ((e = bd_extrap_cells(b->c, b->phi, b->st, b->cells(b->cc), b->done_c, b->counters, (volatile int *)b->hint,
+    if ((e = bd_extrap_cells(b->c, b->phi, b->st, b->cells(b->cc), b->done_c, b->counters, b->xlist, (volatile int *)b->hint,
                              &b->waves_c, b->stream)))
         return e;
-    if ((e = bd_extrap_nodes(b->c, b->phi, b->st, b->nst, b->nodes(b->nc), b->done_n, b->nkey, b->counters,
+    if ((e = bd_extrap_nodes(b->c, b->phi, b->st, b->nst, b->nodes(b->nc), b->done_n, b->nkey, b->counters, b->xlist,
                              (volatile int *)b->hint, &b->waves_n, b->stream)))
         return e;
     if ((e = read_int(b, b->counters + 2, &fails))) return e;
