@@ -350,6 +350,10 @@ int stz_bd_last_solve(stz_bd *b, stz_solve_info *info);
  * cl cc cr ul uc ur, each a b c d row-major) and the source.  Any pointer
  * may be NULL. */
 int stz_bd_interface_velocity(stz_bd *b, double *vp);
+/* The band rebuilt around the unmoved interface (levelset::move_and_smooth
+ * with dt = 0, levelset.hh:159-205): interface cells keep phi, the rest of
+ * the band is marched again; statuses and node statuses follow. */
+int stz_bd_remarch(stz_bd *b);
 /* Solver settings: pre/post Gauss-Seidel sweeps per level of the V-cycle
  * preconditioner (default 1, 1 as tgmg.hh:439), sweeps on its coarsest
  * level (20), the

@@ -421,6 +421,7 @@ class RefBd:
                 "operator": (None, [vp, d, _ip, _dp, _dp]),
                 "interface_velocity": (None, [vp, _dp]),
                 "set_bc_smooth": (None, [vp, d]),
+                "remarch": (None, [vp]),
             }
             f = {}
             for name, (res, args) in sig.items():
@@ -529,3 +530,19 @@ class RefBd:
         a = np.zeros(self.m * self.n)
         self._f["interface_velocity"](self._h, a)
         return a.reshape(self.cell_shape)
+
+    def remarch(self):
+        self._f["remarch"](self._h)
+
+
+def bd_remarch(phi, status, dx, dy, delta):
+    """the round-parallel band rebuild of bd_oracle.c on copies of phi and
+    status; returns (phi, status, rounds)"""
+    lib = C.CDLL(ORACLE_SO)
+    lib.orc_bd_remarch.restype = C.c_int
+    lib.orc_bd_remarch.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _ip]
+    p = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    s = np.ascontiguousarray(status, dtype=np.int32).copy()
+    n, m = p.shape
+    r = lib.orc_bd_remarch(m, n, float(dx), float(dy), float(delta), p.ravel(), s.ravel())
+    return p, s, r

@@ -214,6 +214,15 @@ void ref_bd_interface_velocity(void *h, double *vp) {
     }
 }
 
+// the band rebuilt around an unmoved interface: levelset::move_and_smooth
+// with dt = 0 (levelset.hh:159-205, post_move_update levelset.cc:701-772),
+// then the node statuses
+void ref_bd_remarch(void *h) {
+    bd_sim &s = *static_cast<ref_bd *>(h)->sim;
+    s.ls.move_and_smooth(&s, 0.0, 0.0);
+    s.set_staggered_status();
+}
+
 // levelset::newton at one cell: closest point (x,y) and success flag
 int ref_bd_newton(void *h, int ij, double *xy) {
     levelset &ls = static_cast<ref_bd *>(h)->sim->ls;

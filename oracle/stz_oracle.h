@@ -82,6 +82,12 @@ void orc_compute_strain(orc_sim *s, int corrected);
 void orc_l2_comparison(orc_sim *a, orc_sim *b, double *l2);
 void orc_initialize_chi_bicubic(orc_sim *s, int mm, int nn, const double *arr, double lo, double sca);
 
+/* free-boundary band rebuild as the GPU runs it (bd_oracle.c): the
+ * interface cells (status 3/4) stay, the rest of the band is marched again
+ * in rounds of width delta; phi and st (m*n, row-major) updated in place;
+ * returns the number of rounds */
+int orc_bd_remarch(int m, int n, double dx, double dy, double delta, double *phi, int *st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -176,6 +176,10 @@ class BdSim:
         check(self._lib.stz_bd_set_stream(self._h, C.c_void_p(stream)))
 
     # ---- diagnostics of single pieces
+    def remarch(self):
+        """the band rebuilt around the unmoved interface"""
+        check(self._lib.stz_bd_remarch(self._h))
+
     def interface_velocity(self):
         a = np.zeros(self.m * self.n)
         check(self._lib.stz_bd_interface_velocity(self._h, a))

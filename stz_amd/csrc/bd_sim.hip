@@ -369,6 +369,22 @@ int stz_bd_interface_velocity(stz_bd *b, double *vp) {
     return STZ_OK;
 }
 
+int stz_bd_remarch(stz_bd *b) {
+    CHECK_BD(b);
+    if (!b->have_band) { set_error("no level set: call stz_bd_set_level_set first"); return STZ_ERR_STATE; }
+    const BdConst &c = b->c;
+    int e;
+    STZ_HIP(hipMemsetAsync(b->vp, 0, (size_t)c.m * c.n * sizeof(double), b->stream));
+    if ((e = bd_ls_advance(c, b->phi, b->vp, b->st, 0.0, b->stream))) return e;
+    if ((e = bd_ls_march(c, b->phi, b->st, b->march_delta, b->dmin, b->counters, (volatile int *)b->hint,
+                         &b->march_rounds, b->stream)))
+        return e;
+    if ((e = bd_ls_finish(c, b->phi, b->vp, b->st, 0, b->stream))) return e;
+    if ((e = bd_ls_node_status(c, b->phi, b->st, b->nst, b->stream))) return e;
+    STZ_HIP(hipStreamSynchronize(b->stream));
+    return STZ_OK;
+}
+
 int stz_bd_set_solver(stz_bd *b, int nu1, int nu2, int bottom_sweeps, int max_iter, double tol) {
     CHECK_BD(b);
     if (nu1 < 0 || nu2 < 0 || nu1 + nu2 < 1 || bottom_sweeps < 1 || max_iter < 1 || !(tol > 0)) {

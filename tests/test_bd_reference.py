@@ -12,6 +12,7 @@ import pytest
 import scipy.sparse as sp
 
 from conftest import load_fixture
+from oracle.pyoracle import bd_remarch
 
 
 @pytest.fixture(scope="module")
@@ -187,3 +188,47 @@ def test_init_bar_statuses(name):
     assert np.array_equal(f["nstatus"], node_status(phi, st))
     inside = st < 4
     assert np.all(f["chi"][inside] == 620 / 21000)
+
+
+def _planar(N, y0):
+    """statuses and phi of a flat interface y = y0: the distance is exact"""
+    dx = 2.0 / N
+    x = -1 + (np.arange(N) + 0.5) * dx
+    X, Y = np.meshgrid(x, x)
+    phi = Y - y0
+    pos = phi > 0
+    opp = np.zeros_like(pos)
+    opp[1:, :] |= pos[1:, :] != pos[:-1, :]
+    opp[:-1, :] |= pos[:-1, :] != pos[1:, :]
+    st = np.where(pos, np.where(opp, 4, 7), np.where(opp, 3, 0)).astype(np.int32)
+    return phi, st, dx
+
+
+@pytest.mark.parametrize("N,y0", [(32, 0.1234), (40, -0.377)])
+def test_remarch_planar_is_the_distance(N, y0):
+    """on a flat interface the rounds march must reproduce the signed
+    distance y - y0 in the whole band (to rounding), with band statuses
+    exactly where |y - y0| < 5 dx"""
+    phi, st, dx = _planar(N, y0)
+    p, s, rounds = bd_remarch(phi, st, dx, dx, 0.3 * dx)
+    band = (s >= 2) & (s <= 5)
+    assert np.array_equal(band, np.abs(phi) < 5 * dx)
+    assert np.abs(p - phi)[band].max() < 1e-13
+    assert np.all(np.abs(p[~band]) == 5 * dx)
+    assert 5 <= rounds <= 40
+
+
+def test_remarch_follows_reference_march(plate):
+    """against the reference's heap march (the fixture's band, built by
+    build_band from the analytic phi): same statuses, most band cells equal
+    to rounding, none off by more than a tenth of a cell"""
+    phi, st = plate["build_phi"], plate["build_status"]
+    dx = 2.0 / phi.shape[1]
+    p, s, _ = bd_remarch(phi, st, dx, dx, 0.3 * dx)
+    assert np.array_equal(s, st)
+    band = (st >= 2) & (st <= 5)
+    d = np.abs(p - phi)[band] / dx
+    assert (d <= 1e-12).mean() > 0.8
+    assert d.max() < 0.1
+    iface = (st == 3) | (st == 4)
+    assert np.array_equal(p[iface], phi[iface])

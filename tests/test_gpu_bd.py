@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 from conftest import load_fixture, rel_err
+from oracle.pyoracle import bd_remarch
 from stz_amd.bd import BdSim
 from stz_amd.problems import bd_params, plate_hole_phi
 
@@ -61,6 +62,20 @@ def test_build_band(plate):
     assert (d <= 1e-12).mean() > 0.7, stats
     assert d.max() < 0.1, stats
     assert np.array_equal(phi[~band], ref[~band])
+
+
+def test_remarch_equals_oracle(plate):
+    """the round-parallel march on the GPU against its CPU restatement
+    (oracle/bd_oracle.c): bit for bit"""
+    g = BdSim(plate["params"])
+    g.upload_level_set(plate["build_phi"], plate["build_status"])
+    g.remarch()
+    st, nst = g.status()
+    dx = 2.0 / st.shape[1]
+    p, s, _ = bd_remarch(plate["build_phi"], plate["build_status"], dx, dx, 0.3 * dx)
+    assert np.array_equal(st, s)
+    assert np.array_equal(g.get_cell("phi"), p)
+    assert np.array_equal(nst, plate["build_nstatus"])
 
 
 def test_interface_velocity(plate):
