@@ -26,6 +26,9 @@ template <typename T> struct LevelArgs {
     // (the rows near the walls differ), so kernels can keep it in registers
     const T *Si;
     int nb;
+    // Si has the sparsity of the top-level operator (offdiag9_top in
+    // mg_node.h): the tiled kernels then skip its zero entries
+    int sparse;
 };
 
 // A whole V-cycle: levels 0..L-1 (0 = top); levels >= lc are run by a single

@@ -76,6 +76,84 @@ __device__ __forceinline__ static void resid_value(const SP &s, T bx, T by, T ax
     ry = by - ay - fmav(s[19], z.y, s[18] * z.x);
 }
 
+// ---------------------------------------------------------------------------
+// The top level's interior stencil (qs_multi.cc:31-54, qs_multi.hh:40-50)
+// couples x to x and y to y only through the four edge neighbours (dc, cl,
+// cr, uc) and x to y only through the four corners (dl, dr, ul, ur); its
+// centre block and that block's inverse are diagonal.  The *_top variants
+// drop exactly the entries that are zero in this pattern and keep the order
+// of the others: a dropped fma(0, z, a) returns a unchanged, and a chain
+// that starts from 0 * z continues with fma(s, z', +-0) = s * z', so on such
+// a stencil they round as offdiag9 / gs_value / resid_value do (up to the
+// sign of a zero) with half the multiply-adds.  sparse_pattern() is the
+// host's test for the pattern.
+// ---------------------------------------------------------------------------
+template <typename T, typename SP>
+__device__ __forceinline__ static void offdiag9_top(const SP &s, const vec2s<T> &dl, const vec2s<T> &dc,
+                                                    const vec2s<T> &dr, const vec2s<T> &cl, const vec2s<T> &cr,
+                                                    const vec2s<T> &ul, const vec2s<T> &uc, const vec2s<T> &ur,
+                                                    T &ax, T &ay) {
+    T xd = s[1] * dl.y, yd = s[2] * dl.x;
+    xd = fmav(s[4], dc.x, xd);
+    yd = fmav(s[7], dc.y, yd);
+    xd = fmav(s[9], dr.y, xd);
+    yd = fmav(s[10], dr.x, yd);
+    T xc = s[12] * cl.x, yc = s[15] * cl.y;
+    xc = fmav(s[20], cr.x, xc);
+    yc = fmav(s[23], cr.y, yc);
+    T xu = s[25] * ul.y, yu = s[26] * ul.x;
+    xu = fmav(s[28], uc.x, xu);
+    yu = fmav(s[31], uc.y, yu);
+    xu = fmav(s[33], ur.y, xu);
+    yu = fmav(s[34], ur.x, yu);
+    ax = (xd + xu) + xc;
+    ay = (yd + yu) + yc;
+}
+
+template <typename T, typename SP>
+__device__ __forceinline__ static vec2s<T> gs_value_top(const SP &s, T bx, T by, T ax, T ay) {
+    vec2s<T> o;
+    o.x = s[36] * (bx - ax);
+    o.y = s[39] * (by - ay);
+    return o;
+}
+
+template <typename T, typename SP>
+__device__ __forceinline__ static void resid_value_top(const SP &s, T bx, T by, T ax, T ay, const vec2s<T> &z,
+                                                       T &rx, T &ry) {
+    rx = bx - ax - s[16] * z.x;
+    ry = by - ay - s[19] * z.y;
+}
+
+// dense or top-pattern arithmetic, chosen at compile time
+template <bool SPARSE, typename T, typename SP>
+__device__ __forceinline__ static void offdiag9_any(const SP &s, const vec2s<T> &dl, const vec2s<T> &dc,
+                                                    const vec2s<T> &dr, const vec2s<T> &cl, const vec2s<T> &cr,
+                                                    const vec2s<T> &ul, const vec2s<T> &uc, const vec2s<T> &ur,
+                                                    T &ax, T &ay) {
+    if constexpr (SPARSE) offdiag9_top<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+    else offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+}
+template <bool SPARSE, typename T, typename SP>
+__device__ __forceinline__ static vec2s<T> gs_value_any(const SP &s, T bx, T by, T ax, T ay) {
+    if constexpr (SPARSE) return gs_value_top<T>(s, bx, by, ax, ay);
+    else return gs_value<T>(s, bx, by, ax, ay);
+}
+template <bool SPARSE, typename T, typename SP>
+__device__ __forceinline__ static void resid_value_any(const SP &s, T bx, T by, T ax, T ay, const vec2s<T> &z,
+                                                       T &rx, T &ry) {
+    if constexpr (SPARSE) resid_value_top<T>(s, bx, by, ax, ay, z, rx, ry);
+    else resid_value<T>(s, bx, by, ax, ay, z, rx, ry);
+}
+
+// host: does the 40-entry stencil s have the top-level pattern?
+inline bool sparse_pattern(const double *s) {
+    static const int zero[] = {0, 3, 5, 6, 8, 11, 13, 14, 17, 18, 21, 22, 24, 27, 29, 30, 32, 35, 37, 38};
+    for (int k : zero)
+        if (s[k] != 0.0) return false;
+    return true;
+}
+
 // the eight neighbours of node (i,j) of a natural-order level (zero outside
 // the walls; x periodic)
 template <typename T>

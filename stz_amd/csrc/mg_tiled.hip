@@ -92,7 +92,7 @@ template <typename T> __device__ __forceinline__ static const T *wall_stencil(co
 // the coarse nodes whose centre lies in the tile (R = P^T, full weighting in
 // x, tgmg.in.cc:626-647), writing Cr's source (and zeroing its z when
 // zero_z), so the V-cycle needs no separate restriction launch.
-template <typename T, int TX, int TY, int S, int MODE, bool FUSE>
+template <typename T, int TX, int TY, int S, int MODE, bool FUSE, bool SP>
 __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
     smooth_tile_kernel(LevelArgs<T> L, T *__restrict__ zout, LevelArgs<T> Cz, LevelArgs<T> Cr, int zero_z) {
     constexpr int H = tile_halo(S, FUSE), RX = TX + 2 * H, RY = TY + 2 * H;
@@ -190,8 +190,8 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
                     offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
                     out = gs_value<T>(s, bx[q][c], by[q][c], ax, ay);
                 } else {
-                    offdiag9<T>(si, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
-                    out = gs_value<T>(si, bx[q][c], by[q][c], ax, ay);
+                    offdiag9_any<SP, T>(si, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+                    out = gs_value_any<SP, T>(si, bx[q][c], by[q][c], ax, ay);
                 }
                 zs[c * NB + blk] = out;
             }
@@ -226,8 +226,8 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
                     offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
                     resid_value<T>(s, bx[q][c], by[q][c], ax, ay, z0, rxs[q][c], rys[q][c]);
                 } else {
-                    offdiag9<T>(si, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
-                    resid_value<T>(si, bx[q][c], by[q][c], ax, ay, z0, rxs[q][c], rys[q][c]);
+                    offdiag9_any<SP, T>(si, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
+                    resid_value_any<SP, T>(si, bx[q][c], by[q][c], ax, ay, z0, rxs[q][c], rys[q][c]);
                 }
             }
         }
@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
 }
 
 // restriction: coarse tile CX x CY of C from fine level F
-template <typename T, int CX, int CY>
+template <typename T, int CX, int CY, bool SP>
 __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, LevelArgs<T> C, int zero_z) {
     // fine residual region: cols [2I0-1, 2I0+2CX-1], rows [2J0-2, 2J0+2CY]
     constexpr int RXr = 2 * CX + 1, RYr = 2 * CY + 3;
@@ -351,8 +351,8 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
                 offdiag9<T>(s, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
                 resid_value<T>(s, F.b[k], F.b[k + 1], ax, ay, z0, r.x, r.y);
             } else {
-                offdiag9<T>(si, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
-                resid_value<T>(si, F.b[k], F.b[k + 1], ax, ay, z0, r.x, r.y);
+                offdiag9_any<SP, T>(si, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
+                resid_value_any<SP, T>(si, F.b[k], F.b[k + 1], ax, ay, z0, r.x, r.y);
             }
         }
         rs[t] = r;
@@ -390,6 +390,7 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
 }
 
 // FP64 residual of the top level (64x16 tiles) -> FP32 source, |r|^2 partials
+template <bool SP>
 __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float *b32, double *partial) {
     constexpr int TX = 64, TY = 16, ZX = TX + 2, ZY = TY + 2, NT = 256;
     __shared__ vec2s<double> zs[ZY * ZX];
@@ -431,8 +432,8 @@ __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float 
             offdiag9<double>(s, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
             resid_value<double>(s, L.b[k], L.b[k + 1], ax, ay, z0, rx, ry);
         } else {
-            offdiag9<double>(si, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
-            resid_value<double>(si, L.b[k], L.b[k + 1], ax, ay, z0, rx, ry);
+            offdiag9_any<SP, double>(si, zd[-1], zd[0], zd[1], zc[-1], zc[1], zu[-1], zu[0], zu[1], ax, ay);
+            resid_value_any<SP, double>(si, L.b[k], L.b[k + 1], ax, ay, z0, rx, ry);
         }
         b32[k] = (float)rx;
         b32[k + 1] = (float)ry;
@@ -470,10 +471,20 @@ static void launch_tile(const LevelArgs<T> &L, T *zout, const LevelArgs<T> &Cz, 
                         hipStream_t st) {
     if (big_tiles(L)) {
         const dim3 g(L.m / 64, (L.n + 31) / 32), b(tile_threads(64, 32, tile_halo(S, FUSE)));
-        hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE, FUSE>), g, b, 0, st, L, zout, Cz, Cr, zero_z);
+        if (L.sparse)
+            hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE, FUSE, true>), g, b, 0, st, L, zout, Cz, Cr,
+                               zero_z);
+        else
+            hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE, FUSE, false>), g, b, 0, st, L, zout, Cz, Cr,
+                               zero_z);
     } else {
         const dim3 g(L.m / 32, (L.n + 15) / 16), b(tile_threads(32, 16, tile_halo(S, FUSE)));
-        hipLaunchKernelGGL((smooth_tile_kernel<T, 32, 16, S, MODE, FUSE>), g, b, 0, st, L, zout, Cz, Cr, zero_z);
+        if (L.sparse)
+            hipLaunchKernelGGL((smooth_tile_kernel<T, 32, 16, S, MODE, FUSE, true>), g, b, 0, st, L, zout, Cz, Cr,
+                               zero_z);
+        else
+            hipLaunchKernelGGL((smooth_tile_kernel<T, 32, 16, S, MODE, FUSE, false>), g, b, 0, st, L, zout, Cz, Cr,
+                               zero_z);
     }
 }
 
@@ -533,10 +544,12 @@ template <typename T> static int restrict_level(const LevelArgs<T> &F, const Lev
     constexpr int CY = sizeof(T) == 8 ? 8 : 16;
     if (((C.m + 31) / 32) * ((C.n + CY - 1) / CY) >= 256) {
         const dim3 g((C.m + 31) / 32, (C.n + CY - 1) / CY);
-        hipLaunchKernelGGL((restrict_tile_kernel<T, 32, CY>), g, dim3(256), 0, st, F, C, zero_z);
+        if (F.sparse) hipLaunchKernelGGL((restrict_tile_kernel<T, 32, CY, true>), g, dim3(256), 0, st, F, C, zero_z);
+        else hipLaunchKernelGGL((restrict_tile_kernel<T, 32, CY, false>), g, dim3(256), 0, st, F, C, zero_z);
     } else {
         const dim3 g((C.m + 15) / 16, (C.n + 7) / 8);
-        hipLaunchKernelGGL((restrict_tile_kernel<T, 16, 8>), g, dim3(256), 0, st, F, C, zero_z);
+        if (F.sparse) hipLaunchKernelGGL((restrict_tile_kernel<T, 16, 8, true>), g, dim3(256), 0, st, F, C, zero_z);
+        else hipLaunchKernelGGL((restrict_tile_kernel<T, 16, 8, false>), g, dim3(256), 0, st, F, C, zero_z);
     }
     STZ_HIP(hipGetLastError());
     return STZ_OK;
@@ -570,7 +583,8 @@ int mg_refine_blocks(const LevelArgs<double> &L) { return ((L.m + 63) / 64) * ((
 
 int mg_refine(const LevelArgs<double> &L, float *b32, double *partial, double *out, hipStream_t st) {
     const dim3 g((L.m + 63) / 64, (L.n + 15) / 16);
-    hipLaunchKernelGGL(refine_kernel, g, dim3(256), 0, st, L, b32, partial);
+    if (L.sparse) hipLaunchKernelGGL(refine_kernel<true>, g, dim3(256), 0, st, L, b32, partial);
+    else hipLaunchKernelGGL(refine_kernel<false>, g, dim3(256), 0, st, L, b32, partial);
     STZ_HIP(hipGetLastError());
     return mg_sum_partials(partial, (int)(g.x * g.y), out, st);
 }

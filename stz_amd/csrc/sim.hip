@@ -22,6 +22,7 @@
 #include "device.h"
 #include "diag.h"
 #include "mg.h"
+#include "mg_node.h"
 
 namespace stz {
 
@@ -380,6 +381,7 @@ static int set_level_stencils(stz_sim *s, DevMG<T> &mg, const std::vector<std::v
         A.cs = H.xinv ? 0 : 1;
         A.Si = mg.S[l] + H.S.size();
         A.nb = H.nb;
+        A.sparse = H.xinv && sparse_pattern(Sx[l].data() + H.S.size()) ? 1 : 0;
     }
     mg.lds = (size_t)coarse_layout(mg.args) * sizeof(T);
     if ((e = set_lds_attr(mg))) return e;
