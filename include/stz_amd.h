@@ -376,6 +376,43 @@ int stz_bd_set_body_force(stz_bd *b, const double *force);
  * loads a body held between two moving grips. */
 int stz_bd_set_edge_velocity(stz_bd *b, double u_left, double u_right);
 
+/* ---- ensembles of independent simulations --------------------------------
+ * No counterpart in the reference, which runs one shear_sim per process
+ * (drivers such as shear_random.cc:106-121 loop over seeds in separate
+ * runs).  An ensemble owns `count` simulations of one precision, each with
+ * its own parameters (grid size included), and steps them together: every
+ * kernel of stz_sim_step_quasistatic runs as ONE launch over all members,
+ * each member's adaptive V-cycle count (tgmg_predict) is decided on the
+ * device, and a step replays from HIP graphs with one host synchronisation.
+ * Every member computes exactly what stz_sim_step_quasistatic computes for
+ * it alone.  Requirements: all members have the same stz_mg_config, with
+ * kernels = 0, nu1 >= 1 and ceil(nu1/2) + ceil(nu2/2) even (nu1 = nu2 is
+ * always fine), and every multigrid level above the workgroup solver is
+ * covered by the tiled kernels (sides a multiple of 32 and at most 512 in
+ * the FP64 form); stz_ensemble_step_quasistatic reports anything else as
+ * STZ_ERR_ARG. */
+typedef struct stz_ensemble stz_ensemble;   /* opaque */
+
+/* count simulations, member k from params[k] (as stz_sim_create). */
+int stz_ensemble_create(const stz_sim_params *params, int count, int precision, stz_ensemble **out);
+int stz_ensemble_destroy(stz_ensemble *e);
+int stz_ensemble_size(stz_ensemble *e, int *count);
+/* Borrowed handle of member k for every stz_sim_* call (fields, time,
+ * initial conditions, diagnostics, single steps); never destroy it. */
+int stz_ensemble_member(stz_ensemble *e, int k, stz_sim **member);
+/* Multigrid settings of every member (stz_sim_set_mg_config on each). */
+int stz_ensemble_set_mg_config(stz_ensemble *e, const stz_mg_config *cfg);
+/* Replay steps from HIP graphs (1, default) or launch them directly (0). */
+int stz_ensemble_use_graphs(stz_ensemble *e, int on);
+/* nsteps quasi-static steps of every member.  Stops after the first step in
+ * which a member's solve fails and returns that status; the failing member
+ * is left as stz_sim_step_quasistatic leaves it, the others have completed
+ * the step. */
+int stz_ensemble_step_quasistatic(stz_ensemble *e, double dt, int nsteps);
+/* Counters since creation: out4 = {steps, cycle slots launched, slots
+ * launched after the planned ones, graph captures}. */
+int stz_ensemble_stats(stz_ensemble *e, int64_t *out4);
+
 #ifdef __cplusplus
 }
 #endif

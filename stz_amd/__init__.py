@@ -7,6 +7,7 @@ reference class; ``problems`` holds the workload parameter sets.
 """
 from . import problems
 from ._lib import StzError, device_count, load
+from .ensemble import Ensemble
 from .sim import FIELD_NAMES, ShearSim
 
-__all__ = ["ShearSim", "StzError", "FIELD_NAMES", "problems", "device_count", "load"]
+__all__ = ["ShearSim", "Ensemble", "StzError", "FIELD_NAMES", "problems", "device_count", "load"]

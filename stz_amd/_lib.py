@@ -167,6 +167,14 @@ _SIGS = {
     "stz_bd_operator": [C.c_void_p, C.c_double, _ip, _dp, _dp],
     "stz_bd_set_body_force": [C.c_void_p, C.c_void_p],
     "stz_bd_set_edge_velocity": [C.c_void_p, C.c_double, C.c_double],
+    "stz_ensemble_create": [C.POINTER(SimParams), C.c_int, C.c_int, C.POINTER(C.c_void_p)],
+    "stz_ensemble_destroy": [C.c_void_p],
+    "stz_ensemble_size": [C.c_void_p, C.POINTER(C.c_int)],
+    "stz_ensemble_member": [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)],
+    "stz_ensemble_set_mg_config": [C.c_void_p, C.POINTER(MGConfig)],
+    "stz_ensemble_use_graphs": [C.c_void_p, C.c_int],
+    "stz_ensemble_step_quasistatic": [C.c_void_p, C.c_double, C.c_int],
+    "stz_ensemble_stats": [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")],
 }
 _RESTYPE = {"stz_last_error": C.c_char_p}
 

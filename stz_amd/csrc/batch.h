@@ -1,0 +1,145 @@
+// batch.h -- batched launches for an ensemble of independent quasi-static
+// simulations.  One launch runs one kernel of the step on every member it
+// lists: its workgroups are split among the members by a prefix table
+// (wg0[e] <= workgroup < wg0[e+1] belongs to entry e), and each workgroup
+// runs the single-simulation kernel body on its member's arrays, so every
+// member computes exactly what its own simulation would.  The per-member
+// tables live in device memory and do not change from step to step, which
+// lets a whole step be captured in a HIP graph; what does change (time, the
+// adaptive V-cycle bookkeeping, which members still cycle) is in EnsState,
+// written only by the device.
+#pragma once
+
+#include <vector>
+
+#include "device.h"
+#include "mg.h"
+
+namespace stz {
+
+// the adaptive V-cycle count of tgmg::solve(type, tgmg_predict&)
+// (tgmg.in.cc:266-301), one state machine per member
+enum { ENS_FIRST = 0, ENS_LOOP = 1, ENS_EXTRA = 2, ENS_DONE = 3, ENS_CORRECTED = 4, ENS_FAILED = 5 };
+
+struct EnsState {
+    double time;       // simulation time (incremented at the start of a step)
+    double tol;        // threshold of the mean-square residual
+    double denom;      // number of top-level nodes: residual = |r|^2 / denom
+    double acc;        // residual at the last test
+    int lim, n, k, remaining;  // tgmg_predict limit, cycles counted, loop batch, cycles left
+    int phase, active;         // ENS_*; 1 while the member runs the current cycle slot
+    int cycles_run, solves, vcount;
+    int status;        // STZ_OK, STZ_ERR_NOT_CONVERGED, STZ_ERR_NAN
+};
+
+// predictor settings shared by all members (tgmg_config.hh:26-45)
+struct EnsPredict {
+    int mult, decay, max_thresh, extra;
+};
+
+// member index of workgroup w
+__device__ __forceinline__ static int batch_entry(const int *wg0, int ne, int w) {
+    int lo = 0, hi = ne - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (wg0[mid] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// one member's cell-grid kernels (advection, chi diffusion, source, correction)
+struct CellEntry {
+    int member, gx;     // gx: workgroups along x (64 x 4 threads each)
+    SimConst c;
+    FieldPtrs f[2];     // plane sets: f[parity] is the current one
+    double *vel, *src;
+    uint32_t *dec;      // decision record or null
+    double xbc;         // wall-row source factor bcs * cxx
+};
+
+// one member's launch of the LDS-tiled smoother (mg_tiled.hip)
+template <typename T> struct TileEntry {
+    int member, gx;
+    LevelArgs<T> L;
+    T *zout;
+    LevelArgs<T> Cz, Cr;
+    int zero_z;
+};
+
+// FP64 residual of a member's top level (refinement or norm)
+struct RefineEntry {
+    int member, gx;
+    LevelArgs<double> L;
+    float *b32;         // mixed: FP32 source of the correction equation
+    double *partial;    // |r|^2 of each workgroup
+};
+
+template <typename T> struct CoarseEntry {
+    int member;
+    CoarseArgs<T> A;
+};
+
+// vel += e (mixed form), count values
+struct CorrEntry {
+    int member, count;
+    double *z;
+    const float *e;
+};
+
+// workgroup-partial sums of one member's residual
+struct NormEntry {
+    int member, np;
+    const double *partial;
+};
+
+// launchers: E and wg0 are device arrays of ne entries (wg0 has ne + 1);
+// total = wg0[ne].  With a non-null state array, members whose `active`
+// flag is clear return at once (cycle slots); correct runs only members in
+// ENS_DONE.
+int batch_time(EnsState *S, int nmem, double dt, hipStream_t st);
+int batch_advect(const CellEntry *E, const int *wg0, int ne, int total, int parity, double dt, hipStream_t st);
+int batch_chi_diffusion(const CellEntry *E, const int *wg0, int ne, int total, int parity, double dt,
+                        hipStream_t st);
+int batch_source(const CellEntry *E, const int *wg0, int ne, int total, int parity, const EnsState *S, double dt,
+                 hipStream_t st);
+int batch_correct(const CellEntry *E, const int *wg0, int ne, int total, int parity, const EnsState *S, double dt,
+                  hipStream_t st);
+
+// tiled smoother launch of variant (S sweeps, mode, fuse, sparse, 64x32 tiles)
+struct TileKey {
+    int sweeps, mode, fuse, sparse, big;
+    bool operator==(const TileKey &o) const {
+        return sweeps == o.sweeps && mode == o.mode && fuse == o.fuse && sparse == o.sparse && big == o.big;
+    }
+};
+template <typename T>
+int batch_tile(const TileKey &key, const TileEntry<T> *E, const int *wg0, int ne, int total, const EnsState *S,
+               hipStream_t st);
+// one launch of a member's tiled V-cycle: phase 0 pre-smoothing (with the
+// fused restriction), 2 post-smoothing; level and launch index within it
+template <typename T> struct TileRec {
+    int phase, level, index;
+    TileKey key;
+    LevelArgs<T> L;
+    T *zout;
+    LevelArgs<T> Cz, Cr;
+    int zero_z;
+};
+// the launches mg_vcycle_tiled would issue for A (mg_tiled.hip)
+template <typename T> int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out);
+
+// mixed: FP64 residual -> FP32 source of the correction equation + norm partials
+int batch_refine(const RefineEntry *E, const int *wg0, int ne, int total, int sparse, const EnsState *S,
+                 hipStream_t st);
+template <typename T> int batch_coarse(const CoarseEntry<T> *E, int ne, size_t lds, const EnsState *S, hipStream_t st);
+template <typename T> int batch_set_lds_limit(size_t bytes);
+// fp64: |r|^2 partials of the top level, blocks as mg_resid_norm
+int batch_resid_norm(const RefineEntry *E, const int *wg0, int ne, int total, const EnsState *S, hipStream_t st);
+int batch_correct64(const CorrEntry *E, const int *wg0, int ne, int total, const EnsState *S, hipStream_t st);
+
+// the per-member V-cycle count (mg.hip): mode 0 starts a solve, mode 1
+// follows a cycle slot, mode 2 marks corrected members
+int batch_control(EnsState *S, const NormEntry *N, int nmem, EnsPredict P, int mode, hipStream_t st);
+
+}  // namespace stz

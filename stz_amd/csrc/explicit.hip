@@ -15,6 +15,7 @@
 // reference wherever no transcendental function is involved (the ENO, upwind
 // and yield decisions in particular); exp/tanh/cosh differ from glibc by at
 // most a few ulp.
+#include "batch.h"
 #include "device.h"
 #include "stz_law.h"
 
@@ -34,11 +35,11 @@ __device__ __forceinline__ static double eno2(double p0, double p1, double p2, d
 // ---------------------------------------------------------------------------
 // advection + plasticity: shear_sim.cc:296-349 (+ update, :356-360)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) advect_kernel(SimConst c, FieldPtrs o, FieldPtrs w, double dt,
-                                                     uint32_t *dec) {
+__device__ __forceinline__ static void advect_body(const SimConst &c, const FieldPtrs &o, const FieldPtrs &w,
+                                                   double dt, uint32_t *dec, int bx, int by) {
     const int m = c.m, n = c.n;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
     if (i >= m || j >= n) return;
     const double hx = 0.5 * c.xsp * dt, hy = 0.5 * c.ysp * dt, lx = c.lx;
     const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
@@ -197,6 +198,10 @@ __global__ void __launch_bounds__(256) advect_kernel(SimConst c, FieldPtrs o, Fi
         bits |= (uint32_t)(trips > 255 ? 255 : trips) << STZ_D_SUBSTEP_SHIFT;
         dec[i + m * j] = bits;
     }
+}
+__global__ void __launch_bounds__(256) advect_kernel(SimConst c, FieldPtrs o, FieldPtrs w, double dt,
+                                                     uint32_t *dec) {
+    advect_body(c, o, w, dt, dec, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -378,12 +383,12 @@ __global__ void __launch_bounds__(256) direct_kernel(SimConst c, FieldPtrs o, Fi
 
 // chi_diffusion, shear_sim.cc:364-386: w.chi holds the post-plastic chi
 // (cchi) and receives the diffused value; o.chi is the previous chi.
-__global__ void __launch_bounds__(256) chi_diffusion_kernel(SimConst c, const double *__restrict__ chi_old,
-                                                            const double *__restrict__ ddev, double *chi_new,
-                                                            double dt) {
+__device__ __forceinline__ static void chi_diffusion_body(const SimConst &c, const double *__restrict__ chi_old,
+                                                          const double *__restrict__ ddev, double *chi_new,
+                                                          double dt, int bx, int by) {
     const int m = c.m, n = c.n;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
     if (i >= m || j >= n) return;
     const double hx = 0.5 * c.xsp * dt, hy = 0.5 * c.ysp * dt;
     const double hxxcd = 2 * c.xsp * hx * c.chi_diff, hyycd = 2 * c.ysp * hy * c.chi_diff,
@@ -400,16 +405,22 @@ __global__ void __launch_bounds__(256) chi_diffusion_kernel(SimConst c, const do
                          + dfacx * (Dr * cr - (Dr + Dl) * cf + Dl * cl)
                          + dfacy * (Du * cu - (Du + Dd) * cf + Dd * cd);
 }
+__global__ void __launch_bounds__(256) chi_diffusion_kernel(SimConst c, const double *__restrict__ chi_old,
+                                                            const double *__restrict__ ddev, double *chi_new,
+                                                            double dt) {
+    chi_diffusion_body(c, chi_old, ddev, chi_new, dt, blockIdx.x, blockIdx.y);
+}
 
 // ---------------------------------------------------------------------------
 // projection source b = div(sigma)/dt and wall rows, shear_sim.cc:399-409,
 // net_force :447-453.  One thread per node (i,j), j in [0,n].
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) source_kernel(SimConst c, FieldPtrs f, double xfac, double yfac,
-                                                     double xbc, double bvel, double *src) {
+__device__ __forceinline__ static void source_body(const SimConst &c, const FieldPtrs &f, double xfac,
+                                                   double yfac, double xbc, double bvel, double *src, int bx,
+                                                   int by) {
     const int m = c.m, n = c.n;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
     if (i >= m || j > n) return;
     double fx, fy;
     if (j == 0) {
@@ -432,6 +443,10 @@ __global__ void __launch_bounds__(256) source_kernel(SimConst c, FieldPtrs f, do
     const int ij = i + m * j;
     src[2 * ij] = fx;
     src[2 * ij + 1] = fy;
+}
+__global__ void __launch_bounds__(256) source_kernel(SimConst c, FieldPtrs f, double xfac, double yfac,
+                                                     double xbc, double bvel, double *src) {
+    source_body(c, f, xfac, yfac, xbc, bvel, src, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -459,11 +474,11 @@ __device__ __forceinline__ static void correct_cell(const SimConst &c, const Fie
     PL(f.v, i, j) = vpy;
 }
 
-__global__ void __launch_bounds__(256) correct_kernel(SimConst c, FieldPtrs f, const double *vel, double dt,
-                                                      double bvel, double bpos) {
+__device__ __forceinline__ static void correct_body(const SimConst &c, const FieldPtrs &f, const double *vel,
+                                                    double dt, double bvel, double bpos, int bx, int by) {
     const int m = c.m, n = c.n;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
     if (i >= m || j >= n) return;
     if (j == 1 || j == n - 2) return;  // done by the wall-row threads
     const double hx = 0.5 * c.xsp * dt, hy = 0.5 * c.ysp * dt;
@@ -492,6 +507,10 @@ __global__ void __launch_bounds__(256) correct_kernel(SimConst c, FieldPtrs f, c
     PL(f.q, i, j) = q;
     PL(f.s, i, j) = s;
     PL(f.tau, i, j) = t;
+}
+__global__ void __launch_bounds__(256) correct_kernel(SimConst c, FieldPtrs f, const double *vel, double dt,
+                                                      double bvel, double bpos) {
+    correct_body(c, f, vel, dt, bvel, bpos, blockIdx.x, blockIdx.y);
 }
 
 // set_boundaries on its own, shear_sim.cc:552-581
@@ -609,6 +628,92 @@ int launch_init_fields(const SimConst &c, const FieldPtrs &f, int chi_case, doub
                        hipStream_t st) {
     dim3 b(64, 4);
     hipLaunchKernelGGL(init_fields_kernel, grid2(c.m, c.n + 2, b), b, 0, st, c, f, chi_case, chi0, chi1);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched launches over the members of an ensemble (batch.h): the same
+// bodies, one member per workgroup range; the wall velocity and position
+// come from the member's device-side time, evaluated as the host evaluates
+// bdry_vel / bdry_pos (shear_sim.hh:120-124)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ static double ens_bvel(const SimConst &c, double t) {
+    return t < 1 ? t * c.u_bdry : c.u_bdry;
+}
+__device__ __forceinline__ static double ens_bpos(const SimConst &c, double t) {
+    return t < 1 ? c.u_bdry * t * t * 0.5 : c.u_bdry * (t - 0.5);
+}
+
+__global__ void __launch_bounds__(256) time_batch_kernel(EnsState *S, int nmem, double dt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nmem) S[k].time += dt;  // shear_sim.cc:283
+}
+
+__global__ void __launch_bounds__(256) advect_batch_kernel(const CellEntry *E, const int *wg0, int ne, int parity,
+                                                           double dt) {
+    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const CellEntry &x = E[e];
+    advect_body(x.c, x.f[parity], x.f[parity ^ 1], dt, x.dec, l % x.gx, l / x.gx);
+}
+
+__global__ void __launch_bounds__(256) chi_diffusion_batch_kernel(const CellEntry *E, const int *wg0, int ne,
+                                                                  int parity, double dt) {
+    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const CellEntry &x = E[e];
+    chi_diffusion_body(x.c, x.f[parity].chi, x.f[parity ^ 1].ddev, x.f[parity ^ 1].chi, dt, l % x.gx, l / x.gx);
+}
+
+__global__ void __launch_bounds__(256) source_batch_kernel(const CellEntry *E, const int *wg0, int ne, int parity,
+                                                           const EnsState *S, double dt) {
+    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const CellEntry &x = E[e];
+    const double xfac = 0.5 / dt * x.c.xsp, yfac = 0.5 / dt * x.c.ysp;
+    source_body(x.c, x.f[parity], xfac, yfac, x.xbc, ens_bvel(x.c, S[x.member].time), x.src, l % x.gx, l / x.gx);
+}
+
+__global__ void __launch_bounds__(256) correct_batch_kernel(const CellEntry *E, const int *wg0, int ne, int parity,
+                                                            const EnsState *S, double dt) {
+    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const CellEntry &x = E[e];
+    const EnsState &st = S[x.member];
+    if (st.phase != ENS_DONE) return;
+    correct_body(x.c, x.f[parity], x.vel, dt, ens_bvel(x.c, st.time), ens_bpos(x.c, st.time), l % x.gx, l / x.gx);
+}
+
+int batch_time(EnsState *S, int nmem, double dt, hipStream_t st) {
+    hipLaunchKernelGGL(time_batch_kernel, dim3((nmem + 255) / 256), dim3(256), 0, st, S, nmem, dt);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int batch_advect(const CellEntry *E, const int *wg0, int ne, int total, int parity, double dt, hipStream_t st) {
+    if (ne == 0) return STZ_OK;
+    hipLaunchKernelGGL(advect_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, wg0, ne, parity, dt);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int batch_chi_diffusion(const CellEntry *E, const int *wg0, int ne, int total, int parity, double dt,
+                        hipStream_t st) {
+    if (ne == 0) return STZ_OK;
+    hipLaunchKernelGGL(chi_diffusion_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, wg0, ne, parity, dt);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int batch_source(const CellEntry *E, const int *wg0, int ne, int total, int parity, const EnsState *S, double dt,
+                 hipStream_t st) {
+    if (ne == 0) return STZ_OK;
+    hipLaunchKernelGGL(source_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, wg0, ne, parity, S, dt);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int batch_correct(const CellEntry *E, const int *wg0, int ne, int total, int parity, const EnsState *S, double dt,
+                  hipStream_t st) {
+    if (ne == 0) return STZ_OK;
+    hipLaunchKernelGGL(correct_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, wg0, ne, parity, S, dt);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }

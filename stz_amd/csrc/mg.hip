@@ -22,6 +22,7 @@
 //                    coarsest level exactly with a precomputed dense inverse
 //   residual norm    wavefront-shuffle reduction to per-block partials, then
 //                    one block sums the partials in a fixed order
+#include "batch.h"
 #include "mg_node.h"
 
 namespace stz {
@@ -65,18 +66,22 @@ __device__ __forceinline__ static double wave_sum(double v) {
     return v;
 }
 
-// block of 256 threads -> one partial
-__device__ __forceinline__ static void block_partial(double v, double *partial) {
+// block of 256 threads -> one partial, stored at partial[slot]
+__device__ __forceinline__ static void block_partial(double v, double *partial, int slot) {
     __shared__ double ws[4];
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) ws[wid] = v;
     __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+    if (threadIdx.x == 0) partial[slot] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+__device__ __forceinline__ static void block_partial(double v, double *partial) {
+    block_partial(v, partial, blockIdx.x);
 }
 
-template <typename T> __global__ void __launch_bounds__(256) resid_norm_kernel(LevelArgs<T> L, double *partial) {
-    const int idx = blockIdx.x * 256 + threadIdx.x;
+template <typename T>
+__device__ __forceinline__ static void resid_norm_body(const LevelArgs<T> &L, double *partial, int blk) {
+    const int idx = blk * 256 + threadIdx.x;
     double v = 0;
     if (idx < L.m * L.n) {
         const int j = idx / L.m, i = idx - j * L.m;
@@ -84,11 +89,15 @@ template <typename T> __global__ void __launch_bounds__(256) resid_norm_kernel(L
         resid_node(L, i, j, rx, ry);
         v = (double)rx * (double)rx + (double)ry * (double)ry;
     }
-    block_partial(v, partial);
+    block_partial(v, partial, blk);
 }
 
-// fixed-order sum of the partials by one block
-__global__ void __launch_bounds__(256) sum_partials_kernel(const double *partial, int np, double *out) {
+template <typename T> __global__ void __launch_bounds__(256) resid_norm_kernel(LevelArgs<T> L, double *partial) {
+    resid_norm_body(L, partial, blockIdx.x);
+}
+
+// fixed-order sum of the partials by one block of 256 threads
+__device__ __forceinline__ static double sum_partials_body(const double *partial, int np) {
     double v = 0;
     for (int k = threadIdx.x; k < np; k += 256) v += partial[k];
     __shared__ double ws[4];
@@ -96,7 +105,12 @@ __global__ void __launch_bounds__(256) sum_partials_kernel(const double *partial
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) ws[wid] = v;
     __syncthreads();
-    if (threadIdx.x == 0) *out = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+    return (ws[0] + ws[1]) + (ws[2] + ws[3]);  // valid in thread 0
+}
+
+__global__ void __launch_bounds__(256) sum_partials_kernel(const double *partial, int np, double *out) {
+    const double v = sum_partials_body(partial, np);
+    if (threadIdx.x == 0) *out = v;
 }
 
 // mixed precision: FP64 residual of the top level -> FP32 source of the
@@ -513,7 +527,7 @@ template <typename T> __device__ __forceinline__ static void load_si(const Level
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kCoarseThreads) coarse_solve_kernel(CoarseArgs<T> A) {
+__device__ __forceinline__ static void coarse_solve_body(const CoarseArgs<T> &A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *sm = reinterpret_cast<T *>(smem_raw);
     const int tid = threadIdx.x, NT = kCoarseThreads;
@@ -650,6 +664,20 @@ __global__ void __launch_bounds__(kCoarseThreads) coarse_solve_kernel(CoarseArgs
             gz[2 * t + 1] = sm[W.zo + k + 1];
         }
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kCoarseThreads) coarse_solve_kernel(CoarseArgs<T> A) {
+    coarse_solve_body(A);
+}
+
+// one workgroup per member of an ensemble (batch.h)
+template <typename T>
+__global__ void __launch_bounds__(kCoarseThreads) coarse_solve_batch_kernel(const CoarseEntry<T> *E,
+                                                                            const EnsState *St) {
+    const CoarseEntry<T> &x = E[blockIdx.x];
+    if (St && !St[x.member].active) return;
+    coarse_solve_body(x.A);
 }
 
 template <typename T> int coarse_layout(CoarseArgs<T> &A) {
@@ -791,9 +819,14 @@ template <typename T> int mg_vcycle(const CoarseArgs<T> &A, hipStream_t st) {
     return STZ_OK;
 }
 
+// The attribute belongs to the kernel, not to a simulation: it only ever
+// grows, so that simulations of different sizes in one process all fit.
 template <typename T> int mg_set_lds_limit(size_t bytes) {
+    static size_t set = 0;
+    if (bytes <= set) return STZ_OK;
     STZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&coarse_solve_kernel<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    set = bytes;
     return STZ_OK;
 }
 template int mg_set_lds_limit<double>(size_t);
@@ -813,5 +846,143 @@ template int mg_resid_norm<double>(const LevelArgs<double> &, double *, double *
 template int mg_resid_norm<float>(const LevelArgs<float> &, double *, double *, hipStream_t);
 template size_t coarse_lds_bytes<double>(const CoarseArgs<double> &);
 template size_t coarse_lds_bytes<float>(const CoarseArgs<float> &);
+
+// ---------------------------------------------------------------------------
+// batched launches over the members of an ensemble (batch.h)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) resid_norm_batch_kernel(const RefineEntry *E, const int *wg0, int ne,
+                                                               const EnsState *St) {
+    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const RefineEntry &x = E[e];
+    if (St && !St[x.member].active) return;
+    resid_norm_body(x.L, x.partial, l);
+}
+
+__global__ void __launch_bounds__(256) correct64_batch_kernel(const CorrEntry *E, const int *wg0, int ne,
+                                                              const EnsState *St) {
+    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const CorrEntry &x = E[e];
+    if (St && !St[x.member].active) return;
+    const int idx = l * 256 + threadIdx.x;
+    if (idx < x.count) x.z[idx] += (double)x.e[idx];
+}
+
+// tgmg::solve(type, tgmg_predict&) (tgmg.in.cc:266-301) as a state machine:
+// the same tests, in the same order, on the same residuals as the host loop
+// of the single simulation, advanced once per cycle slot.  acc is the
+// residual after the cycles counted so far.
+__device__ static void ens_resolve(EnsState &st, double acc, const EnsPredict &P) {
+    for (;;) {
+        if (st.phase == ENS_FIRST || st.phase == ENS_LOOP) {
+            if (st.remaining > 0) {
+                st.active = 1;
+                return;
+            }
+            st.acc = acc;
+            if (acc < st.tol) {
+                if (st.phase == ENS_FIRST && st.lim > 0) st.lim -= 1 + st.lim / P.decay;
+                st.phase = ENS_EXTRA;
+                st.remaining = P.extra;
+                continue;
+            }
+            st.k += 1;
+            st.lim += st.k * P.mult;
+            if (st.lim > P.max_thresh) {
+                st.phase = ENS_FAILED;
+                st.status = STZ_ERR_NOT_CONVERGED;
+                st.active = 0;
+                return;
+            }
+            st.n += st.k;
+            st.remaining = st.k;
+            st.phase = ENS_LOOP;
+            st.active = 1;
+            return;
+        }
+        if (st.phase == ENS_EXTRA) {
+            if (st.remaining > 0) {
+                st.active = 1;
+                return;
+            }
+            st.phase = ENS_DONE;
+            st.solves++;
+            st.vcount += st.n;
+            st.status = isnan(st.acc) ? STZ_ERR_NAN : STZ_OK;
+        }
+        st.active = 0;
+        return;
+    }
+}
+
+// one workgroup per member: mode 0 starts the solve from the residual of the
+// step's first evaluation, mode 1 counts the slot's cycle of every member
+// that ran it and tests its residual, mode 2 marks members corrected
+__global__ void __launch_bounds__(256) control_batch_kernel(EnsState *S, const NormEntry *N, EnsPredict P,
+                                                            int mode) {
+    const NormEntry &x = N[blockIdx.x];
+    EnsState &st = S[x.member];
+    if (mode == 2) {
+        if (threadIdx.x == 0 && st.phase == ENS_DONE) st.phase = ENS_CORRECTED;
+        return;
+    }
+    if (mode == 1 && !st.active) return;
+    const double sum = sum_partials_body(x.partial, x.np);
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
+        st.n = st.lim / P.mult;
+        st.k = 0;
+        st.remaining = st.n;
+        st.phase = ENS_FIRST;
+        st.cycles_run = 0;
+        st.status = STZ_OK;
+    } else {
+        st.remaining--;
+        st.cycles_run++;
+    }
+    ens_resolve(st, sum / st.denom, P);
+}
+
+template <typename T> int batch_coarse(const CoarseEntry<T> *E, int ne, size_t lds, const EnsState *St,
+                                       hipStream_t st) {
+    if (ne == 0) return STZ_OK;
+    hipLaunchKernelGGL(coarse_solve_batch_kernel<T>, dim3(ne), dim3(kCoarseThreads), lds, st, E, St);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+template int batch_coarse<double>(const CoarseEntry<double> *, int, size_t, const EnsState *, hipStream_t);
+template int batch_coarse<float>(const CoarseEntry<float> *, int, size_t, const EnsState *, hipStream_t);
+
+template <typename T> int batch_set_lds_limit(size_t bytes) {
+    static size_t set = 0;
+    if (bytes <= set) return STZ_OK;
+    STZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&coarse_solve_batch_kernel<T>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    set = bytes;
+    return STZ_OK;
+}
+template int batch_set_lds_limit<double>(size_t);
+template int batch_set_lds_limit<float>(size_t);
+
+int batch_resid_norm(const RefineEntry *E, const int *wg0, int ne, int total, const EnsState *St, hipStream_t st) {
+    if (ne == 0) return STZ_OK;
+    hipLaunchKernelGGL(resid_norm_batch_kernel<double>, dim3(total), dim3(256), 0, st, E, wg0, ne, St);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int batch_correct64(const CorrEntry *E, const int *wg0, int ne, int total, const EnsState *St, hipStream_t st) {
+    if (ne == 0) return STZ_OK;
+    hipLaunchKernelGGL(correct64_batch_kernel, dim3(total), dim3(256), 0, st, E, wg0, ne, St);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int batch_control(EnsState *S, const NormEntry *N, int nmem, EnsPredict P, int mode, hipStream_t st) {
+    if (nmem == 0) return STZ_OK;
+    hipLaunchKernelGGL(control_batch_kernel, dim3(nmem), dim3(256), 0, st, S, N, P, mode);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
 
 }  // namespace stz

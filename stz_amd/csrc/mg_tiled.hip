@@ -28,6 +28,7 @@
 // smoother.
 #include <utility>
 
+#include "batch.h"
 #include "mg_node.h"
 
 namespace stz {
@@ -93,8 +94,9 @@ template <typename T> __device__ __forceinline__ static const T *wall_stencil(co
 // x, tgmg.in.cc:626-647), writing Cr's source (and zeroing its z when
 // zero_z), so the V-cycle needs no separate restriction launch.
 template <typename T, int TX, int TY, int S, int MODE, bool FUSE, bool SP>
-__global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
-    smooth_tile_kernel(LevelArgs<T> L, T *__restrict__ zout, LevelArgs<T> Cz, LevelArgs<T> Cr, int zero_z) {
+__device__ __forceinline__ static void smooth_tile_body(const LevelArgs<T> &L, T *__restrict__ zout,
+                                                        const LevelArgs<T> &Cz, const LevelArgs<T> &Cr, int zero_z,
+                                                        int tbx, int tby) {
     constexpr int H = tile_halo(S, FUSE), RX = TX + 2 * H, RY = TY + 2 * H;
     constexpr int BX = RX / 2, BY = RY / 2, NB = BX * BY, NT = tile_threads(TX, TY, H);
     constexpr int BPT = (NB + NT - 1) / NT;
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
     __shared__ T ss[2 * kMaxWall * 40];
     const int tid = threadIdx.x;
     const int m = L.m, n = L.n, nb = L.nb;
-    const int i0 = blockIdx.x * TX - H, j0 = blockIdx.y * TY - H;
+    const int i0 = tbx * TX - H, j0 = tby * TY - H;
 
     // interior stencil: wave-uniform, held in registers
     T si[40];
@@ -265,7 +267,7 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
     // coarse nodes of this tile: I in [fi0/2, fi0/2 + TX/2), J with its fine
     // centre row fy0(J) in [fj0, fj0 + TY) (for an even fine height the last
     // coarse row is centred on the last fine row)
-    const int fi0 = blockIdx.x * TX, fj0 = blockIdx.y * TY;
+    const int fi0 = tbx * TX, fj0 = tby * TY;
     for (int t = tid; t < (TX / 2) * (TY / 2 + 1); t += NT) {
         const int tI = t % (TX / 2), lr = t / (TX / 2);
         int J;
@@ -298,6 +300,23 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
             Cr.z[k + 1] = 0;
         }
     }
+}
+
+template <typename T, int TX, int TY, int S, int MODE, bool FUSE, bool SP>
+__global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
+    smooth_tile_kernel(LevelArgs<T> L, T *__restrict__ zout, LevelArgs<T> Cz, LevelArgs<T> Cr, int zero_z) {
+    smooth_tile_body<T, TX, TY, S, MODE, FUSE, SP>(L, zout, Cz, Cr, zero_z, blockIdx.x, blockIdx.y);
+}
+
+// the same over the members of an ensemble (batch.h); members whose slot
+// flag is clear return at once
+template <typename T, int TX, int TY, int S, int MODE, bool FUSE, bool SP>
+__global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
+    smooth_tile_batch_kernel(const TileEntry<T> *E, const int *wg0, int ne, const EnsState *St) {
+    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const TileEntry<T> &x = E[e];
+    if (St && !St[x.member].active) return;
+    smooth_tile_body<T, TX, TY, S, MODE, FUSE, SP>(x.L, x.zout, x.Cz, x.Cr, x.zero_z, l % x.gx, l / x.gx);
 }
 
 // restriction: coarse tile CX x CY of C from fine level F
@@ -391,13 +410,14 @@ __global__ void __launch_bounds__(256) restrict_tile_kernel(LevelArgs<T> F, Leve
 
 // FP64 residual of the top level (64x16 tiles) -> FP32 source, |r|^2 partials
 template <bool SP>
-__global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float *b32, double *partial) {
+__device__ __forceinline__ static void refine_body(const LevelArgs<double> &L, float *b32, double *partial,
+                                                   int bx, int by, int gx) {
     constexpr int TX = 64, TY = 16, ZX = TX + 2, ZY = TY + 2, NT = 256;
     __shared__ vec2s<double> zs[ZY * ZX];
     __shared__ double ss[2 * kMaxWall * 40];
     __shared__ double ws[4];
     const int tid = threadIdx.x, m = L.m, n = L.n, nb = L.nb;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int i0 = bx * TX, j0 = by * TY;
     double si[40];
 #pragma unroll
     for (int k = 0; k < 40; k++) si[k] = L.Si[k];
@@ -443,7 +463,21 @@ __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float 
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if ((tid & 63) == 0) ws[tid >> 6] = acc;
     __syncthreads();
-    if (tid == 0) partial[blockIdx.x + gridDim.x * blockIdx.y] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+    if (tid == 0) partial[bx + gx * by] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+template <bool SP>
+__global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float *b32, double *partial) {
+    refine_body<SP>(L, b32, partial, blockIdx.x, blockIdx.y, gridDim.x);
+}
+
+template <bool SP>
+__global__ void __launch_bounds__(256) refine_batch_kernel(const RefineEntry *E, const int *wg0, int ne,
+                                                           const EnsState *St) {
+    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const RefineEntry &x = E[e];
+    if (St && !St[x.member].active) return;
+    refine_body<SP>(x.L, x.b32, x.partial, l % x.gx, l / x.gx, x.gx);
 }
 
 // ---------------------------------------------------------------------------
@@ -597,5 +631,137 @@ template int mg_vcycle_tiled<double>(CoarseArgs<double> &, bool, hipStream_t);
 template int mg_vcycle_tiled<float>(CoarseArgs<float> &, bool, hipStream_t);
 template int mg_smooth_level<double>(LevelArgs<double> &, int, hipStream_t);
 template int mg_smooth_level<float>(LevelArgs<float> &, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// batched launchers (batch.h).  Tile shape does not change what a tiled
+// launch computes (every node sees the same colour order and operands), so
+// batches use 32x16 tiles, or 64x32 when key.big
+// ---------------------------------------------------------------------------
+template <typename T, int S, int MODE, bool FUSE, bool SP>
+static void launch_tile_batch(bool big, const TileEntry<T> *E, const int *wg0, int ne, int total,
+                              const EnsState *St, hipStream_t st) {
+    if (big)
+        hipLaunchKernelGGL((smooth_tile_batch_kernel<T, 64, 32, S, MODE, FUSE, SP>), dim3(total),
+                           dim3(tile_threads(64, 32, tile_halo(S, FUSE))), 0, st, E, wg0, ne, St);
+    else
+        hipLaunchKernelGGL((smooth_tile_batch_kernel<T, 32, 16, S, MODE, FUSE, SP>), dim3(total),
+                           dim3(tile_threads(32, 16, tile_halo(S, FUSE))), 0, st, E, wg0, ne, St);
+}
+
+template <typename T, int S, int MODE>
+static void launch_tile_batch_fs(const TileKey &k, const TileEntry<T> *E, const int *wg0, int ne, int total,
+                                 const EnsState *St, hipStream_t st) {
+    if (k.fuse) {
+        if (k.sparse) launch_tile_batch<T, S, MODE, true, true>(k.big, E, wg0, ne, total, St, st);
+        else launch_tile_batch<T, S, MODE, true, false>(k.big, E, wg0, ne, total, St, st);
+    } else {
+        if (k.sparse) launch_tile_batch<T, S, MODE, false, true>(k.big, E, wg0, ne, total, St, st);
+        else launch_tile_batch<T, S, MODE, false, false>(k.big, E, wg0, ne, total, St, st);
+    }
+}
+
+template <typename T, int S>
+static void launch_tile_batch_m(const TileKey &k, const TileEntry<T> *E, const int *wg0, int ne, int total,
+                                const EnsState *St, hipStream_t st) {
+    if (k.mode == TILE_ZERO) launch_tile_batch_fs<T, S, TILE_ZERO>(k, E, wg0, ne, total, St, st);
+    else if (k.mode == TILE_PROLONG) launch_tile_batch_fs<T, S, TILE_PROLONG>(k, E, wg0, ne, total, St, st);
+    else launch_tile_batch_fs<T, S, TILE_LOAD>(k, E, wg0, ne, total, St, st);
+}
+
+template <typename T>
+int batch_tile(const TileKey &key, const TileEntry<T> *E, const int *wg0, int ne, int total, const EnsState *St,
+               hipStream_t st) {
+    if (ne == 0) return STZ_OK;
+    if (key.sweeps == 2) launch_tile_batch_m<T, 2>(key, E, wg0, ne, total, St, st);
+    else if (key.sweeps == 1) launch_tile_batch_m<T, 1>(key, E, wg0, ne, total, St, st);
+    else {
+        set_error("batched smoother launch with %d sweeps", key.sweeps);
+        return STZ_ERR_ARG;
+    }
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+template int batch_tile<double>(const TileKey &, const TileEntry<double> *, const int *, int, int, const EnsState *,
+                                hipStream_t);
+template int batch_tile<float>(const TileKey &, const TileEntry<float> *, const int *, int, int, const EnsState *,
+                               hipStream_t);
+
+int batch_refine(const RefineEntry *E, const int *wg0, int ne, int total, int sparse, const EnsState *St,
+                 hipStream_t st) {
+    if (ne == 0) return STZ_OK;
+    if (sparse) hipLaunchKernelGGL(refine_batch_kernel<true>, dim3(total), dim3(256), 0, st, E, wg0, ne, St);
+    else hipLaunchKernelGGL(refine_batch_kernel<false>, dim3(total), dim3(256), 0, st, E, wg0, ne, St);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+// the tiled V-cycle of one member as a list of launches, for the ensemble:
+// the same sequence mg_vcycle_tiled issues (and the same z/z2 exchanges),
+// recorded instead of launched.  Fails for a level the batched path does
+// not cover (per-colour kernels, separate restriction).
+template <typename T>
+static int plan_level(LevelArgs<T> &L, const LevelArgs<T> *C, int sweeps, int mode, const LevelArgs<T> *R,
+                      int level, int phase, std::vector<TileRec<T>> &out) {
+    if (!tiled_ok(L)) {
+        set_error("level %d (%d x %d) is not covered by the tiled kernels", level, L.m, L.n);
+        return STZ_ERR_ARG;
+    }
+    const LevelArgs<T> Cv = C ? *C : L;
+    const LevelArgs<T> Rv = R ? *R : L;
+    const int zz = R ? coarse_needs_zero(*R) : 0;
+    int idx = 0;
+    while (sweeps > 0) {
+        const int s = sweeps >= 2 ? 2 : 1;
+        const bool fuse = R && s == sweeps;
+        TileRec<T> r;
+        r.phase = phase;
+        r.level = level;
+        r.index = idx++;
+        r.key = TileKey{s, mode, fuse ? 1 : 0, L.sparse ? 1 : 0, 0};
+        r.L = L;
+        r.zout = L.z2;
+        r.Cz = Cv;
+        r.Cr = Rv;
+        r.zero_z = zz;
+        out.push_back(r);
+        std::swap(L.z, L.z2);
+        sweeps -= s;
+        mode = TILE_LOAD;
+    }
+    return STZ_OK;
+}
+
+template <typename T> int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out) {
+    int e;
+    out.clear();
+    // the z/z2 exchanges of one cycle must cancel on every level, so that
+    // the recorded pointers hold for every cycle of every member; the
+    // restriction rides on the last pre-smoothing launch
+    if ((((A.nu1 + 1) / 2 + (A.nu2 + 1) / 2) & 1) || A.nu1 < 1) {
+        set_error("batched V-cycle needs nu1 >= 1 and an even number of smoother launches per level (nu1 %d, "
+                  "nu2 %d)", A.nu1, A.nu2);
+        return STZ_ERR_ARG;
+    }
+    if (A.lc < 1) {
+        set_error("batched V-cycle needs at least one grid level above the workgroup solver");
+        return STZ_ERR_ARG;
+    }
+    for (int l = 0; l < A.lc; l++) {
+        if (!fuse_ok(A.lv[l])) {
+            set_error("level %d (%d x %d) would need a separate restriction launch", l, A.lv[l].m, A.lv[l].n);
+            return STZ_ERR_ARG;
+        }
+        if ((e = plan_level(A.lv[l], (const LevelArgs<T> *)nullptr, A.nu1,
+                            l == 0 ? (top_zero ? TILE_ZERO : TILE_LOAD) : TILE_ZERO, &A.lv[l + 1], l, 0, out)))
+            return e;
+    }
+    for (int l = A.lc; l >= 1; l--)
+        if ((e = plan_level(A.lv[l - 1], &A.lv[l], A.nu2, TILE_PROLONG, (const LevelArgs<T> *)nullptr, l - 1, 2,
+                            out)))
+            return e;
+    return STZ_OK;
+}
+template int mg_vcycle_plan<double>(CoarseArgs<double>, bool, std::vector<TileRec<double>> &);
+template int mg_vcycle_plan<float>(CoarseArgs<float>, bool, std::vector<TileRec<float>> &);
 
 }  // namespace stz

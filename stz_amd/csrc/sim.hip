@@ -23,6 +23,7 @@
 #include "diag.h"
 #include "mg.h"
 #include "mg_node.h"
+#include "sim_internal.h"
 
 namespace stz {
 
@@ -34,16 +35,6 @@ void set_error(const char *fmt, ...) {
     va_end(ap);
 }
 
-// ---------------------------------------------------------------------------
-// host-side operator hierarchy
-// ---------------------------------------------------------------------------
-struct HostLevel {
-    int m, n;
-    int nb = 0;              // rows at each wall whose stencil is not the interior one
-    bool xinv;               // stencil independent of i
-    std::vector<double> S;   // 40 per row (xinv) or per node
-    const double *at(int i, int j) const { return &S[(size_t)(xinv ? j : (i + m * j)) * 40]; }
-};
 
 // 1D prolongation parents (tgmg.in.cc:552-621): periodic axis with unwrapped
 // fine index f; returns count, unwrapped coarse indices and weights
@@ -205,29 +196,6 @@ static bool bottom_inverse(const HostLevel &B, std::vector<double> &inv) {
     return true;
 }
 
-// ---------------------------------------------------------------------------
-// device hierarchy for one precision
-// ---------------------------------------------------------------------------
-template <typename T> struct DevMG {
-    int L = 0;
-    CoarseArgs<T> args{};
-    T *S[kMaxLevels] = {};
-    T *zb[kMaxLevels] = {};  // z and b of levels (own allocations)
-    T *pack = nullptr;       // constant part of the workgroup solver's LDS image
-    size_t pack_elems = 0;
-    size_t lds = 0;
-
-    void free_all() {
-        for (int l = 0; l < kMaxLevels; l++) {
-            if (S[l]) (void)hipFree(S[l]);
-            if (zb[l]) (void)hipFree(zb[l]);
-            S[l] = zb[l] = nullptr;
-        }
-        if (pack) (void)hipFree(pack);
-        pack = nullptr;
-        pack_elems = 0;
-    }
-};
 
 template <typename T> static int upload(T **dst, const std::vector<double> &src) {
     std::vector<T> tmp(src.begin(), src.end());
@@ -240,50 +208,6 @@ template <typename T> static int upload(T **dst, const std::vector<double> &src)
 
 using namespace stz;
 
-// ---------------------------------------------------------------------------
-// the simulation object
-// ---------------------------------------------------------------------------
-struct stz_sim {
-    stz_sim_params p;
-    SimConst c;
-    int precision;
-    int device;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    double time = 0;
-    // fields
-    double *planes = nullptr;
-    FieldPtrs cur{}, alt{};
-    double *cu = nullptr;              // c_field::cu: diagnostic output plane
-    double *un = nullptr, *vn = nullptr;  // new u, v of the direct step
-    double *vel = nullptr, *src = nullptr;
-    uint32_t *dec = nullptr;
-    bool rec_dec = false;
-    // multigrid
-    std::vector<HostLevel> hl;
-    double viscdt_prev = -1;
-    DevMG<double> mg64;
-    DevMG<float> mg32;
-    double *partial = nullptr, *dnorm = nullptr, *hnorm = nullptr;
-    int nblocks_norm = 0;
-    double acc_base = 0;
-    stz_mg_config cfg{};
-    int lim = 0, solves = 0, vcount = 0;
-    stz_solve_info last{};
-
-    LevelArgs<double> top64() const {
-        LevelArgs<double> L = mg64.args.lv[0];
-        L.m = c.m;
-        L.n = c.n + 1;
-        L.z = vel;
-        L.b = src;
-        L.S = mg64.S[0];
-        L.rs = 1;
-        L.cs = 0;
-        L.xodd = c.m & 1;
-        return L;
-    }
-};
 
 static void set_defaults(stz_mg_config &g) {
     std::memset(&g, 0, sizeof g);
@@ -399,7 +323,7 @@ static int set_level_stencils(stz_sim *s, DevMG<T> &mg, const std::vector<std::v
 }
 
 // (re)build operators for viscosity/dt = viscdt (qs_multi::init, qs_multi.cc:31-54)
-static int mg_build(stz_sim *s, double viscdt) {
+int mg_build(stz_sim *s, double viscdt) {
     if (viscdt == s->viscdt_prev) return STZ_OK;
     s->viscdt_prev = viscdt;
     top_stencil(s->c, viscdt, s->hl[0]);

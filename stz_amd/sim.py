@@ -31,8 +31,25 @@ class ShearSim:
         self._h = h
         self._lib = lib
 
+    @classmethod
+    def _borrow(cls, owner, handle, params, precision):
+        """a member of an ensemble: the ensemble owns the handle"""
+        self = cls.__new__(cls)
+        self.params = dict(params)
+        self._p = _lib.make_params(params)
+        self.m, self.n = self._p.m, self._p.n
+        self.precision = precision
+        self._h = handle
+        self._lib = _lib.load()
+        self._owner = owner
+        return self
+
     # ---- lifetime
     def close(self):
+        if getattr(self, "_owner", None) is not None:
+            self._h = None
+            self._owner = None
+            return
         if getattr(self, "_h", None):
             self._lib.stz_sim_destroy(self._h)
             self._h = None
