@@ -383,7 +383,8 @@ int stz_bd_set_edge_velocity(stz_bd *b, double u_left, double u_right);
  * its own parameters (grid size included), and steps them together: every
  * kernel of stz_sim_step_quasistatic runs as ONE launch over all members,
  * each member's adaptive V-cycle count (tgmg_predict) is decided on the
- * device, and a step replays from HIP graphs with one host synchronisation.
+ * device, and a step needs one host synchronisation (optionally replayed
+ * from a HIP graph).
  * Every member computes exactly what stz_sim_step_quasistatic computes for
  * it alone.  Requirements: all members have the same stz_mg_config, with
  * kernels = 0, nu1 >= 1 and ceil(nu1/2) + ceil(nu2/2) even (nu1 = nu2 is
@@ -402,7 +403,9 @@ int stz_ensemble_size(stz_ensemble *e, int *count);
 int stz_ensemble_member(stz_ensemble *e, int k, stz_sim **member);
 /* Multigrid settings of every member (stz_sim_set_mg_config on each). */
 int stz_ensemble_set_mg_config(stz_ensemble *e, const stz_mg_config *cfg);
-/* Replay steps from HIP graphs (1, default) or launch them directly (0). */
+/* Replay steps from HIP graphs (1) or launch them directly (0, default: on
+ * ROCm 7.2 the same launches replayed from a graph took about 0.8 ms more
+ * per step of a 256-member ensemble than issued directly on the stream). */
 int stz_ensemble_use_graphs(stz_ensemble *e, int on);
 /* nsteps quasi-static steps of every member.  Stops after the first step in
  * which a member's solve fails and returns that status; the failing member
@@ -412,6 +415,12 @@ int stz_ensemble_step_quasistatic(stz_ensemble *e, double dt, int nsteps);
 /* Counters since creation: out4 = {steps, cycle slots launched, slots
  * launched after the planned ones, graph captures}. */
 int stz_ensemble_stats(stz_ensemble *e, int64_t *out4);
+/* The tile-shape tuner: when the tables are built (first step, new dt or
+ * settings), each batched smoother launch is timed with 32x16 tiles and,
+ * where the level widths allow, 64x32 tiles, and the faster kept (the shape
+ * changes speed only, never results).  out2 = {launches kept at 32x16,
+ * launches kept at 64x32} of the current tables. */
+int stz_ensemble_tile_choice(stz_ensemble *e, int32_t *out2);
 
 #ifdef __cplusplus
 }

@@ -175,6 +175,7 @@ _SIGS = {
     "stz_ensemble_use_graphs": [C.c_void_p, C.c_int],
     "stz_ensemble_step_quasistatic": [C.c_void_p, C.c_double, C.c_int],
     "stz_ensemble_stats": [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")],
+    "stz_ensemble_tile_choice": [C.c_void_p, _ip],
 }
 _RESTYPE = {"stz_last_error": C.c_char_p}
 

@@ -1,7 +1,7 @@
 // batch.h -- batched launches for an ensemble of independent quasi-static
 // simulations.  One launch runs one kernel of the step on every member it
-// lists: its workgroups are split among the members by a prefix table
-// (wg0[e] <= workgroup < wg0[e+1] belongs to entry e), and each workgroup
+// lists: its workgroups are split among the members by a table that gives
+// each workgroup its entry and its index within that entry, and each workgroup
 // runs the single-simulation kernel body on its member's arrays, so every
 // member computes exactly what its own simulation would.  The per-member
 // tables live in device memory and do not change from step to step, which
@@ -28,6 +28,7 @@ struct EnsState {
     double acc;        // residual at the last test
     int lim, n, k, remaining;  // tgmg_predict limit, cycles counted, loop batch, cycles left
     int phase, active;         // ENS_*; 1 while the member runs the current cycle slot
+    int resid;                 // 1 if the slot's cycle is followed by a residual evaluation
     int cycles_run, solves, vcount;
     int status;        // STZ_OK, STZ_ERR_NOT_CONVERGED, STZ_ERR_NAN
 };
@@ -35,17 +36,13 @@ struct EnsState {
 // predictor settings shared by all members (tgmg_config.hh:26-45)
 struct EnsPredict {
     int mult, decay, max_thresh, extra;
+    int mixed;  // residual after every cycle but the last (mixed) or only before a test (FP64)
 };
 
-// member index of workgroup w
-__device__ __forceinline__ static int batch_entry(const int *wg0, int ne, int w) {
-    int lo = 0, hi = ne - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (wg0[mid] <= w) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+// entry and workgroup index within it of workgroup w: map holds one
+// (entry, local index) pair per workgroup, a single scalar load
+__device__ __forceinline__ static int2 batch_slot(const int *map, int w) {
+    return reinterpret_cast<const int2 *>(map)[w];
 }
 
 // one member's cell-grid kernels (advection, chi diffusion, source, correction)
@@ -93,17 +90,17 @@ struct NormEntry {
     const double *partial;
 };
 
-// launchers: E and wg0 are device arrays of ne entries (wg0 has ne + 1);
-// total = wg0[ne].  With a non-null state array, members whose `active`
-// flag is clear return at once (cycle slots); correct runs only members in
-// ENS_DONE.
+// launchers: E is a device array of ne entries, map one (entry, index)
+// pair per workgroup, total workgroups.  With a non-null state array,
+// members whose `active` flag is clear return at once (cycle slots; the
+// residual launches test `resid`); correct runs only members in ENS_DONE.
 int batch_time(EnsState *S, int nmem, double dt, hipStream_t st);
-int batch_advect(const CellEntry *E, const int *wg0, int ne, int total, int parity, double dt, hipStream_t st);
-int batch_chi_diffusion(const CellEntry *E, const int *wg0, int ne, int total, int parity, double dt,
+int batch_advect(const CellEntry *E, const int *map, int ne, int total, int parity, double dt, hipStream_t st);
+int batch_chi_diffusion(const CellEntry *E, const int *map, int ne, int total, int parity, double dt,
                         hipStream_t st);
-int batch_source(const CellEntry *E, const int *wg0, int ne, int total, int parity, const EnsState *S, double dt,
+int batch_source(const CellEntry *E, const int *map, int ne, int total, int parity, const EnsState *S, double dt,
                  hipStream_t st);
-int batch_correct(const CellEntry *E, const int *wg0, int ne, int total, int parity, const EnsState *S, double dt,
+int batch_correct(const CellEntry *E, const int *map, int ne, int total, int parity, const EnsState *S, double dt,
                   hipStream_t st);
 
 // tiled smoother launch of variant (S sweeps, mode, fuse, sparse, 64x32 tiles)
@@ -114,7 +111,7 @@ struct TileKey {
     }
 };
 template <typename T>
-int batch_tile(const TileKey &key, const TileEntry<T> *E, const int *wg0, int ne, int total, const EnsState *S,
+int batch_tile(const TileKey &key, const TileEntry<T> *E, const int *map, int ne, int total, const EnsState *S,
                hipStream_t st);
 // one launch of a member's tiled V-cycle: phase 0 pre-smoothing (with the
 // fused restriction), 2 post-smoothing; level and launch index within it
@@ -130,13 +127,13 @@ template <typename T> struct TileRec {
 template <typename T> int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out);
 
 // mixed: FP64 residual -> FP32 source of the correction equation + norm partials
-int batch_refine(const RefineEntry *E, const int *wg0, int ne, int total, int sparse, const EnsState *S,
+int batch_refine(const RefineEntry *E, const int *map, int ne, int total, int sparse, const EnsState *S,
                  hipStream_t st);
 template <typename T> int batch_coarse(const CoarseEntry<T> *E, int ne, size_t lds, const EnsState *S, hipStream_t st);
 template <typename T> int batch_set_lds_limit(size_t bytes);
 // fp64: |r|^2 partials of the top level, blocks as mg_resid_norm
-int batch_resid_norm(const RefineEntry *E, const int *wg0, int ne, int total, const EnsState *S, hipStream_t st);
-int batch_correct64(const CorrEntry *E, const int *wg0, int ne, int total, const EnsState *S, hipStream_t st);
+int batch_resid_norm(const RefineEntry *E, const int *map, int ne, int total, const EnsState *S, hipStream_t st);
+int batch_correct64(const CorrEntry *E, const int *map, int ne, int total, const EnsState *S, hipStream_t st);
 
 // the per-member V-cycle count (mg.hip): mode 0 starts a solve, mode 1
 // follows a cycle slot, mode 2 marks corrected members

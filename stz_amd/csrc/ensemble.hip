@@ -15,8 +15,9 @@
 // announce, and tail, then synchronises once and reads the members' states;
 // a member that needs more cycles than planned gets further slots and
 // another tail.  Launch arguments never change between steps (the state they
-// depend on is in EnsState on the device), so head/slot/tail are captured
-// once as HIP graphs and replayed.
+// depend on is in EnsState on the device), so a step is captured once per
+// step parity and slot count as one HIP graph and replayed; further slots
+// and tails replay graphs of their own.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,7 +45,7 @@ struct DevBuf {
 
 template <typename T> struct DevTable {
     T *E = nullptr;
-    int *wg0 = nullptr;
+    int *map = nullptr;  // (entry, workgroup within it) per workgroup
     int ne = 0, total = 0;
 };
 
@@ -66,7 +67,7 @@ struct stz_ensemble {
     int device = 0, precision = 0;
     std::vector<stz_sim *> mem;
     hipStream_t stream = nullptr;
-    bool use_graphs = true;
+    bool use_graphs = false;  // direct launches measured faster than graph replay
     // tables (valid for dt_tab and the members' plane-set parity below)
     bool built = false;
     double dt_tab = 0;
@@ -87,18 +88,21 @@ struct stz_ensemble {
     // state: device array and a pinned host copy
     EnsState *dst = nullptr, *hst = nullptr;
     // graphs
-    hipGraphExec_t g_head[2] = {nullptr, nullptr}, g_tail[2] = {nullptr, nullptr}, g_slot = nullptr;
+    hipGraphExec_t g_tail[2] = {nullptr, nullptr}, g_slot = nullptr;
+    std::map<std::pair<int, int>, hipGraphExec_t> g_step;  // whole step by (parity, planned slots)
     int64_t stats[4] = {0, 0, 0, 0};
+    int tuned[2] = {0, 0};        // launch positions that kept 32x16 / 64x32 tiles
 
     int nmem() const { return (int)mem.size(); }
     void free_graphs() {
         for (int p = 0; p < 2; p++) {
-            if (g_head[p]) (void)hipGraphExecDestroy(g_head[p]);
             if (g_tail[p]) (void)hipGraphExecDestroy(g_tail[p]);
-            g_head[p] = g_tail[p] = nullptr;
+            g_tail[p] = nullptr;
         }
         if (g_slot) (void)hipGraphExecDestroy(g_slot);
         g_slot = nullptr;
+        for (auto &g : g_step) (void)hipGraphExecDestroy(g.second);
+        g_step.clear();
     }
     void free_tables() {
         free_graphs();
@@ -106,6 +110,7 @@ struct stz_ensemble {
         bufs.clear();
         plan64 = VCyclePlan<double>();
         plan32 = VCyclePlan<float>();
+        tuned[0] = tuned[1] = 0;
         built = false;
     }
 };
@@ -123,16 +128,25 @@ template <typename T> int upload_vec(stz_ensemble *e, const std::vector<T> &h, T
     return STZ_OK;
 }
 
-// entries with their workgroup counts -> device table with prefix
+// workgroup -> (entry, index within the entry) for entries of wgs[k] workgroups
+int make_map(stz_ensemble *e, const std::vector<int> &wgs, int **map, int *total) {
+    std::vector<int> m;
+    for (size_t k = 0; k < wgs.size(); k++)
+        for (int w = 0; w < wgs[k]; w++) {
+            m.push_back((int)k);
+            m.push_back(w);
+        }
+    *total = (int)(m.size() / 2);
+    return upload_vec(e, m, map);
+}
+
+// entries with their workgroup counts -> device table
 template <typename T>
 int make_table(stz_ensemble *e, const std::vector<T> &E, const std::vector<int> &wgs, DevTable<T> &t) {
-    std::vector<int> wg0(E.size() + 1, 0);
-    for (size_t k = 0; k < E.size(); k++) wg0[k + 1] = wg0[k] + wgs[k];
     int rc;
     if ((rc = upload_vec(e, E, &t.E))) return rc;
-    if ((rc = upload_vec(e, wg0, &t.wg0))) return rc;
+    if ((rc = make_map(e, wgs, &t.map, &t.total))) return rc;
     t.ne = (int)E.size();
-    t.total = wg0.back();
     return STZ_OK;
 }
 
@@ -140,6 +154,35 @@ int make_table(stz_ensemble *e, const std::vector<T> &E, const std::vector<int> 
 template <typename T> void tile_grid(const LevelArgs<T> &L, bool big, int &gx, int &gy) {
     gx = big ? L.m / 64 : L.m / 32;
     gy = big ? (L.n + 31) / 32 : (L.n + 15) / 16;
+}
+
+// average time of one pass over `ls` (ungated launches, after one untimed
+// pass).  Between steps the buffers a tiled launch writes (a level's z
+// buffers, the coarser level's source) are rewritten by the next cycle
+// before they are read, except the FP64 top level's z, which is vel: the
+// caller keeps that aside.
+template <typename T> int time_launches(stz_ensemble *e, const std::vector<TileLaunch<T>> &ls, double *us) {
+    hipEvent_t a, b;
+    STZ_HIP(hipEventCreate(&a));
+    STZ_HIP(hipEventCreate(&b));
+    int rc = STZ_OK;
+    for (int rep = 0; rep < 4 && !rc; rep++) {
+        if (rep == 1) (void)hipEventRecord(a, e->stream);
+        for (const TileLaunch<T> &l : ls)
+            if ((rc = batch_tile<T>(l.key, l.t.E, l.t.map, l.t.ne, l.t.total, nullptr, e->stream))) break;
+    }
+    float ms = 0;
+    if (!rc) {
+        (void)hipEventRecord(b, e->stream);
+        if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+            set_error("timing the tile launches failed");
+            rc = STZ_ERR_HIP;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *us = 1e3 * ms / 3;
+    return rc;
 }
 
 // every member's V-cycle plan, launches grouped by position: pre-smoothing
@@ -180,28 +223,62 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
         for (auto &rk : g.second)
             if (std::find(keys.begin(), keys.end(), rk.first.key) == keys.end()) keys.push_back(rk.first.key);
         for (const TileKey &key : keys) {
-            std::vector<TileEntry<T>> E;
-            std::vector<int> wgs;
-            for (auto &rk : g.second) {
-                if (!(rk.first.key == key)) continue;
-                const TileRec<T> &r = rk.first;
-                TileEntry<T> te;
-                te.member = rk.second;
-                int gx, gy;
-                tile_grid(r.L, key.big != 0, gx, gy);
-                te.gx = gx;
-                te.L = r.L;
-                te.zout = r.zout;
-                te.Cz = r.Cz;
-                te.Cr = r.Cr;
-                te.zero_z = r.zero_z;
-                E.push_back(te);
-                wgs.push_back(gx * gy);
+            std::vector<const TileRec<T> *> R;
+            std::vector<int> who;
+            for (auto &rk : g.second)
+                if (rk.first.key == key) {
+                    R.push_back(&rk.first);
+                    who.push_back(rk.second);
+                }
+            // candidates: 32x16 tiles for all; 64x32 tiles where the level
+            // width allows them (the rest 32x16).  Tile shape does not change
+            // what a launch computes, only how much of the halo is recomputed
+            // and how many workgroups share the chip, so the faster is kept.
+            std::vector<std::vector<TileLaunch<T>>> cands;
+            for (int big = 0; big < 2; big++) {
+                std::vector<TileLaunch<T>> cand;
+                for (int part = 0; part < 2; part++) {
+                    const bool want_big = big && part == 0;
+                    std::vector<TileEntry<T>> E;
+                    std::vector<int> wgs;
+                    for (size_t q = 0; q < R.size(); q++) {
+                        const TileRec<T> &r = *R[q];
+                        const bool can_big = r.L.m % 64 == 0;
+                        if ((big ? (can_big ? 0 : 1) : 0) != part) continue;
+                        TileEntry<T> te;
+                        te.member = who[q];
+                        int gx, gy;
+                        tile_grid(r.L, want_big, gx, gy);
+                        te.gx = gx;
+                        te.L = r.L;
+                        te.zout = r.zout;
+                        te.Cz = r.Cz;
+                        te.Cr = r.Cr;
+                        te.zero_z = r.zero_z;
+                        E.push_back(te);
+                        wgs.push_back(gx * gy);
+                    }
+                    if (E.empty()) continue;
+                    TileLaunch<T> tl;
+                    tl.key = key;
+                    tl.key.big = want_big ? 1 : 0;
+                    if ((rc = make_table(e, E, wgs, tl.t))) return rc;
+                    cand.push_back(tl);
+                }
+                bool any_big = false;
+                for (auto &tl : cand) any_big |= tl.key.big != 0;
+                if (big == 0 || any_big) cands.push_back(cand);
             }
-            TileLaunch<T> tl;
-            tl.key = key;
-            if ((rc = make_table(e, E, wgs, tl.t))) return rc;
-            (std::get<0>(g.first) == 0 ? P.down : P.up).push_back(tl);
+            int best = 0;
+            if (cands.size() > 1) {
+                double t[2];
+                for (int c = 0; c < 2; c++)
+                    if ((rc = time_launches<T>(e, cands[c], &t[c]))) return rc;
+                best = t[1] < t[0] ? 1 : 0;
+            }
+            e->tuned[best]++;
+            auto &dst = std::get<0>(g.first) == 0 ? P.down : P.up;
+            dst.insert(dst.end(), cands[best].begin(), cands[best].end());
         }
     }
     if ((rc = upload_vec(e, coarse, &P.coarse))) return rc;
@@ -233,6 +310,7 @@ int build_tables(stz_ensemble *e, double dt) {
     e->pred.decay = e->cfg.predict_decay;
     e->pred.max_thresh = e->cfg.predict_mult * e->cfg.predict_max_iters;
     e->pred.extra = e->cfg.extra_cycles;
+    e->pred.mixed = e->precision == STZ_PREC_MIXED ? 1 : 0;
     // operators of every member for this dt (projection_impl)
     for (int k = 0; k < N; k++)
         if ((rc = mg_build(e->mem[k], e->mem[k]->p.viscosity / dt))) return rc;
@@ -313,17 +391,34 @@ int build_tables(stz_ensemble *e, double dt) {
     if ((rc = make_table(e, cells, wcell, e->cells))) return rc;
     e->srcs = e->cells;
     {
-        std::vector<int> wg0(N + 1, 0);
-        for (int k = 0; k < N; k++) wg0[k + 1] = wg0[k] + wsrc[k];
-        if ((rc = upload_vec(e, wg0, &e->srcs.wg0))) return rc;
-        e->srcs.total = wg0.back();
+        if ((rc = make_map(e, wsrc, &e->srcs.map, &e->srcs.total))) return rc;
     }
     if ((rc = make_table(e, chis, wchi, e->chis))) return rc;
     if ((rc = make_table(e, refs, wref, e->refs))) return rc;
     if ((rc = make_table(e, corrs, wcorr, e->corrs))) return rc;
     if ((rc = upload_vec(e, norms, &e->norms))) return rc;
-    if (e->precision == STZ_PREC_MIXED) rc = build_plan<float>(e, e->plan32, &stz_sim::mg32, true);
-    else rc = build_plan<double>(e, e->plan64, &stz_sim::mg64, false);
+    if (e->precision == STZ_PREC_MIXED) {
+        rc = build_plan<float>(e, e->plan32, &stz_sim::mg32, true);
+    } else {
+        // the tile-shape timing runs the FP64 top level's last smoothing
+        // launch, which writes vel: keep the velocities aside meanwhile
+        std::vector<size_t> off(N + 1, 0);
+        for (int k = 0; k < N; k++) off[k + 1] = off[k] + 2 * (size_t)e->mem[k]->c.m * (e->mem[k]->c.n + 1);
+        double *keep = nullptr;
+        STZ_HIP(hipMalloc(&keep, off[N] * sizeof(double)));
+        for (int k = 0; k < N && !rc; k++)
+            if (hipMemcpyAsync(keep + off[k], e->mem[k]->vel, (off[k + 1] - off[k]) * sizeof(double),
+                               hipMemcpyDeviceToDevice, e->stream) != hipSuccess)
+                rc = STZ_ERR_HIP;
+        if (!rc) rc = build_plan<double>(e, e->plan64, &stz_sim::mg64, false);
+        for (int k = 0; k < N; k++)
+            if (hipMemcpyAsync(e->mem[k]->vel, keep + off[k], (off[k + 1] - off[k]) * sizeof(double),
+                               hipMemcpyDeviceToDevice, e->stream) != hipSuccess && !rc)
+                rc = STZ_ERR_HIP;
+        if (hipStreamSynchronize(e->stream) != hipSuccess && !rc) rc = STZ_ERR_HIP;
+        (void)hipFree(keep);
+        if (rc == STZ_ERR_HIP) set_error("saving the velocities around the tile timing failed");
+    }
     if (rc) return rc;
     e->dt_tab = dt;
     e->parity = 0;
@@ -338,13 +433,13 @@ int launch_head(stz_ensemble *e, int parity, double dt) {
     const int N = e->nmem();
     if ((rc = batch_time(e->dst, N, dt, st))) return rc;
     const DevTable<CellEntry> &C = e->cells, &X = e->chis, &S = e->srcs;
-    if ((rc = batch_advect(C.E, C.wg0, C.ne, C.total, parity, dt, st))) return rc;
-    if ((rc = batch_chi_diffusion(X.E, X.wg0, X.ne, X.total, parity, dt, st))) return rc;
+    if ((rc = batch_advect(C.E, C.map, C.ne, C.total, parity, dt, st))) return rc;
+    if ((rc = batch_chi_diffusion(X.E, X.map, X.ne, X.total, parity, dt, st))) return rc;
     // the advected plane set is current from here on
-    if ((rc = batch_source(S.E, S.wg0, S.ne, S.total, parity ^ 1, e->dst, dt, st))) return rc;
+    if ((rc = batch_source(S.E, S.map, S.ne, S.total, parity ^ 1, e->dst, dt, st))) return rc;
     const DevTable<RefineEntry> &R = e->refs;
-    if (e->precision == STZ_PREC_MIXED) rc = batch_refine(R.E, R.wg0, R.ne, R.total, e->top_sparse, nullptr, st);
-    else rc = batch_resid_norm(R.E, R.wg0, R.ne, R.total, nullptr, st);
+    if (e->precision == STZ_PREC_MIXED) rc = batch_refine(R.E, R.map, R.ne, R.total, e->top_sparse, nullptr, st);
+    else rc = batch_resid_norm(R.E, R.map, R.ne, R.total, nullptr, st);
     if (rc) return rc;
     return batch_control(e->dst, e->norms, N, e->pred, 0, st);
 }
@@ -352,10 +447,10 @@ int launch_head(stz_ensemble *e, int parity, double dt) {
 template <typename T> int launch_vcycle(stz_ensemble *e, const VCyclePlan<T> &P) {
     int rc;
     for (const TileLaunch<T> &l : P.down)
-        if ((rc = batch_tile<T>(l.key, l.t.E, l.t.wg0, l.t.ne, l.t.total, e->dst, e->stream))) return rc;
+        if ((rc = batch_tile<T>(l.key, l.t.E, l.t.map, l.t.ne, l.t.total, e->dst, e->stream))) return rc;
     if ((rc = batch_coarse<T>(P.coarse, P.ncoarse, P.lds, e->dst, e->stream))) return rc;
     for (const TileLaunch<T> &l : P.up)
-        if ((rc = batch_tile<T>(l.key, l.t.E, l.t.wg0, l.t.ne, l.t.total, e->dst, e->stream))) return rc;
+        if ((rc = batch_tile<T>(l.key, l.t.E, l.t.map, l.t.ne, l.t.total, e->dst, e->stream))) return rc;
     return STZ_OK;
 }
 
@@ -366,11 +461,11 @@ int launch_slot(stz_ensemble *e) {
     if (e->precision == STZ_PREC_MIXED) {
         if ((rc = launch_vcycle<float>(e, e->plan32))) return rc;
         const DevTable<CorrEntry> &K = e->corrs;
-        if ((rc = batch_correct64(K.E, K.wg0, K.ne, K.total, e->dst, st))) return rc;
-        if ((rc = batch_refine(R.E, R.wg0, R.ne, R.total, e->top_sparse, e->dst, st))) return rc;
+        if ((rc = batch_correct64(K.E, K.map, K.ne, K.total, e->dst, st))) return rc;
+        if ((rc = batch_refine(R.E, R.map, R.ne, R.total, e->top_sparse, e->dst, st))) return rc;
     } else {
         if ((rc = launch_vcycle<double>(e, e->plan64))) return rc;
-        if ((rc = batch_resid_norm(R.E, R.wg0, R.ne, R.total, e->dst, st))) return rc;
+        if ((rc = batch_resid_norm(R.E, R.map, R.ne, R.total, e->dst, st))) return rc;
     }
     return batch_control(e->dst, e->norms, e->nmem(), e->pred, 1, st);
 }
@@ -378,7 +473,7 @@ int launch_slot(stz_ensemble *e) {
 int launch_tail(stz_ensemble *e, int parity, double dt) {
     int rc;
     const DevTable<CellEntry> &C = e->cells;
-    if ((rc = batch_correct(C.E, C.wg0, C.ne, C.total, parity ^ 1, e->dst, dt, e->stream))) return rc;
+    if ((rc = batch_correct(C.E, C.map, C.ne, C.total, parity ^ 1, e->dst, dt, e->stream))) return rc;
     return batch_control(e->dst, e->norms, e->nmem(), e->pred, 2, e->stream);
 }
 
@@ -406,21 +501,40 @@ template <typename F> int capture(stz_ensemble *e, hipGraphExec_t *out, F fn) {
     return STZ_OK;
 }
 
-int run_head(stz_ensemble *e, int parity, double dt) {
-    if (!e->use_graphs) return launch_head(e, parity, dt);
-    int rc;
-    if (!e->g_head[parity] && (rc = capture(e, &e->g_head[parity], [&] { return launch_head(e, parity, dt); })))
-        return rc;
-    STZ_HIP(hipGraphLaunch(e->g_head[parity], e->stream));
-    return STZ_OK;
-}
-
 int run_slot(stz_ensemble *e) {
     e->stats[1]++;
     if (!e->use_graphs) return launch_slot(e);
     int rc;
     if (!e->g_slot && (rc = capture(e, &e->g_slot, [&] { return launch_slot(e); }))) return rc;
     STZ_HIP(hipGraphLaunch(e->g_slot, e->stream));
+    return STZ_OK;
+}
+
+// head, `slots` cycle slots and tail as one graph
+int run_step(stz_ensemble *e, int parity, int slots, double dt) {
+    int rc;
+    e->stats[1] += slots;
+    if (!e->use_graphs) {
+        if ((rc = launch_head(e, parity, dt))) return rc;
+        for (int q = 0; q < slots; q++)
+            if ((rc = launch_slot(e))) return rc;
+        return launch_tail(e, parity, dt);
+    }
+    const std::pair<int, int> key(parity, slots);
+    auto it = e->g_step.find(key);
+    if (it == e->g_step.end()) {
+        hipGraphExec_t g = nullptr;
+        rc = capture(e, &g, [&] {
+            int r;
+            if ((r = launch_head(e, parity, dt))) return r;
+            for (int q = 0; q < slots; q++)
+                if ((r = launch_slot(e))) return r;
+            return launch_tail(e, parity, dt);
+        });
+        if (rc) return rc;
+        it = e->g_step.emplace(key, g).first;
+    }
+    STZ_HIP(hipGraphLaunch(it->second, e->stream));
     return STZ_OK;
 }
 
@@ -526,6 +640,13 @@ int stz_ensemble_use_graphs(stz_ensemble *e, int on) {
     return STZ_OK;
 }
 
+int stz_ensemble_tile_choice(stz_ensemble *e, int32_t *out2) {
+    if (!e || !out2) { set_error("null argument"); return STZ_ERR_ARG; }
+    out2[0] = e->tuned[0];
+    out2[1] = e->tuned[1];
+    return STZ_OK;
+}
+
 int stz_ensemble_stats(stz_ensemble *e, int64_t *out4) {
     if (!e || !out4) { set_error("null argument"); return STZ_ERR_ARG; }
     for (int k = 0; k < 4; k++) out4[k] = e->stats[k];
@@ -573,10 +694,7 @@ int stz_ensemble_step_quasistatic(stz_ensemble *e, double dt, int nsteps) {
     for (int step = 0; step < nsteps && status == STZ_OK; step++) {
         const int p = e->parity;
         const int plan = planned_slots(e);
-        if ((rc = run_head(e, p, dt))) return rc;
-        for (int q = 0; q < plan; q++)
-            if ((rc = run_slot(e))) return rc;
-        if ((rc = run_tail(e, p, dt))) return rc;
+        if ((rc = run_step(e, p, plan, dt))) return rc;
         if ((rc = sync_state(e))) return rc;
         // members whose predictor asked for more cycles than planned
         for (;;) {

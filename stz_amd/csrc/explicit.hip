@@ -650,35 +650,54 @@ __global__ void __launch_bounds__(256) time_batch_kernel(EnsState *S, int nmem, 
     if (k < nmem) S[k].time += dt;  // shear_sim.cc:283
 }
 
-__global__ void __launch_bounds__(256) advect_batch_kernel(const CellEntry *E, const int *wg0, int ne, int parity,
+__global__ void __launch_bounds__(256) advect_batch_kernel(const CellEntry *E, const int *map, int ne, int parity,
                                                            double dt) {
-    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const int2 el = batch_slot(map, blockIdx.x);
+    const int e = el.x, l = el.y;
+    // local copies: the body's stores cannot alias them, so each field is
+    // loaded once
     const CellEntry &x = E[e];
-    advect_body(x.c, x.f[parity], x.f[parity ^ 1], dt, x.dec, l % x.gx, l / x.gx);
+    const SimConst c = x.c;
+    const FieldPtrs o = x.f[parity], w = x.f[parity ^ 1];
+    const int gx = x.gx;
+    advect_body(c, o, w, dt, x.dec, l % gx, l / gx);
 }
 
-__global__ void __launch_bounds__(256) chi_diffusion_batch_kernel(const CellEntry *E, const int *wg0, int ne,
+__global__ void __launch_bounds__(256) chi_diffusion_batch_kernel(const CellEntry *E, const int *map, int ne,
                                                                   int parity, double dt) {
-    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const int2 el = batch_slot(map, blockIdx.x);
+    const int e = el.x, l = el.y;
     const CellEntry &x = E[e];
-    chi_diffusion_body(x.c, x.f[parity].chi, x.f[parity ^ 1].ddev, x.f[parity ^ 1].chi, dt, l % x.gx, l / x.gx);
+    const SimConst c = x.c;
+    const FieldPtrs o = x.f[parity], w = x.f[parity ^ 1];
+    const int gx = x.gx;
+    chi_diffusion_body(c, o.chi, w.ddev, w.chi, dt, l % gx, l / gx);
 }
 
-__global__ void __launch_bounds__(256) source_batch_kernel(const CellEntry *E, const int *wg0, int ne, int parity,
+__global__ void __launch_bounds__(256) source_batch_kernel(const CellEntry *E, const int *map, int ne, int parity,
                                                            const EnsState *S, double dt) {
-    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const int2 el = batch_slot(map, blockIdx.x);
+    const int e = el.x, l = el.y;
     const CellEntry &x = E[e];
-    const double xfac = 0.5 / dt * x.c.xsp, yfac = 0.5 / dt * x.c.ysp;
-    source_body(x.c, x.f[parity], xfac, yfac, x.xbc, ens_bvel(x.c, S[x.member].time), x.src, l % x.gx, l / x.gx);
+    const SimConst c = x.c;
+    const FieldPtrs f = x.f[parity];
+    const int gx = x.gx;
+    const double xfac = 0.5 / dt * c.xsp, yfac = 0.5 / dt * c.ysp;
+    source_body(c, f, xfac, yfac, x.xbc, ens_bvel(c, S[x.member].time), x.src, l % gx, l / gx);
 }
 
-__global__ void __launch_bounds__(256) correct_batch_kernel(const CellEntry *E, const int *wg0, int ne, int parity,
+__global__ void __launch_bounds__(256) correct_batch_kernel(const CellEntry *E, const int *map, int ne, int parity,
                                                             const EnsState *S, double dt) {
-    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const int2 el = batch_slot(map, blockIdx.x);
+    const int e = el.x, l = el.y;
     const CellEntry &x = E[e];
     const EnsState &st = S[x.member];
     if (st.phase != ENS_DONE) return;
-    correct_body(x.c, x.f[parity], x.vel, dt, ens_bvel(x.c, st.time), ens_bpos(x.c, st.time), l % x.gx, l / x.gx);
+    const SimConst c = x.c;
+    const FieldPtrs f = x.f[parity];
+    const int gx = x.gx;
+    const double t = st.time;
+    correct_body(c, f, x.vel, dt, ens_bvel(c, t), ens_bpos(c, t), l % gx, l / gx);
 }
 
 int batch_time(EnsState *S, int nmem, double dt, hipStream_t st) {
@@ -687,33 +706,33 @@ int batch_time(EnsState *S, int nmem, double dt, hipStream_t st) {
     return STZ_OK;
 }
 
-int batch_advect(const CellEntry *E, const int *wg0, int ne, int total, int parity, double dt, hipStream_t st) {
+int batch_advect(const CellEntry *E, const int *map, int ne, int total, int parity, double dt, hipStream_t st) {
     if (ne == 0) return STZ_OK;
-    hipLaunchKernelGGL(advect_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, wg0, ne, parity, dt);
+    hipLaunchKernelGGL(advect_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, map, ne, parity, dt);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
 
-int batch_chi_diffusion(const CellEntry *E, const int *wg0, int ne, int total, int parity, double dt,
+int batch_chi_diffusion(const CellEntry *E, const int *map, int ne, int total, int parity, double dt,
                         hipStream_t st) {
     if (ne == 0) return STZ_OK;
-    hipLaunchKernelGGL(chi_diffusion_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, wg0, ne, parity, dt);
+    hipLaunchKernelGGL(chi_diffusion_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, map, ne, parity, dt);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
 
-int batch_source(const CellEntry *E, const int *wg0, int ne, int total, int parity, const EnsState *S, double dt,
+int batch_source(const CellEntry *E, const int *map, int ne, int total, int parity, const EnsState *S, double dt,
                  hipStream_t st) {
     if (ne == 0) return STZ_OK;
-    hipLaunchKernelGGL(source_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, wg0, ne, parity, S, dt);
+    hipLaunchKernelGGL(source_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, map, ne, parity, S, dt);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
 
-int batch_correct(const CellEntry *E, const int *wg0, int ne, int total, int parity, const EnsState *S, double dt,
+int batch_correct(const CellEntry *E, const int *map, int ne, int total, int parity, const EnsState *S, double dt,
                   hipStream_t st) {
     if (ne == 0) return STZ_OK;
-    hipLaunchKernelGGL(correct_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, wg0, ne, parity, S, dt);
+    hipLaunchKernelGGL(correct_batch_kernel, dim3(total), dim3(64, 4), 0, st, E, map, ne, parity, S, dt);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }

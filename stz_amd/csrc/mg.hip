@@ -851,17 +851,20 @@ template size_t coarse_lds_bytes<float>(const CoarseArgs<float> &);
 // batched launches over the members of an ensemble (batch.h)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) resid_norm_batch_kernel(const RefineEntry *E, const int *wg0, int ne,
+__global__ void __launch_bounds__(256) resid_norm_batch_kernel(const RefineEntry *E, const int *map, int ne,
                                                                const EnsState *St) {
-    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const int2 el = batch_slot(map, blockIdx.x);
+    const int e = el.x, l = el.y;
     const RefineEntry &x = E[e];
-    if (St && !St[x.member].active) return;
-    resid_norm_body(x.L, x.partial, l);
+    if (St && !St[x.member].resid) return;
+    const LevelArgs<double> L = x.L;
+    resid_norm_body(L, x.partial, l);
 }
 
-__global__ void __launch_bounds__(256) correct64_batch_kernel(const CorrEntry *E, const int *wg0, int ne,
+__global__ void __launch_bounds__(256) correct64_batch_kernel(const CorrEntry *E, const int *map, int ne,
                                                               const EnsState *St) {
-    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const int2 el = batch_slot(map, blockIdx.x);
+    const int e = el.x, l = el.y;
     const CorrEntry &x = E[e];
     if (St && !St[x.member].active) return;
     const int idx = l * 256 + threadIdx.x;
@@ -941,6 +944,12 @@ __global__ void __launch_bounds__(256) control_batch_kernel(EnsState *S, const N
         st.cycles_run++;
     }
     ens_resolve(st, sum / st.denom, P);
+    // the single simulation evaluates the residual after every mixed cycle
+    // but the solve's last (one_cycle's need_resid), and in FP64 only where
+    // it tests it (top_l2)
+    const bool last_extra = st.phase == ENS_EXTRA && st.remaining == 1;
+    const bool before_test = st.phase != ENS_EXTRA && st.remaining == 1;
+    st.resid = st.active && (P.mixed ? !last_extra : before_test) ? 1 : 0;
 }
 
 template <typename T> int batch_coarse(const CoarseEntry<T> *E, int ne, size_t lds, const EnsState *St,
@@ -964,16 +973,16 @@ template <typename T> int batch_set_lds_limit(size_t bytes) {
 template int batch_set_lds_limit<double>(size_t);
 template int batch_set_lds_limit<float>(size_t);
 
-int batch_resid_norm(const RefineEntry *E, const int *wg0, int ne, int total, const EnsState *St, hipStream_t st) {
+int batch_resid_norm(const RefineEntry *E, const int *map, int ne, int total, const EnsState *St, hipStream_t st) {
     if (ne == 0) return STZ_OK;
-    hipLaunchKernelGGL(resid_norm_batch_kernel<double>, dim3(total), dim3(256), 0, st, E, wg0, ne, St);
+    hipLaunchKernelGGL(resid_norm_batch_kernel<double>, dim3(total), dim3(256), 0, st, E, map, ne, St);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
 
-int batch_correct64(const CorrEntry *E, const int *wg0, int ne, int total, const EnsState *St, hipStream_t st) {
+int batch_correct64(const CorrEntry *E, const int *map, int ne, int total, const EnsState *St, hipStream_t st) {
     if (ne == 0) return STZ_OK;
-    hipLaunchKernelGGL(correct64_batch_kernel, dim3(total), dim3(256), 0, st, E, wg0, ne, St);
+    hipLaunchKernelGGL(correct64_batch_kernel, dim3(total), dim3(256), 0, st, E, map, ne, St);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }

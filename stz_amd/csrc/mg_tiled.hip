@@ -312,11 +312,14 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
 // flag is clear return at once
 template <typename T, int TX, int TY, int S, int MODE, bool FUSE, bool SP>
 __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
-    smooth_tile_batch_kernel(const TileEntry<T> *E, const int *wg0, int ne, const EnsState *St) {
-    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    smooth_tile_batch_kernel(const TileEntry<T> *E, const int *map, int ne, const EnsState *St) {
+    const int2 el = batch_slot(map, blockIdx.x);
+    const int e = el.x, l = el.y;
     const TileEntry<T> &x = E[e];
     if (St && !St[x.member].active) return;
-    smooth_tile_body<T, TX, TY, S, MODE, FUSE, SP>(x.L, x.zout, x.Cz, x.Cr, x.zero_z, l % x.gx, l / x.gx);
+    const LevelArgs<T> L = x.L, Cz = x.Cz, Cr = x.Cr;
+    const int gx = x.gx;
+    smooth_tile_body<T, TX, TY, S, MODE, FUSE, SP>(L, x.zout, Cz, Cr, x.zero_z, l % gx, l / gx);
 }
 
 // restriction: coarse tile CX x CY of C from fine level F
@@ -472,12 +475,15 @@ __global__ void __launch_bounds__(256) refine_kernel(LevelArgs<double> L, float 
 }
 
 template <bool SP>
-__global__ void __launch_bounds__(256) refine_batch_kernel(const RefineEntry *E, const int *wg0, int ne,
+__global__ void __launch_bounds__(256) refine_batch_kernel(const RefineEntry *E, const int *map, int ne,
                                                            const EnsState *St) {
-    const int e = batch_entry(wg0, ne, blockIdx.x), l = blockIdx.x - wg0[e];
+    const int2 el = batch_slot(map, blockIdx.x);
+    const int e = el.x, l = el.y;
     const RefineEntry &x = E[e];
-    if (St && !St[x.member].active) return;
-    refine_body<SP>(x.L, x.b32, x.partial, l % x.gx, l / x.gx, x.gx);
+    if (St && !St[x.member].resid) return;
+    const LevelArgs<double> L = x.L;
+    const int gx = x.gx;
+    refine_body<SP>(L, x.b32, x.partial, l % gx, l / gx, gx);
 }
 
 // ---------------------------------------------------------------------------
@@ -638,42 +644,42 @@ template int mg_smooth_level<float>(LevelArgs<float> &, int, hipStream_t);
 // batches use 32x16 tiles, or 64x32 when key.big
 // ---------------------------------------------------------------------------
 template <typename T, int S, int MODE, bool FUSE, bool SP>
-static void launch_tile_batch(bool big, const TileEntry<T> *E, const int *wg0, int ne, int total,
+static void launch_tile_batch(bool big, const TileEntry<T> *E, const int *map, int ne, int total,
                               const EnsState *St, hipStream_t st) {
     if (big)
         hipLaunchKernelGGL((smooth_tile_batch_kernel<T, 64, 32, S, MODE, FUSE, SP>), dim3(total),
-                           dim3(tile_threads(64, 32, tile_halo(S, FUSE))), 0, st, E, wg0, ne, St);
+                           dim3(tile_threads(64, 32, tile_halo(S, FUSE))), 0, st, E, map, ne, St);
     else
         hipLaunchKernelGGL((smooth_tile_batch_kernel<T, 32, 16, S, MODE, FUSE, SP>), dim3(total),
-                           dim3(tile_threads(32, 16, tile_halo(S, FUSE))), 0, st, E, wg0, ne, St);
+                           dim3(tile_threads(32, 16, tile_halo(S, FUSE))), 0, st, E, map, ne, St);
 }
 
 template <typename T, int S, int MODE>
-static void launch_tile_batch_fs(const TileKey &k, const TileEntry<T> *E, const int *wg0, int ne, int total,
+static void launch_tile_batch_fs(const TileKey &k, const TileEntry<T> *E, const int *map, int ne, int total,
                                  const EnsState *St, hipStream_t st) {
     if (k.fuse) {
-        if (k.sparse) launch_tile_batch<T, S, MODE, true, true>(k.big, E, wg0, ne, total, St, st);
-        else launch_tile_batch<T, S, MODE, true, false>(k.big, E, wg0, ne, total, St, st);
+        if (k.sparse) launch_tile_batch<T, S, MODE, true, true>(k.big, E, map, ne, total, St, st);
+        else launch_tile_batch<T, S, MODE, true, false>(k.big, E, map, ne, total, St, st);
     } else {
-        if (k.sparse) launch_tile_batch<T, S, MODE, false, true>(k.big, E, wg0, ne, total, St, st);
-        else launch_tile_batch<T, S, MODE, false, false>(k.big, E, wg0, ne, total, St, st);
+        if (k.sparse) launch_tile_batch<T, S, MODE, false, true>(k.big, E, map, ne, total, St, st);
+        else launch_tile_batch<T, S, MODE, false, false>(k.big, E, map, ne, total, St, st);
     }
 }
 
 template <typename T, int S>
-static void launch_tile_batch_m(const TileKey &k, const TileEntry<T> *E, const int *wg0, int ne, int total,
+static void launch_tile_batch_m(const TileKey &k, const TileEntry<T> *E, const int *map, int ne, int total,
                                 const EnsState *St, hipStream_t st) {
-    if (k.mode == TILE_ZERO) launch_tile_batch_fs<T, S, TILE_ZERO>(k, E, wg0, ne, total, St, st);
-    else if (k.mode == TILE_PROLONG) launch_tile_batch_fs<T, S, TILE_PROLONG>(k, E, wg0, ne, total, St, st);
-    else launch_tile_batch_fs<T, S, TILE_LOAD>(k, E, wg0, ne, total, St, st);
+    if (k.mode == TILE_ZERO) launch_tile_batch_fs<T, S, TILE_ZERO>(k, E, map, ne, total, St, st);
+    else if (k.mode == TILE_PROLONG) launch_tile_batch_fs<T, S, TILE_PROLONG>(k, E, map, ne, total, St, st);
+    else launch_tile_batch_fs<T, S, TILE_LOAD>(k, E, map, ne, total, St, st);
 }
 
 template <typename T>
-int batch_tile(const TileKey &key, const TileEntry<T> *E, const int *wg0, int ne, int total, const EnsState *St,
+int batch_tile(const TileKey &key, const TileEntry<T> *E, const int *map, int ne, int total, const EnsState *St,
                hipStream_t st) {
     if (ne == 0) return STZ_OK;
-    if (key.sweeps == 2) launch_tile_batch_m<T, 2>(key, E, wg0, ne, total, St, st);
-    else if (key.sweeps == 1) launch_tile_batch_m<T, 1>(key, E, wg0, ne, total, St, st);
+    if (key.sweeps == 2) launch_tile_batch_m<T, 2>(key, E, map, ne, total, St, st);
+    else if (key.sweeps == 1) launch_tile_batch_m<T, 1>(key, E, map, ne, total, St, st);
     else {
         set_error("batched smoother launch with %d sweeps", key.sweeps);
         return STZ_ERR_ARG;
@@ -686,11 +692,11 @@ template int batch_tile<double>(const TileKey &, const TileEntry<double> *, cons
 template int batch_tile<float>(const TileKey &, const TileEntry<float> *, const int *, int, int, const EnsState *,
                                hipStream_t);
 
-int batch_refine(const RefineEntry *E, const int *wg0, int ne, int total, int sparse, const EnsState *St,
+int batch_refine(const RefineEntry *E, const int *map, int ne, int total, int sparse, const EnsState *St,
                  hipStream_t st) {
     if (ne == 0) return STZ_OK;
-    if (sparse) hipLaunchKernelGGL(refine_batch_kernel<true>, dim3(total), dim3(256), 0, st, E, wg0, ne, St);
-    else hipLaunchKernelGGL(refine_batch_kernel<false>, dim3(total), dim3(256), 0, st, E, wg0, ne, St);
+    if (sparse) hipLaunchKernelGGL(refine_batch_kernel<true>, dim3(total), dim3(256), 0, st, E, map, ne, St);
+    else hipLaunchKernelGGL(refine_batch_kernel<false>, dim3(total), dim3(256), 0, st, E, map, ne, St);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }

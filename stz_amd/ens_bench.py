@@ -1,0 +1,126 @@
+"""Benchmark of the batched ensemble: many independent simple-shear
+simulations of different sizes stepped together on one GPU.
+
+    python -m stz_amd.ens_bench [--members 256] [--steps 100] [--warmup 60]
+                                [--form mixed] [--seed 0] [--serial-steps 3]
+
+Workload: `members` simulations, the side of each drawn uniformly from
+{64, 128, 256, 384, 512} (numpy PCG64, `seed`), chi = 600 K + 20 K seeded
+Gaussian field (correlation length 2 cells, seed per member), zero initial
+stress and velocity, dt = 0.2 (the 1024^2 workload's step), V(2,2), residual
+threshold 1e-4 x the reference's, FP32 multigrid inside FP64 refinement.
+
+Prints one JSON line: aggregate cell-steps/s of the batched step (direct
+launches, the library's default), the same replayed from HIP graphs, and the
+members stepped one at a time through the single-simulation path
+(`serial-steps` steps each) for comparison.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+SIDES = (64, 128, 256, 384, 512)
+
+
+def build(members, seed, form, nu, tol):
+    from . import problems
+    from .ensemble import Ensemble
+    from .initial import simple_shear_state
+    rng = np.random.default_rng(seed)
+    sides = [int(s) for s in rng.choice(SIDES, size=members)]
+    params = [problems.simple_shear(s) for s in sides]
+    ens = Ensemble(params, form)
+    for k, p in enumerate(params):
+        ens[k].set_fields(simple_shear_state(p, 600.0, 20.0, 2.0, seed=1000 + seed * members + k))
+        ens[k].set_vel(np.zeros((p["n"] + 1, p["m"], 2)))
+        ens[k].time = 0.0
+    ens.set_mg_config(nu1=nu, nu2=nu, tol_mult=tol)
+    return ens, sides
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--form", default="mixed", choices=("mixed", "fp64"))
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--nu", type=int, default=2)
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--serial-steps", type=int, default=3)
+    args = ap.parse_args()
+    from . import problems
+    dt = problems.T_END / 5000
+
+    t0 = time.perf_counter()
+    ens, sides = build(args.members, args.seed, args.form, args.nu, args.tol)
+    cells = sum(s * s for s in sides)
+    print("ens_bench: %d members, %.2fM cells, set-up %.1f s" % (len(sides), cells / 1e6, time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+
+    def timed(nsteps, graphs):
+        ens.use_graphs(graphs)
+        ens[0].synchronize()
+        t = time.perf_counter()
+        ens.step_forward_quasistatic(dt, nsteps)
+        return time.perf_counter() - t
+
+    # the predictors start at 8 cycles (tgmg_config.hh:26) and settle within
+    # a few tens of steps; the two launch modes are timed in alternating
+    # halves after the warm-up so that both see the same phase of the run
+    timed(args.warmup, False)
+    half = max(1, args.steps // 2)
+    el, el_d, cyc_g = 0.0, 0.0, []
+    s0 = ens.stats()
+    for rnd in range(2):
+        for m in ens:
+            m.avg_iters()
+        el += timed(half, False)
+        cyc_g.append(float(np.mean([m.avg_iters() for m in ens])))
+        el_d += timed(half, True)
+    s1 = ens.stats()
+    nsteps = 2 * half
+    cyc = float(np.mean(cyc_g))
+    slots = (s1["slots"] - s0["slots"]) / (2 * nsteps)
+    extra = (s1["extra_slots"] - s0["extra_slots"]) / (2 * nsteps)
+    print("ens_bench: direct  %.2f ms/step, %.3e cell-steps/s, %.2f cycles/solve, %.2f slots/step "
+          "(%.2f beyond plan), tiles %s" % (1e3 * el / nsteps, cells * nsteps / el, cyc, slots, extra,
+                                            ens.tile_choice()), file=sys.stderr, flush=True)
+    print("ens_bench: graphs  %.2f ms/step" % (1e3 * el_d / nsteps), file=sys.stderr, flush=True)
+
+    # the same members one at a time through stz_sim_step_quasistatic
+    for m in ens:
+        m.avg_iters()
+    t = time.perf_counter()
+    for m in ens:
+        m.step_forward_quasistatic(dt, args.serial_steps)
+    el_s = time.perf_counter() - t
+    cyc_s = float(np.mean([m.avg_iters() for m in ens]))
+    serial = cells * args.serial_steps / el_s
+    print("ens_bench: one at a time %.2f ms per ensemble step, %.3e cell-steps/s, %.2f cycles/solve" %
+          (1e3 * el_s / args.serial_steps, serial, cyc_s), file=sys.stderr, flush=True)
+
+    value = cells * nsteps / el
+    hist = {str(s): sides.count(s) for s in SIDES}
+    print(json.dumps({
+        "metric": "ensemble cell-steps/s (batched quasi-static step, all members)",
+        "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": nsteps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / nsteps, "higher_is_better": True,
+        "dtype": "f32+f64" if args.form == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": "ensemble of %d simple-shear simulations, sides %s, chi 600K+20K (corr 2 cells), "
+                               "dt %g, V(%d,%d), threshold %g x reference" %
+                               (len(sides), hist, dt, args.nu, args.nu, args.tol),
+                   "cells": cells, "cycles_per_solve": cyc, "slots_per_step": slots,
+                   "slots_beyond_plan_per_step": extra,
+                   "graph_replay_ms_per_step": 1e3 * el_d / nsteps, "tiles": ens.tile_choice(),
+                   "one_at_a_time_cycles_per_solve": cyc_s,
+                   "one_at_a_time_cell_steps_per_s": serial,
+                   "batched_over_one_at_a_time": value / serial}}))
+    ens.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -6,8 +6,9 @@ No counterpart in the reference, which runs one ``shear_sim`` per process.
 per-simulation call (fields, time, initial conditions, diagnostics);
 ``step_forward_quasistatic(dt, nsteps)`` advances all members with one
 batched launch per kernel of the step, the members' adaptive V-cycle counts
-decided on the device and the step replayed from HIP graphs.  Each member
-ends exactly where ``ShearSim.step_forward_quasistatic`` would take it.
+decided on the device and one host synchronisation per step (optionally
+replayed from a HIP graph, ``use_graphs``).  Each member ends exactly where
+``ShearSim.step_forward_quasistatic`` would take it.
 """
 import ctypes as C
 
@@ -70,6 +71,7 @@ class Ensemble:
         check(self._lib.stz_ensemble_set_mg_config(self._h, C.byref(g)))
 
     def use_graphs(self, on=True):
+        """replay each step from a HIP graph (default: direct launches)"""
         check(self._lib.stz_ensemble_use_graphs(self._h, 1 if on else 0))
 
     def step_forward_quasistatic(self, dt, nsteps=1):
@@ -80,3 +82,9 @@ class Ensemble:
         out = np.zeros(4, dtype=np.int64)
         check(self._lib.stz_ensemble_stats(self._h, out))
         return dict(zip(("steps", "slots", "extra_slots", "captures"), (int(v) for v in out)))
+
+    def tile_choice(self):
+        """{launches kept at 32x16 tiles, at 64x32} by the tuner"""
+        out = np.zeros(2, dtype=np.int32)
+        check(self._lib.stz_ensemble_tile_choice(self._h, out))
+        return {"32x16": int(out[0]), "64x32": int(out[1])}

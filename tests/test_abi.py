@@ -95,3 +95,23 @@ def test_bad_arguments_rejected_before_device_use():
     assert b"m >= 8" in lib.stz_last_error()
     assert lib.stz_bd_destroy(None) == _lib.OK
     assert lib.stz_bd_step_quasistatic(None, 1.0, 1) == _lib.ERR_ARG
+
+
+def test_no_device_fails_loudly_ensemble():
+    if stz_amd.device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(stz_amd.StzError) as e:
+        stz_amd.Ensemble([stz_amd.problems.simple_shear(64)] * 2)
+    assert e.value.code == _lib.ERR_NO_DEVICE
+
+
+def test_ensemble_bad_arguments():
+    lib = _lib.load()
+    h = C.c_void_p()
+    p = _lib.make_params(stz_amd.problems.simple_shear(64))
+    assert lib.stz_ensemble_create(C.byref(p), 0, 1, C.byref(h)) == _lib.ERR_ARG
+    assert lib.stz_ensemble_create(None, 2, 1, C.byref(h)) == _lib.ERR_ARG
+    assert lib.stz_ensemble_destroy(None) == _lib.OK
+    assert lib.stz_ensemble_step_quasistatic(None, 1.0, 1) == _lib.ERR_ARG
+    n = C.c_int()
+    assert lib.stz_ensemble_size(None, C.byref(n)) == _lib.ERR_ARG

@@ -65,8 +65,9 @@ def test_members_equal_single_simulations(form, graphs):
     st = ens.stats()
     print(form, graphs, st, [g.last_solve()["cycles"] for g in singles])
     assert st["steps"] == 6
-    # head and tail for each step parity, one cycle slot
-    assert st["captures"] == (5 if graphs else 0), st
+    # graphs: one per step parity and planned slot count, and the slot and
+    # tail graphs of members that need more cycles than planned
+    assert (st["captures"] > 0) if graphs else st["captures"] == 0, st
     ens.close()
 
 
