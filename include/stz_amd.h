@@ -383,8 +383,8 @@ int stz_bd_set_edge_velocity(stz_bd *b, double u_left, double u_right);
  * its own parameters (grid size included), and steps them together: every
  * kernel of stz_sim_step_quasistatic runs as ONE launch over all members,
  * each member's adaptive V-cycle count (tgmg_predict) is decided on the
- * device, and a step needs one host synchronisation (optionally replayed
- * from a HIP graph).
+ * device, and a step replays from a HIP graph with one host
+ * synchronisation.
  * Every member computes exactly what stz_sim_step_quasistatic computes for
  * it alone.  Requirements: all members have the same stz_mg_config, with
  * kernels = 0, nu1 >= 1 and ceil(nu1/2) + ceil(nu2/2) even (nu1 = nu2 is
@@ -403,9 +403,8 @@ int stz_ensemble_size(stz_ensemble *e, int *count);
 int stz_ensemble_member(stz_ensemble *e, int k, stz_sim **member);
 /* Multigrid settings of every member (stz_sim_set_mg_config on each). */
 int stz_ensemble_set_mg_config(stz_ensemble *e, const stz_mg_config *cfg);
-/* Replay steps from HIP graphs (1) or launch them directly (0, default: on
- * ROCm 7.2 the same launches replayed from a graph took about 0.8 ms more
- * per step of a 256-member ensemble than issued directly on the stream). */
+/* Replay steps from HIP graphs (1, default) or launch them directly (0);
+ * the launches and results are the same. */
 int stz_ensemble_use_graphs(stz_ensemble *e, int on);
 /* nsteps quasi-static steps of every member.  Stops after the first step in
  * which a member's solve fails and returns that status; the failing member

@@ -67,7 +67,7 @@ struct stz_ensemble {
     int device = 0, precision = 0;
     std::vector<stz_sim *> mem;
     hipStream_t stream = nullptr;
-    bool use_graphs = false;  // direct launches measured faster than graph replay
+    bool use_graphs = true;
     // tables (valid for dt_tab and the members' plane-set parity below)
     bool built = false;
     double dt_tab = 0;

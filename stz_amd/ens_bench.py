@@ -1,8 +1,12 @@
 """Benchmark of the batched ensemble: many independent simple-shear
 simulations of different sizes stepped together on one GPU.
 
-    python -m stz_amd.ens_bench [--members 256] [--steps 100] [--warmup 60]
+    python -m stz_amd.ens_bench [--members 256] [--steps 200] [--warmup 60]
                                 [--form mixed] [--seed 0] [--serial-steps 3]
+                                [--one-mode]
+
+`--warmup 0 --steps 1000 --one-mode` times the whole 1000-step run of
+every member from t = 0.
 
 Workload: `members` simulations, the side of each drawn uniformly from
 {64, 128, 256, 384, 512} (numpy PCG64, `seed`), chi = 600 K + 20 K seeded
@@ -10,9 +14,9 @@ Gaussian field (correlation length 2 cells, seed per member), zero initial
 stress and velocity, dt = 0.2 (the 1024^2 workload's step), V(2,2), residual
 threshold 1e-4 x the reference's, FP32 multigrid inside FP64 refinement.
 
-Prints one JSON line: aggregate cell-steps/s of the batched step (direct
-launches, the library's default), the same replayed from HIP graphs, and the
-members stepped one at a time through the single-simulation path
+Prints one JSON line: aggregate cell-steps/s of the batched step replayed
+from HIP graphs (the library's default), the same with direct launches, and
+the members stepped one at a time through the single-simulation path
 (`serial-steps` steps each) for comparison.
 """
 import argparse
@@ -44,13 +48,16 @@ def build(members, seed, form, nu, tol):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--members", type=int, default=256)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--form", default="mixed", choices=("mixed", "fp64"))
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--nu", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--serial-steps", type=int, default=3)
+    ap.add_argument("--block", type=int, default=10, help="steps per block of one launch mode")
+    ap.add_argument("--one-mode", action="store_true",
+                    help="time every step replayed from graphs (no direct-launch comparison)")
     args = ap.parse_args()
     from . import problems
     dt = problems.T_END / 5000
@@ -69,27 +76,33 @@ def main():
         return time.perf_counter() - t
 
     # the predictors start at 8 cycles (tgmg_config.hh:26) and settle within
-    # a few tens of steps; the two launch modes are timed in alternating
-    # halves after the warm-up so that both see the same phase of the run
-    timed(args.warmup, False)
-    half = max(1, args.steps // 2)
+    # a few tens of steps, and the cycle count keeps drifting with the
+    # loading: the two launch modes alternate in blocks of `block` steps so
+    # that both see the same phases of the run
+    if args.warmup:
+        timed(args.warmup, True)
+    block = args.steps if args.one_mode else max(1, min(args.block, args.steps // 2))
+    rounds = 1 if args.one_mode else max(1, args.steps // (2 * block))
     el, el_d, cyc_g = 0.0, 0.0, []
     s0 = ens.stats()
-    for rnd in range(2):
+    for rnd in range(rounds):
         for m in ens:
             m.avg_iters()
-        el += timed(half, False)
+        el += timed(block, True)
         cyc_g.append(float(np.mean([m.avg_iters() for m in ens])))
-        el_d += timed(half, True)
+        if not args.one_mode:
+            el_d += timed(block, False)
     s1 = ens.stats()
-    nsteps = 2 * half
+    nsteps = rounds * block
     cyc = float(np.mean(cyc_g))
-    slots = (s1["slots"] - s0["slots"]) / (2 * nsteps)
-    extra = (s1["extra_slots"] - s0["extra_slots"]) / (2 * nsteps)
-    print("ens_bench: direct  %.2f ms/step, %.3e cell-steps/s, %.2f cycles/solve, %.2f slots/step "
+    timed_steps = nsteps if args.one_mode else 2 * nsteps
+    slots = (s1["slots"] - s0["slots"]) / timed_steps
+    extra = (s1["extra_slots"] - s0["extra_slots"]) / timed_steps
+    print("ens_bench: graphs  %.2f ms/step, %.3e cell-steps/s, %.2f cycles/solve, %.2f slots/step "
           "(%.2f beyond plan), tiles %s" % (1e3 * el / nsteps, cells * nsteps / el, cyc, slots, extra,
                                             ens.tile_choice()), file=sys.stderr, flush=True)
-    print("ens_bench: graphs  %.2f ms/step" % (1e3 * el_d / nsteps), file=sys.stderr, flush=True)
+    if not args.one_mode:
+        print("ens_bench: direct  %.2f ms/step" % (1e3 * el_d / nsteps), file=sys.stderr, flush=True)
 
     # the same members one at a time through stz_sim_step_quasistatic
     for m in ens:
@@ -115,7 +128,8 @@ def main():
                                (len(sides), hist, dt, args.nu, args.nu, args.tol),
                    "cells": cells, "cycles_per_solve": cyc, "slots_per_step": slots,
                    "slots_beyond_plan_per_step": extra,
-                   "graph_replay_ms_per_step": 1e3 * el_d / nsteps, "tiles": ens.tile_choice(),
+                   "direct_launch_ms_per_step": None if args.one_mode else 1e3 * el_d / nsteps,
+                   "tiles": ens.tile_choice(),
                    "one_at_a_time_cycles_per_solve": cyc_s,
                    "one_at_a_time_cell_steps_per_s": serial,
                    "batched_over_one_at_a_time": value / serial}}))

@@ -6,8 +6,9 @@ No counterpart in the reference, which runs one ``shear_sim`` per process.
 per-simulation call (fields, time, initial conditions, diagnostics);
 ``step_forward_quasistatic(dt, nsteps)`` advances all members with one
 batched launch per kernel of the step, the members' adaptive V-cycle counts
-decided on the device and one host synchronisation per step (optionally
-replayed from a HIP graph, ``use_graphs``).  Each member ends exactly where
+decided on the device, and each step replayed from a HIP graph with one host
+synchronisation (``use_graphs(False)`` launches directly).  Each member ends
+exactly where
 ``ShearSim.step_forward_quasistatic`` would take it.
 """
 import ctypes as C
@@ -71,7 +72,7 @@ class Ensemble:
         check(self._lib.stz_ensemble_set_mg_config(self._h, C.byref(g)))
 
     def use_graphs(self, on=True):
-        """replay each step from a HIP graph (default: direct launches)"""
+        """replay each step from a HIP graph (default) or launch directly"""
         check(self._lib.stz_ensemble_use_graphs(self._h, 1 if on else 0))
 
     def step_forward_quasistatic(self, dt, nsteps=1):
