@@ -62,6 +62,7 @@ template <typename T> struct TileEntry {
     T *zout;
     LevelArgs<T> Cz, Cr;
     int zero_z;
+    double *corr;  // mixed top level's last launch: vel += result
 };
 
 // FP64 residual of a member's top level (refinement or norm)
@@ -75,13 +76,6 @@ struct RefineEntry {
 template <typename T> struct CoarseEntry {
     int member;
     CoarseArgs<T> A;
-};
-
-// vel += e (mixed form), count values
-struct CorrEntry {
-    int member, count;
-    double *z;
-    const float *e;
 };
 
 // workgroup-partial sums of one member's residual
@@ -122,9 +116,12 @@ template <typename T> struct TileRec {
     T *zout;
     LevelArgs<T> Cz, Cr;
     int zero_z;
+    double *corr;
 };
-// the launches mg_vcycle_tiled would issue for A (mg_tiled.hip)
-template <typename T> int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out);
+// the launches mg_vcycle_tiled(A, top_zero, st, corr) would issue
+// (mg_tiled.hip)
+template <typename T>
+int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out, double *corr);
 
 // mixed: FP64 residual -> FP32 source of the correction equation + norm partials
 int batch_refine(const RefineEntry *E, const int *map, int ne, int total, int sparse, const EnsState *S,
@@ -133,7 +130,6 @@ template <typename T> int batch_coarse(const CoarseEntry<T> *E, int ne, size_t l
 template <typename T> int batch_set_lds_limit(size_t bytes);
 // fp64: |r|^2 partials of the top level, blocks as mg_resid_norm
 int batch_resid_norm(const RefineEntry *E, const int *map, int ne, int total, const EnsState *S, hipStream_t st);
-int batch_correct64(const CorrEntry *E, const int *map, int ne, int total, const EnsState *S, hipStream_t st);
 
 // the per-member V-cycle count (mg.hip): mode 0 starts a solve, mode 1
 // follows a cycle slot, mode 2 marks corrected members

@@ -79,7 +79,6 @@ struct stz_ensemble {
     std::vector<DevBuf> bufs;
     DevTable<CellEntry> cells, srcs, chis;
     DevTable<RefineEntry> refs;
-    DevTable<CorrEntry> corrs;
     NormEntry *norms = nullptr;
     int top_sparse = 0;
     VCyclePlan<double> plan64;
@@ -159,8 +158,8 @@ template <typename T> void tile_grid(const LevelArgs<T> &L, bool big, int &gx, i
 // average time of one pass over `ls` (ungated launches, after one untimed
 // pass).  Between steps the buffers a tiled launch writes (a level's z
 // buffers, the coarser level's source) are rewritten by the next cycle
-// before they are read, except the FP64 top level's z, which is vel: the
-// caller keeps that aside.
+// before they are read, except vel (the FP64 top level's z, or the target
+// of the mixed correction): the caller keeps that aside.
 template <typename T> int time_launches(stz_ensemble *e, const std::vector<TileLaunch<T>> &ls, double *us) {
     hipEvent_t a, b;
     STZ_HIP(hipEventCreate(&a));
@@ -201,7 +200,8 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
         A.nu1 = e->cfg.nu1;
         A.nu2 = e->cfg.nu2;
         std::vector<TileRec<T>> recs;
-        if ((rc = mg_vcycle_plan(A, top_zero, recs))) {
+        // mixed: the top level's last launch applies vel += e (one_cycle)
+        if ((rc = mg_vcycle_plan(A, top_zero, recs, top_zero ? s->vel : nullptr))) {
             char msg[400];
             std::snprintf(msg, sizeof msg, "ensemble member %d (%d x %d): %s", k, s->c.m, s->c.n, stz_last_error());
             set_error("%s", msg);
@@ -255,6 +255,7 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
                         te.Cz = r.Cz;
                         te.Cr = r.Cr;
                         te.zero_z = r.zero_z;
+                        te.corr = r.corr;
                         E.push_back(te);
                         wgs.push_back(gx * gy);
                     }
@@ -316,9 +317,8 @@ int build_tables(stz_ensemble *e, double dt) {
         if ((rc = mg_build(e->mem[k], e->mem[k]->p.viscosity / dt))) return rc;
 
     std::vector<CellEntry> cells, chis;
-    std::vector<int> wcell, wsrc, wchi, wref, wcorr;
+    std::vector<int> wcell, wsrc, wchi, wref;
     std::vector<RefineEntry> refs;
-    std::vector<CorrEntry> corrs;
     std::vector<NormEntry> norms;
     e->cur0.assign(N, nullptr);
     e->c0.resize(N);
@@ -371,13 +371,6 @@ int build_tables(stz_ensemble *e, double dt) {
                 set_error("ensemble members differ in the top-level stencil pattern");
                 return STZ_ERR_ARG;
             }
-            CorrEntry co;
-            co.member = k;
-            co.count = 2 * c.m * (c.n + 1);
-            co.z = s->vel;
-            co.e = s->mg32.args.lv[0].z;
-            corrs.push_back(co);
-            wcorr.push_back((co.count + 255) / 256);
         } else {
             // mg_resid_norm's grid: 256 nodes per block
             re.gx = (t.m * t.n + 255) / 256;
@@ -395,13 +388,11 @@ int build_tables(stz_ensemble *e, double dt) {
     }
     if ((rc = make_table(e, chis, wchi, e->chis))) return rc;
     if ((rc = make_table(e, refs, wref, e->refs))) return rc;
-    if ((rc = make_table(e, corrs, wcorr, e->corrs))) return rc;
     if ((rc = upload_vec(e, norms, &e->norms))) return rc;
-    if (e->precision == STZ_PREC_MIXED) {
-        rc = build_plan<float>(e, e->plan32, &stz_sim::mg32, true);
-    } else {
-        // the tile-shape timing runs the FP64 top level's last smoothing
-        // launch, which writes vel: keep the velocities aside meanwhile
+    {
+        // the tile-shape timing runs the top level's last smoothing launch,
+        // which writes vel (FP64: it is the level's z; mixed: vel += e):
+        // keep the velocities aside meanwhile
         std::vector<size_t> off(N + 1, 0);
         for (int k = 0; k < N; k++) off[k + 1] = off[k] + 2 * (size_t)e->mem[k]->c.m * (e->mem[k]->c.n + 1);
         double *keep = nullptr;
@@ -410,7 +401,10 @@ int build_tables(stz_ensemble *e, double dt) {
             if (hipMemcpyAsync(keep + off[k], e->mem[k]->vel, (off[k + 1] - off[k]) * sizeof(double),
                                hipMemcpyDeviceToDevice, e->stream) != hipSuccess)
                 rc = STZ_ERR_HIP;
-        if (!rc) rc = build_plan<double>(e, e->plan64, &stz_sim::mg64, false);
+        if (!rc) {
+            if (e->precision == STZ_PREC_MIXED) rc = build_plan<float>(e, e->plan32, &stz_sim::mg32, true);
+            else rc = build_plan<double>(e, e->plan64, &stz_sim::mg64, false);
+        }
         for (int k = 0; k < N; k++)
             if (hipMemcpyAsync(e->mem[k]->vel, keep + off[k], (off[k + 1] - off[k]) * sizeof(double),
                                hipMemcpyDeviceToDevice, e->stream) != hipSuccess && !rc)
@@ -459,9 +453,8 @@ int launch_slot(stz_ensemble *e) {
     hipStream_t st = e->stream;
     const DevTable<RefineEntry> &R = e->refs;
     if (e->precision == STZ_PREC_MIXED) {
+        // vel += e is applied by the top level's last smoothing launch
         if ((rc = launch_vcycle<float>(e, e->plan32))) return rc;
-        const DevTable<CorrEntry> &K = e->corrs;
-        if ((rc = batch_correct64(K.E, K.map, K.ne, K.total, e->dst, st))) return rc;
         if ((rc = batch_refine(R.E, R.map, R.ne, R.total, e->top_sparse, e->dst, st))) return rc;
     } else {
         if ((rc = launch_vcycle<double>(e, e->plan64))) return rc;

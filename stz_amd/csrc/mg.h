@@ -79,7 +79,12 @@ template <typename T> bool tiled_ok(const LevelArgs<T> &L);
 // one V-cycle with the tiled kernels where a level allows them; swaps the
 // z/z2 pointers of the levels it smooths; top_zero starts the top level
 // from z = 0 (correction equation)
-template <typename T> int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipStream_t st);
+// corr (mixed form): the top level's last smoothing launch adds its result
+// to this FP64 array instead of storing it (the refinement's vel += e);
+// *corrected says whether it did (the top level was tiled)
+template <typename T>
+int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipStream_t st, double *corr = nullptr,
+                    int *corrected = nullptr);
 // `sweeps` smoothing sweeps on one level (tiled where possible; swaps z/z2)
 template <typename T> int mg_smooth_level(LevelArgs<T> &L, int sweeps, hipStream_t st);
 // residual of the FP64 top level -> FP32 source, |r|^2 partials summed to *out

@@ -861,16 +861,6 @@ __global__ void __launch_bounds__(256) resid_norm_batch_kernel(const RefineEntry
     resid_norm_body(L, x.partial, l);
 }
 
-__global__ void __launch_bounds__(256) correct64_batch_kernel(const CorrEntry *E, const int *map, int ne,
-                                                              const EnsState *St) {
-    const int2 el = batch_slot(map, blockIdx.x);
-    const int e = el.x, l = el.y;
-    const CorrEntry &x = E[e];
-    if (St && !St[x.member].active) return;
-    const int idx = l * 256 + threadIdx.x;
-    if (idx < x.count) x.z[idx] += (double)x.e[idx];
-}
-
 // tgmg::solve(type, tgmg_predict&) (tgmg.in.cc:266-301) as a state machine:
 // the same tests, in the same order, on the same residuals as the host loop
 // of the single simulation, advanced once per cycle slot.  acc is the
@@ -976,13 +966,6 @@ template int batch_set_lds_limit<float>(size_t);
 int batch_resid_norm(const RefineEntry *E, const int *map, int ne, int total, const EnsState *St, hipStream_t st) {
     if (ne == 0) return STZ_OK;
     hipLaunchKernelGGL(resid_norm_batch_kernel<double>, dim3(total), dim3(256), 0, st, E, map, ne, St);
-    STZ_HIP(hipGetLastError());
-    return STZ_OK;
-}
-
-int batch_correct64(const CorrEntry *E, const int *map, int ne, int total, const EnsState *St, hipStream_t st) {
-    if (ne == 0) return STZ_OK;
-    hipLaunchKernelGGL(correct64_batch_kernel, dim3(total), dim3(256), 0, st, E, map, ne, St);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }

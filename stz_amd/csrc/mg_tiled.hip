@@ -96,7 +96,7 @@ template <typename T> __device__ __forceinline__ static const T *wall_stencil(co
 template <typename T, int TX, int TY, int S, int MODE, bool FUSE, bool SP>
 __device__ __forceinline__ static void smooth_tile_body(const LevelArgs<T> &L, T *__restrict__ zout,
                                                         const LevelArgs<T> &Cz, const LevelArgs<T> &Cr, int zero_z,
-                                                        int tbx, int tby) {
+                                                        double *corr, int tbx, int tby) {
     constexpr int H = tile_halo(S, FUSE), RX = TX + 2 * H, RY = TY + 2 * H;
     constexpr int BX = RX / 2, BY = RY / 2, NB = BX * BY, NT = tile_threads(TX, TY, H);
     constexpr int BPT = (NB + NT - 1) / NT;
@@ -242,8 +242,13 @@ __device__ __forceinline__ static void smooth_tile_body(const LevelArgs<T> &L, T
         if (gj >= n) continue;
         const size_t k = 2 * ((size_t)gi + (size_t)m * gj);
         const vec2s<T> v = zs[((li & 1) + 2 * (lj & 1)) * NB + (lj >> 1) * BX + (li >> 1)];
-        zout[k] = v.x;
-        zout[k + 1] = v.y;
+        if (corr) {  // the mixed form's correction vel += e, in place of storing e
+            corr[k] += (double)v.x;
+            corr[k + 1] += (double)v.y;
+        } else {
+            zout[k] = v.x;
+            zout[k + 1] = v.y;
+        }
     }
     if (!FUSE) return;
 
@@ -304,8 +309,9 @@ __device__ __forceinline__ static void smooth_tile_body(const LevelArgs<T> &L, T
 
 template <typename T, int TX, int TY, int S, int MODE, bool FUSE, bool SP>
 __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
-    smooth_tile_kernel(LevelArgs<T> L, T *__restrict__ zout, LevelArgs<T> Cz, LevelArgs<T> Cr, int zero_z) {
-    smooth_tile_body<T, TX, TY, S, MODE, FUSE, SP>(L, zout, Cz, Cr, zero_z, blockIdx.x, blockIdx.y);
+    smooth_tile_kernel(LevelArgs<T> L, T *__restrict__ zout, LevelArgs<T> Cz, LevelArgs<T> Cr, int zero_z,
+                       double *corr) {
+    smooth_tile_body<T, TX, TY, S, MODE, FUSE, SP>(L, zout, Cz, Cr, zero_z, corr, blockIdx.x, blockIdx.y);
 }
 
 // the same over the members of an ensemble (batch.h); members whose slot
@@ -319,7 +325,7 @@ __global__ void __launch_bounds__(tile_threads(TX, TY, tile_halo(S, FUSE)))
     if (St && !St[x.member].active) return;
     const LevelArgs<T> L = x.L, Cz = x.Cz, Cr = x.Cr;
     const int gx = x.gx;
-    smooth_tile_body<T, TX, TY, S, MODE, FUSE, SP>(L, x.zout, Cz, Cr, x.zero_z, l % gx, l / gx);
+    smooth_tile_body<T, TX, TY, S, MODE, FUSE, SP>(L, x.zout, Cz, Cr, x.zero_z, x.corr, l % gx, l / gx);
 }
 
 // restriction: coarse tile CX x CY of C from fine level F
@@ -508,32 +514,32 @@ template <typename T> static bool fuse_ok(const LevelArgs<T> &L) { return sizeof
 
 template <typename T, int S, int MODE, bool FUSE>
 static void launch_tile(const LevelArgs<T> &L, T *zout, const LevelArgs<T> &Cz, const LevelArgs<T> &Cr, int zero_z,
-                        hipStream_t st) {
+                        double *corr, hipStream_t st) {
     if (big_tiles(L)) {
         const dim3 g(L.m / 64, (L.n + 31) / 32), b(tile_threads(64, 32, tile_halo(S, FUSE)));
         if (L.sparse)
             hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE, FUSE, true>), g, b, 0, st, L, zout, Cz, Cr,
-                               zero_z);
+                               zero_z, corr);
         else
             hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE, FUSE, false>), g, b, 0, st, L, zout, Cz, Cr,
-                               zero_z);
+                               zero_z, corr);
     } else {
         const dim3 g(L.m / 32, (L.n + 15) / 16), b(tile_threads(32, 16, tile_halo(S, FUSE)));
         if (L.sparse)
             hipLaunchKernelGGL((smooth_tile_kernel<T, 32, 16, S, MODE, FUSE, true>), g, b, 0, st, L, zout, Cz, Cr,
-                               zero_z);
+                               zero_z, corr);
         else
             hipLaunchKernelGGL((smooth_tile_kernel<T, 32, 16, S, MODE, FUSE, false>), g, b, 0, st, L, zout, Cz, Cr,
-                               zero_z);
+                               zero_z, corr);
     }
 }
 
 template <typename T, int S, bool FUSE>
 static void launch_tile_mode(int mode, const LevelArgs<T> &L, T *zout, const LevelArgs<T> &Cz, const LevelArgs<T> &Cr,
-                             int zero_z, hipStream_t st) {
-    if (mode == TILE_ZERO) launch_tile<T, S, TILE_ZERO, FUSE>(L, zout, Cz, Cr, zero_z, st);
-    else if (mode == TILE_PROLONG) launch_tile<T, S, TILE_PROLONG, FUSE>(L, zout, Cz, Cr, zero_z, st);
-    else launch_tile<T, S, TILE_LOAD, FUSE>(L, zout, Cz, Cr, zero_z, st);
+                             int zero_z, double *corr, hipStream_t st) {
+    if (mode == TILE_ZERO) launch_tile<T, S, TILE_ZERO, FUSE>(L, zout, Cz, Cr, zero_z, corr, st);
+    else if (mode == TILE_PROLONG) launch_tile<T, S, TILE_PROLONG, FUSE>(L, zout, Cz, Cr, zero_z, corr, st);
+    else launch_tile<T, S, TILE_LOAD, FUSE>(L, zout, Cz, Cr, zero_z, corr, st);
 }
 
 // the coarse z must start at zero unless its first smoothing is tiled
@@ -548,9 +554,11 @@ template <typename T> static int coarse_needs_zero(const LevelArgs<T> &C) { retu
 // kernels in place.
 template <typename T>
 static int smooth_level(LevelArgs<T> &L, const LevelArgs<T> *C, int sweeps, int mode, hipStream_t st,
-                        const LevelArgs<T> *R = nullptr, int *restricted = nullptr) {
+                        const LevelArgs<T> *R = nullptr, int *restricted = nullptr, double *corr = nullptr,
+                        int *corrected = nullptr) {
     int e;
     if (restricted) *restricted = 0;
+    if (corrected) *corrected = 0;
     if (!tiled_ok(L)) {
         if (mode == TILE_PROLONG && (e = mg_prolong(*C, L, st))) return e;
         if (mode == TILE_ZERO && (e = mg_fill_zero(L.z, 2 * (size_t)L.m * L.n, st))) return e;
@@ -562,14 +570,16 @@ static int smooth_level(LevelArgs<T> &L, const LevelArgs<T> *C, int sweeps, int 
     while (sweeps > 0) {
         const int s = sweeps >= 2 ? 2 : 1;
         const bool fuse = R && s == sweeps;
+        double *cr = s == sweeps ? corr : nullptr;
         if (s == 2) {
-            if (fuse) launch_tile_mode<T, 2, true>(mode, L, L.z2, Cv, Rv, zz, st);
-            else launch_tile_mode<T, 2, false>(mode, L, L.z2, Cv, Rv, zz, st);
+            if (fuse) launch_tile_mode<T, 2, true>(mode, L, L.z2, Cv, Rv, zz, cr, st);
+            else launch_tile_mode<T, 2, false>(mode, L, L.z2, Cv, Rv, zz, cr, st);
         } else {
-            if (fuse) launch_tile_mode<T, 1, true>(mode, L, L.z2, Cv, Rv, zz, st);
-            else launch_tile_mode<T, 1, false>(mode, L, L.z2, Cv, Rv, zz, st);
+            if (fuse) launch_tile_mode<T, 1, true>(mode, L, L.z2, Cv, Rv, zz, cr, st);
+            else launch_tile_mode<T, 1, false>(mode, L, L.z2, Cv, Rv, zz, cr, st);
         }
         if (fuse && restricted) *restricted = 1;
+        if (cr && corrected) *corrected = 1;
         std::swap(L.z, L.z2);
         sweeps -= s;
         mode = TILE_LOAD;
@@ -595,8 +605,10 @@ template <typename T> static int restrict_level(const LevelArgs<T> &F, const Lev
     return STZ_OK;
 }
 
-template <typename T> int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipStream_t st) {
+template <typename T> int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipStream_t st, double *corr,
+                                          int *corrected) {
     int e;
+    if (corrected) *corrected = 0;
     for (int l = 0; l < A.lc; l++) {
         // pre-smoothing, fused with the restriction into level l+1 when tiled
         int done = 0;
@@ -611,7 +623,9 @@ template <typename T> int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipSt
         return e;
     if ((e = mg_coarse(A, st))) return e;
     for (int l = A.lc; l >= 1; l--)
-        if ((e = smooth_level(A.lv[l - 1], &A.lv[l], A.nu2, TILE_PROLONG, st))) return e;
+        if ((e = smooth_level(A.lv[l - 1], &A.lv[l], A.nu2, TILE_PROLONG, st, (const LevelArgs<T> *)nullptr,
+                              nullptr, l == 1 ? corr : nullptr, l == 1 ? corrected : nullptr)))
+            return e;
     return STZ_OK;
 }
 
@@ -633,8 +647,8 @@ template bool wall_ok<double>(const LevelArgs<double> &);
 template bool wall_ok<float>(const LevelArgs<float> &);
 template bool tiled_ok<double>(const LevelArgs<double> &);
 template bool tiled_ok<float>(const LevelArgs<float> &);
-template int mg_vcycle_tiled<double>(CoarseArgs<double> &, bool, hipStream_t);
-template int mg_vcycle_tiled<float>(CoarseArgs<float> &, bool, hipStream_t);
+template int mg_vcycle_tiled<double>(CoarseArgs<double> &, bool, hipStream_t, double *, int *);
+template int mg_vcycle_tiled<float>(CoarseArgs<float> &, bool, hipStream_t, double *, int *);
 template int mg_smooth_level<double>(LevelArgs<double> &, int, hipStream_t);
 template int mg_smooth_level<float>(LevelArgs<float> &, int, hipStream_t);
 
@@ -707,7 +721,7 @@ int batch_refine(const RefineEntry *E, const int *map, int ne, int total, int sp
 // not cover (per-colour kernels, separate restriction).
 template <typename T>
 static int plan_level(LevelArgs<T> &L, const LevelArgs<T> *C, int sweeps, int mode, const LevelArgs<T> *R,
-                      int level, int phase, std::vector<TileRec<T>> &out) {
+                      int level, int phase, std::vector<TileRec<T>> &out, double *corr = nullptr) {
     if (!tiled_ok(L)) {
         set_error("level %d (%d x %d) is not covered by the tiled kernels", level, L.m, L.n);
         return STZ_ERR_ARG;
@@ -729,6 +743,7 @@ static int plan_level(LevelArgs<T> &L, const LevelArgs<T> *C, int sweeps, int mo
         r.Cz = Cv;
         r.Cr = Rv;
         r.zero_z = zz;
+        r.corr = s == sweeps ? corr : nullptr;
         out.push_back(r);
         std::swap(L.z, L.z2);
         sweeps -= s;
@@ -737,7 +752,8 @@ static int plan_level(LevelArgs<T> &L, const LevelArgs<T> *C, int sweeps, int mo
     return STZ_OK;
 }
 
-template <typename T> int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out) {
+template <typename T>
+int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out, double *corr) {
     int e;
     out.clear();
     // the z/z2 exchanges of one cycle must cancel on every level, so that
@@ -763,11 +779,11 @@ template <typename T> int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::ve
     }
     for (int l = A.lc; l >= 1; l--)
         if ((e = plan_level(A.lv[l - 1], &A.lv[l], A.nu2, TILE_PROLONG, (const LevelArgs<T> *)nullptr, l - 1, 2,
-                            out)))
+                            out, l == 1 ? corr : nullptr)))
             return e;
     return STZ_OK;
 }
-template int mg_vcycle_plan<double>(CoarseArgs<double>, bool, std::vector<TileRec<double>> &);
-template int mg_vcycle_plan<float>(CoarseArgs<float>, bool, std::vector<TileRec<float>> &);
+template int mg_vcycle_plan<double>(CoarseArgs<double>, bool, std::vector<TileRec<double>> &, double *);
+template int mg_vcycle_plan<float>(CoarseArgs<float>, bool, std::vector<TileRec<float>> &, double *);
 
 }  // namespace stz

@@ -410,12 +410,13 @@ static int one_cycle(stz_sim *s, bool need_resid) {
     CoarseArgs<float> &A = s->mg32.args;
     A.nu1 = s->cfg.nu1;
     A.nu2 = s->cfg.nu2;
+    int corrected = 0;  // vel += e done by the top level's last tiled launch
     if (tiled) {
-        if ((e = mg_vcycle_tiled(A, true, s->stream))) return e;
+        if ((e = mg_vcycle_tiled(A, true, s->stream, s->vel, &corrected))) return e;
     } else {
         if ((e = mg_vcycle(A, s->stream))) return e;
     }
-    if ((e = mg_correct64(s->vel, A.lv[0].z, 2 * s->c.m * (s->c.n + 1), s->stream))) return e;
+    if (!corrected && (e = mg_correct64(s->vel, A.lv[0].z, 2 * s->c.m * (s->c.n + 1), s->stream))) return e;
     s->last.refinements++;
     if (need_resid) return refine_resid(s);
     return STZ_OK;
