@@ -421,6 +421,37 @@ int stz_ensemble_stats(stz_ensemble *e, int64_t *out4);
  * launches kept at 64x32} of the current tables. */
 int stz_ensemble_tile_choice(stz_ensemble *e, int32_t *out2);
 
+/* ---- adjoint of the FP64 step ------------------------------------------
+ * No counterpart in the reference.  Reverse-mode gradients of a function of
+ * the final state with respect to the initial state and the STZ parameters,
+ * through nsteps of stz_sim_step_quasistatic.  Each discrete choice of the
+ * forward step (upwind side, ENO branch, one-sided wall rule, yield branch,
+ * plastic substep count) is the forward's own, taken again on recomputation;
+ * the multigrid solve is differentiated as an exact solve (the adjoint
+ * solve reuses the forward operator, whose interior block is symmetric);
+ * states are kept every `checkpoint_every` steps and each segment is
+ * recomputed during the reverse sweep.  FP64 form, linear athermal STZ law
+ * with t_model = -1, no chi diffusion. */
+typedef struct stz_adjoint stz_adjoint;   /* opaque */
+
+/* Replace the STZ law's parameters (c0, Omegaeps0, ...; stz_model.hh:22-26)
+ * of a simulation; chi_inf of the plastic term is stz_sim_set_chi_inf. */
+int stz_sim_set_model(stz_sim *sim, const stz_model_params *mp);
+int stz_adjoint_create(stz_sim *sim, int checkpoint_every, stz_adjoint **out);
+int stz_adjoint_destroy(stz_adjoint *a);
+/* Step the simulation nsteps times, keeping checkpoints. */
+int stz_adjoint_forward(stz_adjoint *a, double dt, int nsteps);
+/* lam_final: dL/d(final state), ten planes in field order (STZ_F_U ..
+ * STZ_F_DDEV, m*(n+2) doubles each).  Writes dL/d(initial state) in the
+ * same layout and grad3 = dL/d(c0, Omegaeps0, chi_inf).  The simulation is
+ * left in its state after the forward run. */
+int stz_adjoint_backward(stz_adjoint *a, const double *lam_final, double *lam_init, double *grad3);
+/* Relative residual of each adjoint solve (default 1e-10). */
+int stz_adjoint_set_solve_accuracy(stz_adjoint *a, double rel);
+/* out6 = {forward seconds, backward seconds, recomputed steps, adjoint
+ * solves, adjoint V-cycles, forward V-cycles}, accumulated. */
+int stz_adjoint_stats(stz_adjoint *a, double *out6);
+
 #ifdef __cplusplus
 }
 #endif

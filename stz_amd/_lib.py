@@ -176,6 +176,13 @@ _SIGS = {
     "stz_ensemble_step_quasistatic": [C.c_void_p, C.c_double, C.c_int],
     "stz_ensemble_stats": [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")],
     "stz_ensemble_tile_choice": [C.c_void_p, _ip],
+    "stz_sim_set_model": [C.c_void_p, C.POINTER(ModelParams)],
+    "stz_adjoint_create": [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)],
+    "stz_adjoint_destroy": [C.c_void_p],
+    "stz_adjoint_forward": [C.c_void_p, C.c_double, C.c_int],
+    "stz_adjoint_backward": [C.c_void_p, _dp, _dp, _dp],
+    "stz_adjoint_set_solve_accuracy": [C.c_void_p, C.c_double],
+    "stz_adjoint_stats": [C.c_void_p, _dp],
 }
 _RESTYPE = {"stz_last_error": C.c_char_p}
 

@@ -1,0 +1,126 @@
+"""Adjoint of the FP64 quasi-static step, and a torch.autograd wrapper.
+
+No counterpart in the reference.  ``Adjoint(sim, checkpoint_every)`` runs
+``forward(dt, nsteps)`` on a ``ShearSim`` (FP64 form) while keeping
+checkpoints, and ``backward(lam_final)`` returns dL/d(initial fields) and
+dL/d(c0, Omegaeps0, chi_inf) for L = <lam_final, final fields>, with every
+discrete choice of the forward (upwind side, ENO branch, yield branch,
+plastic substep count) frozen.  ``QSRun`` exposes the same as a
+torch.autograd.Function of the initial chi field and the three parameters.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+PARAMS = ("c0", "Omegaeps0", "chi_inf")
+
+
+class Adjoint:
+    def __init__(self, sim, checkpoint_every=25):
+        if sim.precision != "fp64":
+            raise ValueError("the adjoint differentiates the FP64 form")
+        self.sim = sim
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        check(self._lib.stz_adjoint_create(sim._h, int(checkpoint_every), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.stz_adjoint_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_solve_accuracy(self, rel):
+        check(self._lib.stz_adjoint_set_solve_accuracy(self._h, float(rel)))
+
+    def forward(self, dt, nsteps):
+        check(self._lib.stz_adjoint_forward(self._h, float(dt), int(nsteps)))
+
+    def backward(self, lam_final):
+        lam_final = np.ascontiguousarray(lam_final, dtype=np.float64)
+        if lam_final.shape != (10,) + self.sim.plane_shape:
+            raise ValueError("lam_final must have shape %s" % ((10,) + self.sim.plane_shape,))
+        lam_init = np.zeros_like(lam_final)
+        g = np.zeros(3)
+        check(self._lib.stz_adjoint_backward(self._h, lam_final, lam_init, g))
+        return lam_init, dict(zip(PARAMS, g))
+
+    def stats(self):
+        out = np.zeros(6)
+        check(self._lib.stz_adjoint_stats(self._h, out))
+        return dict(zip(("forward_s", "backward_s", "recomputed_steps", "adjoint_solves",
+                         "adjoint_cycles", "forward_cycles"), out))
+
+
+def set_params(sim, c0=None, Omegaeps0=None, chi_inf=None):
+    """change the STZ law's c0 / Omegaeps0 and the plastic term's chi_inf"""
+    mp = dict(sim.params["stz"])
+    if c0 is not None:
+        mp["c0"] = float(c0)
+    if Omegaeps0 is not None:
+        mp["Omegaeps0"] = float(Omegaeps0)
+    sim.params["stz"] = mp
+    p = _lib.make_params(sim.params)
+    check(_lib.load().stz_sim_set_model(sim._h, C.byref(p.stz)))
+    if chi_inf is not None:
+        sim.set_chi_inf(float(chi_inf))
+
+
+def chi_planes(chi, planes):
+    """planes with chi (n x m, cells) in rows 0..n-1 and the ghost rows
+    extrapolated linearly (shear_sim.cc:552-581)"""
+    out = np.array(planes, dtype=np.float64, copy=True)
+    n = chi.shape[0]
+    out[6][1:n + 1] = chi
+    out[6][0] = 2 * chi[0] - chi[1]
+    out[6][n + 1] = 2 * chi[n - 1] - chi[n - 2]
+    return out
+
+
+def chi_planes_adjoint(lam_planes):
+    """adjoint of chi_planes with respect to chi"""
+    lc = lam_planes[6]
+    n = lc.shape[0] - 2
+    g = lc[1:n + 1].copy()
+    g[0] += 2 * lc[0]
+    g[1] -= lc[0]
+    g[n - 1] += 2 * lc[n + 1]
+    g[n - 2] -= lc[n + 1]
+    return g
+
+
+def make_qs_run():
+    """torch.autograd.Function: (chi [n, m], theta [3], ctx data) -> final
+    field planes [10, n+2, m]; backward through the hand-written adjoint"""
+    import torch
+
+    class QSRun(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, chi, theta, sim, base_planes, vel0, t0, dt, nsteps, every):
+            set_params(sim, *[float(v) for v in theta.detach().cpu().numpy()])
+            sim.set_fields(chi_planes(chi.detach().cpu().numpy(), base_planes))
+            sim.set_vel(vel0)
+            sim.time = t0
+            adj = Adjoint(sim, every)
+            adj.forward(dt, nsteps)
+            ctx.adj = adj
+            return torch.from_numpy(sim.get_fields())
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            lam_init, g = ctx.adj.backward(grad_out.detach().cpu().numpy())
+            gchi = torch.from_numpy(chi_planes_adjoint(lam_init))
+            gth = torch.tensor([g[k] for k in PARAMS], dtype=torch.float64)
+            ctx.adj.close()
+            return gchi, gth, None, None, None, None, None, None, None
+
+    return QSRun

@@ -1,0 +1,150 @@
+"""GPU: the hand-written adjoint of the FP64 step (stz_adjoint_*) against
+central finite differences of the same forward step on the GPU.
+
+f(x, theta) = <lam, final fields> for a random lam.  The adjoint gives
+df/dx and df/d(c0, Omegaeps0, chi_inf); a central difference along a random
+direction v (fields) or e (parameters) must agree within 1e-5 relative,
+away from kinks: every step's decision bits (upwind sides, ENO branches,
+wall rules, yield branch, plastic substep counts) must be the same in the
++h and -h runs at two step sizes, and the two differences must agree, else
+the direction is redrawn.  The adjoint is also independent of the
+checkpoint interval (recomputation repeats the forward exactly)."""
+import numpy as np
+import pytest
+
+from stategen import random_state
+from stz_amd import ShearSim
+from stz_amd.adjoint import PARAMS, Adjoint, set_params
+from stz_amd.problems import simple_shear
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6      # forward solve threshold, x the reference's
+DT, STEPS = 0.5, 3
+
+
+def case(m=32, seed=5):
+    p = simple_shear(m, u_bdry=2e-3)
+    planes, vel = random_state(p, seed, ties=False)
+    return p, planes, vel
+
+
+def forward(p, planes, vel, theta=None, decisions=True):
+    g = ShearSim(p, "fp64")
+    g.set_fields(planes)
+    g.set_vel(vel)
+    g.time = 1.2
+    g.set_mg_config(tol_mult=TOL)
+    if theta is not None:
+        set_params(g, **theta)
+    g.record_decisions(decisions)
+    bits = []
+    for _ in range(STEPS):
+        g.step_forward_quasistatic(DT, 1)
+        bits.append(g.get_decisions().copy())
+    F = g.get_fields()
+    g.close()
+    return F, bits
+
+
+def gradient(p, planes, vel, lam, every=2):
+    g = ShearSim(p, "fp64")
+    g.set_fields(planes)
+    g.set_vel(vel)
+    g.time = 1.2
+    g.set_mg_config(tol_mult=TOL)
+    a = Adjoint(g, every)
+    a.forward(DT, STEPS)
+    lam_init, gth = a.backward(lam)
+    st = a.stats()
+    a.close()
+    g.close()
+    return lam_init, gth, st
+
+
+def smooth(rng, shape, modes=3):
+    n2, m = shape
+    x = np.arange(m) / m
+    y = np.arange(n2) / n2
+    f = np.zeros(shape)
+    for _ in range(modes):
+        a, b = rng.integers(1, 4, 2)
+        f += rng.normal() * np.cos(2 * np.pi * a * x[None, :] + rng.uniform(0, 6.3)) * \
+            np.cos(np.pi * b * y[:, None] + rng.uniform(0, 6.3))
+    return f
+
+
+def fd(p, planes, vel, lam, dx, dth, h):
+    """central difference of <lam, final fields> along (dx, dth); None when
+    a decision of any step differs between the +h and -h runs"""
+    base = {k: p["stz"][k] for k in PARAMS}
+    th_p = {k: base[k] + h * dth.get(k, 0.0) for k in base}
+    th_m = {k: base[k] - h * dth.get(k, 0.0) for k in base}
+    Fp, bp = forward(p, planes + h * dx, vel, th_p)
+    Fm, bm = forward(p, planes - h * dx, vel, th_m)
+    if not all(np.array_equal(a, b) for a, b in zip(bp, bm)):
+        return None
+    return (np.sum(lam * Fp) - np.sum(lam * Fm)) / (2 * h)
+
+
+def check_direction(p, planes, vel, lam, dirs, label, h=1e-7):
+    """away from kinks: the decisions agree at h and 4h and the two central
+    differences agree to 1e-6 (no hidden switch inside the plastic
+    substeps; a switch moves a difference by far more)"""
+    for attempt, (dx, dth, ad) in enumerate(dirs):
+        f1 = fd(p, planes, vel, lam, dx, dth, h)
+        f4 = fd(p, planes, vel, lam, dx, dth, 4 * h)
+        smooth_here = f1 is not None and f4 is not None and abs(f1 - f4) <= 1e-6 * max(abs(f1), 1e-300)
+        print("%s attempt %d: fd(h) %s fd(4h) %s ad %.10e smooth %s" % (label, attempt, f1, f4, ad, smooth_here),
+              flush=True)
+        if not smooth_here:
+            continue
+        rel = abs(f1 - ad) / max(abs(f1), abs(ad), 1e-300)
+        print("%s: rel %.2e" % (label, rel), flush=True)
+        assert rel < 1e-5, (label, f1, ad, rel)
+        return
+    pytest.skip("%s: every direction drawn is near a kink" % label)
+
+
+@pytest.mark.parametrize("which", ["chi", "fields", "params"])
+def test_adjoint_matches_central_differences(which):
+    print("case adjoint", which, flush=True)
+    p, planes, vel = case()
+    rng = np.random.default_rng(17)
+    lam = rng.standard_normal(planes.shape)
+    lam_init, gth, st = gradient(p, planes, vel, lam)
+    print("adjoint stats", st, "grad theta", gth, flush=True)
+    scales = np.array([np.max(np.abs(planes[k])) for k in range(10)])
+
+    def field_dirs():
+        for _ in range(6):
+            dx = np.zeros_like(planes)
+            for k in ([6] if which == "chi" else range(9)):
+                dx[k] = (scales[k] if scales[k] > 0 else 1.0) * smooth(rng, planes[k].shape)
+            yield dx, {}, float(np.sum(lam_init * dx))
+
+    if which == "params":
+        for k in PARAMS:
+            e = {k: p["stz"][k]}
+            check_direction(p, planes, vel, lam, iter([(np.zeros_like(planes), e, gth[k] * e[k])]), "param " + k)
+    else:
+        # every field at once meets more near-ties (the reference map is
+        # nearly linear), so a smaller step
+        check_direction(p, planes, vel, lam, field_dirs(), which, h=1e-7 if which == "chi" else 2e-8)
+
+
+def test_adjoint_checkpoint_interval_does_not_matter():
+    """recomputing from checkpoints repeats the forward exactly: the
+    gradient is the same for every checkpoint interval"""
+    print("case adjoint checkpoints", flush=True)
+    p, planes, vel = case()
+    lam = np.random.default_rng(2).standard_normal(planes.shape)
+    ref = None
+    for every in (1, 2, 3):
+        li, g, _ = gradient(p, planes, vel, lam, every=every)
+        if ref is None:
+            ref = (li, g)
+            continue
+        assert np.array_equal(li, ref[0]) or np.max(np.abs(li - ref[0])) <= 1e-12 * np.max(np.abs(ref[0]))
+        for k in PARAMS:
+            assert abs(g[k] - ref[1][k]) <= 1e-12 * abs(ref[1][k])
