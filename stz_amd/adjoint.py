@@ -75,6 +75,14 @@ def set_params(sim, c0=None, Omegaeps0=None, chi_inf=None):
         sim.set_chi_inf(float(chi_inf))
 
 
+def reset_predictor(sim):
+    """start the adaptive V-cycle count afresh (tgmg_predict's initial
+    limit), so that repeated runs from the same state take the same cycles"""
+    init = sim.mg_config().predict_init
+    sim.set_mg_config(predict_init=init + 1)
+    sim.set_mg_config(predict_init=init)
+
+
 def chi_planes(chi, planes):
     """planes with chi (n x m, cells) in rows 0..n-1 and the ghost rows
     extrapolated linearly (shear_sim.cc:552-581)"""
@@ -105,14 +113,19 @@ def make_qs_run():
 
     class QSRun(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, chi, theta, sim, base_planes, vel0, t0, dt, nsteps, every):
+        def forward(ctx, chi, theta, sim, base_planes, vel0, t0, dt, nsteps, every, rel=1e-10):
             set_params(sim, *[float(v) for v in theta.detach().cpu().numpy()])
             sim.set_fields(chi_planes(chi.detach().cpu().numpy(), base_planes))
             sim.set_vel(vel0)
             sim.time = t0
-            adj = Adjoint(sim, every)
-            adj.forward(dt, nsteps)
-            ctx.adj = adj
+            reset_predictor(sim)
+            ctx.adj = None
+            if ctx.needs_input_grad[0] or ctx.needs_input_grad[1]:
+                ctx.adj = Adjoint(sim, every)
+                ctx.adj.set_solve_accuracy(rel)
+                ctx.adj.forward(dt, nsteps)
+            else:
+                sim.step_forward_quasistatic(dt, nsteps)
             return torch.from_numpy(sim.get_fields())
 
         @staticmethod
@@ -121,6 +134,6 @@ def make_qs_run():
             gchi = torch.from_numpy(chi_planes_adjoint(lam_init))
             gth = torch.tensor([g[k] for k in PARAMS], dtype=torch.float64)
             ctx.adj.close()
-            return gchi, gth, None, None, None, None, None, None, None
+            return gchi, gth, None, None, None, None, None, None, None, None
 
     return QSRun
