@@ -106,6 +106,48 @@ def chi_planes_adjoint(lam_planes):
     return g
 
 
+def hencky_strain(planes, params):
+    """shear_sim::compute_strain (shear_sim.cc:778-824) as a differentiable
+    torch function of the field planes [10, n+2, m]: the deviatoric modulus
+    of the Hencky strain at the n x m cell centres, in the form with the
+    y derivative taken over rows j+1 and j-1 in the interior rows and the
+    periodic shift added at both seam columns (the reference's own form
+    gives 0 on the interior rows; stz_sim_compute_strain(corrected = 1)
+    computes this one on the device).
+
+    With C = F^T F (F the inverse of the reference map's gradient),
+    eigenvalues l1 >= l2 > 0, the Hencky strain (1/2) log C has eigenvalues
+    (1/2) log l1 and (1/2) log l2 and devmod (mat.hh:51-53) of a symmetric
+    2 x 2 tensor is |e1 - e2| / sqrt(2), so the value is
+    log(l1 / l2) / (2 sqrt 2) = atanh(sqrt((C00 - C11)^2 + 4 C01^2) / tr C) / sqrt 2,
+    which needs no eigenvectors."""
+    import torch
+    m, n = params["m"], params["n"]
+    lx = params["bx"] - params["ax"]
+    xsp = m / lx
+    ysp = n / (params["by"] - params["ay"])
+    X = planes[7][1:n + 2]          # rows 0..n
+    Y = planes[8][1:n + 2]
+    Xc, Yc = X[:n], Y[:n]
+    wrap = torch.zeros(m, dtype=planes.dtype)
+    wrap[0] = lx
+    wrap[m - 1] = lx
+    Xx = (torch.roll(Xc, -1, 1) - torch.roll(Xc, 1, 1) + wrap) * (0.5 * xsp)
+    Yx = (torch.roll(Yc, -1, 1) - torch.roll(Yc, 1, 1)) * (0.5 * xsp)
+
+    def ddy(Z):
+        return torch.cat([ysp * (Z[1:2] - Z[0:1]), (0.5 * ysp) * (Z[2:n] - Z[0:n - 2]),
+                          ysp * (Z[n:n + 1] - Z[n - 1:n])])
+    Xy, Yy = ddy(X), ddy(Y)
+    J = 1 / (Xx * Yy - Xy * Yx)
+    F00, F01, F10, F11 = Yy * J, -Xy * J, -Yx * J, Xx * J
+    C00 = F00 * F00 + F10 * F10
+    C01 = F00 * F01 + F10 * F11
+    C11 = F01 * F01 + F11 * F11
+    disc = torch.sqrt((C00 - C11) ** 2 + 4 * C01 * C01)
+    return torch.atanh(disc / (C00 + C11)) / np.sqrt(2.0)
+
+
 def make_qs_run():
     """torch.autograd.Function: (chi [n, m], theta [3], ctx data) -> final
     field planes [10, n+2, m]; backward through the hand-written adjoint"""

@@ -1,22 +1,24 @@
 """Calibration driver: fit the initial chi field and the STZ parameters
-(c0, Omega*eps0, chi_inf) to a target shear-strain field, with gradients
-from the hand-written adjoint of the FP64 step (stz_amd.adjoint).
+(c0, Omega*eps0, chi_inf) to a target strain field, with gradients from the
+hand-written adjoint of the FP64 step (stz_amd.adjoint).
 
     python -m stz_amd.calibrate [--size 256] [--steps 400] [--every 25]
                                 [--iters 8] [--fd-dirs 16] [--fd-step 1e-6]
 
 Workload: simple shear, `size`^2 cells, `steps` steps of dt = 2.5 from
 t = 0 to t = 1000 (the end time of the 1024^2 workload: elastic loading,
-yield near t = 425, plastic flow).  The target is the final shear strain of a forward run from a hidden
-chi field (620 K + 25 K seeded Gaussian field, correlation length 3 cells)
-and the default parameters, plus 1 % Gaussian noise.  The fit starts from a
-uniform 620 K field and parameters 10 % off; the loss is the mean-square
-misfit of the linearised shear strain -(dX/dy + dY/dx)/2 of the reference
-map, cell by cell.  Prints one JSON line: loss history, the cost of an
-adjoint (backward) run relative to a forward run, and the agreement of the
-gradient with FP64 central differences along `fd-dirs` random directions:
-over a window of a few steps from a plastic state (with the number of
-decisions each difference crosses), and over the whole run.
+yield near t = 425, plastic flow).  The target is the final strain field of
+a forward run from a hidden chi field (620 K + 25 K seeded Gaussian field,
+correlation length 3 cells) and the default parameters, plus 1 % Gaussian
+noise.  The fit starts from a uniform 620 K field and parameters 10 % off;
+the loss is the mean-square misfit, cell by cell, of the deviatoric modulus
+of the Hencky strain that shear_sim::compute_strain writes
+(shear_sim.cc:778-824; the form with central y differences,
+stz_amd.adjoint.hencky_strain).  Prints one JSON line: loss history, the
+cost of an adjoint (backward) run relative to a forward run, and the
+agreement of the gradient with FP64 central differences along `fd-dirs`
+random directions: over a window of a few steps from a plastic state (with
+the number of decisions each difference crosses), and over the whole run.
 """
 import argparse
 import json
@@ -28,19 +30,6 @@ import numpy as np
 # decision bits that are jumps of the step map (include/stz_amd.h): all but
 # the plastic substep count in bits 24..31
 SWITCHES = np.uint32(0x00FFFFFF)
-
-def strain(planes, params, torch):
-    """linearised shear strain at the cell centres from the node fields X, Y"""
-    m, n = params["m"], params["n"]
-    dx = (params["bx"] - params["ax"]) / m
-    dy = (params["by"] - params["ay"]) / n
-    X = planes[7][1:n + 2]
-    Y = planes[8][1:n + 2]
-    Xr = torch.roll(X, -1, dims=1)
-    Yr = torch.roll(Y, -1, dims=1)
-    dXdy = 0.5 * ((X[1:] - X[:-1]) + (Xr[1:] - Xr[:-1])) / dy
-    dYdx = 0.5 * ((Yr[:-1] - Y[:-1]) + (Yr[1:] - Y[1:])) / dx
-    return -0.5 * (dXdy + dYdx)
 
 
 def main():
@@ -61,7 +50,7 @@ def main():
 
     import torch
     from . import problems
-    from .adjoint import PARAMS, make_qs_run
+    from .adjoint import PARAMS, hencky_strain, make_qs_run
     from .initial import random_chi, simple_shear_state
     from .sim import ShearSim
 
@@ -82,7 +71,7 @@ def main():
     # target: hidden chi, default parameters, 1 % noise
     chi_true = torch.from_numpy(random_chi(p, 620.0, 25.0, 3.0, seed=12345))
     with torch.no_grad():
-        g_true = strain(run(chi_true, torch.from_numpy(theta_true)), p, torch)
+        g_true = hencky_strain(run(chi_true, torch.from_numpy(theta_true)), p)
     rng = np.random.default_rng(99)
     target = g_true + 0.01 * float(g_true.std()) * torch.from_numpy(rng.standard_normal(tuple(g_true.shape)))
 
@@ -93,7 +82,7 @@ def main():
 
     def loss_of(chi_K, logs):
         F = run(chi_K / TZ, theta0 * torch.exp(logs))
-        return torch.mean((strain(F, p, torch) - target) ** 2)
+        return torch.mean((hencky_strain(F, p) - target) ** 2)
 
     # gradient check 1: a window of `window` steps from the plastic state the
     # fit starts from at t = 500, f = <lam, fields after the window>, central

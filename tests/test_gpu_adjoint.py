@@ -148,3 +148,61 @@ def test_adjoint_checkpoint_interval_does_not_matter():
         assert np.array_equal(li, ref[0]) or np.max(np.abs(li - ref[0])) <= 1e-12 * np.max(np.abs(ref[0]))
         for k in PARAMS:
             assert abs(g[k] - ref[1][k]) <= 1e-12 * abs(ref[1][k])
+
+
+def test_autograd_run_with_strain_loss():
+    """QSRun (the torch.autograd.Function over the initial chi field and the
+    three parameters) under the calibration's loss, a weighted sum of the
+    Hencky strain of the final state: its gradient along a chi direction and
+    along each parameter against central differences of the same loss, away
+    from kinks as above.  The strain of a state near the identity map is
+    computed with a cancellation (C00 - C11 of C = F^T F near I), which
+    leaves a rounding noise of about 1e-15 in the loss: the steps tried
+    here start 20 times longer than above and shrink when the longer step
+    crosses a kink."""
+    import torch
+    from stz_amd.adjoint import chi_planes, hencky_strain, make_qs_run
+    print("case adjoint autograd strain loss", flush=True)
+    p, planes, vel = case()
+    n, m = p["n"], p["m"]
+    rng = np.random.default_rng(23)
+    w = torch.from_numpy(rng.standard_normal((n, m)))
+    theta = np.array([p["stz"][k] for k in PARAMS])
+    chi0 = planes[6][1:n + 1].copy()
+    sim = ShearSim(p, "fp64")
+    sim.set_mg_config(tol_mult=TOL)
+    chi_t = torch.from_numpy(chi0.copy()).requires_grad_(True)
+    th_t = torch.from_numpy(theta.copy()).requires_grad_(True)
+    F = make_qs_run().apply(chi_t, th_t, sim, planes, vel, 1.2, DT, STEPS, 2, 1e-10)
+    loss = torch.sum(w * hencky_strain(F, p))
+    loss.backward()
+    sim.close()
+    g_chi, g_th = chi_t.grad.numpy(), th_t.grad.numpy()
+
+    def f(chi, th):
+        Fk, bits = forward(p, chi_planes(chi, planes), vel, dict(zip(PARAMS, th)))
+        return float(torch.sum(w * hencky_strain(torch.from_numpy(Fk), p))), bits
+
+    def central(dchi, dth, h):
+        fp, bp = f(chi0 + h * dchi, theta + h * dth)
+        fm, bm = f(chi0 - h * dchi, theta - h * dth)
+        if not all(np.array_equal(a, b) for a, b in zip(bp, bm)):
+            return None
+        return (fp - fm) / (2 * h)
+
+    def check(k, dchi, dth):
+        ad = float(np.sum(g_chi * dchi) + np.dot(g_th, dth))
+        for h in (2e-6, 5e-7, 1e-7):
+            f1, f4 = central(dchi, dth, h), central(dchi, dth, 4 * h)
+            ok = f1 is not None and f4 is not None and abs(f1 - f4) <= 1e-6 * max(abs(f1), 1e-300)
+            print("direction %s h %g: fd(h) %s fd(4h) %s ad %.10e smooth %s" % (k, h, f1, f4, ad, ok), flush=True)
+            if ok:
+                rel = abs(f1 - ad) / max(abs(f1), abs(ad), 1e-300)
+                assert rel < 1e-5, (k, f1, ad, rel)
+                return True
+        return False
+
+    chi_ok = any(check("chi %d" % k, np.max(np.abs(chi0)) * smooth(rng, (n, m)), np.zeros(3))
+                 for k in range(4))
+    par_ok = [check(PARAMS[k], np.zeros((n, m)), theta * np.eye(3)[k]) for k in range(3)]
+    assert chi_ok and sum(par_ok) >= 2, "too few directions away from kinks"
