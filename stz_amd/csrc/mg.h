@@ -88,6 +88,7 @@ int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipStream_t st, double *cor
 // `sweeps` smoothing sweeps on one level (tiled where possible; swaps z/z2)
 template <typename T> int mg_smooth_level(LevelArgs<T> &L, int sweeps, hipStream_t st);
 // residual of the FP64 top level -> FP32 source, |r|^2 partials summed to *out
+// (out == nullptr: partials only)
 int mg_refine(const LevelArgs<double> &L, float *b32, double *partial, double *out, hipStream_t st);
 int mg_refine_blocks(const LevelArgs<double> &L);
 

@@ -545,10 +545,6 @@ template <typename T> static bool big_tiles(const LevelArgs<T> &L) {
     return L.m % 64 == 0 && (L.m / 64) * ((L.n + 31) / 32) >= 256;
 }
 
-// Fusing the restriction into the last pre-smoothing launch pays except for
-// FP64 64x32 tiles, whose widened halo needs two blocks per thread and
-// leaves one workgroup per CU
-template <typename T> static bool fuse_ok(const LevelArgs<T> &L) { return sizeof(T) == 4 || !big_tiles(L); }
 
 template <typename T, int S, int MODE, bool FUSE>
 static void launch_tile(const LevelArgs<T> &L, T *zout, const LevelArgs<T> &Cz, const LevelArgs<T> &Cr, int zero_z,
@@ -651,8 +647,7 @@ template <typename T> int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipSt
         // pre-smoothing, fused with the restriction into level l+1 when tiled
         int done = 0;
         if ((e = smooth_level(A.lv[l], (const LevelArgs<T> *)nullptr, A.nu1,
-                              l == 0 ? (top_zero ? TILE_ZERO : TILE_LOAD) : TILE_ZERO, st,
-                              fuse_ok(A.lv[l]) ? &A.lv[l + 1] : nullptr, &done)))
+                              l == 0 ? (top_zero ? TILE_ZERO : TILE_LOAD) : TILE_ZERO, st, &A.lv[l + 1], &done)))
             return e;
         if (!done && (e = restrict_level(A.lv[l], A.lv[l + 1], st))) return e;
     }
@@ -678,7 +673,7 @@ int mg_refine(const LevelArgs<double> &L, float *b32, double *partial, double *o
     if (L.sparse) hipLaunchKernelGGL(refine_kernel<true>, g, dim3(256), 0, st, L, b32, partial);
     else hipLaunchKernelGGL(refine_kernel<false>, g, dim3(256), 0, st, L, b32, partial);
     STZ_HIP(hipGetLastError());
-    return mg_sum_partials(partial, (int)(g.x * g.y), out, st);
+    return out ? mg_sum_partials(partial, (int)(g.x * g.y), out, st) : STZ_OK;
 }
 
 template bool wall_ok<double>(const LevelArgs<double> &);
@@ -807,10 +802,6 @@ int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out,
         return STZ_ERR_ARG;
     }
     for (int l = 0; l < A.lc; l++) {
-        if (!fuse_ok(A.lv[l])) {
-            set_error("level %d (%d x %d) would need a separate restriction launch", l, A.lv[l].m, A.lv[l].n);
-            return STZ_ERR_ARG;
-        }
         if ((e = plan_level(A.lv[l], (const LevelArgs<T> *)nullptr, A.nu1,
                             l == 0 ? (top_zero ? TILE_ZERO : TILE_LOAD) : TILE_ZERO, &A.lv[l + 1], l, 0, out)))
             return e;

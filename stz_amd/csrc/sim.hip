@@ -361,10 +361,13 @@ int mg_build(stz_sim *s, double viscdt) {
 
 // mixed form: FP64 residual of the top level -> FP32 source of the next
 // correction equation (and zero correction for the per-colour path)
-static int refine_resid(stz_sim *s) {
+// FP64 residual -> FP32 source of the correction equation; with `norm` also
+// its mean square into *dnorm (the host reads it only at convergence tests,
+// so the other residuals skip the summing launch)
+static int refine_resid(stz_sim *s, bool norm = true) {
     LevelArgs<double> t = s->top64();
     LevelArgs<float> &f = s->mg32.args.lv[0];
-    if (s->cfg.kernels == 0 && wall_ok(t)) return mg_refine(t, f.b, s->partial, s->dnorm, s->stream);
+    if (s->cfg.kernels == 0 && wall_ok(t)) return mg_refine(t, f.b, s->partial, norm ? s->dnorm : nullptr, s->stream);
     return mg_resid64_to32(t, f.b, f.z, s->partial, s->dnorm, s->stream);
 }
 
@@ -396,7 +399,7 @@ static int top_home(stz_sim *s) {
 // one cycle: FP64 V-cycle on vel, or (mixed) FP32 V-cycle on the correction
 // equation whose source the last residual evaluation wrote, then vel += e
 // and a fresh FP64 residual (the next cycle's source)
-static int one_cycle(stz_sim *s, bool need_resid) {
+static int one_cycle(stz_sim *s, bool need_resid, bool need_norm = true) {
     int e;
     const bool tiled = s->cfg.kernels == 0;
     if (s->precision == STZ_PREC_FP64) {
@@ -418,14 +421,16 @@ static int one_cycle(stz_sim *s, bool need_resid) {
     }
     if (!corrected && (e = mg_correct64(s->vel, A.lv[0].z, 2 * s->c.m * (s->c.n + 1), s->stream))) return e;
     s->last.refinements++;
-    if (need_resid) return refine_resid(s);
+    if (need_resid) return refine_resid(s, need_norm);
     return STZ_OK;
 }
 
+// k cycles; with resid_after the residual after the last one, whose norm
+// the convergence test reads
 static int run_cycles(stz_sim *s, int k, bool resid_after) {
     int e;
     for (int i = 0; i < k; i++)
-        if ((e = one_cycle(s, resid_after || i + 1 < k))) return e;
+        if ((e = one_cycle(s, resid_after || i + 1 < k, resid_after && i + 1 == k))) return e;
     s->last.total_cycles += k;
     return STZ_OK;
 }
@@ -439,11 +444,12 @@ static int mg_solve_impl(stz_sim *s) {
     std::memset(&s->last, 0, sizeof s->last);
     s->last.tolerance = tol;
     double acc;
+    int n = s->lim / g.predict_mult;
     if (s->precision == STZ_PREC_MIXED) {  // first source of the correction equation
-        if ((e = refine_resid(s))) return e;
+        // (its norm is the first test's when no cycle comes before it)
+        if ((e = refine_resid(s, n == 0))) return e;
         s->last.refinements = 0;
     }
-    int n = s->lim / g.predict_mult;
     if ((e = run_cycles(s, n, true))) return e;
     if (s->precision == STZ_PREC_MIXED) {
         STZ_HIP(hipStreamSynchronize(s->stream));
