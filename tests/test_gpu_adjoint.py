@@ -74,26 +74,48 @@ def smooth(rng, shape, modes=3):
     return f
 
 
-def fd(p, planes, vel, lam, dx, dth, h):
+def oracle_forward(p, planes, vel, theta):
+    """the same steps through the FP64 restatement of the reference's step
+    (oracle/, on the CPU), with its own decision bits"""
+    from oracle.pyoracle import Oracle
+    q = dict(p)
+    q["stz"] = dict(p["stz"], c0=theta["c0"], Omegaeps0=theta["Omegaeps0"])
+    o = Oracle(q)
+    o.set_fields(planes)
+    o.set_vel(vel)
+    o.time = 1.2
+    o.set_tol_mult(TOL)
+    o.set_chi_inf(theta["chi_inf"])
+    o.record_decisions(True)
+    bits = []
+    for _ in range(STEPS):
+        assert o.step(DT, 1) == 0
+        bits.append(o.get_decisions().copy())
+    F = o.get_fields()
+    o.close()
+    return F, bits
+
+
+def fd(p, planes, vel, lam, dx, dth, h, fwd=forward):
     """central difference of <lam, final fields> along (dx, dth); None when
     a decision of any step differs between the +h and -h runs"""
     base = {k: p["stz"][k] for k in PARAMS}
     th_p = {k: base[k] + h * dth.get(k, 0.0) for k in base}
     th_m = {k: base[k] - h * dth.get(k, 0.0) for k in base}
-    Fp, bp = forward(p, planes + h * dx, vel, th_p)
-    Fm, bm = forward(p, planes - h * dx, vel, th_m)
+    Fp, bp = fwd(p, planes + h * dx, vel, th_p)
+    Fm, bm = fwd(p, planes - h * dx, vel, th_m)
     if not all(np.array_equal(a, b) for a, b in zip(bp, bm)):
         return None
     return (np.sum(lam * Fp) - np.sum(lam * Fm)) / (2 * h)
 
 
-def check_direction(p, planes, vel, lam, dirs, label, h=1e-7):
+def check_direction(p, planes, vel, lam, dirs, label, h=1e-7, fwd=forward):
     """away from kinks: the decisions agree at h and 4h and the two central
     differences agree to 1e-6 (no hidden switch inside the plastic
     substeps; a switch moves a difference by far more)"""
     for attempt, (dx, dth, ad) in enumerate(dirs):
-        f1 = fd(p, planes, vel, lam, dx, dth, h)
-        f4 = fd(p, planes, vel, lam, dx, dth, 4 * h)
+        f1 = fd(p, planes, vel, lam, dx, dth, h, fwd)
+        f4 = fd(p, planes, vel, lam, dx, dth, 4 * h, fwd)
         smooth_here = f1 is not None and f4 is not None and abs(f1 - f4) <= 1e-6 * max(abs(f1), 1e-300)
         print("%s attempt %d: fd(h) %s fd(4h) %s ad %.10e smooth %s" % (label, attempt, f1, f4, ad, smooth_here),
               flush=True)
@@ -131,6 +153,31 @@ def test_adjoint_matches_central_differences(which):
         # every field at once meets more near-ties (the reference map is
         # nearly linear), so a smaller step
         check_direction(p, planes, vel, lam, field_dirs(), which, h=1e-7 if which == "chi" else 2e-8)
+
+
+def test_adjoint_matches_differences_of_the_reference_step():
+    """the GPU adjoint against central differences of the checker's FP64
+    restatement of the reference's step (the oracle, CPU): an independent
+    FP64 reference gradient along a chi direction and each parameter, away
+    from kinks by the oracle's own decision bits"""
+    print("case adjoint vs oracle differences", flush=True)
+    p, planes, vel = case()
+    rng = np.random.default_rng(29)
+    lam = rng.standard_normal(planes.shape)
+    lam_init, gth, _ = gradient(p, planes, vel, lam)
+    scale = np.max(np.abs(planes[6]))
+
+    def chi_dirs():
+        for _ in range(6):
+            dx = np.zeros_like(planes)
+            dx[6] = scale * smooth(rng, planes[6].shape)
+            yield dx, {}, float(np.sum(lam_init * dx))
+
+    check_direction(p, planes, vel, lam, chi_dirs(), "oracle chi", fwd=oracle_forward)
+    for k in PARAMS:
+        e = {k: p["stz"][k]}
+        check_direction(p, planes, vel, lam, iter([(np.zeros_like(planes), e, gth[k] * e[k])]), "oracle " + k,
+                        fwd=oracle_forward)
 
 
 def test_adjoint_checkpoint_interval_does_not_matter():
