@@ -360,6 +360,12 @@ int stz_bd_remarch(stz_bd *b);
  * BiCGStab iteration cap (500) and the mean-square residual threshold
  * (1e-20, qs_multi_bd.cc:16). */
 int stz_bd_set_solver(stz_bd *b, int nu1, int nu2, int bottom_sweeps, int max_iter, double tol);
+/* Precision of the projection solve's preconditioner: STZ_PREC_FP64 (the
+ * default) or STZ_PREC_MIXED (each V-cycle in FP32 on FP32 copies of the
+ * Galerkin levels; BiCGStab, its residuals and the convergence test stay
+ * FP64, so the solve stops at the same threshold).  No counterpart in the
+ * reference, which solves in FP64 throughout (qs_multi_bd.cc:12-196). */
+int stz_bd_set_precision(stz_bd *b, int precision);
 /* Work counts of the last step: out4 = {march rounds (both sides), cell
  * extrapolation waves, node extrapolation waves, BiCGStab iterations}. */
 int stz_bd_step_stats(stz_bd *b, int32_t *out4);

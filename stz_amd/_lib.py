@@ -164,6 +164,7 @@ _SIGS = {
     "stz_bd_step_stats": [C.c_void_p, _ip],
     "stz_bd_remarch": [C.c_void_p],
     "stz_bd_set_solver": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
+    "stz_bd_set_precision": [C.c_void_p, C.c_int],
     "stz_bd_operator": [C.c_void_p, C.c_double, _ip, _dp, _dp],
     "stz_bd_set_body_force": [C.c_void_p, C.c_void_p],
     "stz_bd_set_edge_velocity": [C.c_void_p, C.c_double, C.c_double],

@@ -164,6 +164,11 @@ class BdSim:
         check(self._lib.stz_bd_set_solver(self._h, int(nu1), int(nu2), int(bottom_sweeps), int(max_iter),
                                           float(tol)))
 
+    def set_precision(self, precision):
+        """"fp64" (default) or "mixed": the projection's V-cycle
+        preconditioner in FP32 inside FP64 BiCGStab"""
+        check(self._lib.stz_bd_set_precision(self._h, {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]))
+
     def step_stats(self):
         a = np.zeros(4, np.int32)
         check(self._lib.stz_bd_step_stats(self._h, a))

@@ -1,15 +1,17 @@
 """Throughput of the free-boundary quasi-static step (bd_sim) on one GPU.
 
     python -m stz_amd.bd_bench [--size 4096] [--steps 10] [--warmup 2]
+                               [--precision mixed|fp64]
 
 Workload (config C3 of this build's workload table): a plate |y| < 0.9 in
 the box [-1,1]^2 with a central hole of radius 0.4 (0.2 L), held by grips on
 the left and right edges that move apart at nominal strain rate 1e-4
 (u = -+1e-4), material of bd_test.cc, chi = 580 K + 30 K seeded Gaussian
-field with correlation length 4 cells inside the plate, dt = 0.5.  Prints
-one JSON line: cell-steps/s over the m*n cells, ms per step and the work
-counts of the last step (march rounds, extrapolation waves, BiCGStab
-iterations).
+field with correlation length 4 cells inside the plate, dt = 0.5; the
+projection's V-cycle preconditioner in FP32 inside FP64 BiCGStab (mixed,
+the default) or in FP64.  Prints one JSON line: cell-steps/s over the m*n
+cells, ms per step and the work counts of the last step (march rounds,
+extrapolation waves, BiCGStab iterations).
 """
 import argparse
 import json
@@ -46,9 +48,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--dt", type=float, default=0.5)
     ap.add_argument("--nu", type=int, default=1)
+    ap.add_argument("--precision", default="mixed", choices=("mixed", "fp64"))
     a = ap.parse_args()
     sim = setup(a.size)
     sim.set_solver(a.nu, a.nu)
+    sim.set_precision(a.precision)
     sim.step_forward_quasistatic(a.dt, a.warmup)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -56,6 +60,7 @@ def main():
     sim.synchronize()
     el = time.perf_counter() - t0
     out = {"workload": "plate with hole, grips, %dx%d cells" % (a.size, a.size), "nu": a.nu,
+           "precision": a.precision,
            "steps": a.steps, "ms_per_step": 1e3 * el / a.steps,
            "cell_steps_per_s": a.size * a.size * a.steps / el,
            "last_solve": sim.last_solve(), "last_step": sim.step_stats()}

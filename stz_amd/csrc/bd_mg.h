@@ -18,14 +18,17 @@ struct BdTop {
 };
 
 // Coarse level.  Rows whose fine rows were all interior rows share one
-// stencil Sc (type -1); the others keep their own 40 doubles in S (type 0).
-struct BdLevel {
+// stencil Sc (type -1); the others keep their own 40 entries in S (type 0).
+// The Galerkin hierarchy is built in FP64; the mixed form keeps an FP32 copy
+// of every level (stencils rounded once per solve) for its preconditioner.
+template <typename T> struct BdLevelT {
     int m = 0, n = 0;
-    double *z = nullptr, *b = nullptr, *r = nullptr;
-    double *S = nullptr;   // 40 per node, read only where type >= 0
-    int *type = nullptr;
-    double *Sc = nullptr;  // 40
+    T *z = nullptr, *b = nullptr, *r = nullptr;
+    T *S = nullptr;        // 40 per node, read only where type >= 0
+    int *type = nullptr;   // shared by the FP64 and FP32 copies
+    T *Sc = nullptr;       // 40
 };
+using BdLevel = BdLevelT<double>;
 
 enum { kR = 0, kRhat, kP, kV, kS, kT, kPhat, kShat, kTmp, kBdVecs };
 constexpr int kMaxPartial = 4096;
@@ -38,18 +41,27 @@ struct BdSolver {
     double *vec[kBdVecs] = {};
     double *partial = nullptr, *sc = nullptr;
     double *hscal = nullptr;   // coherent pinned
+    // mixed form: the V-cycle (the preconditioner) in FP32 on FP32 copies of
+    // the levels; BiCGStab, its residuals and the convergence test stay FP64
+    bool mixed = false;
+    BdLevelT<float> lv32[kMaxLevels];
+    float *top32[3] = {};      // top-level source, correction, residual
 
     static int dims_for(int tm, int tn, std::vector<std::pair<int, int>> &out);
     int init(int tm, int tn);
     void free_all();
+    int set_mixed(bool on);
     int setup(const BdTop &t, hipStream_t s);          // Galerkin hierarchy
     int solve(double *x, const double *b, double acc, int max_iter, stz_solve_info &info, hipStream_t s);
     int l2_error(const double *x, const double *b, double *out, hipStream_t s);
     int vcycle(const double *rhs, double *z, hipStream_t s);
 
   private:
-    int smooth_top(double *z, const double *b, int sweeps, hipStream_t s);
-    int smooth_level(int l, int sweeps, hipStream_t s);
+    int alloc_mixed();
+    void free_mixed();
+    template <typename T> int vcycle_t(const T *rhs, T *z, BdLevelT<T> *L, T *r0, hipStream_t s);
+    template <typename T> int smooth_top(T *z, const T *b, int sweeps, hipStream_t s);
+    template <typename T> int smooth_level(const BdLevelT<T> &L, bool bottom, int sweeps, hipStream_t s);
     int dot(const double *a, const double *b, int slot, int op, int host_slot, hipStream_t s);
 };
 

@@ -32,9 +32,10 @@ namespace {
 constexpr int kThreads = 256;
 inline dim3 blocks(long n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
 
-struct V2 {
-    double x, y;
+template <typename T> struct V2T {
+    T x, y;
 };
+using V2 = V2T<double>;
 
 // one entry (2x2 block k of row (i,j)) of the top operator, as the
 // reference's a_xx accessors give it (qs_multi_bd.hh:31-39)
@@ -67,80 +68,84 @@ __device__ __forceinline__ void top_entry(const BdTop &t, int ij, int k, double 
 }
 
 // off-diagonal part of row ij applied to z (qs_multi_bd::mul_a,
-// qs_multi_bd.hh:41-56) and the diagonal block
-__device__ __forceinline__ V2 top_offdiag(const BdTop &t, const double *z, int ij, int d) {
+// qs_multi_bd.hh:41-56) and the diagonal block; in T, with the operator's
+// FP64 constants rounded to T
+template <typename T> __device__ __forceinline__ V2T<T> top_offdiag(const BdTop &t, const T *z, int ij, int d) {
     const int m = t.m;
     if (d == -1) {
-        const double *zp = z + 2 * (size_t)ij;
-        const double *zl = zp - 2, *zr = zp + 2, *zd = zp - 2 * m, *zu = zp + 2 * m;
-        const double *zdl = zd - 2, *zdr = zd + 2, *zul = zu - 2, *zur = zu + 2;
-        return V2{t.Kmx * (zl[0] + zr[0]) + t.my * (zd[0] + zu[0]) + t.qK * (zdl[1] + zur[1] - zdr[1] - zul[1]),
-                  t.mx * (zl[1] + zr[1]) + t.Kmy * (zd[1] + zu[1]) + t.qK * (zdl[0] + zur[0] - zdr[0] - zul[0])};
+        const T *zp = z + 2 * (size_t)ij;
+        const T *zl = zp - 2, *zr = zp + 2, *zd = zp - 2 * m, *zu = zp + 2 * m;
+        const T *zdl = zd - 2, *zdr = zd + 2, *zul = zu - 2, *zur = zu + 2;
+        const T Kmx = (T)t.Kmx, Kmy = (T)t.Kmy, mx = (T)t.mx, my = (T)t.my, qK = (T)t.qK;
+        return V2T<T>{Kmx * (zl[0] + zr[0]) + my * (zd[0] + zu[0]) + qK * (zdl[1] + zur[1] - zdr[1] - zul[1]),
+                      mx * (zl[1] + zr[1]) + Kmy * (zd[1] + zu[1]) + qK * (zdl[0] + zur[0] - zdr[0] - zul[0])};
     }
-    if (d < 0) return V2{0, 0};
+    if (d < 0) return V2T<T>{0, 0};
     const double *a = t.fmats + 32 * (size_t)d;
     const int off[8] = {-m - 1, -m, -m + 1, -1, 1, m - 1, m, m + 1};
-    double rx = 0, ry = 0;
+    T rx = 0, ry = 0;
     for (int q = 0; q < 8; q++) {
-        const double *zz = z + 2 * (size_t)(ij + off[q]);
-        rx += a[4 * q] * zz[0] + a[4 * q + 1] * zz[1];
-        ry += a[4 * q + 2] * zz[0] + a[4 * q + 3] * zz[1];
+        const T *zz = z + 2 * (size_t)(ij + off[q]);
+        rx += (T)a[4 * q] * zz[0] + (T)a[4 * q + 1] * zz[1];
+        ry += (T)a[4 * q + 2] * zz[0] + (T)a[4 * q + 3] * zz[1];
     }
-    return V2{rx, ry};
+    return V2T<T>{rx, ry};
 }
 __device__ __forceinline__ double top_diag_scale(const BdTop &t, int d) {
     return d == -1 ? 1.0 : (d == -2 ? t.bcs : (d == -3 ? t.ocs : t.ncs));
 }
 
-__device__ __forceinline__ const double *lvl_stencil(const BdLevel &L, int ij) {
+template <typename T> __device__ __forceinline__ const T *lvl_stencil(const BdLevelT<T> &L, int ij) {
     return L.type[ij] < 0 ? L.Sc : L.S + 40 * (size_t)ij;
 }
 
-__device__ __forceinline__ V2 lvl_offdiag(const BdLevel &L, const double *z, int i, int j) {
-    const double *s = lvl_stencil(L, i + L.m * j);
-    double rx = 0, ry = 0;
+template <typename T> __device__ __forceinline__ V2T<T> lvl_offdiag(const BdLevelT<T> &L, const T *z, int i, int j) {
+    const T *s = lvl_stencil(L, i + L.m * j);
+    T rx = 0, ry = 0;
     for (int k = 0; k < 9; k++) {
         if (k == 4) continue;
         const int ii = i + k % 3 - 1, jj = j + k / 3 - 1;
         if (ii < 0 || ii >= L.m || jj < 0 || jj >= L.n) continue;
-        const double *zz = z + 2 * (size_t)(ii + L.m * jj);
+        const T *zz = z + 2 * (size_t)(ii + L.m * jj);
         rx += s[4 * k] * zz[0] + s[4 * k + 1] * zz[1];
         ry += s[4 * k + 2] * zz[0] + s[4 * k + 3] * zz[1];
     }
-    return V2{rx, ry};
+    return V2T<T>{rx, ry};
 }
 
 // ---------------------------------------------------------------------------
 // Gauss-Seidel sweeps, one colour (i&1, j&1) per launch
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) gs_top_kernel(BdTop t, double *__restrict__ z,
-                                                          const double *__restrict__ b, int colour) {
+template <typename T>
+__global__ void __launch_bounds__(kThreads) gs_top_kernel(BdTop t, T *__restrict__ z, const T *__restrict__ b,
+                                                          int colour) {
     const int hm = (t.m + 1 - (colour & 1)) >> 1, hn = (t.n + 1 - (colour >> 1)) >> 1;
     const int q = blockIdx.x * kThreads + threadIdx.x;
     if (q >= hm * hn) return;
     const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + t.m * j;
     const int d = t.rowt[ij];
-    const V2 o = top_offdiag(t, z, ij, d);
+    const V2T<T> o = top_offdiag(t, z, ij, d);
     const double sc = top_diag_scale(t, d);
-    z[2 * (size_t)ij] = (b[2 * (size_t)ij] - o.x) / (sc * t.cxx);
-    z[2 * (size_t)ij + 1] = (b[2 * (size_t)ij + 1] - o.y) / (sc * t.cyy);
+    z[2 * (size_t)ij] = (b[2 * (size_t)ij] - o.x) / (T)(sc * t.cxx);
+    z[2 * (size_t)ij + 1] = (b[2 * (size_t)ij + 1] - o.y) / (T)(sc * t.cyy);
 }
 
-__global__ void __launch_bounds__(kThreads) gs_level_kernel(BdLevel L, int colour) {
+template <typename T> __global__ void __launch_bounds__(kThreads) gs_level_kernel(BdLevelT<T> L, int colour) {
     const int hm = (L.m + 1 - (colour & 1)) >> 1, hn = (L.n + 1 - (colour >> 1)) >> 1;
     const int q = blockIdx.x * kThreads + threadIdx.x;
     if (q >= hm * hn) return;
     const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + L.m * j;
-    const V2 o = lvl_offdiag(L, L.z, i, j);
-    const double rx = L.b[2 * (size_t)ij] - o.x, ry = L.b[2 * (size_t)ij + 1] - o.y;
-    const double *iv = lvl_stencil(L, ij) + 36;
+    const V2T<T> o = lvl_offdiag(L, L.z, i, j);
+    const T rx = L.b[2 * (size_t)ij] - o.x, ry = L.b[2 * (size_t)ij + 1] - o.y;
+    const T *iv = lvl_stencil(L, ij) + 36;
     L.z[2 * (size_t)ij] = iv[0] * rx + iv[1] * ry;
     L.z[2 * (size_t)ij + 1] = iv[2] * rx + iv[3] * ry;
 }
 
 // the bottom level in one workgroup: z in LDS, `sweeps` four-colour sweeps
-__global__ void __launch_bounds__(1024) gs_bottom_kernel(BdLevel L, int sweeps) {
-    extern __shared__ double zs[];
+template <typename T> __global__ void __launch_bounds__(1024) gs_bottom_kernel(BdLevelT<T> L, int sweeps) {
+    extern __shared__ __align__(16) unsigned char bd_smem[];
+    T *zs = reinterpret_cast<T *>(bd_smem);
     const int N = L.m * L.n;
     for (int q = threadIdx.x; q < 2 * N; q += blockDim.x) zs[q] = L.z[q];
     __syncthreads();
@@ -149,9 +154,9 @@ __global__ void __launch_bounds__(1024) gs_bottom_kernel(BdLevel L, int sweeps) 
             const int hm = (L.m + 1 - (colour & 1)) >> 1, hn = (L.n + 1 - (colour >> 1)) >> 1;
             for (int q = threadIdx.x; q < hm * hn; q += blockDim.x) {
                 const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + L.m * j;
-                const V2 o = lvl_offdiag(L, zs, i, j);
-                const double rx = L.b[2 * ij] - o.x, ry = L.b[2 * ij + 1] - o.y;
-                const double *iv = lvl_stencil(L, ij) + 36;
+                const V2T<T> o = lvl_offdiag(L, zs, i, j);
+                const T rx = L.b[2 * ij] - o.x, ry = L.b[2 * ij + 1] - o.y;
+                const T *iv = lvl_stencil(L, ij) + 36;
                 zs[2 * ij] = iv[0] * rx + iv[1] * ry;
                 zs[2 * ij + 1] = iv[2] * rx + iv[3] * ry;
             }
@@ -161,23 +166,23 @@ __global__ void __launch_bounds__(1024) gs_bottom_kernel(BdLevel L, int sweeps) 
 }
 
 // r = b - A z on the top level; optional per-block |r|^2 partial sums
-__global__ void __launch_bounds__(kThreads) resid_top_kernel(BdTop t, const double *__restrict__ z,
-                                                             const double *__restrict__ b, double *__restrict__ r,
-                                                             double *__restrict__ partial) {
+template <typename T>
+__global__ void __launch_bounds__(kThreads) resid_top_kernel(BdTop t, const T *__restrict__ z, const T *__restrict__ b,
+                                                             T *__restrict__ r, double *__restrict__ partial) {
     __shared__ double red[kThreads / 64];
     const int ij = blockIdx.x * kThreads + threadIdx.x;
     double acc = 0;
     if (ij < t.m * t.n) {
         const int d = t.rowt[ij];
-        const V2 o = top_offdiag(t, z, ij, d);
+        const V2T<T> o = top_offdiag(t, z, ij, d);
         const double sc = top_diag_scale(t, d);
-        const double rx = b[2 * (size_t)ij] - o.x - sc * t.cxx * z[2 * (size_t)ij];
-        const double ry = b[2 * (size_t)ij + 1] - o.y - sc * t.cyy * z[2 * (size_t)ij + 1];
+        const T rx = b[2 * (size_t)ij] - o.x - (T)(sc * t.cxx) * z[2 * (size_t)ij];
+        const T ry = b[2 * (size_t)ij + 1] - o.y - (T)(sc * t.cyy) * z[2 * (size_t)ij + 1];
         if (r) {
             r[2 * (size_t)ij] = rx;
             r[2 * (size_t)ij + 1] = ry;
         }
-        acc = rx * rx + ry * ry;
+        acc = (double)rx * rx + (double)ry * ry;
     }
     if (!partial) return;
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
@@ -202,13 +207,13 @@ __global__ void __launch_bounds__(kThreads) apply_top_kernel(BdTop t, const doub
     y[2 * (size_t)ij + 1] = o.y + sc * t.cyy * x[2 * (size_t)ij + 1];
 }
 
-__global__ void __launch_bounds__(kThreads) resid_level_kernel(BdLevel L) {
+template <typename T> __global__ void __launch_bounds__(kThreads) resid_level_kernel(BdLevelT<T> L) {
     const int ij = blockIdx.x * kThreads + threadIdx.x;
     if (ij >= L.m * L.n) return;
     const int i = ij % L.m, j = ij / L.m;
-    const V2 o = lvl_offdiag(L, L.z, i, j);
-    const double *s = lvl_stencil(L, ij) + 16;
-    const double zx = L.z[2 * (size_t)ij], zy = L.z[2 * (size_t)ij + 1];
+    const V2T<T> o = lvl_offdiag(L, L.z, i, j);
+    const T *s = lvl_stencil(L, ij) + 16;
+    const T zx = L.z[2 * (size_t)ij], zy = L.z[2 * (size_t)ij + 1];
     L.r[2 * (size_t)ij] = L.b[2 * (size_t)ij] - o.x - (s[0] * zx + s[1] * zy);
     L.r[2 * (size_t)ij + 1] = L.b[2 * (size_t)ij + 1] - o.y - (s[2] * zx + s[3] * zy);
 }
@@ -228,12 +233,13 @@ __device__ __forceinline__ double weight_of(int np, const int *c, const double *
 }
 
 // b_c = P^T r_f (restriction with the unnormalised weights 1, 1/2, 1/4)
-__global__ void __launch_bounds__(kThreads) restrict_kernel(const double *__restrict__ rf, int fm, int fn,
-                                                            double *__restrict__ bc, int cm, int cn) {
+template <typename T>
+__global__ void __launch_bounds__(kThreads) restrict_kernel(const T *__restrict__ rf, int fm, int fn,
+                                                            T *__restrict__ bc, int cm, int cn) {
     const int IJ = blockIdx.x * kThreads + threadIdx.x;
     if (IJ >= cm * cn) return;
     const int I = IJ % cm, J = IJ / cm;
-    double sx = 0, sy = 0;
+    T sx = 0, sy = 0;
     int c[2];
     double w[2];
     for (int fy = 2 * J - 1; fy <= 2 * J + 1; fy++) {
@@ -242,9 +248,9 @@ __global__ void __launch_bounds__(kThreads) restrict_kernel(const double *__rest
         for (int fx = 2 * I - 1; fx <= 2 * I + 1; fx++) {
             const double wx = weight_of(parents(fx, fm, cm, c, w), c, w, I);
             if (wx == 0) continue;
-            const double *r = rf + 2 * (size_t)(fx + fm * fy);
-            sx += wx * wy * r[0];
-            sy += wx * wy * r[1];
+            const T *r = rf + 2 * (size_t)(fx + fm * fy);
+            sx += (T)(wx * wy) * r[0];
+            sy += (T)(wx * wy) * r[1];
         }
     }
     bc[2 * (size_t)IJ] = sx;
@@ -252,19 +258,20 @@ __global__ void __launch_bounds__(kThreads) restrict_kernel(const double *__rest
 }
 
 // z_f += P z_c
-__global__ void __launch_bounds__(kThreads) prolong_kernel(const double *__restrict__ zc, int cm, int cn,
-                                                           double *__restrict__ zf, int fm, int fn) {
+template <typename T>
+__global__ void __launch_bounds__(kThreads) prolong_kernel(const T *__restrict__ zc, int cm, int cn,
+                                                           T *__restrict__ zf, int fm, int fn) {
     const int ij = blockIdx.x * kThreads + threadIdx.x;
     if (ij >= fm * fn) return;
     const int i = ij % fm, j = ij / fm;
     int cx[2], cy[2];
     double wx[2], wy[2];
     const int nx = parents(i, fm, cm, cx, wx), ny = parents(j, fn, cn, cy, wy);
-    double sx = 0, sy = 0;
+    T sx = 0, sy = 0;
     for (int b = 0; b < ny; b++)
         for (int a = 0; a < nx; a++) {
-            const double w = wx[a] * wy[b];
-            const double *z = zc + 2 * (size_t)(cx[a] + cm * cy[b]);
+            const T w = (T)(wx[a] * wy[b]);
+            const T *z = zc + 2 * (size_t)(cx[a] + cm * cy[b]);
             sx += w * z[0];
             sy += w * z[1];
         }
@@ -385,9 +392,16 @@ __global__ void rap_const_kernel(Src src, BdLevel C) {
     for (int q = 0; q < 40; q++) C.Sc[q] = o[q];
 }
 
-__global__ void __launch_bounds__(kThreads) fill_kernel(double *p, long n, double v) {
+template <typename T> __global__ void __launch_bounds__(kThreads) fill_kernel(T *p, long n, T v) {
     const long q = (long)blockIdx.x * kThreads + threadIdx.x;
     if (q < n) p[q] = v;
+}
+
+// precision changes at the mixed preconditioner's boundary
+template <typename S, typename D>
+__global__ void __launch_bounds__(kThreads) convert_kernel(const S *__restrict__ a, D *__restrict__ b, long n) {
+    const long q = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (q < n) b[q] = (D)a[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -480,6 +494,7 @@ int BdSolver::dims_for(int tm, int tn, std::vector<std::pair<int, int>> &out) {
 }
 
 int BdSolver::init(int tm, int tn) {
+    const bool was_mixed = mixed;
     free_all();
     top_m = tm;
     top_n = tn;
@@ -502,10 +517,57 @@ int BdSolver::init(int tm, int tn) {
         STZ_HIP(hipMalloc(&L.type, (size_t)L.m * L.n * sizeof(int)));
         STZ_HIP(hipMalloc(&L.Sc, 40 * sizeof(double)));
     }
+    return was_mixed ? set_mixed(true) : STZ_OK;
+}
+
+int BdSolver::alloc_mixed() {
+    const size_t nv = 2 * (size_t)top_m * top_n;
+    for (int k = 0; k < 3; k++) STZ_HIP(hipMalloc(&top32[k], nv * sizeof(float)));
+    for (int l = 0; l < nl; l++) {
+        BdLevelT<float> &L = lv32[l];
+        L.m = lv[l].m;
+        L.n = lv[l].n;
+        const size_t n2 = 2 * (size_t)L.m * L.n;
+        STZ_HIP(hipMalloc(&L.z, n2 * sizeof(float)));
+        STZ_HIP(hipMalloc(&L.b, n2 * sizeof(float)));
+        STZ_HIP(hipMalloc(&L.r, n2 * sizeof(float)));
+        STZ_HIP(hipMalloc(&L.S, 20 * n2 * sizeof(float)));
+        STZ_HIP(hipMalloc(&L.Sc, 40 * sizeof(float)));
+        L.type = lv[l].type;
+    }
+    return STZ_OK;
+}
+
+void BdSolver::free_mixed() {
+    for (int k = 0; k < 3; k++) {
+        if (top32[k]) (void)hipFree(top32[k]);
+        top32[k] = nullptr;
+    }
+    for (int l = 0; l < kMaxLevels; l++) {
+        BdLevelT<float> &L = lv32[l];
+        for (float *q : {L.z, L.b, L.r, L.S, L.Sc})
+            if (q) (void)hipFree(q);
+        L = BdLevelT<float>{};
+    }
+}
+
+// FP32 copies for the mixed form's preconditioner (allocated once; the
+// stencils are rounded from the FP64 hierarchy at every setup)
+int BdSolver::set_mixed(bool on) {
+    if (on && !top32[0]) {
+        const int e = alloc_mixed();
+        if (e) {
+            free_mixed();
+            return e;
+        }
+    }
+    if (!on) free_mixed();
+    mixed = on;
     return STZ_OK;
 }
 
 void BdSolver::free_all() {
+    free_mixed();
     for (int k = 0; k < kBdVecs; k++) {
         if (vec[k]) (void)hipFree(vec[k]);
         vec[k] = nullptr;
@@ -545,73 +607,92 @@ int BdSolver::setup(const BdTop &t, hipStream_t s) {
         }
         STZ_HIP(hipGetLastError());
     }
+    if (mixed)
+        for (int l = 0; l < nl; l++) {
+            const long ns = 40L * lv[l].m * lv[l].n;
+            hipLaunchKernelGGL((convert_kernel<double, float>), blocks(ns), dim3(kThreads), 0, s, lv[l].S, lv32[l].S, ns);
+            hipLaunchKernelGGL((convert_kernel<double, float>), dim3(1), dim3(kThreads), 0, s, lv[l].Sc, lv32[l].Sc,
+                               40L);
+            STZ_HIP(hipGetLastError());
+        }
     return STZ_OK;
 }
 
-int BdSolver::smooth_top(double *z, const double *b, int sweeps, hipStream_t s) {
+template <typename T> int BdSolver::smooth_top(T *z, const T *b, int sweeps, hipStream_t s) {
     for (int k = 0; k < sweeps; k++)
         for (int c = 0; c < 4; c++) {
             const long cnt = (long)((top_m + 1 - (c & 1)) >> 1) * ((top_n + 1 - (c >> 1)) >> 1);
-            hipLaunchKernelGGL(gs_top_kernel, blocks(cnt), dim3(kThreads), 0, s, top, z, b, c);
+            hipLaunchKernelGGL(gs_top_kernel<T>, blocks(cnt), dim3(kThreads), 0, s, top, z, b, c);
         }
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
 
-int BdSolver::smooth_level(int l, int sweeps, hipStream_t s) {
-    const BdLevel &L = lv[l];
-    const size_t lds = 2 * (size_t)L.m * L.n * sizeof(double);
-    if (l == nl - 1 && lds <= 64 * 1024) {
-        hipLaunchKernelGGL(gs_bottom_kernel, dim3(1), dim3(1024), lds, s, L, sweeps);
+template <typename T> int BdSolver::smooth_level(const BdLevelT<T> &L, bool bottom, int sweeps, hipStream_t s) {
+    const size_t lds = 2 * (size_t)L.m * L.n * sizeof(T);
+    if (bottom && lds <= 64 * 1024) {
+        hipLaunchKernelGGL(gs_bottom_kernel<T>, dim3(1), dim3(1024), lds, s, L, sweeps);
         STZ_HIP(hipGetLastError());
         return STZ_OK;
     }
     for (int k = 0; k < sweeps; k++)
         for (int c = 0; c < 4; c++) {
             const long cnt = (long)((L.m + 1 - (c & 1)) >> 1) * ((L.n + 1 - (c >> 1)) >> 1);
-            hipLaunchKernelGGL(gs_level_kernel, blocks(cnt), dim3(kThreads), 0, s, L, c);
+            hipLaunchKernelGGL(gs_level_kernel<T>, blocks(cnt), dim3(kThreads), 0, s, L, c);
         }
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
 
-// one V-cycle with zero start on the top level: z ~= A^-1 rhs
-int BdSolver::vcycle(const double *rhs, double *z, hipStream_t s) {
+// one V-cycle with zero start on the top level, in precision T: z ~= A^-1 rhs
+template <typename T> int BdSolver::vcycle_t(const T *rhs, T *z, BdLevelT<T> *Lv, T *r0, hipStream_t s) {
     const long n2 = 2L * top_m * top_n;
-    hipLaunchKernelGGL(fill_kernel, blocks(n2), dim3(kThreads), 0, s, z, n2, 0.0);
+    hipLaunchKernelGGL(fill_kernel<T>, blocks(n2), dim3(kThreads), 0, s, z, n2, (T)0);
     int e = smooth_top(z, rhs, nu1, s);
     if (e) return e;
     if (nl > 0) {
-        double *r0 = vec[kTmp];
-        hipLaunchKernelGGL(resid_top_kernel, blocks((long)top_m * top_n), dim3(kThreads), 0, s, top, z, rhs, r0,
+        hipLaunchKernelGGL(resid_top_kernel<T>, blocks((long)top_m * top_n), dim3(kThreads), 0, s, top, z, rhs, r0,
                            (double *)nullptr);
-        const double *rf = r0;
+        const T *rf = r0;
         int fm = top_m, fn = top_n;
         for (int l = 0; l < nl; l++) {
-            BdLevel &L = lv[l];
-            hipLaunchKernelGGL(restrict_kernel, blocks((long)L.m * L.n), dim3(kThreads), 0, s, rf, fm, fn, L.b, L.m,
+            BdLevelT<T> &L = Lv[l];
+            hipLaunchKernelGGL(restrict_kernel<T>, blocks((long)L.m * L.n), dim3(kThreads), 0, s, rf, fm, fn, L.b, L.m,
                                L.n);
             const long c2 = 2L * L.m * L.n;
-            hipLaunchKernelGGL(fill_kernel, blocks(c2), dim3(kThreads), 0, s, L.z, c2, 0.0);
+            hipLaunchKernelGGL(fill_kernel<T>, blocks(c2), dim3(kThreads), 0, s, L.z, c2, (T)0);
             if (l == nl - 1) break;
-            if ((e = smooth_level(l, nu1, s))) return e;
-            hipLaunchKernelGGL(resid_level_kernel, blocks((long)L.m * L.n), dim3(kThreads), 0, s, L);
+            if ((e = smooth_level(L, false, nu1, s))) return e;
+            hipLaunchKernelGGL(resid_level_kernel<T>, blocks((long)L.m * L.n), dim3(kThreads), 0, s, L);
             rf = L.r;
             fm = L.m;
             fn = L.n;
         }
-        if ((e = smooth_level(nl - 1, 2 * bottom_half, s))) return e;
+        if ((e = smooth_level(Lv[nl - 1], true, 2 * bottom_half, s))) return e;
         for (int l = nl - 1; l >= 0; l--) {
-            const BdLevel &C = lv[l];
-            double *zf = l == 0 ? z : lv[l - 1].z;
-            const int fm2 = l == 0 ? top_m : lv[l - 1].m, fn2 = l == 0 ? top_n : lv[l - 1].n;
-            hipLaunchKernelGGL(prolong_kernel, blocks((long)fm2 * fn2), dim3(kThreads), 0, s, C.z, C.m, C.n, zf,
+            const BdLevelT<T> &C = Lv[l];
+            T *zf = l == 0 ? z : Lv[l - 1].z;
+            const int fm2 = l == 0 ? top_m : Lv[l - 1].m, fn2 = l == 0 ? top_n : Lv[l - 1].n;
+            hipLaunchKernelGGL(prolong_kernel<T>, blocks((long)fm2 * fn2), dim3(kThreads), 0, s, C.z, C.m, C.n, zf,
                                fm2, fn2);
-            if (l > 0 && (e = smooth_level(l - 1, nu2, s))) return e;
+            if (l > 0 && (e = smooth_level(Lv[l - 1], false, nu2, s))) return e;
         }
         STZ_HIP(hipGetLastError());
     }
     return smooth_top(z, rhs, nu2, s);
+}
+
+// the preconditioner: in FP64, or (mixed) rhs rounded to FP32, the V-cycle
+// in FP32, the result widened back
+int BdSolver::vcycle(const double *rhs, double *z, hipStream_t s) {
+    if (!mixed) return vcycle_t<double>(rhs, z, lv, vec[kTmp], s);
+    const long n2 = 2L * top_m * top_n;
+    hipLaunchKernelGGL((convert_kernel<double, float>), blocks(n2), dim3(kThreads), 0, s, rhs, top32[0], n2);
+    const int e = vcycle_t<float>(top32[0], top32[1], lv32, top32[2], s);
+    if (e) return e;
+    hipLaunchKernelGGL((convert_kernel<float, double>), blocks(n2), dim3(kThreads), 0, s, top32[1], z, n2);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
 }
 
 int BdSolver::dot(const double *a, const double *b, int slot, int op, int host_slot, hipStream_t s) {
@@ -640,7 +721,7 @@ int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_so
     // restart: r = b - A x, r_hat = r, rho = (r,r)
     auto restart = [&](bool recompute, double &rr) -> int {
         if (recompute)
-            hipLaunchKernelGGL(resid_top_kernel, blocks(mn), b1, 0, s, top, x, b, r, (double *)nullptr);
+            hipLaunchKernelGGL(resid_top_kernel<double>, blocks(mn), b1, 0, s, top, x, b, r, (double *)nullptr);
         STZ_HIP(hipMemcpyAsync(rh, r, n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
         int e2 = dot(r, r, 0, 0, 0, s);
         if (e2) return e2;
@@ -715,7 +796,7 @@ int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_so
 int BdSolver::l2_error(const double *x, const double *b, double *out, hipStream_t s) {
     const long mn = (long)top_m * top_n;
     double *r = vec[kTmp];
-    hipLaunchKernelGGL(resid_top_kernel, blocks(mn), dim3(kThreads), 0, s, top, x, b, r, (double *)nullptr);
+    hipLaunchKernelGGL(resid_top_kernel<double>, blocks(mn), dim3(kThreads), 0, s, top, x, b, r, (double *)nullptr);
     int e = dot(r, r, 7, 0, 0, s);
     if (e) return e;
     STZ_HIP(hipStreamSynchronize(s));

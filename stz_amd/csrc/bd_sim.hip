@@ -399,6 +399,16 @@ int stz_bd_set_solver(stz_bd *b, int nu1, int nu2, int bottom_sweeps, int max_it
     return STZ_OK;
 }
 
+int stz_bd_set_precision(stz_bd *b, int precision) {
+    CHECK_BD(b);
+    if (precision != STZ_PREC_FP64 && precision != STZ_PREC_MIXED) {
+        set_error("unknown precision %d", precision);
+        return STZ_ERR_ARG;
+    }
+    STZ_HIP(hipStreamSynchronize(b->stream));
+    return b->mg.set_mixed(precision == STZ_PREC_MIXED);
+}
+
 int stz_bd_step_stats(stz_bd *b, int32_t *out4) {
     CHECK_BD(b);
     if (!out4) { set_error("null argument"); return STZ_ERR_ARG; }

@@ -130,6 +130,28 @@ def test_projection(plate):
         assert rel_err(g.get_cell(k), plate["proj_" + k]) < 1e-7, k
 
 
+def test_projection_mixed_preconditioner(plate):
+    """the mixed form (FP32 V-cycle preconditioner inside FP64 BiCGStab)
+    reaches the same threshold and the same solution as the FP64 form, and
+    switching back gives the FP64 form's result bit for bit"""
+    print("case bd projection mixed", flush=True)
+    out = {}
+    for form in ("fp64", "mixed", "fp64"):
+        g = injected(plate)
+        g.set_precision(form)
+        g.post_process(True)
+        g.projection_step(float(plate["dt"]))
+        info = g.last_solve()
+        print(form, info, flush=True)
+        assert info["status"] == 0 and info["residual"] <= 1e-20
+        if form in out:
+            assert np.array_equal(g.get_vel(), out[form])
+        out[form] = g.get_vel()
+        g.close()
+    assert rel_err(out["mixed"], plate["proj_vel"]) < 1e-7
+    assert rel_err(out["mixed"], out["fp64"]) < 1e-7
+
+
 def test_advection(plate):
     g = injected(plate)
     g.post_process(True)
