@@ -135,7 +135,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (N=%d requested); the step "
+        print("bench.py: this benchmark runs on one GPU only (N=%d requested); the step "
               "would shard into y-slabs across GPUs, which is not built" % args.gpus)
         sys.exit(2)
 

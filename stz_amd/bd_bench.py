@@ -3,7 +3,7 @@
     python -m stz_amd.bd_bench [--size 4096] [--steps 10] [--warmup 2]
                                [--precision mixed|fp64]
 
-Workload (config C3 of this build's workload table): a plate |y| < 0.9 in
+Workload (plate_hole4096 in stz_amd.problems): a plate |y| < 0.9 in
 the box [-1,1]^2 with a central hole of radius 0.4 (0.2 L), held by grips on
 the left and right edges that move apart at nominal strain rate 1e-4
 (u = -+1e-4), material of bd_test.cc, chi = 580 K + 30 K seeded Gaussian
@@ -29,7 +29,7 @@ def setup(N, seed=4):
     sim = BdSim(p)
     sim.set_level_set(problems.plate_hole_phi(N))
     st, _ = sim.status()
-    w = problems.BD_WORKLOADS["C3"]
+    w = problems.BD_WORKLOADS["plate_hole4096"]
     chi = random_chi(p, w["chi_mean"], w["chi_std"], w["corr"], seed)
     sim.set_cell("chi", np.where(st < 4, chi, 0.0))
     X, Y = sim.node_coords()

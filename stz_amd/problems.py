@@ -59,9 +59,9 @@ def simple_shear(m, n=None, u_bdry=1e-4, t_scale=1.0, viscosity=0.02,
 # Simple-shear workloads: (cells per side, steps, chi mean K, chi std K,
 # correlation length in cells)
 WORKLOADS = {
-    "C1": dict(N=128, steps=2000, chi_mean=600.0, chi_std=20.0, corr=2.0),
-    "C2": dict(N=1024, steps=5000, chi_mean=600.0, chi_std=25.0, corr=3.0),
-    "C5": dict(N=256, steps=400, chi_mean=620.0, chi_std=25.0, corr=3.0),
+    "shear128": dict(N=128, steps=2000, chi_mean=600.0, chi_std=20.0, corr=2.0),
+    "shear1024": dict(N=1024, steps=5000, chi_mean=600.0, chi_std=25.0, corr=3.0),
+    "calibration256": dict(N=256, steps=400, chi_mean=620.0, chi_std=25.0, corr=3.0),
 }
 T_END = 1000.0
 
@@ -108,8 +108,8 @@ def bd_params(m, n=None, box=(-1.0, 1.0, -1.0, 1.0), viscosity=0.02,
     The reference's free-boundary runs all use these moduli (K/mu = 3).  With
     the stiffer simple-shear set (K/mu = 7.2) the free-surface rows of
     qs_multi_bd make Gauss-Seidel V-cycles diverge on plate geometries, the
-    reference's as well as this build's, so the free-boundary workloads keep
-    the reference's material."""
+    the reference's as well as the ones here, so the free-boundary workloads
+    keep the reference's material."""
     n = m if n is None else n
     ax, bx, ay, by = box
     return {
@@ -136,10 +136,10 @@ def plate_hole_phi(m, n=None, box=(-1.0, 1.0, -1.0, 1.0), half_height=0.9,
     return np.maximum(np.abs(Y) - half_height, R - np.sqrt(X * X + Y * Y))
 
 
-# C3: plate with a hole pulled apart by grips on the left and right edges at
+# plate with a hole pulled apart by grips on the left and right edges at
 # nominal strain rate 1e-4 (grip speed 1e-4 * L/2), chi = 580 K + 30 K
 # Gaussian field with correlation length 4 cells
 BD_WORKLOADS = {
-    "C3": dict(N=4096, steps=2000, chi_mean=580.0, chi_std=30.0, corr=4.0,
+    "plate_hole4096": dict(N=4096, steps=2000, chi_mean=580.0, chi_std=30.0, corr=4.0,
                strain_rate=1e-4),
 }
