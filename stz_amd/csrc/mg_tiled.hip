@@ -150,7 +150,8 @@ __device__ __forceinline__ static void smooth_tile_body(const LevelArgs<T> &L, T
 #pragma unroll
     for (int k = 0; k < 40; k++) si[k] = L.Si[k];
     // wall rows' stencils, only if this tile reaches them
-    if (j0 < nb || j0 + RY > n - nb) load_wall_stencils(ss, L, tid, NT);
+    const bool walls = j0 < nb || j0 + RY > n - nb;
+    if (walls) load_wall_stencils(ss, L, tid, NT);
     // z into LDS (zero outside the level) and into the owner's registers,
     // b into registers
     T bx[BPT][4], by[BPT][4];
@@ -224,7 +225,7 @@ __device__ __forceinline__ static void smooth_tile_body(const LevelArgs<T> &L, T
                 else tile_neighbours_own<T, BX, NB, 3>(zs, blk, zo[q], dl, dc, dr, cl, cr, ul, uc, ur);
                 T ax, ay;
                 vec2s<T> out;
-                if (gj < nb || gj >= n - nb) {
+                if (walls && (gj < nb || gj >= n - nb)) {
                     const T *s = wall_stencil(ss, n, nb, gj);
                     offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
                     out = gs_value<T>(s, bx[q][c], by[q][c], ax, ay);
@@ -255,13 +256,13 @@ __device__ __forceinline__ static void smooth_tile_body(const LevelArgs<T> &L, T
                 const int gj = j0 + lj;
                 if (li < H - 1 || li > H + TX || lj < H - 1 || lj > H + TY || gj < 0 || gj >= n) continue;
                 vec2s<T> dl, dc, dr, cl, cr, ul, uc, ur;
-                if (c == 0) tile_neighbours<T, BX, NB, 0>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
-                else if (c == 1) tile_neighbours<T, BX, NB, 1>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
-                else if (c == 2) tile_neighbours<T, BX, NB, 2>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
-                else tile_neighbours<T, BX, NB, 3>(zs, blk, dl, dc, dr, cl, cr, ul, uc, ur);
-                const vec2s<T> z0 = zs[c * NB + blk];
+                if (c == 0) tile_neighbours_own<T, BX, NB, 0>(zs, blk, zo[q], dl, dc, dr, cl, cr, ul, uc, ur);
+                else if (c == 1) tile_neighbours_own<T, BX, NB, 1>(zs, blk, zo[q], dl, dc, dr, cl, cr, ul, uc, ur);
+                else if (c == 2) tile_neighbours_own<T, BX, NB, 2>(zs, blk, zo[q], dl, dc, dr, cl, cr, ul, uc, ur);
+                else tile_neighbours_own<T, BX, NB, 3>(zs, blk, zo[q], dl, dc, dr, cl, cr, ul, uc, ur);
+                const vec2s<T> z0 = zo[q][c];
                 T ax, ay;
-                if (gj < nb || gj >= n - nb) {
+                if (walls && (gj < nb || gj >= n - nb)) {
                     const T *s = wall_stencil(ss, n, nb, gj);
                     offdiag9<T>(s, dl, dc, dr, cl, cr, ul, uc, ur, ax, ay);
                     resid_value<T>(s, bx[q][c], by[q][c], ax, ay, z0, rxs[q][c], rys[q][c]);
