@@ -214,3 +214,46 @@ def test_every_solve_meets_its_threshold(form):
     print(form, "steps whose first test came before any cycle:", zero_first, flush=True)
     assert zero_first > 0
     g.close()
+
+
+def test_shear128_run_against_the_compiled_reference():
+    """The 128^2 simple-shear workload (initialize_random chi 600 K + 20 K,
+    correlation 2 cells, 2000 steps of dt 0.5 to t = 1000) at the
+    reference's own threshold, both forms on the GPU against the compiled
+    reference on the CPU, every 100 steps.  The solves stop at different
+    points inside the threshold (the smoothers differ), so the bars follow
+    the physics: 1e-8 during elastic loading (t <= 400), 1e-4 while plastic
+    flow spreads (t <= 850), and 1e-1 once the shear band has formed, where
+    the instability amplifies any difference (two FP64 runs with different
+    smoothers included)."""
+    from oracle.pyoracle import Reference, reference_available
+    if not reference_available():
+        pytest.skip("compiled reference not present")
+    from stz_amd import problems
+    print("case shear128 vs reference", flush=True)
+    p = problems.simple_shear(128)
+    w = problems.WORKLOADS["shear128"]
+    dt = problems.T_END / w["steps"]
+    sims = {}
+    for form in ("fp64", "mixed"):
+        g = ShearSim(p, form)
+        g.initialize_random(w["corr"], w["chi_mean"], w["chi_std"], seed=1)
+        g.set_vel(np.zeros((p["n"] + 1, p["m"], 2)))
+        g.time = 0.0
+        sims[form] = g
+    r = Reference(p)
+    r.initialize_random(w["corr"], w["chi_mean"], w["chi_std"], 1)
+    r.time = 0.0
+    for k in range(w["steps"] // 100):
+        r.step(dt, 100)
+        F = r.get_fields()
+        bar = 1e-8 if r.time <= 400 else (1e-4 if r.time <= 850 else 1e-1)
+        for form, g in sims.items():
+            g.step_forward_quasistatic(dt, 100)
+            ge = group_errors(g.get_fields(), F)
+            print("t %.0f %s %s" % (r.time, form, {a: "%.1e" % b for a, b in ge.items()}), flush=True)
+            for grp in ("stress", "chi", "velocity"):
+                assert ge[grp] < bar, (r.time, form, grp, ge)
+    for g in sims.values():
+        g.close()
+    r.close()
