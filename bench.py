@@ -96,7 +96,7 @@ def cpu_baseline(params, samples, dt, tol, phase_steps, budget_s=10.0, max_steps
     from oracle import pyoracle
     kind = "reference" if pyoracle.reference_available() else "port"
     cls = pyoracle.Reference if kind == "reference" else pyoracle.Oracle
-    per_step, desc = {}, []
+    per_step, desc, finals = {}, [], {}
     for phase, (fields, vel, t) in samples.items():
         sim = cls(params)
         sim.set_fields(fields)
@@ -111,6 +111,7 @@ def cpu_baseline(params, samples, dt, tol, phase_steps, budget_s=10.0, max_steps
         el = time.perf_counter() - t0
         per_step[phase] = el / steps
         desc.append("%s: %d steps from t=%.0f, %.3f s/step" % (phase, steps, t, el / steps))
+        finals[phase] = (1 + steps, sim.get_fields())
         sim.close()
     total_t = sum(phase_steps[p] * per_step[p] for p in per_step)
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
@@ -120,7 +121,28 @@ def cpu_baseline(params, samples, dt, tol, phase_steps, budget_s=10.0, max_steps
     return {"value": cells * sum(phase_steps.values()) / total_t, "unit": "cell-steps/s", "cores": cores,
             "kind": kind,
             "sample": "%dx%d simple shear at the same threshold; %s; weighted by %s steps per phase" %
-                      (params["m"], params["n"], "; ".join(desc), phase_steps)}
+                      (params["m"], params["n"], "; ".join(desc), phase_steps)}, finals
+
+
+def gpu_vs_cpu(params, samples, finals, dt, nu, tol):
+    """the GPU FP64 form stepped from each CPU sample's start for the same
+    number of steps: relative difference per field group from the CPU's
+    final state (parity at the full size)"""
+    from stz_amd import ShearSim
+    out = {}
+    for phase, (fields, vel, t) in samples.items():
+        nsteps, ref = finals[phase]
+        g = ShearSim(params, "fp64")
+        g.set_fields(fields)
+        g.set_vel(vel)
+        g.time = t
+        g.set_mg_config(nu1=nu, nu2=nu, tol_mult=tol)
+        g.step_forward_quasistatic(dt, nsteps)
+        f = g.get_fields()
+        g.close()
+        out[phase] = {"steps": nsteps, "stress": group_err(f, ref, [2, 3, 4, 5]), "chi": group_err(f, ref, [6]),
+                      "velocity": group_err(f, ref, [0, 1]), "refmap": group_err(f, ref, [7, 8])}
+    return out
 
 
 def main():
@@ -209,13 +231,15 @@ def main():
         if tj.get("kernel") == "smooth_tile_kernel<float,64,32,2,0>" and tj.get("level", "").startswith("%dx" % N):
             traffic = tj.get("traffic_bytes_per_launch")
 
-    cpu = None
+    cpu, vs_cpu = None, None
     if not args.no_cpu and elastic_at in res["fp64"]["kept"] and plastic_at in res["fp64"]["kept"]:
         # steps before the first yield (t ~ 425 at this loading rate) are elastic
         n_el = min(total, int(425.0 / dt))
         samples = {"elastic": res["fp64"]["kept"][elastic_at], "plastic": res["fp64"]["kept"][plastic_at]}
-        cpu = cpu_baseline(params, samples, dt, args.tol, {"elastic": n_el, "plastic": total - n_el})
+        cpu, finals = cpu_baseline(params, samples, dt, args.tol, {"elastic": n_el, "plastic": total - n_el})
         log("bench: cpu baseline %s" % cpu)
+        vs_cpu = gpu_vs_cpu(params, samples, finals, dt, args.nu, args.tol)
+        log("bench: GPU FP64 vs the CPU (%s) from the same states: %s" % (cpu["kind"], vs_cpu))
 
     mixed, fp64 = res["mixed"], res["fp64"]
     line = {
@@ -240,6 +264,7 @@ def main():
             "mixed_over_fp64": mixed["value"] / fp64["value"],
             "mixed_vs_fp64_rel_err": errs,
             "cycles_per_solve": {"mixed": mixed["cycles"], "fp64": fp64["cycles"]},
+            "fp64_vs_cpu_same_start": vs_cpu,
             "smoother_fp64_GBps": achieved64,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
