@@ -22,6 +22,8 @@
 //                    coarsest level exactly with a precomputed dense inverse
 //   residual norm    wavefront-shuffle reduction to per-block partials, then
 //                    one block sums the partials in a fixed order
+#include <cstring>
+
 #include "batch.h"
 #include "mg_node.h"
 
@@ -630,11 +632,13 @@ __device__ __forceinline__ static void coarse_solve_body(const CoarseArgs<T> &A)
             sm[ro + 2 * t + 1] = sm[B.bo + k + 1];
         }
         __syncthreads();
+        // in FP64 in both forms: the inverse is held in doubles
+        const double *bi = reinterpret_cast<const double *>(sm + A.binv_off);
         for (int r = tid; r < N; r += NT) {
-            T acc = 0;
-            for (int c = 0; c < N; c++) acc = fmav(sm[A.binv_off + c * N + r], sm[ro + c], acc);
+            double acc = 0;
+            for (int c = 0; c < N; c++) acc = fma(bi[c * N + r], (double)sm[ro + c], acc);
             const int node = r >> 1, j = node / B.m, i = node - j * B.m;
-            sm[B.zo + cidx(B, i, j) + (r & 1)] = acc;
+            sm[B.zo + cidx(B, i, j) + (r & 1)] = (T)acc;
         }
         __syncthreads();
     }
@@ -687,9 +691,12 @@ template <typename T> int coarse_layout(CoarseArgs<T> &A) {
         off += stencil_count(A.lv[l]);
         off = (off + 3) & ~3;
     }
+    // the coarsest level's dense inverse in doubles (two elements per value
+    // when T is float), 8-byte aligned
     const int N = 2 * A.lv[A.L - 1].m * A.lv[A.L - 1].n;
+    constexpr int per = (int)(sizeof(double) / sizeof(T));
     A.binv_off = off;
-    off += N * N;
+    off += per * N * N;
     off = (off + 3) & ~3;
     A.pack_elems = off;
     for (int l = A.lc; l < A.L; l++) {
@@ -718,8 +725,12 @@ void coarse_pack(const CoarseArgs<T> &A, const std::vector<double> *S, const std
         for (int k = 0; k < cnt; k++) out[(size_t)A.soff[l] + k] = (T)S[l][(size_t)k];
     }
     const int N = 2 * A.lv[A.L - 1].m * A.lv[A.L - 1].n;
+    double *bi = reinterpret_cast<double *>(out.data() + A.binv_off);
     for (int r = 0; r < N; r++)
-        for (int c = 0; c < N; c++) out[(size_t)A.binv_off + (size_t)c * N + r] = (T)binv[(size_t)r * N + c];
+        for (int c = 0; c < N; c++) {
+            const double v = binv[(size_t)r * N + c];
+            std::memcpy(bi + (size_t)c * N + r, &v, sizeof v);
+        }
 }
 template void coarse_pack<double>(const CoarseArgs<double> &, const std::vector<double> *, const std::vector<double> &,
                                   std::vector<double> &);
