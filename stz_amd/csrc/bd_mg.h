@@ -45,7 +45,7 @@ struct BdSolver {
     // the levels; BiCGStab, its residuals and the convergence test stay FP64
     bool mixed = false;
     BdLevelT<float> lv32[kMaxLevels];
-    float *top32[3] = {};      // top-level source, correction, residual
+    float *top32[3] = {};      // top-level preconditioned vectors p^, s^ and the residual
 
     static int dims_for(int tm, int tn, std::vector<std::pair<int, int>> &out);
     int init(int tm, int tn);
@@ -59,8 +59,8 @@ struct BdSolver {
   private:
     int alloc_mixed();
     void free_mixed();
-    template <typename T> int vcycle_t(const T *rhs, T *z, BdLevelT<T> *L, T *r0, hipStream_t s);
-    template <typename T> int smooth_top(T *z, const T *b, int sweeps, hipStream_t s);
+    template <typename T, typename TB> int vcycle_t(const TB *rhs, T *z, BdLevelT<T> *L, T *r0, hipStream_t s);
+    template <typename T, typename TB> int smooth_top(T *z, const TB *b, int sweeps, hipStream_t s);
     template <typename T> int smooth_level(const BdLevelT<T> &L, bool bottom, int sweeps, hipStream_t s);
     int dot(const double *a, const double *b, int slot, int op, int host_slot, hipStream_t s);
 };

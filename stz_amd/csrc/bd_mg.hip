@@ -70,24 +70,27 @@ __device__ __forceinline__ void top_entry(const BdTop &t, int ij, int k, double 
 // off-diagonal part of row ij applied to z (qs_multi_bd::mul_a,
 // qs_multi_bd.hh:41-56) and the diagonal block; in T, with the operator's
 // FP64 constants rounded to T
-template <typename T> __device__ __forceinline__ V2T<T> top_offdiag(const BdTop &t, const T *z, int ij, int d) {
+template <typename T, typename TZ = T>
+__device__ __forceinline__ V2T<T> top_offdiag(const BdTop &t, const TZ *z, int ij, int d) {
     const int m = t.m;
     if (d == -1) {
-        const T *zp = z + 2 * (size_t)ij;
-        const T *zl = zp - 2, *zr = zp + 2, *zd = zp - 2 * m, *zu = zp + 2 * m;
-        const T *zdl = zd - 2, *zdr = zd + 2, *zul = zu - 2, *zur = zu + 2;
+        const TZ *zp = z + 2 * (size_t)ij;
+        const TZ *zl = zp - 2, *zr = zp + 2, *zd = zp - 2 * m, *zu = zp + 2 * m;
+        const TZ *zdl = zd - 2, *zdr = zd + 2, *zul = zu - 2, *zur = zu + 2;
         const T Kmx = (T)t.Kmx, Kmy = (T)t.Kmy, mx = (T)t.mx, my = (T)t.my, qK = (T)t.qK;
-        return V2T<T>{Kmx * (zl[0] + zr[0]) + my * (zd[0] + zu[0]) + qK * (zdl[1] + zur[1] - zdr[1] - zul[1]),
-                      mx * (zl[1] + zr[1]) + Kmy * (zd[1] + zu[1]) + qK * (zdl[0] + zur[0] - zdr[0] - zul[0])};
+        return V2T<T>{Kmx * ((T)zl[0] + (T)zr[0]) + my * ((T)zd[0] + (T)zu[0]) +
+                          qK * ((T)zdl[1] + (T)zur[1] - (T)zdr[1] - (T)zul[1]),
+                      mx * ((T)zl[1] + (T)zr[1]) + Kmy * ((T)zd[1] + (T)zu[1]) +
+                          qK * ((T)zdl[0] + (T)zur[0] - (T)zdr[0] - (T)zul[0])};
     }
     if (d < 0) return V2T<T>{0, 0};
     const double *a = t.fmats + 32 * (size_t)d;
     const int off[8] = {-m - 1, -m, -m + 1, -1, 1, m - 1, m, m + 1};
     T rx = 0, ry = 0;
     for (int q = 0; q < 8; q++) {
-        const T *zz = z + 2 * (size_t)(ij + off[q]);
-        rx += (T)a[4 * q] * zz[0] + (T)a[4 * q + 1] * zz[1];
-        ry += (T)a[4 * q + 2] * zz[0] + (T)a[4 * q + 3] * zz[1];
+        const TZ *zz = z + 2 * (size_t)(ij + off[q]);
+        rx += (T)a[4 * q] * (T)zz[0] + (T)a[4 * q + 1] * (T)zz[1];
+        ry += (T)a[4 * q + 2] * (T)zz[0] + (T)a[4 * q + 3] * (T)zz[1];
     }
     return V2T<T>{rx, ry};
 }
@@ -116,18 +119,18 @@ template <typename T> __device__ __forceinline__ V2T<T> lvl_offdiag(const BdLeve
 // ---------------------------------------------------------------------------
 // Gauss-Seidel sweeps, one colour (i&1, j&1) per launch
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(kThreads) gs_top_kernel(BdTop t, T *__restrict__ z, const T *__restrict__ b,
+template <typename T, typename TB>
+__global__ void __launch_bounds__(kThreads) gs_top_kernel(BdTop t, T *__restrict__ z, const TB *__restrict__ b,
                                                           int colour) {
     const int hm = (t.m + 1 - (colour & 1)) >> 1, hn = (t.n + 1 - (colour >> 1)) >> 1;
     const int q = blockIdx.x * kThreads + threadIdx.x;
     if (q >= hm * hn) return;
     const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + t.m * j;
     const int d = t.rowt[ij];
-    const V2T<T> o = top_offdiag(t, z, ij, d);
+    const V2T<T> o = top_offdiag<T>(t, z, ij, d);
     const double sc = top_diag_scale(t, d);
-    z[2 * (size_t)ij] = (b[2 * (size_t)ij] - o.x) / (T)(sc * t.cxx);
-    z[2 * (size_t)ij + 1] = (b[2 * (size_t)ij + 1] - o.y) / (T)(sc * t.cyy);
+    z[2 * (size_t)ij] = ((T)b[2 * (size_t)ij] - o.x) / (T)(sc * t.cxx);
+    z[2 * (size_t)ij + 1] = ((T)b[2 * (size_t)ij + 1] - o.y) / (T)(sc * t.cyy);
 }
 
 template <typename T> __global__ void __launch_bounds__(kThreads) gs_level_kernel(BdLevelT<T> L, int colour) {
@@ -166,18 +169,18 @@ template <typename T> __global__ void __launch_bounds__(1024) gs_bottom_kernel(B
 }
 
 // r = b - A z on the top level; optional per-block |r|^2 partial sums
-template <typename T>
-__global__ void __launch_bounds__(kThreads) resid_top_kernel(BdTop t, const T *__restrict__ z, const T *__restrict__ b,
+template <typename T, typename TB>
+__global__ void __launch_bounds__(kThreads) resid_top_kernel(BdTop t, const T *__restrict__ z, const TB *__restrict__ b,
                                                              T *__restrict__ r, double *__restrict__ partial) {
     __shared__ double red[kThreads / 64];
     const int ij = blockIdx.x * kThreads + threadIdx.x;
     double acc = 0;
     if (ij < t.m * t.n) {
         const int d = t.rowt[ij];
-        const V2T<T> o = top_offdiag(t, z, ij, d);
+        const V2T<T> o = top_offdiag<T>(t, z, ij, d);
         const double sc = top_diag_scale(t, d);
-        const T rx = b[2 * (size_t)ij] - o.x - (T)(sc * t.cxx) * z[2 * (size_t)ij];
-        const T ry = b[2 * (size_t)ij + 1] - o.y - (T)(sc * t.cyy) * z[2 * (size_t)ij + 1];
+        const T rx = (T)b[2 * (size_t)ij] - o.x - (T)(sc * t.cxx) * z[2 * (size_t)ij];
+        const T ry = (T)b[2 * (size_t)ij + 1] - o.y - (T)(sc * t.cyy) * z[2 * (size_t)ij + 1];
         if (r) {
             r[2 * (size_t)ij] = rx;
             r[2 * (size_t)ij + 1] = ry;
@@ -195,16 +198,18 @@ __global__ void __launch_bounds__(kThreads) resid_top_kernel(BdTop t, const T *_
     }
 }
 
-// y = A x on the top level
-__global__ void __launch_bounds__(kThreads) apply_top_kernel(BdTop t, const double *__restrict__ x,
+// y = A x on the top level, in FP64 (x FP64, or the mixed form's FP32
+// preconditioned vector)
+template <typename TX>
+__global__ void __launch_bounds__(kThreads) apply_top_kernel(BdTop t, const TX *__restrict__ x,
                                                              double *__restrict__ y) {
     const int ij = blockIdx.x * kThreads + threadIdx.x;
     if (ij >= t.m * t.n) return;
     const int d = t.rowt[ij];
-    const V2 o = top_offdiag(t, x, ij, d);
+    const V2 o = top_offdiag<double, TX>(t, x, ij, d);
     const double sc = top_diag_scale(t, d);
-    y[2 * (size_t)ij] = o.x + sc * t.cxx * x[2 * (size_t)ij];
-    y[2 * (size_t)ij + 1] = o.y + sc * t.cyy * x[2 * (size_t)ij + 1];
+    y[2 * (size_t)ij] = o.x + sc * t.cxx * (double)x[2 * (size_t)ij];
+    y[2 * (size_t)ij + 1] = o.y + sc * t.cyy * (double)x[2 * (size_t)ij + 1];
 }
 
 template <typename T> __global__ void __launch_bounds__(kThreads) resid_level_kernel(BdLevelT<T> L) {
@@ -460,15 +465,16 @@ __global__ void __launch_bounds__(kThreads) bicg_s_kernel(double *s, const doubl
     if (q < n) s[q] = r[q] - sc[2] * v[q];
 }
 // x += alpha phat + omega shat; r = s - omega t; rho_prev = rho
-__global__ void __launch_bounds__(kThreads) bicg_x_kernel(double *x, double *r, const double *__restrict__ ph,
-                                                          const double *__restrict__ sh,
+template <typename TP>
+__global__ void __launch_bounds__(kThreads) bicg_x_kernel(double *x, double *r, const TP *__restrict__ ph,
+                                                          const TP *__restrict__ sh,
                                                           const double *__restrict__ s,
                                                           const double *__restrict__ t, double *sc, long n) {
     const long q = (long)blockIdx.x * kThreads + threadIdx.x;
     if (q == 0) sc[1] = sc[0];
     if (q >= n) return;
     const double a = sc[2], w = sc[3];
-    x[q] += a * ph[q] + w * sh[q];
+    x[q] += a * (double)ph[q] + w * (double)sh[q];
     r[q] = s[q] - w * t[q];
 }
 
@@ -618,11 +624,11 @@ int BdSolver::setup(const BdTop &t, hipStream_t s) {
     return STZ_OK;
 }
 
-template <typename T> int BdSolver::smooth_top(T *z, const T *b, int sweeps, hipStream_t s) {
+template <typename T, typename TB> int BdSolver::smooth_top(T *z, const TB *b, int sweeps, hipStream_t s) {
     for (int k = 0; k < sweeps; k++)
         for (int c = 0; c < 4; c++) {
             const long cnt = (long)((top_m + 1 - (c & 1)) >> 1) * ((top_n + 1 - (c >> 1)) >> 1);
-            hipLaunchKernelGGL(gs_top_kernel<T>, blocks(cnt), dim3(kThreads), 0, s, top, z, b, c);
+            hipLaunchKernelGGL((gs_top_kernel<T, TB>), blocks(cnt), dim3(kThreads), 0, s, top, z, b, c);
         }
     STZ_HIP(hipGetLastError());
     return STZ_OK;
@@ -644,15 +650,17 @@ template <typename T> int BdSolver::smooth_level(const BdLevelT<T> &L, bool bott
     return STZ_OK;
 }
 
-// one V-cycle with zero start on the top level, in precision T: z ~= A^-1 rhs
-template <typename T> int BdSolver::vcycle_t(const T *rhs, T *z, BdLevelT<T> *Lv, T *r0, hipStream_t s) {
+// one V-cycle with zero start on the top level, in precision T (the source
+// read in its own precision TB): z ~= A^-1 rhs
+template <typename T, typename TB>
+int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, hipStream_t s) {
     const long n2 = 2L * top_m * top_n;
     hipLaunchKernelGGL(fill_kernel<T>, blocks(n2), dim3(kThreads), 0, s, z, n2, (T)0);
     int e = smooth_top(z, rhs, nu1, s);
     if (e) return e;
     if (nl > 0) {
-        hipLaunchKernelGGL(resid_top_kernel<T>, blocks((long)top_m * top_n), dim3(kThreads), 0, s, top, z, rhs, r0,
-                           (double *)nullptr);
+        hipLaunchKernelGGL((resid_top_kernel<T, TB>), blocks((long)top_m * top_n), dim3(kThreads), 0, s, top, z, rhs,
+                           r0, (double *)nullptr);
         const T *rf = r0;
         int fm = top_m, fn = top_n;
         for (int l = 0; l < nl; l++) {
@@ -682,17 +690,10 @@ template <typename T> int BdSolver::vcycle_t(const T *rhs, T *z, BdLevelT<T> *Lv
     return smooth_top(z, rhs, nu2, s);
 }
 
-// the preconditioner: in FP64, or (mixed) rhs rounded to FP32, the V-cycle
-// in FP32, the result widened back
+// the FP64 preconditioner (the mixed form applies vcycle_t<float, double>
+// inside solve and keeps its result in FP32)
 int BdSolver::vcycle(const double *rhs, double *z, hipStream_t s) {
-    if (!mixed) return vcycle_t<double>(rhs, z, lv, vec[kTmp], s);
-    const long n2 = 2L * top_m * top_n;
-    hipLaunchKernelGGL((convert_kernel<double, float>), blocks(n2), dim3(kThreads), 0, s, rhs, top32[0], n2);
-    const int e = vcycle_t<float>(top32[0], top32[1], lv32, top32[2], s);
-    if (e) return e;
-    hipLaunchKernelGGL((convert_kernel<float, double>), blocks(n2), dim3(kThreads), 0, s, top32[1], z, n2);
-    STZ_HIP(hipGetLastError());
-    return STZ_OK;
+    return vcycle_t<double, double>(rhs, z, lv, vec[kTmp], s);
 }
 
 int BdSolver::dot(const double *a, const double *b, int slot, int op, int host_slot, hipStream_t s) {
@@ -721,7 +722,8 @@ int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_so
     // restart: r = b - A x, r_hat = r, rho = (r,r)
     auto restart = [&](bool recompute, double &rr) -> int {
         if (recompute)
-            hipLaunchKernelGGL(resid_top_kernel<double>, blocks(mn), b1, 0, s, top, x, b, r, (double *)nullptr);
+            hipLaunchKernelGGL((resid_top_kernel<double, double>), blocks(mn), b1, 0, s, top, x, b, r,
+                               (double *)nullptr);
         STZ_HIP(hipMemcpyAsync(rh, r, n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
         int e2 = dot(r, r, 0, 0, 0, s);
         if (e2) return e2;
@@ -746,15 +748,30 @@ int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_so
     for (int total = 1; total <= max_iter; total++) {
         hipLaunchKernelGGL(bicg_p_kernel, g, b1, 0, s, p, r, v, sc, n2, first ? 1 : 0);
         first = false;
-        if ((e = vcycle(p, ph, s))) return e;
-        hipLaunchKernelGGL(apply_top_kernel, blocks(mn), b1, 0, s, top, ph, v);
+        // p^ = M p: FP64, or (mixed) an FP32 V-cycle reading p in FP64 and
+        // leaving p^ in FP32 for the FP64 kernels that use it
+        if (mixed) {
+            if ((e = vcycle_t<float, double>(p, top32[0], lv32, top32[2], s))) return e;
+            hipLaunchKernelGGL(apply_top_kernel<float>, blocks(mn), b1, 0, s, top, top32[0], v);
+        } else {
+            if ((e = vcycle(p, ph, s))) return e;
+            hipLaunchKernelGGL(apply_top_kernel<double>, blocks(mn), b1, 0, s, top, ph, v);
+        }
         if ((e = dot(rh, v, 4, 2, -1, s))) return e;
         hipLaunchKernelGGL(bicg_s_kernel, g, b1, 0, s, sv, r, v, sc, n2);
-        if ((e = vcycle(sv, sh, s))) return e;
-        hipLaunchKernelGGL(apply_top_kernel, blocks(mn), b1, 0, s, top, sh, t);
+        if (mixed) {
+            if ((e = vcycle_t<float, double>(sv, top32[1], lv32, top32[2], s))) return e;
+            hipLaunchKernelGGL(apply_top_kernel<float>, blocks(mn), b1, 0, s, top, top32[1], t);
+        } else {
+            if ((e = vcycle(sv, sh, s))) return e;
+            hipLaunchKernelGGL(apply_top_kernel<double>, blocks(mn), b1, 0, s, top, sh, t);
+        }
         if ((e = dot(t, sv, 5, 0, -1, s))) return e;
         if ((e = dot(t, t, 6, 3, -1, s))) return e;
-        hipLaunchKernelGGL(bicg_x_kernel, g, b1, 0, s, x, r, ph, sh, sv, t, sc, n2);
+        if (mixed)
+            hipLaunchKernelGGL(bicg_x_kernel<float>, g, b1, 0, s, x, r, top32[0], top32[1], sv, t, sc, n2);
+        else
+            hipLaunchKernelGGL(bicg_x_kernel<double>, g, b1, 0, s, x, r, ph, sh, sv, t, sc, n2);
         STZ_HIP(hipGetLastError());
         info.cycles = info.total_cycles = 2 * total;
         if ((e = dot(r, r, 7, 0, 0, s))) return e;      // (r,r) -> host[0]
@@ -796,7 +813,8 @@ int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_so
 int BdSolver::l2_error(const double *x, const double *b, double *out, hipStream_t s) {
     const long mn = (long)top_m * top_n;
     double *r = vec[kTmp];
-    hipLaunchKernelGGL(resid_top_kernel<double>, blocks(mn), dim3(kThreads), 0, s, top, x, b, r, (double *)nullptr);
+    hipLaunchKernelGGL((resid_top_kernel<double, double>), blocks(mn), dim3(kThreads), 0, s, top, x, b, r,
+                       (double *)nullptr);
     int e = dot(r, r, 7, 0, 0, s);
     if (e) return e;
     STZ_HIP(hipStreamSynchronize(s));
