@@ -184,6 +184,33 @@ def test_three_steps_statistical(plate):
         assert rel_err(g.get_node(k), plate["steps_" + k]) < 5e-2, k
 
 
+def test_three_steps_mixed_against_fp64(plate):
+    """whole steps of the mixed form against the FP64 form from the same
+    state: the march and the extrapolation are the same code in both, so
+    the fields differ only by where the two solves stop inside the
+    threshold"""
+    print("case bd three steps mixed vs fp64", flush=True)
+    out = {}
+    for form in ("fp64", "mixed"):
+        g = injected(plate)
+        g.set_precision(form)
+        g.post_process(True)
+        g.step_forward_quasistatic(float(plate["dt"]), int(plate["steps"]))
+        st, _ = g.status()
+        out[form] = (st, {k: g.get_cell(k) for k in CELLS}, {k: g.get_node(k) for k in ("u", "v")})
+        g.close()
+    assert np.array_equal(out["mixed"][0], out["fp64"][0])
+    inside = out["fp64"][0] < 4
+    for k in CELLS:
+        e = rel_err(out["mixed"][1][k][inside], out["fp64"][1][k][inside])
+        print(k, "%.1e" % e, flush=True)
+        assert e < 1e-6, k
+    for k in ("u", "v"):
+        e = rel_err(out["mixed"][2][k], out["fp64"][2][k])
+        print(k, "%.1e" % e, flush=True)
+        assert e < 1e-6, k
+
+
 @pytest.mark.parametrize("name", ["bd_bar1", "bd_bar2", "bd_bar3"])
 def test_init_bar(name):
     f = load_fixture(name)
