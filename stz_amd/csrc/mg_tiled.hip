@@ -545,6 +545,10 @@ template <typename T> bool tiled_ok(const LevelArgs<T> &L) { return wall_ok(L) &
 template <typename T> static bool big_tiles(const LevelArgs<T> &L) {
     return L.m % 64 == 0 && (L.m / 64) * ((L.n + 31) / 32) >= 256;
 }
+// 16x16 tiles on the smallest tiled levels (fewer than 100 tiles of 32x16):
+// twice the workgroups of lighter work, 0.6 % (mixed) and 1.7 % (FP64) off
+// the 1024^2 step
+template <typename T> static bool small_tiles(const LevelArgs<T> &L) { return (L.m / 32) * ((L.n + 15) / 16) < 100; }
 
 
 template <typename T, int S, int MODE, bool FUSE>
@@ -557,6 +561,14 @@ static void launch_tile(const LevelArgs<T> &L, T *zout, const LevelArgs<T> &Cz, 
                                zero_z, corr);
         else
             hipLaunchKernelGGL((smooth_tile_kernel<T, 64, 32, S, MODE, FUSE, false>), g, b, 0, st, L, zout, Cz, Cr,
+                               zero_z, corr);
+    } else if (small_tiles(L)) {
+        const dim3 g(L.m / 16, (L.n + 15) / 16), b(tile_threads(16, 16, tile_halo(S, FUSE)));
+        if (L.sparse)
+            hipLaunchKernelGGL((smooth_tile_kernel<T, 16, 16, S, MODE, FUSE, true>), g, b, 0, st, L, zout, Cz, Cr,
+                               zero_z, corr);
+        else
+            hipLaunchKernelGGL((smooth_tile_kernel<T, 16, 16, S, MODE, FUSE, false>), g, b, 0, st, L, zout, Cz, Cr,
                                zero_z, corr);
     } else {
         const dim3 g(L.m / 32, (L.n + 15) / 16), b(tile_threads(32, 16, tile_halo(S, FUSE)));
