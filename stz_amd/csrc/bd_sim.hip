@@ -197,9 +197,11 @@ int stz_bd_create(const stz_bd_params *p, stz_bd **out) {
         set_error("no HIP device visible");
         return STZ_ERR_NO_DEVICE;
     }
+    int dev = 0;
+    STZ_HIP(hipGetDevice(&dev));
     stz_bd *b = new stz_bd();
+    b->device = dev;
     b->p = *p;
-    STZ_HIP(hipGetDevice(&b->device));
     BdConst &c = b->c;
     // bd_sim.cc:25-53
     c.m = p->m;

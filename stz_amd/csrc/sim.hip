@@ -541,10 +541,12 @@ int stz_sim_create(const stz_sim_params *p, int precision, stz_sim **out) {
         set_error("no HIP device visible");
         return STZ_ERR_NO_DEVICE;
     }
+    int dev = 0;
+    STZ_HIP(hipGetDevice(&dev));
     stz_sim *s = new stz_sim();
+    s->device = dev;
     s->p = *p;
     s->precision = precision;
-    STZ_HIP(hipGetDevice(&s->device));
     SimConst &c = s->c;
     // shear_sim.cc:32-36
     c.m = p->m;
