@@ -1,4 +1,4 @@
-"""ctypes access to the CPU checkers -- TEST INFRASTRUCTURE ONLY.
+"""ctypes access to the CPU oracles -- TEST INFRASTRUCTURE ONLY.
 
 Two libraries, both built by oracle/Makefile:
   * ``liboracle.so``      -- this repository's FP64 restatement of the
@@ -8,7 +8,7 @@ Two libraries, both built by oracle/Makefile:
     build time; it is optional everywhere).
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use
-this module, and only as the checker.
+this module, and only as the oracle.
 """
 import ctypes as C
 import os
@@ -100,7 +100,7 @@ def _bind(lib, prefix):
 
 
 class _Sim:
-    """Common driver over either checker library."""
+    """Common driver over either oracle library."""
 
     def __init__(self, params):
         self.params = params
@@ -108,7 +108,7 @@ class _Sim:
         self.m, self.n = self._p.m, self._p.n
         self._h = self._f["create"](C.byref(self._p))
         if not self._h:
-            raise RuntimeError("checker create failed")
+            raise RuntimeError("oracle create failed")
 
     def close(self):
         if self._h:

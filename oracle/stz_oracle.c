@@ -1,7 +1,7 @@
 /*
  * stz_oracle.c -- CPU FP64 restatement of the reference quasi-static STZ step.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker used by tests/, by
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle used by tests/, by
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; the product
  * (stz_amd/) never links or calls it.
  *

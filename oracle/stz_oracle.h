@@ -1,6 +1,6 @@
 /*
  * stz_oracle.h -- CPU FP64 restatement of the reference's quasi-static STZ
- * step, used ONLY as the checker by the tests, smoke() and bench.py's CPU
+ * step, used ONLY as the oracle by the tests, smoke() and bench.py's CPU
  * baseline.  Nothing in the product links or calls this.
  *
  * Parity status of each part (what pins it):

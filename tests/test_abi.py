@@ -54,7 +54,7 @@ def test_struct_layout_matches_header(tmp_path):
         assert int(got[cname]) == C.sizeof(py), cname
         for fname, _ in py._fields_:
             assert int(got["%s.%s" % (cname, fname)]) == getattr(py, fname).offset, (cname, fname)
-    # the checker's own copy of the parameter structs agrees too
+    # the oracle's own copy of the parameter structs agrees too
     assert C.sizeof(pyoracle.SimParams) == C.sizeof(_lib.SimParams)
     for fname, _ in _lib.SimParams._fields_:
         assert getattr(pyoracle.SimParams, fname).offset == getattr(_lib.SimParams, fname).offset

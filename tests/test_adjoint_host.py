@@ -1,5 +1,5 @@
 """CPU: the host side of the adjoint driver.  The differentiable Hencky
-strain used as a calibration target against the checker's
+strain used as a calibration target against the oracle's
 compute_strain (corrected form, itself pinned by the closed form of a
 uniform simple shear and, in its reference form, by the reference's
 fixture), and the chi-plane embedding against its adjoint (dot-product

@@ -1,6 +1,6 @@
 """The C++ drop-in headers (stz_amd/host/*.hh): a driver written like the
 reference's sb drivers (tests/cpp/facade_demo.cc) compiles against them,
-fails loudly without a device, and on the GPU produces the FP64 checker's
+fails loudly without a device, and on the GPU produces the FP64 oracle's
 state for the same init_fields -> host edit -> upload -> steps sequence."""
 import os
 import subprocess
@@ -49,7 +49,7 @@ def test_facade_fails_loudly_without_device(demo, tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("m,n,chi_case", [(32, 32, 0), (40, 24, 2)])
-def test_facade_driver_matches_checker(demo, tmp_path, m, n, chi_case):
+def test_facade_driver_matches_oracle(demo, tmp_path, m, n, chi_case):
     print("case facade %dx%d chi_case %d" % (m, n, chi_case), flush=True)
     p = simple_shear(m, n, u_bdry=2e-3)
     # equal spacing in x and y: the reference's smoother operator
@@ -79,7 +79,7 @@ def test_facade_driver_matches_checker(demo, tmp_path, m, n, chi_case):
         assert o.step(dt) == 0
     assert t == pytest.approx(0.25 + steps * dt, rel=1e-15)
     errs = group_errors(planes, o.get_fields())
-    print("facade vs checker", {k: "%.1e" % v for k, v in errs.items()})
+    print("facade vs oracle", {k: "%.1e" % v for k, v in errs.items()})
     assert max(errs.values()) < 1e-10, errs
     ov = o.get_vel()
     assert np.max(np.abs(vel - ov)) <= 1e-10 * np.max(np.abs(ov))

@@ -156,7 +156,7 @@ def test_adjoint_matches_central_differences(which):
 
 
 def test_adjoint_matches_differences_of_the_reference_step():
-    """the GPU adjoint against central differences of the checker's FP64
+    """the GPU adjoint against central differences of the oracle's FP64
     restatement of the reference's step (the oracle, CPU): an independent
     FP64 reference gradient along a chi direction and each parameter, away
     from kinks by the oracle's own decision bits"""

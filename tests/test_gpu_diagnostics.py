@@ -1,6 +1,6 @@
 """GPU parity of the frame diagnostics, initial conditions and the direct
 step (through the C ABI) against the reference's fixtures and the FP64
-checker.  Bars: per-node values without transcendentals bit-identical;
+oracle.  Bars: per-node values without transcendentals bit-identical;
 reductions (different summation order) within 1e-12 relative; values built
 on exp/log/sqrt within 1e-12 relative."""
 import os
