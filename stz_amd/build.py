@@ -7,6 +7,7 @@ step turns it off with a pragma to keep the reference's rounding).
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -31,13 +32,16 @@ def _run(cmd):
 
 def build(verbose=False):
     os.makedirs(BUILD_DIR, exist_ok=True)
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(BUILD_DIR, src.replace(".hip", ".o"))
-        out = _run([HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
-        if verbose and out.strip():
-            print(out)
-        objs.append(obj)
+    objs = [os.path.join(BUILD_DIR, src.replace(".hip", ".o")) for src in SOURCES]
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1)), 8))
+    with ThreadPoolExecutor(jobs) as pool:
+        outs = list(pool.map(lambda so: _run([HIPCC] + FLAGS + ["-c", os.path.join(CSRC, so[0]),
+                                                                "-o", so[1]]),
+                             zip(SOURCES, objs)))
+    if verbose:
+        for out in outs:
+            if out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
