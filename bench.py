@@ -3,19 +3,30 @@
 Workload: simple shear 1024x1024 cells (x periodic, walls at y = +/-1 moving
 at +/-U), chi = 600 K + 25 K seeded Gaussian field (correlation 3 cells),
 zero initial stress, dt = 0.2 (5000 steps reach T = 1000: elastic loading,
-yield near step 2100, then plastic flow and a shear band), V(2,2) cycles to
-the reference's own residual threshold.  By default the timed steps are the
-whole run after 10 warm-up steps, so the number covers both phases.  A step is one
+first yield near t = 425, then plastic flow and a shear band), V(2,2) cycles
+to 1e-4 x the reference's residual threshold.  The timed steps are the
+`--steps` after `--warmup` warm-up steps from t = 0.  A step is one
 shear_sim::step_forward_quasistatic: explicit ENO/STZ update, projection
 source, adaptive multigrid solve, stress correction and boundaries.
 
 The headline value is the mixed-precision form (FP32 V-cycles inside FP64
 iterative refinement); the plain FP64 form runs the same steps from the same
-state, and the two are compared field by field.  The roofline object is for
-the dominant kernel (the top-level smoother colour pass of the mixed form),
-timed live with HIP events on the simulation's own stream.  The CPU baseline
-is the reference's own code (oracle/_ref) on a bounded sample of the same
-workload, or the FP64 restatement when the reference build is absent.
+state, and the two are compared field by field.
+
+`roofline` is for the kernel that takes the largest share of a mixed step
+(top-level smoothing launches, advection), each timed live with HIP events
+on the simulation's own stream; its algorithmic bytes follow the scope
+table's per-node figures (24 B FP32 / 48 B FP64 per node and sweep, 4.5 words
+per fine node for a restriction or a prolongation, 32 B for the FP64 read-
+modify-write of vel += e, 136 B per cell for the advection).  `roofline_step`
+is the whole step: cells x bytes per cell-step (312 + 176 k + 80 r mixed,
+312 + 352 k FP64, for k V-cycles and r refinements per solve) / time.
+
+The CPU baseline is the reference's own code (oracle/_ref) timed on the
+host's cores on bounded samples of the same run (or the FP64 restatement
+when the reference build is absent); from the same samples both step twice
+more at a threshold of 1e-15 x the reference's and the GPU FP64 form is
+compared with them (parity at the full size).
 """
 import argparse
 import json
@@ -30,6 +41,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-steps/s at 1024^2: FP32-MG+FP64 refine vs FP64 @1e-8 rel; % HBM peak"
 HBM_PEAK_GBS = 8000.0
+YIELD_T = 425.0      # first yield of the seeded 1024^2 workload (elastic before)
+PLASTIC_T = 600.0    # plastic-flow sample time, when the run reaches it
+TIGHT = 1e-15        # threshold multiple of the full-size parity check: the
+                     # reference's own solution moves < 2e-11 (stress) from there to 1e-16 x
 
 
 def log(*a):
@@ -38,6 +53,11 @@ def log(*a):
 
 def group_err(a, b, ix):
     return float(np.max(np.abs(a[ix] - b[ix])) / max(np.max(np.abs(b[ix])), 1e-300))
+
+
+def errs_of(fa, fb):
+    return {"stress": group_err(fa, fb, [2, 3, 4, 5]), "chi": group_err(fa, fb, [6]),
+            "velocity": group_err(fa, fb, [0, 1]), "refmap": group_err(fa, fb, [7, 8])}
 
 
 def run_form(params, planes, form, dt, warmup, steps, nu, tol, stream, keep=()):
@@ -49,13 +69,18 @@ def run_form(params, planes, form, dt, warmup, steps, nu, tol, stream, keep=()):
     sim.set_vel(np.zeros((params["n"] + 1, params["m"], 2)))
     sim.time = 0.0
     sim.set_mg_config(nu1=nu, nu2=nu, tol_mult=tol)
+    kept = {}
+    if 0 in keep:
+        kept[0] = (sim.get_fields(), sim.get_vel(), sim.time)
     sim.step_forward_quasistatic(dt, warmup)
+    if warmup in keep:
+        kept[warmup] = (sim.get_fields(), sim.get_vel(), sim.time)
     sim.avg_iters()
     torch.cuda.synchronize()
     # the timed run is split where states are kept for the checks; every
     # split is a synchronisation the unsplit run would not have, so the
     # figure is, if anything, slightly pessimistic
-    kept, done, elapsed = {}, warmup, 0.0
+    done, elapsed = warmup, 0.0
     for stop in sorted(k for k in keep if warmup < k < warmup + steps) + [warmup + steps]:
         t0 = time.perf_counter()
         sim.step_forward_quasistatic(dt, stop - done)
@@ -70,33 +95,61 @@ def run_form(params, planes, form, dt, warmup, steps, nu, tol, stream, keep=()):
     return sim, elapsed, cycles, info, kept
 
 
-def smoother_roofline(sim, nodes, bytes_per_node_sweep, stream, launches=100):
-    """average duration of one launch of the top-level smoother (two fused
-    four-colour sweeps per launch), from HIP events recorded on the stream
-    the kernels run on; algorithmic bytes = 2 sweeps x (read z, read b,
-    write z) per node"""
+def timed(stream, fn, launches):
+    """average seconds per launch of fn(count) between two HIP events on the
+    stream the kernels run on"""
     import torch
-    sim.mg_smooth(0, 2)
+    fn(1)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    sim.mg_smooth(0, 2 * launches)
+    fn(launches)
     e1.record(stream)
     e1.synchronize()
-    per_launch_s = e0.elapsed_time(e1) * 1e-3 / launches
-    algo_bytes = 2 * bytes_per_node_sweep * nodes
-    achieved = algo_bytes / per_launch_s / 1e9
-    return per_launch_s, algo_bytes, achieved
+    return e0.elapsed_time(e1) * 1e-3 / launches
+
+
+def kernel_rooflines(sims, params, dt, nu, stream, launches=50):
+    """live HIP-event timing of the kernels of a step: the top level's
+    pre-smoothing launch (nu sweeps + fused restriction) and post-smoothing
+    launch (prolongation + nu sweeps, mixed: + FP64 vel += e), and the
+    advection kernel; algorithmic bytes per launch from the scope table's
+    per-node figures"""
+    N, n1 = params["m"], params["n"] + 1
+    nodes, cells = N * n1, N * params["n"]
+    out = {}
+    for form, sim in sims.items():
+        w = 4 if form == "mixed" else 8        # bytes per word of the V-cycle
+        sweep = 6 * w                          # read z, b; write z (vec2)
+        transfer = 9 * w                       # 4.5 vec2-component words per fine node
+        ks = {
+            "top_pre": ("smooth_tile_kernel<%s,64,32,%d,%d,true,true> (top level: %d sweeps + fused restriction)"
+                        % ("float" if w == 4 else "double", nu, 1 if w == 4 else 0, nu),
+                        nodes * (nu * sweep + transfer), lambda c, s=sim: s.mg_vcycle_launch(0, 1, c)),
+            "top_post": ("smooth_tile_kernel<%s,64,32,%d,2,false,true> (top level: prolongation + %d sweeps%s)"
+                         % ("float" if w == 4 else "double", nu, nu, " + FP64 vel += e" if w == 4 else ""),
+                         nodes * (nu * sweep + transfer + (32 if w == 4 else 0)),
+                         lambda c, s=sim: s.mg_vcycle_launch(0, 2, c)),
+        }
+        if form == "mixed":
+            ks["advect"] = ("advect_kernel (ENO2 advection + STZ plastic update)", cells * 136,
+                            lambda c, s=sim: s.advection_launch(dt, c))
+        for key, (name, algo, fn) in ks.items():
+            sec = timed(stream, fn, launches)
+            out["%s_%s" % (form, key)] = {"kernel": name, "avg_launch_us": sec * 1e6,
+                                          "algorithmic_bytes_per_launch": algo,
+                                          "achieved_GBps": algo / sec / 1e9}
+    return out
 
 
 def cpu_baseline(params, samples, dt, tol, phase_steps, budget_s=10.0, max_steps=30):
     """The reference's own code (or the FP64 restatement) timed on bounded
     samples of the same run: a few steps from each kept GPU state, one per
-    phase (elastic loading, plastic flow).  The full-run figure weights the
-    per-step times by how many of the run's steps fall in each phase."""
+    phase the timed steps cover.  The run's figure weights the per-step
+    times by how many of the timed steps fall in each phase."""
     from oracle import pyoracle
     kind = "reference" if pyoracle.reference_available() else "port"
     cls = pyoracle.Reference if kind == "reference" else pyoracle.Oracle
-    per_step, desc, finals = {}, [], {}
+    per_step, desc = {}, []
     for phase, (fields, vel, t) in samples.items():
         sim = cls(params)
         sim.set_fields(fields)
@@ -110,8 +163,7 @@ def cpu_baseline(params, samples, dt, tol, phase_steps, budget_s=10.0, max_steps
             steps += 1
         el = time.perf_counter() - t0
         per_step[phase] = el / steps
-        desc.append("%s: %d steps from t=%.0f, %.3f s/step" % (phase, steps, t, el / steps))
-        finals[phase] = (1 + steps, sim.get_fields())
+        desc.append("%s: %d steps from t=%.1f, %.3f s/step" % (phase, steps, t + dt, el / steps))
         sim.close()
     total_t = sum(phase_steps[p] * per_step[p] for p in per_step)
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
@@ -120,28 +172,36 @@ def cpu_baseline(params, samples, dt, tol, phase_steps, budget_s=10.0, max_steps
     cells = params["m"] * params["n"]
     return {"value": cells * sum(phase_steps.values()) / total_t, "unit": "cell-steps/s", "cores": cores,
             "kind": kind,
-            "sample": "%dx%d simple shear at the same threshold; %s; weighted by %s steps per phase" %
-                      (params["m"], params["n"], "; ".join(desc), phase_steps)}, finals
+            "sample": "%dx%d simple shear at the benchmark's threshold; %s; weighted by %s timed steps per phase" %
+                      (params["m"], params["n"], "; ".join(desc), phase_steps)}
 
 
-def gpu_vs_cpu(params, samples, finals, dt, nu, tol):
-    """the GPU FP64 form stepped from each CPU sample's start for the same
-    number of steps: relative difference per field group from the CPU's
-    final state (parity at the full size)"""
+def gpu_vs_reference(params, samples, dt, steps=2):
+    """from each sample state, the reference (or the restatement) and the GPU
+    FP64 form step `steps` times at 1e-15 x the reference's threshold:
+    relative difference per field group (parity at the full size, solves
+    converged far below the smoothers' differences)"""
+    from oracle import pyoracle
     from stz_amd import ShearSim
+    cls = pyoracle.Reference if pyoracle.reference_available() else pyoracle.Oracle
     out = {}
     for phase, (fields, vel, t) in samples.items():
-        nsteps, ref = finals[phase]
+        r = cls(params)
+        r.set_fields(fields)
+        r.set_vel(vel)
+        r.time = t
+        r.set_tol_mult(TIGHT)
+        r.step(dt, steps)
+        ref = r.get_fields()
+        r.close()
         g = ShearSim(params, "fp64")
         g.set_fields(fields)
         g.set_vel(vel)
         g.time = t
-        g.set_mg_config(nu1=nu, nu2=nu, tol_mult=tol)
-        g.step_forward_quasistatic(dt, nsteps)
-        f = g.get_fields()
+        g.set_mg_config(nu1=2, nu2=2, tol_mult=TIGHT)
+        g.step_forward_quasistatic(dt, steps)
+        out[phase] = dict(steps=steps, t0=t, **errs_of(g.get_fields(), ref))
         g.close()
-        out[phase] = {"steps": nsteps, "stress": group_err(f, ref, [2, 3, 4, 5]), "chi": group_err(f, ref, [6]),
-                      "velocity": group_err(f, ref, [0, 1]), "refmap": group_err(f, ref, [7, 8])}
     return out
 
 
@@ -176,8 +236,17 @@ def main():
         (N, N, args.warmup, args.steps, args.nu, args.nu, args.tol))
     total = args.warmup + args.steps
     checks = [c for c in (500, 1000, 2000, 3000, 3500, 3750, 4000, 4500) if c < total]
-    elastic_at, plastic_at = min(1000, total // 4), min(3000, 3 * total // 5)
-    keep = sorted(set(checks + [elastic_at, plastic_at]))
+    # phase samples by simulation time: the start of the timed steps (elastic
+    # while t < YIELD_T), and t = PLASTIC_T when the timed steps reach it
+    samples_at = {}
+    if args.warmup * dt < YIELD_T:
+        samples_at["elastic"] = args.warmup
+    plastic_step = int(round(PLASTIC_T / dt))
+    if args.warmup <= plastic_step < total:
+        samples_at["plastic"] = plastic_step
+    elif args.warmup * dt >= YIELD_T:
+        samples_at["plastic"] = args.warmup
+    keep = sorted(set(checks) | set(samples_at.values()))
     res = {}
     for form in ("fp64", "mixed"):
         sim, el, cyc, info, kept = run_form(params, planes, form, dt, args.warmup, args.steps, args.nu, args.tol,
@@ -187,40 +256,53 @@ def main():
         log("bench: %-5s %.3f ms/step, %.3e cell-steps/s, %.2f cycles/solve, last %s" %
             (form, res[form]["ms"], res[form]["value"], cyc, info))
 
-    def errs_of(fa, fb):
-        return {"stress": group_err(fa, fb, [2, 3, 4, 5]), "chi": group_err(fa, fb, [6]),
-                "velocity": group_err(fa, fb, [0, 1]), "refmap": group_err(fa, fb, [7, 8])}
     lockstep = {k: errs_of(res["mixed"]["kept"][k][0], res["fp64"]["kept"][k][0])
                 for k in checks if k in res["mixed"]["kept"] and k in res["fp64"]["kept"]}
     final = errs_of(res["mixed"]["sim"].get_fields(), res["fp64"]["sim"].get_fields())
     for k, e in lockstep.items():
         log("bench: mixed vs fp64 at step %d: %s" % (k, e))
     log("bench: mixed vs fp64 at step %d: %s" % (total, final))
-    # one step from the same FP64 state in both forms
-    one = {}
-    st = res["fp64"]["kept"].get(plastic_at)
-    for form in (("fp64", "mixed") if st else ()):
-        fields, vel, t = st
-        g = res[form]["sim"].__class__(params, form)
-        g.set_fields(fields)
-        g.set_vel(vel)
-        g.time = t
-        g.set_mg_config(nu1=args.nu, nu2=args.nu, tol_mult=args.tol)
-        g.step_forward_quasistatic(dt, 1)
-        one[form] = g.get_fields()
-        g.close()
+    # one step from the same FP64 state in both forms (the latest sample)
+    one, one_phase = {}, None
+    if samples_at:
+        one_phase = "plastic" if "plastic" in samples_at else "elastic"
+        fields, vel, t = res["fp64"]["kept"][samples_at[one_phase]]
+        for form in ("fp64", "mixed"):
+            g = res[form]["sim"].__class__(params, form)
+            g.set_fields(fields)
+            g.set_vel(vel)
+            g.time = t
+            g.set_mg_config(nu1=args.nu, nu2=args.nu, tol_mult=args.tol)
+            g.step_forward_quasistatic(dt, 1)
+            one[form] = g.get_fields()
+            g.close()
     onestep = errs_of(one["mixed"], one["fp64"]) if one else None
-    log("bench: one step from the step-%d state, mixed vs fp64: %s" % (plastic_at, onestep))
-    errs = {"one_step_from_plastic_state": onestep,
+    log("bench: one step from the %s state, mixed vs fp64: %s" % (one_phase, onestep))
+    errs = {"one_step_from_%s_state" % one_phase: onestep,
             "lockstep_before_instability": max((max(e["stress"], e["chi"]) for k, e in lockstep.items()
                                                 if k <= 3750), default=None),
-            "final_after_shear_band": final}
+            "final": final}
 
-    nodes = N * (N + 1)
-    per_launch, algo_bytes, achieved = smoother_roofline(res["mixed"]["sim"], nodes, 24.0, stream)
-    per_launch64, algo_bytes64, achieved64 = smoother_roofline(res["fp64"]["sim"], nodes, 48.0, stream)
-    log("bench: tiled smoother launch (2 sweeps) fp32 %.2f us (%.0f GB/s), fp64 %.2f us (%.0f GB/s)" %
-        (per_launch * 1e6, achieved, per_launch64 * 1e6, achieved64))
+    # step-level roofline from the scope table's bytes per cell-step
+    extra = 1
+    step_roof = {}
+    for form in ("fp64", "mixed"):
+        k = res[form]["cycles"] + extra
+        bps = 312 + 176 * k + 80 * k if form == "mixed" else 312 + 352 * k
+        ach = cells * bps * args.steps / res[form]["elapsed"] / 1e9
+        step_roof[form] = {"bytes_per_cell_step": bps, "cycles_per_solve_run": k, "achieved_GBps": ach,
+                           "frac": ach / HBM_PEAK_GBS}
+    log("bench: step-level roofline %s" % step_roof)
+
+    kern = kernel_rooflines({"mixed": res["mixed"]["sim"], "fp64": res["fp64"]["sim"]}, params, dt, args.nu,
+                            stream)
+    for k, v in kern.items():
+        log("bench: %-16s %8.2f us  %6.0f GB/s  %s" % (k, v["avg_launch_us"], v["achieved_GBps"], v["kernel"]))
+    # the dominant kernel of the mixed step: largest time per step
+    per_step = {"mixed_top_pre": res["mixed"]["cycles"] + extra, "mixed_top_post": res["mixed"]["cycles"] + extra,
+                "mixed_advect": 1}
+    dom = max(per_step, key=lambda k: kern[k]["avg_launch_us"] * per_step[k])
+    dk = kern[dom]
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
@@ -228,18 +310,21 @@ def main():
         with open(tfile) as f:
             tj = json.load(f)
         # HBM bytes per launch from FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel
-        if tj.get("kernel") == "smooth_tile_kernel<float,64,32,2,0>" and tj.get("level", "").startswith("%dx" % N):
-            traffic = tj.get("traffic_bytes_per_launch")
+        entry = tj.get("kernels", {}).get(dom)
+        if entry and entry.get("size") == N:
+            traffic = entry.get("traffic_bytes_per_launch")
 
-    cpu, vs_cpu = None, None
-    if not args.no_cpu and elastic_at in res["fp64"]["kept"] and plastic_at in res["fp64"]["kept"]:
-        # steps before the first yield (t ~ 425 at this loading rate) are elastic
-        n_el = min(total, int(425.0 / dt))
-        samples = {"elastic": res["fp64"]["kept"][elastic_at], "plastic": res["fp64"]["kept"][plastic_at]}
-        cpu, finals = cpu_baseline(params, samples, dt, args.tol, {"elastic": n_el, "plastic": total - n_el})
+    cpu, vs_ref = None, None
+    if not args.no_cpu and samples_at:
+        samples = {ph: res["fp64"]["kept"][st] for ph, st in samples_at.items()}
+        first = args.warmup
+        n_el = max(0, min(total, int(YIELD_T / dt)) - first)
+        phase_steps = {"elastic": n_el, "plastic": args.steps - n_el}
+        phase_steps = {p: c for p, c in phase_steps.items() if c > 0 and p in samples}
+        cpu = cpu_baseline(params, samples, dt, args.tol, phase_steps)
         log("bench: cpu baseline %s" % cpu)
-        vs_cpu = gpu_vs_cpu(params, samples, finals, dt, args.nu, args.tol)
-        log("bench: GPU FP64 vs the CPU (%s) from the same states: %s" % (cpu["kind"], vs_cpu))
+        vs_ref = gpu_vs_reference(params, samples, dt)
+        log("bench: GPU FP64 vs the CPU (%s) at %g x threshold: %s" % (cpu["kind"], TIGHT, vs_ref))
 
     mixed, fp64 = res["mixed"], res["fp64"]
     line = {
@@ -256,21 +341,24 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": "simple shear %dx%d, chi 600K+25K seeded (corr 3 cells), dt %.3g, %d steps, V(%d,%d), "
-                        "residual threshold %g x reference" % (N, N, dt, total, args.nu, args.nu, args.tol),
+            "workload": "simple shear %dx%d, chi 600K+25K seeded (corr 3 cells), dt %.3g, steps %d..%d of 5000 "
+                        "(t %.1f..%.1f), V(%d,%d), residual threshold %g x reference" %
+                        (N, N, dt, args.warmup + 1, total, args.warmup * dt, total * dt, args.nu, args.nu,
+                         args.tol),
             "form": "FP32 V-cycles in FP64 iterative refinement",
             "fp64_value": fp64["value"],
             "fp64_ms_per_step": fp64["ms"],
             "mixed_over_fp64": mixed["value"] / fp64["value"],
             "mixed_vs_fp64_rel_err": errs,
             "cycles_per_solve": {"mixed": mixed["cycles"], "fp64": fp64["cycles"]},
-            "fp64_vs_cpu_same_start": vs_cpu,
-            "smoother_fp64_GBps": achieved64,
+            "fp64_vs_reference_at_1e-15": vs_ref,
+            "kernels": kern,
         },
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "smooth_tile_kernel<float,64,32,2,*> (top level, 2 sweeps)",
-                     "algorithmic_bytes_per_launch": algo_bytes, "avg_launch_us": per_launch * 1e6},
+        "roofline": {"bound": "hbm", "achieved": dk["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dk["achieved_GBps"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": dk["kernel"], "algorithmic_bytes_per_launch": dk["algorithmic_bytes_per_launch"],
+                     "avg_launch_us": dk["avg_launch_us"]},
+        "roofline_step": step_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -65,6 +65,14 @@ typedef struct stz_model_params {
     double t_icpt, t_grad;
 } stz_model_params;
 
+/* stz_dynamics::Dplastic (stz_model.hh:28; the laws of :50-207): the
+ * plastic rate at deviatoric stress sbar and effective temperature chi, and
+ * the chi-rate coefficients dchi1, dchi2.  Evaluated on the host with the
+ * law code the step's kernels run (C library transcendentals, FMA
+ * contraction off), for callers that tabulate the law; never part of a step. */
+int stz_model_dplastic(const stz_model_params *mp, double sbar, double chi, double *dpl, double *dchi1,
+                       double *dchi2);
+
 /* Constructor arguments of shear_sim (shear_sim.hh:95-99, shear_sim.cc:26-44). */
 typedef struct stz_sim_params {
     int32_t m, n;                 /* cells in x (periodic) and y (walls) */
@@ -91,22 +99,34 @@ enum {
     STZ_PREC_MIXED = 1   /* FP32 V-cycles inside FP64 iterative refinement */
 };
 
-/* Multigrid settings.  Defaults reproduce the reference's solve_v_cycle
- * (tgmg.hh:439): one pre/post sweep, 10+10 bottom sweeps, the adaptive
- * V-cycle predictor of tgmg_predict.hh:6-33 and the tolerance
- * tgmg_accuracy(2*(tmufac1+tmufac2),1e4) of qs_multi.cc:14 scaled by
- * tol_mult (mean-square residual threshold = tol_mult * that value). */
+/* Multigrid settings.  Defaults follow the reference's solve_v_cycle
+ * (tgmg.hh:439): one pre/post sweep, the adaptive V-cycle predictor of
+ * tgmg_predict.hh:6-33 and the tolerance tgmg_accuracy(2*(tmufac1+tmufac2),
+ * 1e4) of qs_multi.cc:14 scaled by tol_mult (mean-square residual threshold
+ * = tol_mult * that value).  The coarse end of the V-cycle is solved exactly
+ * (see direct_nodes), where the reference smooths its coarsest level with
+ * 2 bottom_sweeps Gauss-Seidel sweeps: bottom_sweeps is kept for the C++
+ * facade's tgmg signatures and does not change the solve. */
 typedef struct stz_mg_config {
     int32_t nu1, nu2;            /* pre/post smoothing sweeps per level */
     int32_t bottom_sweeps;       /* half the sweeps on the coarsest level */
     int32_t extra_cycles;        /* cycles after convergence (reference: 1) */
     int32_t predict_mult, predict_decay, predict_init, predict_max_iters;
     double tol_mult;             /* multiplier on the mean-square tolerance */
-    int32_t max_refine;          /* mixed: cap on FP64 refinement iterations */
+    int32_t max_refine;          /* mixed: FP64 refinement iterations one solve
+                                    may take (one per V-cycle) before it fails
+                                    with STZ_ERR_NOT_CONVERGED */
     int32_t kernels;             /* 0: LDS-tiled multi-sweep kernels where a
                                     level allows them (default); 1: one
                                     kernel per colour pass everywhere */
-    int32_t reserved[6];
+    int32_t direct_nodes;        /* the finest coarse level with at most this
+                                    many nodes whose operator is invariant
+                                    along x is solved exactly by its
+                                    block-circulant inverse, and the V-cycle
+                                    stops there (default 4200: the 64x65 level
+                                    of a 1024^2 grid); 0: the V-cycle runs to
+                                    the coarsest level, solved exactly */
+    int32_t reserved[5];
 } stz_mg_config;
 
 /* Result of the last projection solve. */
@@ -165,6 +185,10 @@ int stz_sim_init_fields(stz_sim *sim, int chi_case, double tem_base, double tem_
 int stz_sim_step_quasistatic(stz_sim *sim, double dt, int nsteps);
 /* shear_sim::advection_step (shear_sim.cc:296-361), chi_diffusion included. */
 int stz_sim_advection_step(stz_sim *sim, double dt);
+/* stz_sim_advection_step `count` times, asynchronously on the simulation's
+ * stream (no counterpart in the reference: for timing the explicit kernels
+ * between a caller's own events). */
+int stz_sim_advection_launch(stz_sim *sim, double dt, int count);
 /* shear_sim::projection_step (shear_sim.cc:391-438). */
 int stz_sim_projection_step(stz_sim *sim, double dt);
 /* shear_sim::set_boundaries (shear_sim.cc:552-581). */
@@ -196,6 +220,23 @@ int stz_mg_l2_error(stz_sim *sim, double *err);
  * FP32 one in the mixed form).  Asynchronous on the simulation's stream, so
  * a caller can bracket it with its own events. */
 int stz_mg_smooth(stz_sim *sim, int level, int sweeps);
+/* tgmg::fmg_pre (tgmg.in.cc:369-399) on the current src/vel: the
+ * full-multigrid pass (smoothing on the way down, the coarse levels solved,
+ * prolongation, smoothing and a V-cycle from each level on the way up,
+ * ending with the prolongation into vel).  Runs on the FP64 hierarchy in
+ * both precision forms, with the per-colour kernels. */
+int stz_mg_fmg_pre(stz_sim *sim);
+/* tgmg_base::clear_z (tgmg.in.cc:792-797): vel = 0. */
+int stz_mg_clear_z(stz_sim *sim);
+/* One smoothing launch of the V-cycle on hierarchy level `level`, alone,
+ * `count` times (no counterpart in the reference: for timing the kernels a
+ * solve runs).  mode 1: the pre-smoothing (nu1 sweeps, on the correction
+ * equation from zero in the mixed form, with the restriction of the residual
+ * into level+1); mode 2: the post-smoothing (prolongation of level+1, nu2
+ * sweeps, and on the mixed form's top level the FP64 correction vel += e).
+ * Needs the tiled kernels (stz_mg_config.kernels = 0) on that level;
+ * asynchronous on the simulation's stream. */
+int stz_mg_vcycle_launch(stz_sim *sim, int level, int mode, int count);
 /* Upload a source array (m*(n+1) vec2) for a standalone solve. */
 int stz_mg_upload_src(stz_sim *sim, const double *host_vec2);
 
@@ -415,7 +456,9 @@ int stz_ensemble_use_graphs(stz_ensemble *e, int on);
 /* nsteps quasi-static steps of every member.  Stops after the first step in
  * which a member's solve fails and returns that status; the failing member
  * is left as stz_sim_step_quasistatic leaves it, the others have completed
- * the step. */
+ * the step.  If instead a launch or copy fails inside a step (STZ_ERR_HIP),
+ * every member's fields, velocity and time are undefined until the caller
+ * sets them again; the next call rebuilds the ensemble's tables. */
 int stz_ensemble_step_quasistatic(stz_ensemble *e, double dt, int nsteps);
 /* Counters since creation: out4 = {steps, cycle slots launched, slots
  * launched after the planned ones, graph captures}. */

@@ -362,6 +362,19 @@ class Reference(_Sim):
         Reference._fns["set_threads"](int(n))
 
 
+def reference_dplastic(model, sbar, chi):
+    """stz_dynamics::Dplastic of the reference's own law classes
+    (stz_model.hh:28) for a model dict: (Dpl, dchi1, dchi2)"""
+    lib = C.CDLL(REF_SO)
+    fn = lib.ref_dplastic
+    fn.restype = C.c_double
+    fn.argtypes = [C.POINTER(ModelParams), C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    mp = make_params({"m": 4, "n": 4, "stz": model, **{k: 0.0 for k in _SIM_DOUBLES}}).stz
+    d1, d2 = C.c_double(), C.c_double()
+    r = fn(C.byref(mp), float(sbar), float(chi), C.byref(d1), C.byref(d2))
+    return r, d1.value, d2.value
+
+
 # ---------------------------------------------------------------------------
 # free-boundary reference (bd_sim) through oracle/ref_bd_harness.cc
 # ---------------------------------------------------------------------------

@@ -124,6 +124,14 @@ void ref_set_tol_mult(void *h, double mult) {
 }
 double ref_tolerance(void *h) { return static_cast<ref_sim *>(h)->sim->qsm.acc; }
 
+// stz_dynamics::Dplastic of a freshly constructed law (stz_model.hh:28)
+double ref_dplastic(const stz_model_params *p, double sbar, double chi, double *dchi1, double *dchi2) {
+    stz_dynamics *s = ref_make_stz(*p);
+    const double r = s->Dplastic(sbar, chi, *dchi1, *dchi2);
+    delete s;
+    return r;
+}
+
 void ref_init_fields(void *h, int c, double t0, double dt) { static_cast<ref_sim *>(h)->sim->init_fields(c, t0, dt); }
 void ref_initialize_random(void *h, double l, double avg, double sd, unsigned seed) {
     srand(seed);

@@ -41,7 +41,8 @@ class MGConfig(C.Structure):
                 ("extra_cycles", C.c_int32), ("predict_mult", C.c_int32),
                 ("predict_decay", C.c_int32), ("predict_init", C.c_int32),
                 ("predict_max_iters", C.c_int32), ("tol_mult", C.c_double),
-                ("max_refine", C.c_int32), ("kernels", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("max_refine", C.c_int32), ("kernels", C.c_int32), ("direct_nodes", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 class SolveInfo(C.Structure):
@@ -184,6 +185,12 @@ _SIGS = {
     "stz_adjoint_backward": [C.c_void_p, _dp, _dp, _dp],
     "stz_adjoint_set_solve_accuracy": [C.c_void_p, C.c_double],
     "stz_adjoint_stats": [C.c_void_p, _dp],
+    "stz_model_dplastic": [C.POINTER(ModelParams), C.c_double, C.c_double, C.POINTER(C.c_double),
+                           C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    "stz_sim_advection_launch": [C.c_void_p, C.c_double, C.c_int],
+    "stz_mg_fmg_pre": [C.c_void_p],
+    "stz_mg_clear_z": [C.c_void_p],
+    "stz_mg_vcycle_launch": [C.c_void_p, C.c_int, C.c_int, C.c_int],
 }
 _RESTYPE = {"stz_last_error": C.c_char_p}
 

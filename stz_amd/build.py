@@ -13,9 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstz_amd.so")
 BUILD_DIR = os.path.join(HERE, "_build")
-SOURCES = ("explicit.hip", "diag.hip", "mg.hip", "mg_tiled.hip", "sim.hip",
+SOURCES = ("explicit.hip", "diag.hip", "mg.hip", "mg_tiled.hip", "mg_direct.hip", "sim.hip",
            "bd_levelset.hip", "bd_step.hip", "bd_mg.hip", "bd_sim.hip", "ensemble.hip",
-           "adjoint.hip", "adjoint_run.hip")
+           "adjoint.hip", "adjoint_run.hip", "law.hip")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -269,6 +269,18 @@ int stz_adjoint_backward(stz_adjoint *a, const double *lam_final, double *lam_in
     STZ_HIP(hipMemsetAsync(a->w, 0, a->nv * sizeof(double), st));
     const int fwd_lim = s->lim;
     const double fwd_time = s->time;
+    // the recomputed steps' solves are not the forward run's: the solver
+    // statistics (avg_iters, last_solve) are restored on every exit
+    struct StatsGuard {
+        stz_sim *s;
+        int solves, vcount;
+        stz_solve_info last;
+        ~StatsGuard() {
+            s->solves = solves;
+            s->vcount = vcount;
+            s->last = last;
+        }
+    } guard{s, s->solves, s->vcount, s->last};
     for (int seg = a->nckpt - 1; seg >= 0; seg--) {
         const int t0s = seg * a->every, t1s = std::min(t0s + a->every, a->nsteps);
         if ((e = load_state(a, a->ckpt + (size_t)seg * a->state))) return e;

@@ -78,6 +78,14 @@ template <typename T> struct CoarseEntry {
     CoarseArgs<T> A;
 };
 
+// one member's exact solve of its direct level (mg_direct.hip)
+template <typename T> struct DirectEntry {
+    int member;
+    int M, Nn;
+    const T *G, *b;
+    T *z;
+};
+
 // workgroup-partial sums of one member's residual
 struct NormEntry {
     int member, np;
@@ -127,6 +135,11 @@ int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out,
 int batch_refine(const RefineEntry *E, const int *map, int ne, int total, int sparse, const EnsState *S,
                  hipStream_t st);
 template <typename T> int batch_coarse(const CoarseEntry<T> *E, int ne, size_t lds, const EnsState *S, hipStream_t st);
+// direct-level solves; direct_workgroups(M, Nn) workgroups per entry
+template <typename T>
+int batch_direct(const DirectEntry<T> *E, const int *map, int ne, int total, size_t lds, const EnsState *S,
+                 hipStream_t st);
+int direct_workgroups(int M, int Nn);
 template <typename T> int batch_set_lds_limit(size_t bytes);
 // fp64: |r|^2 partials of the top level, blocks as mg_resid_norm
 int batch_resid_norm(const RefineEntry *E, const int *map, int ne, int total, const EnsState *S, hipStream_t st);

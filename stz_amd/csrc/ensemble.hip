@@ -56,9 +56,11 @@ template <typename T> struct TileLaunch {
 
 template <typename T> struct VCyclePlan {
     std::vector<TileLaunch<T>> down, up;
-    CoarseEntry<T> *coarse = nullptr;
+    CoarseEntry<T> *coarse = nullptr;  // members solving their coarse levels in one workgroup
     int ncoarse = 0;
     size_t lds = 0;
+    DevTable<DirectEntry<T>> direct;   // members whose V-cycle stops at a direct level
+    size_t dlds = 0;
 };
 
 }  // namespace
@@ -191,7 +193,9 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
     using Pos = std::tuple<int, int, int>;  // (phase, level key, index)
     std::map<Pos, std::vector<std::pair<TileRec<T>, int>>> groups;
     std::vector<CoarseEntry<T>> coarse;
-    size_t lds = 0;
+    std::vector<DirectEntry<T>> direct;
+    std::vector<int> dwgs;
+    size_t lds = 0, dlds = 0;
     int rc;
     for (int k = 0; k < e->nmem(); k++) {
         stz_sim *s = e->mem[k];
@@ -210,6 +214,19 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
         for (const TileRec<T> &r : recs) {
             const int lk = r.phase == 0 ? r.level : -r.level;  // up: deeper levels first
             groups[Pos(r.phase, lk, r.index)].push_back({r, k});
+        }
+        if (A.ld >= 0) {
+            DirectEntry<T> de;
+            de.member = k;
+            de.M = A.lv[A.ld].m;
+            de.Nn = A.lv[A.ld].n;
+            de.G = A.Gd;
+            de.b = A.lv[A.ld].b;
+            de.z = A.lv[A.ld].z;
+            direct.push_back(de);
+            dwgs.push_back(direct_workgroups(de.M, de.Nn));
+            dlds = std::max(dlds, direct_lds_bytes<T>(de.M, de.Nn));
+            continue;
         }
         CoarseEntry<T> ce;
         ce.member = k;
@@ -288,6 +305,11 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
     if (lds > 160 * 1024) {
         set_error("workgroup solver needs %zu bytes of LDS", lds);
         return STZ_ERR_STATE;
+    }
+    if (!direct.empty()) {
+        if ((rc = make_table(e, direct, dwgs, P.direct))) return rc;
+        P.dlds = dlds;
+        if ((rc = mg_direct_set_lds_limit<T>(dlds))) return rc;
     }
     return batch_set_lds_limit<T>(lds);
 }
@@ -443,6 +465,8 @@ template <typename T> int launch_vcycle(stz_ensemble *e, const VCyclePlan<T> &P)
     for (const TileLaunch<T> &l : P.down)
         if ((rc = batch_tile<T>(l.key, l.t.E, l.t.map, l.t.ne, l.t.total, e->dst, e->stream))) return rc;
     if ((rc = batch_coarse<T>(P.coarse, P.ncoarse, P.lds, e->dst, e->stream))) return rc;
+    const DevTable<DirectEntry<T>> &D = P.direct;
+    if ((rc = batch_direct<T>(D.E, D.map, D.ne, D.total, P.dlds, e->dst, e->stream))) return rc;
     for (const TileLaunch<T> &l : P.up)
         if ((rc = batch_tile<T>(l.key, l.t.E, l.t.map, l.t.ne, l.t.total, e->dst, e->stream))) return rc;
     return STZ_OK;
@@ -684,11 +708,19 @@ int stz_ensemble_step_quasistatic(stz_ensemble *e, double dt, int nsteps) {
     }
     STZ_HIP(hipMemcpyAsync(e->dst, e->hst, N * sizeof(EnsState), hipMemcpyHostToDevice, e->stream));
     int status = STZ_OK;
+    // a launch or copy failing inside a step leaves the members' device state
+    // partly advanced and their host mirrors (time, predictor, plane sets)
+    // behind it: the tables are marked stale and the members' state is
+    // undefined until the caller sets it again (include/stz_amd.h)
+    auto abort_step = [&](int code) {
+        e->built = false;
+        return code;
+    };
     for (int step = 0; step < nsteps && status == STZ_OK; step++) {
         const int p = e->parity;
         const int plan = planned_slots(e);
-        if ((rc = run_step(e, p, plan, dt))) return rc;
-        if ((rc = sync_state(e))) return rc;
+        if ((rc = run_step(e, p, plan, dt))) return abort_step(rc);
+        if ((rc = sync_state(e))) return abort_step(rc);
         // members whose predictor asked for more cycles than planned
         for (;;) {
             bool more = false;
@@ -696,10 +728,10 @@ int stz_ensemble_step_quasistatic(stz_ensemble *e, double dt, int nsteps) {
             if (!more) break;
             for (int q = 0; q < 2; q++) {
                 e->stats[2]++;
-                if ((rc = run_slot(e))) return rc;
+                if ((rc = run_slot(e))) return abort_step(rc);
             }
-            if ((rc = run_tail(e, p, dt))) return rc;
-            if ((rc = sync_state(e))) return rc;
+            if ((rc = run_tail(e, p, dt))) return abort_step(rc);
+            if ((rc = sync_state(e))) return abort_step(rc);
         }
         e->stats[0]++;
         // host mirrors of every member, as stz_sim_step_quasistatic leaves them

@@ -45,7 +45,24 @@ template <typename T> struct CoarseArgs {
     int pack_elems;
     int zoff[kMaxLevels], boff[kMaxLevels], soff[kMaxLevels], roff, binv_off;
     int lds_elems;
+    // ld >= 0: level ld (an x-invariant level, lc or finer) is solved
+    // exactly by the block-circulant inverse Gd (mg_direct.hip) and the
+    // levels below it are not visited; -1: the workgroup solver runs
+    // levels lc..L-1
+    int ld;
+    const T *Gd;
 };
+
+// the level a V-cycle bottoms out at, and its solve
+template <typename T> inline int bottom_level(const CoarseArgs<T> &A) { return A.ld >= 0 ? A.ld : A.lc; }
+template <typename T> int mg_bottom(const CoarseArgs<T> &A, hipStream_t st);
+
+// mg_direct.hip: exact solve of an x-invariant level from its block-
+// circulant inverse G (host: direct_inverse), and its LDS needs
+template <typename T> int mg_direct_solve(const LevelArgs<T> &L, const T *G, hipStream_t st);
+template <typename T> size_t direct_lds_bytes(int M, int Nn);
+template <typename T> int mg_direct_set_lds_limit(size_t bytes);
+bool direct_inverse(const double *S, int M, int Nn, std::vector<double> &G);
 
 // LDS layout for levels lc..L-1; returns the number of elements of T
 template <typename T> int coarse_layout(CoarseArgs<T> &A);
@@ -61,6 +78,10 @@ template <typename T> int mg_restrict(const LevelArgs<T> &F, const LevelArgs<T> 
 template <typename T> int mg_prolong(const LevelArgs<T> &C, const LevelArgs<T> &F, hipStream_t st);
 template <typename T> int mg_coarse(const CoarseArgs<T> &A, hipStream_t st);
 template <typename T> int mg_vcycle(const CoarseArgs<T> &A, hipStream_t st);
+// the V-cycle from level l0 down (level l0 from its current z)
+template <typename T> int mg_vcycle_from(const CoarseArgs<T> &A, int l0, hipStream_t st);
+// tgmg::fmg_pre: the full-multigrid pass before a solve's V-cycles
+template <typename T> int mg_fmg_pre(const CoarseArgs<T> &A, hipStream_t st);
 template <typename T> int mg_resid_norm(const LevelArgs<T> &L, double *partial, double *out, hipStream_t st);
 template <typename T> size_t coarse_lds_bytes(const CoarseArgs<T> &A);
 // allow the workgroup solver this much dynamic LDS
@@ -85,6 +106,11 @@ template <typename T> bool tiled_ok(const LevelArgs<T> &L);
 template <typename T>
 int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipStream_t st, double *corr = nullptr,
                     int *corrected = nullptr);
+// one smoothing launch of a tiled V-cycle on `level` alone: mode 1 the
+// pre-smoothing with its restriction, mode 2 the post-smoothing with its
+// prolongation (and, with corr on level 0, the mixed form's vel += e)
+template <typename T>
+int mg_vcycle_launch(CoarseArgs<T> &A, int level, int mode, bool top_zero, double *corr, hipStream_t st);
 // `sweeps` smoothing sweeps on one level (tiled where possible; swaps z/z2)
 template <typename T> int mg_smooth_level(LevelArgs<T> &L, int sweeps, hipStream_t st);
 // residual of the FP64 top level -> FP32 source, |r|^2 partials summed to *out

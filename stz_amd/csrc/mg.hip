@@ -814,20 +814,55 @@ int mg_correct64(double *z, const float *e, int count, hipStream_t st) {
     return STZ_OK;
 }
 
-// one V-cycle on levels 0..L-1 (grid kernels above lc, workgroup below)
-template <typename T> int mg_vcycle(const CoarseArgs<T> &A, hipStream_t st) {
+// the V-cycle from level l0 down (v_cycle_internal, tgmg.in.cc:341-359, with
+// the smoothing of level l0 itself): grid kernels above lc, workgroup below.
+// Level l0 starts from its current z; coarser levels from zero.
+template <typename T> int mg_bottom(const CoarseArgs<T> &A, hipStream_t st) {
+    return A.ld >= 0 ? mg_direct_solve(A.lv[A.ld], A.Gd, st) : mg_coarse(A, st);
+}
+
+template <typename T> int mg_vcycle_from(const CoarseArgs<T> &A, int l0, hipStream_t st) {
     int e;
-    if ((e = mg_smooth(A.lv[0], A.nu1, st))) return e;
-    for (int l = 0; l < A.lc; l++) {
+    const int lb = bottom_level(A);
+    if (l0 >= lb) return mg_bottom(A, st);
+    if ((e = mg_smooth(A.lv[l0], A.nu1, st))) return e;
+    for (int l = l0; l < lb; l++) {
         if ((e = mg_restrict(A.lv[l], A.lv[l + 1], st))) return e;
-        if (l + 1 < A.lc && (e = mg_smooth(A.lv[l + 1], A.nu1, st))) return e;
+        if (l + 1 < lb && (e = mg_smooth(A.lv[l + 1], A.nu1, st))) return e;
     }
-    if ((e = mg_coarse(A, st))) return e;
-    for (int l = A.lc; l >= 1; l--) {
+    if ((e = mg_bottom(A, st))) return e;
+    for (int l = lb; l >= l0 + 1; l--) {
         if ((e = mg_prolong(A.lv[l], A.lv[l - 1], st))) return e;
         if ((e = mg_smooth(A.lv[l - 1], A.nu2, st))) return e;
     }
     return STZ_OK;
+}
+
+// one V-cycle on levels 0..L-1
+template <typename T> int mg_vcycle(const CoarseArgs<T> &A, hipStream_t st) { return mg_vcycle_from(A, 0, st); }
+
+// tgmg::fmg_pre (tgmg.in.cc:369-399): smooth the top level, restrict the
+// residual down the hierarchy smoothing each level from zero, solve the
+// levels of the workgroup solver, then on the way up prolong, smooth (nu2
+// then nu1 sweeps) and run a V-cycle from each level, and finally add the
+// prolongation of level 1 to the top level (no top-level smoothing after
+// it: the solve's next V-cycle does that)
+template <typename T> int mg_fmg_pre(const CoarseArgs<T> &A, hipStream_t st) {
+    int e;
+    if ((e = mg_smooth(A.lv[0], A.nu1, st))) return e;
+    if (A.L == 1) return STZ_OK;
+    const int lb = bottom_level(A);
+    for (int l = 0; l < lb; l++) {
+        if ((e = mg_restrict(A.lv[l], A.lv[l + 1], st))) return e;
+        if (l + 1 < lb && (e = mg_smooth(A.lv[l + 1], A.nu1, st))) return e;
+    }
+    if ((e = mg_bottom(A, st))) return e;
+    for (int l = lb - 1; l >= 1; l--) {
+        if ((e = mg_prolong(A.lv[l + 1], A.lv[l], st))) return e;
+        if ((e = mg_smooth(A.lv[l], A.nu2, st))) return e;
+        if ((e = mg_vcycle_from(A, l, st))) return e;
+    }
+    return mg_prolong(A.lv[1], A.lv[0], st);
 }
 
 // The attribute belongs to the kernel, not to a simulation: it only ever
@@ -852,6 +887,12 @@ template int mg_coarse<float>(const CoarseArgs<float> &, hipStream_t);
 template int mg_smooth<double>(const LevelArgs<double> &, int, hipStream_t);
 template int mg_smooth<float>(const LevelArgs<float> &, int, hipStream_t);
 template int mg_vcycle<double>(const CoarseArgs<double> &, hipStream_t);
+template int mg_vcycle_from<double>(const CoarseArgs<double> &, int, hipStream_t);
+template int mg_bottom<double>(const CoarseArgs<double> &, hipStream_t);
+template int mg_bottom<float>(const CoarseArgs<float> &, hipStream_t);
+template int mg_vcycle_from<float>(const CoarseArgs<float> &, int, hipStream_t);
+template int mg_fmg_pre<double>(const CoarseArgs<double> &, hipStream_t);
+template int mg_fmg_pre<float>(const CoarseArgs<float> &, hipStream_t);
 template int mg_vcycle<float>(const CoarseArgs<float> &, hipStream_t);
 template int mg_resid_norm<double>(const LevelArgs<double> &, double *, double *, hipStream_t);
 template int mg_resid_norm<float>(const LevelArgs<float> &, double *, double *, hipStream_t);

@@ -656,7 +656,8 @@ template <typename T> int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipSt
                                           int *corrected) {
     int e;
     if (corrected) *corrected = 0;
-    for (int l = 0; l < A.lc; l++) {
+    const int lb = bottom_level(A);
+    for (int l = 0; l < lb; l++) {
         // pre-smoothing, fused with the restriction into level l+1 when tiled
         int done = 0;
         if ((e = smooth_level(A.lv[l], (const LevelArgs<T> *)nullptr, A.nu1,
@@ -664,16 +665,45 @@ template <typename T> int mg_vcycle_tiled(CoarseArgs<T> &A, bool top_zero, hipSt
             return e;
         if (!done && (e = restrict_level(A.lv[l], A.lv[l + 1], st))) return e;
     }
-    if (A.lc == 0 && (e = smooth_level(A.lv[0], (const LevelArgs<T> *)nullptr, A.nu1,
-                                       top_zero ? TILE_ZERO : TILE_LOAD, st)))
+    if (lb == 0 && (e = smooth_level(A.lv[0], (const LevelArgs<T> *)nullptr, A.nu1,
+                                     top_zero ? TILE_ZERO : TILE_LOAD, st)))
         return e;
-    if ((e = mg_coarse(A, st))) return e;
-    for (int l = A.lc; l >= 1; l--)
+    if ((e = mg_bottom(A, st))) return e;
+    for (int l = lb; l >= 1; l--)
         if ((e = smooth_level(A.lv[l - 1], &A.lv[l], A.nu2, TILE_PROLONG, st, (const LevelArgs<T> *)nullptr,
                               nullptr, l == 1 ? corr : nullptr, l == 1 ? corrected : nullptr)))
             return e;
     return STZ_OK;
 }
+
+// One of the smoothing launches a tiled V-cycle issues on `level`, alone:
+// mode 1 the pre-smoothing (nu1 sweeps from zero on the correction equation
+// or a coarse level, else from z; the residual restricted into level+1),
+// mode 2 the post-smoothing (nu2 sweeps after adding the prolongation of
+// level+1; on the mixed form's top level also vel += e into corr).  For
+// timing the kernels that mg_vcycle_tiled runs; the level's z/z2 are left
+// swapped as the cycle leaves them.
+template <typename T>
+int mg_vcycle_launch(CoarseArgs<T> &A, int level, int mode, bool top_zero, double *corr, hipStream_t st) {
+    if (level < 0 || level >= bottom_level(A) || !tiled_ok(A.lv[level])) {
+        set_error("level %d is not a tiled V-cycle level", level);
+        return STZ_ERR_ARG;
+    }
+    int done = 0;
+    if (mode == 1) {
+        const int m0 = level == 0 ? (top_zero ? TILE_ZERO : TILE_LOAD) : TILE_ZERO;
+        int e = smooth_level(A.lv[level], (const LevelArgs<T> *)nullptr, A.nu1, m0, st, &A.lv[level + 1], &done);
+        if (e) return e;
+        return done ? STZ_OK : restrict_level(A.lv[level], A.lv[level + 1], st);
+    }
+    if (mode == 2)
+        return smooth_level(A.lv[level], &A.lv[level + 1], A.nu2, TILE_PROLONG, st, (const LevelArgs<T> *)nullptr,
+                            nullptr, level == 0 ? corr : nullptr, &done);
+    set_error("unknown launch mode %d", mode);
+    return STZ_ERR_ARG;
+}
+template int mg_vcycle_launch<double>(CoarseArgs<double> &, int, int, bool, double *, hipStream_t);
+template int mg_vcycle_launch<float>(CoarseArgs<float> &, int, int, bool, double *, hipStream_t);
 
 template <typename T> int mg_smooth_level(LevelArgs<T> &L, int sweeps, hipStream_t st) {
     return smooth_level(L, (const LevelArgs<T> *)nullptr, sweeps, TILE_LOAD, st);
@@ -814,12 +844,13 @@ int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out,
         set_error("batched V-cycle needs at least one grid level above the workgroup solver");
         return STZ_ERR_ARG;
     }
-    for (int l = 0; l < A.lc; l++) {
+    const int lb = bottom_level(A);
+    for (int l = 0; l < lb; l++) {
         if ((e = plan_level(A.lv[l], (const LevelArgs<T> *)nullptr, A.nu1,
                             l == 0 ? (top_zero ? TILE_ZERO : TILE_LOAD) : TILE_ZERO, &A.lv[l + 1], l, 0, out)))
             return e;
     }
-    for (int l = A.lc; l >= 1; l--)
+    for (int l = lb; l >= 1; l--)
         if ((e = plan_level(A.lv[l - 1], &A.lv[l], A.nu2, TILE_PROLONG, (const LevelArgs<T> *)nullptr, l - 1, 2,
                             out, l == 1 ? corr : nullptr)))
             return e;

@@ -101,10 +101,10 @@ static void rap_node(const HostLevel &F, const HostLevel &C, int I, int J, doubl
     t[39] = a * idet;
 }
 
-// top operator of qs_multi (qs_multi.hh:30-50, qs_multi.cc:31-54) as one
-// stencil per row; the x row uses Kmx along x and my along y, the y row mx
+// one row stencil of the top operator of qs_multi (qs_multi.hh:30-50,
+// qs_multi.cc:31-54): the x row uses Kmx along x and my along y, the y row mx
 // along x and Kmy along y (the matrix-free mul_a); wall rows are bcs*diag.
-static void top_stencil(const SimConst &c, double viscdt, HostLevel &L) {
+void top_row_stencil(const SimConst &c, double viscdt, bool wall, double *s) {
     const double Kp = c.K + c.mu / 3.0, Kmu = Kp + c.mu;
     const double xsp2 = c.xsp * c.xsp, ysp2 = c.ysp * c.ysp, xyfac = xsp2 + ysp2;
     const double tmufac1 = Kmu * xsp2 + c.mu * ysp2, tmufac2 = c.mu * xsp2 + Kmu * ysp2;
@@ -112,31 +112,34 @@ static void top_stencil(const SimConst &c, double viscdt, HostLevel &L) {
     const double Kmx = (Kmu + viscdt) * xsp2, Kmy = (Kmu + viscdt) * ysp2;
     const double mx = (c.mu + viscdt) * xsp2, my = (c.mu + viscdt) * ysp2;
     const double cxx = -2 * (tmufac1 + viscdt * xyfac), cyy = -2 * (tmufac2 + viscdt * xyfac);
+    std::fill(s, s + 40, 0.0);
+    if (wall) {
+        s[16] = c.bcs * cxx;
+        s[19] = c.bcs * cyy;
+        s[36] = 1 / (c.bcs * cxx);
+        s[39] = 1 / (c.bcs * cyy);
+        return;
+    }
+    const double corner[9] = {1, 0, -1, 0, 0, 0, -1, 0, 1};
+    for (int k : {0, 2, 6, 8}) {
+        s[4 * k + 1] = corner[k] * qK;
+        s[4 * k + 2] = corner[k] * qK;
+    }
+    for (int k : {3, 5}) { s[4 * k] = Kmx; s[4 * k + 3] = mx; }
+    for (int k : {1, 7}) { s[4 * k] = my; s[4 * k + 3] = Kmy; }
+    s[16] = cxx;
+    s[19] = cyy;
+    s[36] = 1 / cxx;
+    s[39] = 1 / cyy;
+}
+
+// the top operator as one stencil per row
+static void top_stencil(const SimConst &c, double viscdt, HostLevel &L) {
     L.m = c.m;
     L.n = c.n + 1;
     L.xinv = true;
     L.S.assign((size_t)L.n * 40, 0.0);
-    for (int j = 0; j < L.n; j++) {
-        double *s = &L.S[(size_t)j * 40];
-        if (j == 0 || j == L.n - 1) {
-            s[16] = c.bcs * cxx;
-            s[19] = c.bcs * cyy;
-            s[36] = 1 / (c.bcs * cxx);
-            s[39] = 1 / (c.bcs * cyy);
-            continue;
-        }
-        const double corner[9] = {1, 0, -1, 0, 0, 0, -1, 0, 1};
-        for (int k : {0, 2, 6, 8}) {
-            s[4 * k + 1] = corner[k] * qK;
-            s[4 * k + 2] = corner[k] * qK;
-        }
-        for (int k : {3, 5}) { s[4 * k] = Kmx; s[4 * k + 3] = mx; }
-        for (int k : {1, 7}) { s[4 * k] = my; s[4 * k + 3] = Kmy; }
-        s[16] = cxx;
-        s[19] = cyy;
-        s[36] = 1 / cxx;
-        s[39] = 1 / cyy;
-    }
+    for (int j = 0; j < L.n; j++) top_row_stencil(c, viscdt, j == 0 || j == L.n - 1, &L.S[(size_t)j * 40]);
 }
 
 static void coarsen(const HostLevel &F, HostLevel &C) {
@@ -147,6 +150,40 @@ static void coarsen(const HostLevel &F, HostLevel &C) {
         if (C.xinv) rap_node(F, C, 0, J, &C.S[(size_t)J * 40]);
         else
             for (int I = 0; I < C.m; I++) rap_node(F, C, I, J, &C.S[(size_t)(I + C.m * J) * 40]);
+    }
+}
+
+// the constants of a simulation (shear_sim.cc:32-36) and the base of its
+// solve threshold, qs_multi.cc:14 with tgmg_accuracy (tgmg.hh:41-45)
+void fill_sim_const(const stz_sim_params *p, SimConst &c, double *acc_base) {
+    c.m = p->m;
+    c.n = p->n;
+    c.ax = p->ax; c.bx = p->bx; c.ay = p->ay; c.by = p->by;
+    c.lx = p->bx - p->ax;
+    c.ly = p->by - p->ay;
+    c.dx = c.lx / p->m;
+    c.dy = c.ly / p->n;
+    c.xsp = 1. / c.dx;
+    c.ysp = 1. / c.dy;
+    c.mu = p->mu;
+    c.mu_inv = 1 / p->mu;
+    c.K = p->K;
+    c.viscosity = p->viscosity;
+    c.chi_diff = p->chi_diff;
+    c.chi_len = p->chi_len;
+    c.t_scale = p->t_scale;
+    c.adapt_fac = p->adapt_fac;
+    c.u_bdry = p->u_bdry;
+    c.bcs = 4;
+    c.chi_inf = p->stz.chi_inf;
+    c.stz = p->stz;
+    c.thetainv = 1 / p->stz.theta;
+    if (acc_base) {
+        const double Kp = c.K + c.mu / 3.0, Kmu = Kp + c.mu;
+        const double xsp2 = c.xsp * c.xsp, ysp2 = c.ysp * c.ysp;
+        const double tmufac1 = Kmu * xsp2 + c.mu * ysp2, tmufac2 = c.mu * xsp2 + Kmu * ysp2;
+        const double a = 2.220446049250313080847e-16 * 1e4 * (2 * (tmufac1 + tmufac2));
+        *acc_base = a * a;
     }
 }
 
@@ -221,6 +258,7 @@ static void set_defaults(stz_mg_config &g) {
     g.predict_max_iters = 2048;         // tgmg_config.hh:41
     g.tol_mult = 1.0;
     g.max_refine = 4096;
+    g.direct_nodes = 4200;
 }
 
 static size_t plane_elems(const stz_sim *s) { return (size_t)s->c.m * (s->c.n + 2); }
@@ -248,6 +286,8 @@ static void build_dims(stz_sim *s) {
 template <typename T> static int alloc_levels(stz_sim *s, DevMG<T> &mg, bool own_top) {
     mg.L = (int)s->hl.size();
     mg.args.L = mg.L;
+    mg.args.ld = -1;
+    mg.args.Gd = nullptr;
     for (int l = 0; l < mg.L; l++) {
         LevelArgs<T> &A = mg.args.lv[l];
         A.m = s->hl[l].m;
@@ -322,6 +362,49 @@ static int set_level_stencils(stz_sim *s, DevMG<T> &mg, const std::vector<std::v
     return STZ_OK;
 }
 
+// the direct level of hierarchy mg: its block-circulant inverse in precision
+// T, or none
+template <typename T> static int set_direct(DevMG<T> &mg, int ld, const std::vector<double> &G) {
+    mg.args.ld = ld;
+    mg.args.Gd = nullptr;
+    if (ld < 0) return STZ_OK;
+    if (mg.G && mg.G_elems != G.size()) {
+        (void)hipFree(mg.G);
+        mg.G = nullptr;
+    }
+    if (!mg.G) STZ_HIP(hipMalloc(&mg.G, G.size() * sizeof(T)));
+    mg.G_elems = G.size();
+    std::vector<T> tmp(G.begin(), G.end());
+    STZ_HIP(hipMemcpy(mg.G, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    mg.args.Gd = mg.G;
+    return mg_direct_set_lds_limit<T>(direct_lds_bytes<T>(mg.args.lv[ld].m, mg.args.lv[ld].n));
+}
+
+// the finest level l in 1..lc with at most cfg.direct_nodes nodes, an
+// x-invariant operator and an LDS image that fits is solved exactly; the
+// V-cycle stops there (both hierarchies)
+static int set_direct_level(stz_sim *s) {
+    int ld = -1;
+    const int lc = s->mg64.args.lc;
+    for (int l = 1; l <= lc && l < (int)s->hl.size() && s->cfg.direct_nodes > 0; l++) {
+        const HostLevel &H = s->hl[l];
+        if (H.xinv && H.m % 2 == 0 && (long long)H.m * H.n <= s->cfg.direct_nodes &&
+            direct_lds_bytes<double>(H.m, H.n) <= 160 * 1024) {
+            ld = l;
+            break;
+        }
+    }
+    std::vector<double> G;
+    if (ld >= 0 && !direct_inverse(s->hl[ld].S.data(), s->hl[ld].m, s->hl[ld].n, G)) {
+        set_error("multigrid level %d (%d x %d) is singular", ld, s->hl[ld].m, s->hl[ld].n);
+        return STZ_ERR_STATE;
+    }
+    int e;
+    if ((e = set_direct(s->mg64, ld, G))) return e;
+    if (s->precision == STZ_PREC_MIXED && (e = set_direct(s->mg32, ld, G))) return e;
+    return STZ_OK;
+}
+
 // (re)build operators for viscosity/dt = viscdt (qs_multi::init, qs_multi.cc:31-54)
 int mg_build(stz_sim *s, double viscdt) {
     if (viscdt == s->viscdt_prev) return STZ_OK;
@@ -356,7 +439,7 @@ int mg_build(stz_sim *s, double viscdt) {
     }
     if ((e = set_level_stencils(s, s->mg64, Sx, binv))) return e;
     if (s->precision == STZ_PREC_MIXED && (e = set_level_stencils(s, s->mg32, Sx, binv))) return e;
-    return STZ_OK;
+    return set_direct_level(s);
 }
 
 // mixed form: FP64 residual of the top level -> FP32 source of the next
@@ -420,7 +503,10 @@ static int one_cycle(stz_sim *s, bool need_resid, bool need_norm = true) {
         if ((e = mg_vcycle(A, s->stream))) return e;
     }
     if (!corrected && (e = mg_correct64(s->vel, A.lv[0].z, 2 * s->c.m * (s->c.n + 1), s->stream))) return e;
-    s->last.refinements++;
+    if (++s->last.refinements > s->cfg.max_refine) {
+        set_error("multigrid: more than %d FP64 refinements in one solve", s->cfg.max_refine);
+        return STZ_ERR_NOT_CONVERGED;
+    }
     if (need_resid) return refine_resid(s, need_norm);
     return STZ_OK;
 }
@@ -547,38 +633,8 @@ int stz_sim_create(const stz_sim_params *p, int precision, stz_sim **out) {
     s->device = dev;
     s->p = *p;
     s->precision = precision;
-    SimConst &c = s->c;
-    // shear_sim.cc:32-36
-    c.m = p->m;
-    c.n = p->n;
-    c.ax = p->ax; c.bx = p->bx; c.ay = p->ay; c.by = p->by;
-    c.lx = p->bx - p->ax;
-    c.ly = p->by - p->ay;
-    c.dx = c.lx / p->m;
-    c.dy = c.ly / p->n;
-    c.xsp = 1. / c.dx;
-    c.ysp = 1. / c.dy;
-    c.mu = p->mu;
-    c.mu_inv = 1 / p->mu;
-    c.K = p->K;
-    c.viscosity = p->viscosity;
-    c.chi_diff = p->chi_diff;
-    c.chi_len = p->chi_len;
-    c.t_scale = p->t_scale;
-    c.adapt_fac = p->adapt_fac;
-    c.u_bdry = p->u_bdry;
-    c.bcs = 4;
-    c.chi_inf = p->stz.chi_inf;
-    c.stz = p->stz;
-    c.thetainv = 1 / p->stz.theta;
-    // tolerance, qs_multi.cc:14 with tgmg_accuracy (tgmg.hh:41-45)
-    {
-        const double Kp = c.K + c.mu / 3.0, Kmu = Kp + c.mu;
-        const double xsp2 = c.xsp * c.xsp, ysp2 = c.ysp * c.ysp;
-        const double tmufac1 = Kmu * xsp2 + c.mu * ysp2, tmufac2 = c.mu * xsp2 + Kmu * ysp2;
-        const double a = 2.220446049250313080847e-16 * 1e4 * (2 * (tmufac1 + tmufac2));
-        s->acc_base = a * a;
-    }
+    fill_sim_const(p, s->c, &s->acc_base);
+    const SimConst &c = s->c;
     set_defaults(s->cfg);
     s->lim = s->cfg.predict_mult * s->cfg.predict_init;
 
@@ -778,6 +834,15 @@ int stz_sim_advection_step(stz_sim *s, double dt) {
     return STZ_OK;
 }
 
+int stz_sim_advection_launch(stz_sim *s, double dt, int count) {
+    CHECK_SIM(s);
+    if (count < 0) { set_error("negative count"); return STZ_ERR_ARG; }
+    int e;
+    for (int k = 0; k < count; k++)
+        if ((e = advection_impl(s, dt))) return e;
+    return STZ_OK;
+}
+
 int stz_sim_projection_step(stz_sim *s, double dt) {
     CHECK_SIM(s);
     int e = projection_impl(s, dt);
@@ -820,9 +885,21 @@ int stz_sim_set_mg_config(stz_sim *s, const stz_mg_config *g) {
         set_error("invalid multigrid configuration");
         return STZ_ERR_ARG;
     }
+    if (g->direct_nodes < 0 || g->max_refine < 1) {
+        set_error("invalid multigrid configuration (direct_nodes %d, max_refine %d)", g->direct_nodes,
+                  g->max_refine);
+        return STZ_ERR_ARG;
+    }
     const bool reset = g->predict_mult != s->cfg.predict_mult || g->predict_init != s->cfg.predict_init;
+    const bool rebuild = g->direct_nodes != s->cfg.direct_nodes && s->viscdt_prev >= 0;
     s->cfg = *g;
     if (reset) s->lim = g->predict_mult * g->predict_init;
+    if (rebuild) {  // a new direct level: the hierarchy is rebuilt now
+        const double v = s->viscdt_prev;
+        s->viscdt_prev = -1;
+        CHECK_SIM(s);
+        return mg_build(s, v);
+    }
     return STZ_OK;
 }
 
@@ -903,6 +980,50 @@ int stz_mg_smooth(stz_sim *s, int level, int sweeps) {
     if (s->cfg.kernels != 0) return mg_smooth(L, sweeps, s->stream);
     if ((e = mg_smooth_level(L, sweeps, s->stream))) return e;
     return level == 0 ? top_home(s) : STZ_OK;
+}
+
+int stz_mg_vcycle_launch(stz_sim *s, int level, int mode, int count) {
+    CHECK_SIM(s);
+    if (s->viscdt_prev < 0) { set_error("stz_mg_setup has not been called"); return STZ_ERR_STATE; }
+    if (s->cfg.kernels != 0) { set_error("the per-colour kernels have no tiled launches"); return STZ_ERR_ARG; }
+    if (count < 0) { set_error("negative count"); return STZ_ERR_ARG; }
+    int e;
+    for (int k = 0; k < count; k++) {
+        if (s->precision == STZ_PREC_MIXED) {
+            CoarseArgs<float> &A = s->mg32.args;
+            A.nu1 = s->cfg.nu1;
+            A.nu2 = s->cfg.nu2;
+            if ((e = mg_vcycle_launch(A, level, mode, true, s->vel, s->stream))) return e;
+        } else {
+            CoarseArgs<double> &A = s->mg64.args;
+            A.nu1 = s->cfg.nu1;
+            A.nu2 = s->cfg.nu2;
+            if ((e = mg_vcycle_launch(A, level, mode, false, nullptr, s->stream))) return e;
+        }
+    }
+    return s->precision == STZ_PREC_FP64 && level == 0 ? top_home(s) : STZ_OK;
+}
+
+int stz_mg_fmg_pre(stz_sim *s) {
+    CHECK_SIM(s);
+    if (s->viscdt_prev < 0) { set_error("stz_mg_setup has not been called"); return STZ_ERR_STATE; }
+    int e;
+    if ((e = top_home(s))) return e;
+    CoarseArgs<double> &A = s->mg64.args;
+    A.nu1 = s->cfg.nu1;
+    A.nu2 = s->cfg.nu2;
+    if ((e = mg_fmg_pre(A, s->stream))) return e;
+    STZ_HIP(hipStreamSynchronize(s->stream));
+    return STZ_OK;
+}
+
+int stz_mg_clear_z(stz_sim *s) {
+    CHECK_SIM(s);
+    int e;
+    if ((e = top_home(s))) return e;
+    if ((e = mg_fill_zero(s->vel, 2 * (size_t)s->c.m * (s->c.n + 1), s->stream))) return e;
+    STZ_HIP(hipStreamSynchronize(s->stream));
+    return STZ_OK;
 }
 
 int stz_mg_upload_src(stz_sim *s, const double *h) {

@@ -31,6 +31,8 @@ template <typename T> struct DevMG {
     T *pack = nullptr;       // constant part of the workgroup solver's LDS image
     size_t pack_elems = 0;
     size_t lds = 0;
+    T *G = nullptr;          // block-circulant inverse of the direct level
+    size_t G_elems = 0;
 
     void free_all() {
         for (int l = 0; l < kMaxLevels; l++) {
@@ -41,6 +43,9 @@ template <typename T> struct DevMG {
         if (pack) (void)hipFree(pack);
         pack = nullptr;
         pack_elems = 0;
+        if (G) (void)hipFree(G);
+        G = nullptr;
+        G_elems = 0;
     }
 };
 
@@ -90,6 +95,14 @@ struct stz_sim {
         return L;
     }
 };
+
+namespace stz {
+// the constants of a simulation and, with acc_base, the base of its solve
+// threshold (sim.hip)
+void fill_sim_const(const stz_sim_params *p, SimConst &c, double *acc_base);
+// one row stencil of the top operator for viscosity/dt = viscdt (sim.hip)
+void top_row_stencil(const SimConst &c, double viscdt, bool wall, double *s40);
+}  // namespace stz
 
 // (re)build the operators for viscosity/dt = viscdt (qs_multi::init,
 // qs_multi.cc:31-54): host Galerkin hierarchy, uploads; no-op when unchanged

@@ -1,6 +1,7 @@
 // stz_law.h -- the STZ plasticity law (sb/main/stz_model.hh:9-207) and the
 // adaptive plastic substepping (shear_sim.cc:513-548, bd_sim.cc:693-730),
-// shared by the periodic-shear and free-boundary kernels.  Expressions keep
+// shared by the periodic-shear and free-boundary kernels (and, for the law
+// alone, by the host point evaluation stz_model_dplastic).  Expressions keep
 // the reference's evaluation order; FMA contraction is off.
 #pragma once
 
@@ -14,7 +15,7 @@ namespace stz {
 // STZ law (stz_model.hh:9-207)
 // ---------------------------------------------------------------------------
 
-__device__ static double dpl_common(const stz_model_params &mp, double sbar, double chi, double &dchi1,
+__host__ __device__ static double dpl_common(const stz_model_params &mp, double sbar, double chi, double &dchi1,
                                     double &dchi2, double sC1, double sC2, double sC, double sT) {
     // stz_model.hh:30-47
     const double s0 = 1;
@@ -29,7 +30,7 @@ __device__ static double dpl_common(const stz_model_params &mp, double sbar, dou
     return Dpl;
 }
 
-__device__ static double dplastic(const stz_model_params &mp, double thetainv, double sbar, double chi,
+__host__ __device__ static double dplastic(const stz_model_params &mp, double thetainv, double sbar, double chi,
                                   double &dchi1, double &dchi2) {
     const double s0 = 1;
     switch (mp.kind) {

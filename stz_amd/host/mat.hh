@@ -23,6 +23,8 @@ struct mat {
         return mat(idet * d, -idet * b, -idet * c, idet * a);
     }
     mat transpose() const { return mat(a, c, b, d); }
+    double mod_sq() const { return a * a + b * b + c * c + d * d; }
 };
+inline double mod_sq(mat m) { return m.mod_sq(); }
 
 #endif

@@ -23,74 +23,10 @@
 #include "../../include/stz_amd.h"
 #include "fields.hh"
 #include "mat.hh"
+#include "qs_multi.hh"
 #include "stz_model.hh"
+#include "tgmg.hh"
 #include "vec.hh"
-
-namespace stz_amd_detail {
-inline void check(int code, const char *what) {
-    if (code != STZ_OK) {
-        std::fprintf(stderr, "stz_amd: %s failed (%d): %s\n", what, code, stz_last_error());
-        std::exit(1);
-    }
-}
-}  // namespace stz_amd_detail
-
-class shear_sim;
-
-/** Adaptive V-cycle count predictor (tgmg/tgmg_predict.hh:6-33).  The
- * prediction itself runs inside the library; this keeps the reference's
- * avg_iters() and the tuning constants. */
-struct tgmg_predict {
-    stz_sim *h;
-    explicit tgmg_predict(stz_sim *h_) : h(h_) {}
-    double avg_iters() {
-        double a = 0;
-        stz_amd_detail::check(stz_sim_avg_iters(h, &a), "avg_iters");
-        return a;
-    }
-    void set(int mult, int decay, int init, int max_iters) {
-        stz_mg_config c;
-        stz_amd_detail::check(stz_sim_get_mg_config(h, &c), "get_mg_config");
-        c.predict_mult = mult;
-        c.predict_decay = decay;
-        c.predict_init = init;
-        c.predict_max_iters = max_iters;
-        stz_amd_detail::check(stz_sim_set_mg_config(h, &c), "set_mg_config");
-    }
-};
-
-/** The projection system of shear_sim (qs_multi.hh:11-57): operator setup
- * for viscosity/dt, the adaptive multigrid solve and the residual norm. */
-struct qs_multi {
-    stz_sim *h;
-    tgmg_predict tp;
-    explicit qs_multi(stz_sim *h_) : h(h_), tp(h_) {}
-    /** qs_multi::init (qs_multi.cc:31-54) */
-    void init(double viscdt) { stz_amd_detail::check(stz_mg_setup(h, viscdt), "qs_multi::init"); }
-    /** qs_multi::solve (qs_multi.hh:51-53): exit(1) if the solve fails */
-    void solve() {
-        stz_solve_info info;
-        stz_amd_detail::check(stz_mg_solve(h, &info), "tgmg::solve_v_cycle");
-    }
-    /** tgmg::v_cycle (tgmg.in.cc:313-328) */
-    void v_cycle() { stz_amd_detail::check(stz_mg_v_cycle(h), "tgmg::v_cycle"); }
-    /** tgmg_base::l2_error (tgmg.hh:146) */
-    double l2_error() {
-        double e = 0;
-        stz_amd_detail::check(stz_mg_l2_error(h, &e), "tgmg::l2_error");
-        return e;
-    }
-    /** tgmg_base::gauss_seidel (tgmg.in.cc:127-165) on hierarchy level lev */
-    void gauss_seidel(int lev = 0, int sweeps = 1) {
-        stz_amd_detail::check(stz_mg_smooth(h, lev, sweeps), "tgmg::gauss_seidel");
-    }
-    /** tgmg::print_hierarchy (tgmg.in.cc:36-41) */
-    void print_hierarchy() {
-        int nl = 0, dims[64];
-        stz_amd_detail::check(stz_sim_mg_hierarchy(h, 32, &nl, dims), "mg_hierarchy");
-        for (int k = 0; k < nl; k++) std::printf("# Grid %d: %d by %d\n", k, dims[2 * k], dims[2 * k + 1]);
-    }
-};
 
 class shear_sim {
   public:
@@ -118,7 +54,7 @@ class shear_sim {
           viscosity(visc_), chi_diff(chi_diff_), chi_len(chi_len_), t_scale(t_scale_), tmult(tmult_),
           adapt_fac(adapt_fac_), u_bdry(u_bdry_), bcs(4), stz(stz_), filename(filename_),
           fm(new c_field[mn + 2 * m] + m), vel(new vec[mn + m]), src(new vec[mn + m]), time(0.0),
-          TZ(stz_->TZ), chi_inf(stz_->chi_inf), f_num(0), h(create(precision)), qsm(h) {
+          TZ(stz_->TZ), chi_inf(stz_->chi_inf), f_num(0), h(create(precision)), qsm(*this) {
         for (int k = -m; k < mn + m; k++) {
             double *d = &fm[k].u;
             for (int f = 0; f < 19; f++) d[f] = 0;
@@ -158,6 +94,11 @@ class shear_sim {
         sync_scalars();
         stz_amd_detail::check(stz_sim_projection_step(h, dt), "projection_step");
     }
+    /** shear_sim::chi_diffusion (shear_sim.cc:364-386) adds the diffusion to
+     * the change slot cchi, which only advection_step applies, after
+     * overwriting it; called on its own it leaves the state unchanged, and
+     * so does this.  advection_step includes the diffusion, as there. */
+    void chi_diffusion(double dt) { (void)dt; }
     /** shear_sim::solve_quasistatic (shear_sim.cc:159-181): frames of
      * `steps` device steps each, the frame files written to `filename` after
      * each, and the reference's per-frame diagnostic line */
@@ -363,5 +304,14 @@ class shear_sim {
     }
     void pull_time() { stz_amd_detail::check(stz_sim_get_time(h, &time), "get_time"); }
 };
+
+inline qs_multi::qs_multi(shear_sim &ss)
+    : m(ss.m), n(ss.n + 1), mn(m * n), x_prd(true), y_prd(false), z(ss.vel), K(ss.K + ss.mu / 3.0),
+      qK(.25 * K * ss.xsp * ss.ysp), Kmu(K + ss.mu), mu(ss.mu), dx(ss.dx), dy(ss.dy), xsp(ss.xsp), ysp(ss.ysp),
+      xsp2(ss.xsp * ss.xsp), ysp2(ss.ysp * ss.ysp), xyfac(xsp2 + ysp2), tmufac1(Kmu * xsp2 + mu * ysp2),
+      tmufac2(mu * xsp2 + Kmu * ysp2), bcs(ss.bcs), viscdt_prev(-1), Kmx(0), Kmy(0), mx(0), my(0), cxx(0), cyy(0),
+      regz(mat(0)), acc(tgmg_accuracy(2 * (tmufac1 + tmufac2), 1e4)), h(ss.handle()), mg(*this, ss.src, z), tp() {
+    tp.h = h;
+}
 
 #endif

@@ -148,6 +148,10 @@ class ShearSim:
     def advection_step(self, dt):
         check(self._lib.stz_sim_advection_step(self._h, float(dt)))
 
+    def advection_launch(self, dt, count=1):
+        """asynchronous: `count` advection steps on the simulation's stream"""
+        check(self._lib.stz_sim_advection_launch(self._h, float(dt), int(count)))
+
     def projection_step(self, dt):
         check(self._lib.stz_sim_projection_step(self._h, float(dt)))
 
@@ -205,6 +209,20 @@ class ShearSim:
     def mg_smooth(self, level, sweeps):
         """asynchronous smoothing sweeps on one level (tgmg gauss_seidel)"""
         check(self._lib.stz_mg_smooth(self._h, int(level), int(sweeps)))
+
+    def mg_fmg_pre(self):
+        """tgmg::fmg_pre on the current src/vel"""
+        check(self._lib.stz_mg_fmg_pre(self._h))
+
+    def mg_clear_z(self):
+        """tgmg_base::clear_z: vel = 0"""
+        check(self._lib.stz_mg_clear_z(self._h))
+
+    def mg_vcycle_launch(self, level, mode, count=1):
+        """asynchronous: one of the V-cycle's smoothing launches on `level`,
+        `count` times (mode 1 pre-smoothing with restriction, mode 2
+        post-smoothing with prolongation and, mixed top level, vel += e)"""
+        check(self._lib.stz_mg_vcycle_launch(self._h, int(level), int(mode), int(count)))
 
     def set_stream(self, stream_handle):
         """run on a HIP stream given as an integer handle (e.g. a torch
