@@ -291,12 +291,28 @@ enum {
     STZ_D_CELL_ONESIDED = 1u << 19, /* staggered y derivative one-sided (wall) */
     STZ_D_YIELD = 1u << 20,      /* Dplastic took the sbar>=1 branch on entry */
     STZ_D_SMALLDEV = 1u << 21,   /* sbar < 1e-12 cut-off (shear_sim.cc:522) */
+    STZ_D_YIELD_LAST = 1u << 22, /* the last plastic loop trip took the sbar>=1 branch */
     STZ_D_SUBSTEP_SHIFT = 24     /* bits 24..31: plastic loop trips, saturating */
 };
 
 /* Recording is off by default (it costs one extra store per cell). */
 int stz_sim_record_decisions(stz_sim *sim, int enable);
 int stz_sim_download_decisions(stz_sim *sim, uint32_t *host_cells);
+
+/* Frozen-decision forward: the next nsteps advection steps of this
+ * simulation (stz_sim_advection_step, stz_sim_step_quasistatic, the
+ * launch forms) take every decision listed above from host_cells instead of
+ * evaluating it, step k from host_cells[k*m*n ..] in the layout of
+ * stz_sim_download_decisions; a cell whose substep count is 255 (saturated)
+ * evaluates its plastic loop as usual.  Replaying the record of a run from
+ * the same state repeats that run bit for bit; from a nearby state it gives
+ * the smooth continuation of the recorded branches, the function whose
+ * derivative the adjoint (stz_adjoint_*) computes.  A step after the last
+ * recorded one fails with STZ_ERR_STATE; nsteps = 0 ends the replay.  The
+ * record is copied to the device before the call returns.  No counterpart in
+ * the reference (which evaluates every decision, shear_sim.cc:296-349,
+ * 513-548, 653-655). */
+int stz_sim_replay_decisions(stz_sim *sim, int nsteps, const uint32_t *host_cells);
 
 /* ======================================================================== *
  * Free-boundary simulation: a body bounded by the zero contour of a level
@@ -435,10 +451,11 @@ int stz_bd_set_edge_velocity(stz_bd *b, double u_left, double u_right);
  * Every member computes exactly what stz_sim_step_quasistatic computes for
  * it alone.  Requirements: all members have the same stz_mg_config, with
  * kernels = 0, nu1 >= 1 and ceil(nu1/2) + ceil(nu2/2) even (nu1 = nu2 is
- * always fine), and every multigrid level above the workgroup solver is
- * covered by the tiled kernels (sides a multiple of 32 and at most 512 in
- * the FP64 form); stz_ensemble_step_quasistatic reports anything else as
- * STZ_ERR_ARG. */
+ * always fine), and every multigrid level a member's V-cycle smooths (the
+ * levels above its direct level, or above its workgroup solver) is covered
+ * by the tiled kernels (widths a multiple of 32: true for every side in
+ * {64, 128, 256, 384, 512}); stz_ensemble_step_quasistatic reports anything
+ * else as STZ_ERR_ARG. */
 typedef struct stz_ensemble stz_ensemble;   /* opaque */
 
 /* count simulations, member k from params[k] (as stz_sim_create). */

@@ -152,15 +152,18 @@ double orc_dplastic(const stz_model_params *mp, double sbar, double chi,
     }
 }
 
-/* shear_sim.cc:513-548 (adaptive_plastic_term); trips = loop iterations */
+/* shear_sim.cc:513-548 (adaptive_plastic_term); trips = loop iterations,
+ * last_yield = the yield test sbar >= 1 (stz_model.hh:108,185) on the last */
 static double plastic_term(const orc_sim *s, double sbar, double *chiv, double *ddev,
-                           double dt, int *trips) {
+                           double dt, int *trips, int *last_yield) {
     const double small_dev_cutoff = 1e-12;
     const double theta = 0;
     int adapt = 1, t = 0;
     double dtl = dt, osbar = sbar, dplas, dchi1, dchi2, adt;
+    *last_yield = 0;
     if (sbar < small_dev_cutoff) { *ddev = 0; *trips = 0; return 0; }
     do {
+        *last_yield = !(sbar < 1);
         dplas = 2 * s->p.t_scale * s->p.mu * orc_dplastic(&s->p.stz, sbar, *chiv, &dchi1, &dchi2);
         if (fabs(dtl * dplas) > s->p.adapt_fac) {
             adt = s->p.adapt_fac / fabs(dplas);
@@ -179,7 +182,8 @@ static double plastic_term(const orc_sim *s, double sbar, double *chiv, double *
 }
 
 double orc_plastic_term(orc_sim *s, double sbar, double *chiv, double *ddev, double dt, int *trips) {
-    return plastic_term(s, sbar, chiv, ddev, dt, trips);
+    int last_yield;
+    return plastic_term(s, sbar, chiv, ddev, dt, trips, &last_yield);
 }
 
 /* shear_sim.cc:653-655 */
@@ -530,12 +534,14 @@ void orc_advection_step(orc_sim *s, double dt) {
         {
             double cchi = C[ij] - uc * f1[4] - vc * f2[4];
             double sbar = sqrt(3 * Q[ij] * Q[ij] + S[ij] * S[ij] + T[ij] * T[ij]);
-            int trips;
-            dtmp = plastic_term(s, sbar, &cchi, &D[ij], dt, &trips);
+            int trips, last_yield;
+            const int athermal = s->p.stz.kind == STZ_MODEL_LINEAR_ATHERMAL ||
+                                 s->p.stz.kind == STZ_MODEL_NONLINEAR_ATHERMAL;
+            dtmp = plastic_term(s, sbar, &cchi, &D[ij], dt, &trips, &last_yield);
             s->cchi[ij] = cchi;
             if (sbar < 1e-12) bits |= STZ_D_SMALLDEV;
-            else if (!(sbar < 1) && (s->p.stz.kind == STZ_MODEL_LINEAR_ATHERMAL ||
-                                     s->p.stz.kind == STZ_MODEL_NONLINEAR_ATHERMAL)) bits |= STZ_D_YIELD;
+            else if (!(sbar < 1) && athermal) bits |= STZ_D_YIELD;
+            if (athermal && last_yield) bits |= STZ_D_YIELD_LAST;
             bits |= (uint32_t)(trips > 255 ? 255 : trips) << STZ_D_SUBSTEP_SHIFT;
         }
         s->cp[ij] = -uc * f1[0] - vc * f2[0];
@@ -1185,8 +1191,8 @@ void orc_step_forward(orc_sim *s, double dt) {
         {
             double cchi = C[ij] - uc * f1[4] - vc * f2[4] + hxxcd * (C[r] + C[l]) + hyycd * (C[u] + C[d]) - hccd * C[ij];
             double sbar = sqrt(3 * Q[ij] * Q[ij] + S[ij] * S[ij] + T[ij] * T[ij]);
-            int trips;
-            dtmp = plastic_term(s, sbar, &cchi, &D[ij], dt, &trips);
+            int trips, last_yield;
+            dtmp = plastic_term(s, sbar, &cchi, &D[ij], dt, &trips, &last_yield);
             s->cchi[ij] = cchi;
         }
         s->cp[ij] = -uc * f1[0] - vc * f2[0] - K * (ux + vy);

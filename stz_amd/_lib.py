@@ -140,6 +140,7 @@ _SIGS = {
     "stz_sim_initialize_chi_bicubic": [C.c_void_p, C.c_int, C.c_int, _dp, C.c_double, C.c_double],
     "stz_sim_record_decisions": [C.c_void_p, C.c_int],
     "stz_sim_download_decisions": [C.c_void_p, _up],
+    "stz_sim_replay_decisions": [C.c_void_p, C.c_int, _up],
     "stz_bd_create": [C.POINTER(BdParams), C.POINTER(C.c_void_p)],
     "stz_bd_destroy": [C.c_void_p],
     "stz_bd_set_stream": [C.c_void_p, C.c_void_p],

@@ -221,6 +221,12 @@ int stz_adjoint_forward(stz_adjoint *a, double dt, int nsteps) {
         return STZ_ERR_ARG;
     }
     if (s->cfg.kernels != 0 && s->cfg.kernels != 1) { set_error("bad kernel setting"); return STZ_ERR_ARG; }
+    if (s->rep) {
+        // the adjoint's own forward and recomputation take their decisions
+        // from the state, as the steps it differentiates did
+        set_error("a decision replay is active on this simulation; end it first");
+        return STZ_ERR_STATE;
+    }
     const int nck = (nsteps + a->every - 1) / a->every;
     if (a->ckpt) (void)hipFree(a->ckpt);
     a->ckpt = nullptr;
@@ -258,6 +264,7 @@ int stz_adjoint_backward(stz_adjoint *a, const double *lam_final, double *lam_in
     if (!a || !lam_final || !lam_init || !grad3) { set_error("null argument"); return STZ_ERR_ARG; }
     if (a->nsteps < 1 || !a->ckpt) { set_error("stz_adjoint_forward has not run"); return STZ_ERR_STATE; }
     stz_sim *s = a->s;
+    if (s->rep) { set_error("a decision replay is active on this simulation; end it first"); return STZ_ERR_STATE; }
     STZ_HIP(hipSetDevice(s->device));
     hipStream_t st = s->stream;
     const double t0 = now();

@@ -82,7 +82,8 @@ template <typename T> struct CoarseEntry {
 template <typename T> struct DirectEntry {
     int member;
     int M, Nn;
-    const T *G, *b;
+    T *D;          // the level's mode inverses, twiddles and mode scratch
+    const T *b;
     T *z;
 };
 
@@ -135,11 +136,13 @@ int mg_vcycle_plan(CoarseArgs<T> A, bool top_zero, std::vector<TileRec<T>> &out,
 int batch_refine(const RefineEntry *E, const int *map, int ne, int total, int sparse, const EnsState *S,
                  hipStream_t st);
 template <typename T> int batch_coarse(const CoarseEntry<T> *E, int ne, size_t lds, const EnsState *S, hipStream_t st);
-// direct-level solves; direct_workgroups(M, Nn) workgroups per entry
+// direct-level solves: a mode launch (direct_mode_workgroups(M, Nn)
+// workgroups per entry) and an inverse launch (direct_inverse_workgroups)
 template <typename T>
-int batch_direct(const DirectEntry<T> *E, const int *map, int ne, int total, size_t lds, const EnsState *S,
-                 hipStream_t st);
-int direct_workgroups(int M, int Nn);
+int batch_direct(const DirectEntry<T> *E, const int *mode_map, int mode_total, const int *inv_map, int inv_total,
+                 int ne, size_t lds, const EnsState *S, hipStream_t st);
+int direct_mode_workgroups(int M, int Nn);
+int direct_inverse_workgroups(int M, int Nn);
 template <typename T> int batch_set_lds_limit(size_t bytes);
 // fp64: |r|^2 partials of the top level, blocks as mg_resid_norm
 int batch_resid_norm(const RefineEntry *E, const int *map, int ne, int total, const EnsState *S, hipStream_t st);

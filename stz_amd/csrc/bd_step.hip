@@ -163,7 +163,8 @@ __global__ void __launch_bounds__(kThreads) advect_cells_kernel(BdConst c, const
     double dtmp = sqrt(3 * f.q * f.q + f.s * f.s + f.tau * f.tau);
     double cchi = f.chi, ddev;
     int trips;
-    dtmp = plastic_term(c, dtmp, cchi, ddev, dt, trips);
+    bool last_yield;
+    dtmp = plastic_term(c, dtmp, cchi, ddev, dt, trips, last_yield);
     cchi -= uc * f1.chi + vc * f2.chi;
     const double cp = -uc * f1.p - vc * f2.p + pfil, cq = -uc * f1.q - vc * f2.q - dtmp * f.q + qfil,
                  cs = -uc * f1.s - vc * f2.s - two_omega * f.tau - dtmp * f.s + sfil,

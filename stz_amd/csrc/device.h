@@ -43,8 +43,9 @@ struct SimConst {
 };
 
 // launchers (explicit.hip)
+// rep: a decision record to replay (one uint32 per cell), or null to evaluate
 int launch_advect(const SimConst &c, const FieldPtrs &old_f, const FieldPtrs &new_f, double dt,
-                  uint32_t *dec, hipStream_t st);
+                  uint32_t *dec, hipStream_t st, const uint32_t *rep = nullptr);
 int launch_direct(const SimConst &c, const FieldPtrs &old_f, const FieldPtrs &new_f, double *un, double *vn,
                   double dt, hipStream_t st);
 int launch_chi_diffusion(const SimConst &c, const FieldPtrs &old_f, const FieldPtrs &new_f, double dt,

@@ -60,6 +60,8 @@ template <typename T> struct VCyclePlan {
     int ncoarse = 0;
     size_t lds = 0;
     DevTable<DirectEntry<T>> direct;   // members whose V-cycle stops at a direct level
+    int *dinv_map = nullptr;           // and the workgroups of their inverse launch
+    int dinv_total = 0;
     size_t dlds = 0;
 };
 
@@ -194,7 +196,7 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
     std::map<Pos, std::vector<std::pair<TileRec<T>, int>>> groups;
     std::vector<CoarseEntry<T>> coarse;
     std::vector<DirectEntry<T>> direct;
-    std::vector<int> dwgs;
+    std::vector<int> dwgs, iwgs;
     size_t lds = 0, dlds = 0;
     int rc;
     for (int k = 0; k < e->nmem(); k++) {
@@ -220,11 +222,12 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
             de.member = k;
             de.M = A.lv[A.ld].m;
             de.Nn = A.lv[A.ld].n;
-            de.G = A.Gd;
+            de.D = A.Gd;
             de.b = A.lv[A.ld].b;
             de.z = A.lv[A.ld].z;
             direct.push_back(de);
-            dwgs.push_back(direct_workgroups(de.M, de.Nn));
+            dwgs.push_back(direct_mode_workgroups(de.M, de.Nn));
+            iwgs.push_back(direct_inverse_workgroups(de.M, de.Nn));
             dlds = std::max(dlds, direct_lds_bytes<T>(de.M, de.Nn));
             continue;
         }
@@ -308,6 +311,7 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
     }
     if (!direct.empty()) {
         if ((rc = make_table(e, direct, dwgs, P.direct))) return rc;
+        if ((rc = make_map(e, iwgs, &P.dinv_map, &P.dinv_total))) return rc;
         P.dlds = dlds;
         if ((rc = mg_direct_set_lds_limit<T>(dlds))) return rc;
     }
@@ -466,7 +470,8 @@ template <typename T> int launch_vcycle(stz_ensemble *e, const VCyclePlan<T> &P)
         if ((rc = batch_tile<T>(l.key, l.t.E, l.t.map, l.t.ne, l.t.total, e->dst, e->stream))) return rc;
     if ((rc = batch_coarse<T>(P.coarse, P.ncoarse, P.lds, e->dst, e->stream))) return rc;
     const DevTable<DirectEntry<T>> &D = P.direct;
-    if ((rc = batch_direct<T>(D.E, D.map, D.ne, D.total, P.dlds, e->dst, e->stream))) return rc;
+    if ((rc = batch_direct<T>(D.E, D.map, D.total, P.dinv_map, P.dinv_total, D.ne, P.dlds, e->dst, e->stream)))
+        return rc;
     for (const TileLaunch<T> &l : P.up)
         if ((rc = batch_tile<T>(l.key, l.t.E, l.t.map, l.t.ne, l.t.total, e->dst, e->stream))) return rc;
     return STZ_OK;

@@ -25,22 +25,36 @@ namespace stz {
 
 #define PL(F, i, j) (F)[(i) + m * ((j) + 1)]
 
-// shear_sim.cc:653-655; br = 1 when the one-sided second-order branch is taken
-__device__ __forceinline__ static double eno2(double p0, double p1, double p2, double p3, uint32_t &br) {
-    bool b = fabs(p0 - 2 * p1 + p2) > fabs(p1 - 2 * p2 + p3);
+// shear_sim.cc:653-655; br = 1 when the one-sided second-order branch is taken.
+// force >= 0 takes that branch (1) or the centred one (0) without the test.
+__device__ __forceinline__ static double eno2(double p0, double p1, double p2, double p3, uint32_t &br,
+                                              int force = -1) {
+    bool b = force < 0 ? fabs(p0 - 2 * p1 + p2) > fabs(p1 - 2 * p2 + p3) : force != 0;
     br = b ? 1u : 0u;
     return b ? 3 * p1 - 4 * p2 + p3 : p0 - p2;
 }
 
 // ---------------------------------------------------------------------------
 // advection + plasticity: shear_sim.cc:296-349 (+ update, :356-360)
+//
+// REPLAY: every decision (upwind sides, ENO branches, the wall rules that
+// follow from the upwind side, the plastic cut-off, substep count and last
+// yield branch) is read from rep, a record of the same cell written by an
+// earlier step, instead of being evaluated.  A substep count of 255 means the
+// record saturated; such a cell evaluates its plastic loop as usual.
 // ---------------------------------------------------------------------------
+template <bool REPLAY>
 __device__ __forceinline__ static void advect_body(const SimConst &c, const FieldPtrs &o, const FieldPtrs &w,
-                                                   double dt, uint32_t *dec, int bx, int by) {
+                                                   double dt, uint32_t *dec, const uint32_t *rep, int bx,
+                                                   int by) {
     const int m = c.m, n = c.n;
     const int i = bx * blockDim.x + threadIdx.x;
     const int j = by * blockDim.y + threadIdx.y;
     if (i >= m || j >= n) return;
+    const uint32_t rb = REPLAY ? rep[i + m * j] : 0u;
+    // the recorded branch of decision bit b, or -1 (evaluate) without replay
+#define RB(b) (REPLAY ? (int)((rb >> (b)) & 1u) : -1)
+#define UP(natural, mask) (REPLAY ? (rb & (mask)) != 0 : (natural))
     const double hx = 0.5 * c.xsp * dt, hy = 0.5 * c.ysp * dt, lx = c.lx;
     const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
     const int ill = i <= 1 ? i + m - 2 : i - 2, irr = i >= m - 2 ? i + 2 - m : i + 2;
@@ -52,26 +66,26 @@ __device__ __forceinline__ static void advect_body(const SimConst &c, const Fiel
         uc = PL(o.u, i, j);
         vc = PL(o.v, i, j);
         const double X0 = PL(o.X, i, j), Y0 = PL(o.Y, i, j);
-        if (uc > 0) {
+        if (UP(uc > 0, STZ_D_NODE_UPX)) {
             bits |= STZ_D_NODE_UPX;
             Xx = hx * eno2(PL(o.X, ir, j) + (i == m - 1 ? lx : 0), X0, PL(o.X, il, j) + (i == 0 ? -lx : 0),
-                           PL(o.X, ill, j) + (i <= 1 ? -lx : 0), br);
+                           PL(o.X, ill, j) + (i <= 1 ? -lx : 0), br, RB(2));
             bits |= br ? STZ_D_NODE_ENO_XX : 0;
-            Yx = hx * eno2(PL(o.Y, ir, j), Y0, PL(o.Y, il, j), PL(o.Y, ill, j), br);
+            Yx = hx * eno2(PL(o.Y, ir, j), Y0, PL(o.Y, il, j), PL(o.Y, ill, j), br, RB(3));
             bits |= br ? STZ_D_NODE_ENO_YX : 0;
         } else {
             Xx = -hx * eno2(PL(o.X, il, j) + (i == 0 ? -lx : 0), X0, PL(o.X, ir, j) + (i == m - 1 ? lx : 0),
-                            PL(o.X, irr, j) + (i >= m - 2 ? lx : 0), br);
+                            PL(o.X, irr, j) + (i >= m - 2 ? lx : 0), br, RB(2));
             bits |= br ? STZ_D_NODE_ENO_XX : 0;
-            Yx = -hx * eno2(PL(o.Y, il, j), Y0, PL(o.Y, ir, j), PL(o.Y, irr, j), br);
+            Yx = -hx * eno2(PL(o.Y, il, j), Y0, PL(o.Y, ir, j), PL(o.Y, irr, j), br, RB(3));
             bits |= br ? STZ_D_NODE_ENO_YX : 0;
         }
-        if (vc > 0) {
+        if (UP(vc > 0, STZ_D_NODE_UPY)) {
             bits |= STZ_D_NODE_UPY;
             if (j > 1) {
-                Xy = hy * eno2(PL(o.X, i, j + 1), X0, PL(o.X, i, j - 1), PL(o.X, i, j - 2), br);
+                Xy = hy * eno2(PL(o.X, i, j + 1), X0, PL(o.X, i, j - 1), PL(o.X, i, j - 2), br, RB(4));
                 bits |= br ? STZ_D_NODE_ENO_XY : 0;
-                Yy = hy * eno2(PL(o.Y, i, j + 1), Y0, PL(o.Y, i, j - 1), PL(o.Y, i, j - 2), br);
+                Yy = hy * eno2(PL(o.Y, i, j + 1), Y0, PL(o.Y, i, j - 1), PL(o.Y, i, j - 2), br, RB(5));
                 bits |= br ? STZ_D_NODE_ENO_YY : 0;
             } else {
                 const double hs = hy * 2;
@@ -81,9 +95,9 @@ __device__ __forceinline__ static void advect_body(const SimConst &c, const Fiel
             }
         } else {
             if (j < n - 1) {
-                Xy = -hy * eno2(PL(o.X, i, j - 1), X0, PL(o.X, i, j + 1), PL(o.X, i, j + 2), br);
+                Xy = -hy * eno2(PL(o.X, i, j - 1), X0, PL(o.X, i, j + 1), PL(o.X, i, j + 2), br, RB(4));
                 bits |= br ? STZ_D_NODE_ENO_XY : 0;
-                Yy = -hy * eno2(PL(o.Y, i, j - 1), Y0, PL(o.Y, i, j + 1), PL(o.Y, i, j + 2), br);
+                Yy = -hy * eno2(PL(o.Y, i, j - 1), Y0, PL(o.Y, i, j + 1), PL(o.Y, i, j + 2), br, RB(5));
                 bits |= br ? STZ_D_NODE_ENO_YY : 0;
             } else {
                 const double hs = -hy * 2;
@@ -132,26 +146,29 @@ __device__ __forceinline__ static void advect_body(const SimConst &c, const Fiel
     double f0[5], f1[5], f2[5];
 #pragma unroll
     for (int k = 0; k < 5; k++) f0[k] = PL(F[k], i, j);
-    if (uc > 0) {
+    if (UP(uc > 0, STZ_D_CELL_UPX)) {
         bits |= STZ_D_CELL_UPX;
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            f1[k] = hx * eno2(PL(F[k], ir, j), f0[k], PL(F[k], il, j), PL(F[k], ill, j), br);
+            f1[k] = hx * eno2(PL(F[k], ir, j), f0[k], PL(F[k], il, j), PL(F[k], ill, j), br,
+                              RB(STZ_D_CELL_ENO_X0 + k));
             bits |= br << (STZ_D_CELL_ENO_X0 + k);
         }
     } else {
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            f1[k] = -hx * eno2(PL(F[k], il, j), f0[k], PL(F[k], ir, j), PL(F[k], irr, j), br);
+            f1[k] = -hx * eno2(PL(F[k], il, j), f0[k], PL(F[k], ir, j), PL(F[k], irr, j), br,
+                               RB(STZ_D_CELL_ENO_X0 + k));
             bits |= br << (STZ_D_CELL_ENO_X0 + k);
         }
     }
-    if (vc > 0) {
+    if (UP(vc > 0, STZ_D_CELL_UPY)) {
         bits |= STZ_D_CELL_UPY;
         if (j > 0) {
 #pragma unroll
             for (int k = 0; k < 5; k++) {
-                f2[k] = hy * eno2(PL(F[k], i, j + 1), f0[k], PL(F[k], i, j - 1), PL(F[k], i, j - 2), br);
+                f2[k] = hy * eno2(PL(F[k], i, j + 1), f0[k], PL(F[k], i, j - 1), PL(F[k], i, j - 2), br,
+                                  RB(STZ_D_CELL_ENO_Y0 + k));
                 bits |= br << (STZ_D_CELL_ENO_Y0 + k);
             }
         } else {
@@ -164,7 +181,8 @@ __device__ __forceinline__ static void advect_body(const SimConst &c, const Fiel
         if (j < n - 1) {
 #pragma unroll
             for (int k = 0; k < 5; k++) {
-                f2[k] = -hy * eno2(PL(F[k], i, j - 1), f0[k], PL(F[k], i, j + 1), PL(F[k], i, j + 2), br);
+                f2[k] = -hy * eno2(PL(F[k], i, j - 1), f0[k], PL(F[k], i, j + 1), PL(F[k], i, j + 2), br,
+                                   RB(STZ_D_CELL_ENO_Y0 + k));
                 bits |= br << (STZ_D_CELL_ENO_Y0 + k);
             }
         } else {
@@ -180,7 +198,14 @@ __device__ __forceinline__ static void advect_body(const SimConst &c, const Fiel
     const double sbar = sqrt(3 * f0[1] * f0[1] + f0[2] * f0[2] + f0[3] * f0[3]);
     double ddev;
     int trips;
-    dtmp = plastic_term(c, sbar, cchi, ddev, dt, trips);
+    bool last_yield;
+    if (REPLAY && (rb >> STZ_D_SUBSTEP_SHIFT) != 255u) {
+        trips = (int)(rb >> STZ_D_SUBSTEP_SHIFT);
+        last_yield = (rb & STZ_D_YIELD_LAST) != 0;
+        dtmp = plastic_term_replay(c, sbar, cchi, ddev, dt, (rb & STZ_D_SMALLDEV) != 0, trips, last_yield);
+    } else {
+        dtmp = plastic_term(c, sbar, cchi, ddev, dt, trips, last_yield);
+    }
     PL(w.ddev, i, j) = ddev;
     PL(w.chi, i, j) = cchi;
 
@@ -191,17 +216,23 @@ __device__ __forceinline__ static void advect_body(const SimConst &c, const Fiel
     PL(w.tau, i, j) = f0[3] + (-uc * f1[3] - vc * f2[3] - two_omega * f0[2] - dtmp * f0[3]);
 
     if (dec) {
-        if (sbar < 1e-12) bits |= STZ_D_SMALLDEV;
-        else if (!(sbar < 1) && (c.stz.kind == STZ_MODEL_LINEAR_ATHERMAL ||
-                                 c.stz.kind == STZ_MODEL_NONLINEAR_ATHERMAL))
-            bits |= STZ_D_YIELD;
+        const bool athermal = c.stz.kind == STZ_MODEL_LINEAR_ATHERMAL || c.stz.kind == STZ_MODEL_NONLINEAR_ATHERMAL;
+        if (REPLAY ? (rb & STZ_D_SMALLDEV) != 0 : sbar < 1e-12) bits |= STZ_D_SMALLDEV;
+        else if (athermal && (REPLAY ? (rb & STZ_D_YIELD) != 0 : !(sbar < 1))) bits |= STZ_D_YIELD;
+        if (athermal && last_yield) bits |= STZ_D_YIELD_LAST;
         bits |= (uint32_t)(trips > 255 ? 255 : trips) << STZ_D_SUBSTEP_SHIFT;
         dec[i + m * j] = bits;
     }
+#undef RB
+#undef UP
 }
 __global__ void __launch_bounds__(256) advect_kernel(SimConst c, FieldPtrs o, FieldPtrs w, double dt,
                                                      uint32_t *dec) {
-    advect_body(c, o, w, dt, dec, blockIdx.x, blockIdx.y);
+    advect_body<false>(c, o, w, dt, dec, nullptr, blockIdx.x, blockIdx.y);
+}
+__global__ void __launch_bounds__(256) advect_replay_kernel(SimConst c, FieldPtrs o, FieldPtrs w, double dt,
+                                                            uint32_t *dec, const uint32_t *rep) {
+    advect_body<true>(c, o, w, dt, dec, rep, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -369,7 +400,8 @@ __global__ void __launch_bounds__(256) direct_kernel(SimConst c, FieldPtrs o, Fi
     const double sbar = sqrt(3 * f0[1] * f0[1] + f0[2] * f0[2] + f0[3] * f0[3]);
     double ddev;
     int trips;
-    dtmp = plastic_term(c, sbar, cchi, ddev, dt, trips);
+    bool last_yield;
+    dtmp = plastic_term(c, sbar, cchi, ddev, dt, trips, last_yield);
     PL(w.chi, i, j) = cchi;
     // f.ddev *= f.dev() (:265), with the stresses not yet updated
     PL(w.ddev, i, j) = ddev * sbar;
@@ -579,9 +611,10 @@ __global__ void init_fields_kernel(SimConst c, FieldPtrs f, int chi_case, double
 static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
 
 int launch_advect(const SimConst &c, const FieldPtrs &o, const FieldPtrs &w, double dt, uint32_t *dec,
-                  hipStream_t st) {
+                  hipStream_t st, const uint32_t *rep) {
     dim3 b(64, 4);
-    hipLaunchKernelGGL(advect_kernel, grid2(c.m, c.n, b), b, 0, st, c, o, w, dt, dec);
+    if (rep) hipLaunchKernelGGL(advect_replay_kernel, grid2(c.m, c.n, b), b, 0, st, c, o, w, dt, dec, rep);
+    else hipLaunchKernelGGL(advect_kernel, grid2(c.m, c.n, b), b, 0, st, c, o, w, dt, dec);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
@@ -660,7 +693,7 @@ __global__ void __launch_bounds__(256) advect_batch_kernel(const CellEntry *E, c
     const SimConst c = x.c;
     const FieldPtrs o = x.f[parity], w = x.f[parity ^ 1];
     const int gx = x.gx;
-    advect_body(c, o, w, dt, x.dec, l % gx, l / gx);
+    advect_body<false>(c, o, w, dt, x.dec, nullptr, l % gx, l / gx);
 }
 
 __global__ void __launch_bounds__(256) chi_diffusion_batch_kernel(const CellEntry *E, const int *map, int ne,

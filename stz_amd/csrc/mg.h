@@ -46,23 +46,25 @@ template <typename T> struct CoarseArgs {
     int zoff[kMaxLevels], boff[kMaxLevels], soff[kMaxLevels], roff, binv_off;
     int lds_elems;
     // ld >= 0: level ld (an x-invariant level, lc or finer) is solved
-    // exactly by the block-circulant inverse Gd (mg_direct.hip) and the
+    // exactly through its Fourier modes (buffer Gd, mg_direct.hip) and the
     // levels below it are not visited; -1: the workgroup solver runs
     // levels lc..L-1
     int ld;
-    const T *Gd;
+    T *Gd;
 };
 
 // the level a V-cycle bottoms out at, and its solve
 template <typename T> inline int bottom_level(const CoarseArgs<T> &A) { return A.ld >= 0 ? A.ld : A.lc; }
 template <typename T> int mg_bottom(const CoarseArgs<T> &A, hipStream_t st);
 
-// mg_direct.hip: exact solve of an x-invariant level from its block-
-// circulant inverse G (host: direct_inverse), and its LDS needs
-template <typename T> int mg_direct_solve(const LevelArgs<T> &L, const T *G, hipStream_t st);
+// mg_direct.hip: exact solve of an x-invariant level through its Fourier
+// modes; D is the level's device buffer of direct_elems(M, Nn) values
+// (host: direct_inverse fills it), with the LDS the mode launch needs
+template <typename T> int mg_direct_solve(const LevelArgs<T> &L, T *D, hipStream_t st);
 template <typename T> size_t direct_lds_bytes(int M, int Nn);
 template <typename T> int mg_direct_set_lds_limit(size_t bytes);
-bool direct_inverse(const double *S, int M, int Nn, std::vector<double> &G);
+size_t direct_elems(int M, int Nn);
+bool direct_inverse(const double *S, int M, int Nn, std::vector<double> &D);
 
 // LDS layout for levels lc..L-1; returns the number of elements of T
 template <typename T> int coarse_layout(CoarseArgs<T> &A);

@@ -4,22 +4,27 @@
 // bilinear Galerkin coarsening of an even-width x-invariant level stays
 // x-invariant).
 //
-// On such a level the inverse is block-circulant in x: z(i,j) = sum over
-// (delta, j') of G[delta](j, j') b(i - delta mod M, j'), with 2x2 blocks.
-// The host forms G once per operator from the level's row stencils by a
-// discrete Fourier transform in x: mode k turns the level into a
-// (2 Nn) x (2 Nn) complex block-tridiagonal system A_k (blocks
-// sum_dx S_j[dx,dy] e^{i theta_k dx}), whose inverse is taken by Gauss-Jordan
-// elimination; G[delta] = (1/M) sum_k inv(A_k) e^{i theta_k delta}, using
-// inv(A_{M-k}) = conj(inv(A_k)).
+// A discrete Fourier transform along x turns such a level of M x Nn nodes
+// into M/2 + 1 independent complex systems of N2 = 2 Nn unknowns (mode k:
+// the row stencils' blocks summed as sum_dx S_j[dx,dy] e^{i theta_k dx},
+// theta_k = 2 pi k / M; the other modes are their conjugates).  The host
+// inverts each once per operator (Gauss-Jordan with partial pivoting) and
+// folds the inverse transform's weights into the inverses; the device then
+// solves the level in two launches:
+//   mode launch     one workgroup per (mode, block of rows): the DFT of the
+//                   source along x for its mode, then its rows of the mode's
+//                   inverse applied to it
+//   inverse launch  one workgroup per row (j, component): the real inverse
+//                   DFT of the solved modes
+// Every sum runs in a fixed order, so the result is deterministic.  Per
+// solve: M N2 (M/2 + 1) complex multiply-adds for the transforms and
+// (M/2 + 1) N2^2 for the modes, instead of a V-cycle's chain of
+// barrier-separated smoothing passes on ever smaller levels.
 //
-// The device applies G in one launch: the V-cycle's coarse solve becomes one
-// parallel matrix-vector product instead of a chain of barrier-separated
-// smoothing passes on ever smaller levels.  Each workgroup owns 32 outputs
-// (i0..i0+31 of one row j, one component c); it stages all of b and its
-// (j,c) slice of G in LDS, each of its 8 waves sums over one eighth of the
-// (j',c') range, and the partial sums meet in LDS in a fixed order, so the
-// result is deterministic.
+// Device buffer of a direct level (direct_elems values of T):
+//   Gh  [k][r][q]  complex inverse of mode k, weighted for the inverse DFT
+//   tw  [k][i]     (cos, sin) of theta_k i
+//   Xh  [k][r]     the solved modes (written by the mode launch)
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -33,77 +38,155 @@
 
 namespace stz {
 
-constexpr int kDirThreads = 256;
-constexpr int kDirGroups = kDirThreads / 32;  // partial sums per output
+constexpr int kModeThreads = 256;
+constexpr int kRowBlocks = 4;      // row blocks per mode in the mode launch
+constexpr int kRowLanes = 8;       // lanes that share one row of a mode's matvec
+constexpr int kInvThreads = 64;
 
-// z of the level from its b; workgroup blk = (j * 2 + c) * ceil(M / 32) +
-// block of 32 outputs i (those beyond M idle but take part in the barriers)
+__host__ __device__ static size_t gh_elems(int M, int Nn) { return (size_t)2 * (M / 2 + 1) * (2 * Nn) * (2 * Nn); }
+__host__ __device__ static size_t tw_elems(int M) { return (size_t)2 * (M / 2 + 1) * M; }
+__host__ __device__ static size_t xh_elems(int M, int Nn) { return (size_t)2 * (M / 2 + 1) * (2 * Nn); }
+size_t direct_elems(int M, int Nn) { return gh_elems(M, Nn) + tw_elems(M) + xh_elems(M, Nn); }
+
+template <typename T> size_t direct_lds_bytes(int M, int Nn) {
+    return (size_t)(M * 2 * Nn + 2 * M + 2 * 2 * Nn) * sizeof(T);
+}
+
+// mode launch body: workgroup blk = k * kRowBlocks + row block
 template <typename T>
-__device__ __forceinline__ static void direct_solve_body(int M, int Nn, const T *__restrict__ G,
-                                                         const T *__restrict__ b, T *__restrict__ z, int blk) {
+__device__ __forceinline__ static void direct_mode_body(int M, int Nn, T *__restrict__ D, const T *__restrict__ b,
+                                                        int blk) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *bs = reinterpret_cast<T *>(smem_raw);  // b, natural order: 2 (i + M j') + c'
     const int N2 = 2 * Nn, nb = M * N2;
-    T *gs = bs + nb;                           // G slice: delta * N2 + q
-    T *ps = gs + nb;                           // partial sums [group][output]
-    const int nib = (M + 31) / 32, ib = blk % nib, jc = blk / nib;
-    const T *Gj = G + (size_t)jc * nb;
-    for (int t = threadIdx.x; t < nb; t += kDirThreads) {
-        bs[t] = b[t];
-        gs[t] = Gj[t];
-    }
+    const int k = blk / kRowBlocks, rb = blk % kRowBlocks;
+    const T *Gh = D + (size_t)2 * k * N2 * N2;
+    const T *tw = D + gh_elems(M, Nn) + (size_t)2 * k * M;
+    T *Xh = D + gh_elems(M, Nn) + tw_elems(M) + (size_t)2 * k * N2;
+    T *bs = reinterpret_cast<T *>(smem_raw);  // b, natural order: 2 (i + M j') + c'
+    T *ts = bs + nb;                          // (cos, sin) of theta_k i
+    T *bh = ts + 2 * M;                       // DFT of the source, per q = 2 j' + c'
+    for (int t = threadIdx.x; t < nb; t += kModeThreads) bs[t] = b[t];
+    for (int t = threadIdx.x; t < 2 * M; t += kModeThreads) ts[t] = tw[t];
     __syncthreads();
-    const int lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
-    const int i = ib * 32 + lane;
-    // this group's share of q = 2 j' + c'
-    const int qper = (N2 + kDirGroups - 1) / kDirGroups, q0 = grp * qper,
-              q1 = i < M ? (q0 + qper < N2 ? q0 + qper : N2) : q0;
-    T acc = 0;
-    for (int d = 0; d < M && q0 < q1; d++) {
-        int is = i - d;
-        is = is < 0 ? is + M : is;
-        const T *g = gs + d * N2;
-        const T *bb = bs + 2 * is;
-        for (int q = q0; q < q1; q++) {
-            const int jp = q >> 1, cp = q & 1;
-            acc = fmav(g[q], bb[2 * M * jp + cp], acc);
+    // bhat(q) = sum_i b(i, q) e^{-i theta_k i}
+    // (two threads per q, each summing half of the i; kModeThreads >= 2 N2
+    // is not required: the loop strides over q)
+    for (int t = threadIdx.x; t < 2 * ((N2 + kModeThreads / 2 - 1) / (kModeThreads / 2)) * (kModeThreads / 2);
+         t += kModeThreads) {
+        const int q = t >> 1, h = t & 1;
+        T re = 0, im = 0;
+        if (q < N2) {
+            const T *bq = bs + 2 * M * (q >> 1) + (q & 1);
+            const int i0 = h * (M / 2), i1 = h ? M : M / 2;
+#pragma unroll 8
+            for (int i = i0; i < i1; i++) {
+                const T v = bq[2 * i];
+                re = fmav(v, ts[2 * i], re);
+                im = fmav(-v, ts[2 * i + 1], im);
+            }
+        }
+        re += __shfl_xor(re, 1, 2);
+        im += __shfl_xor(im, 1, 2);
+        if (q < N2 && h == 0) {
+            bh[2 * q] = re;
+            bh[2 * q + 1] = im;
         }
     }
-    ps[grp * 32 + lane] = acc;
     __syncthreads();
-    if (grp == 0 && i < M) {
-        T s = ps[lane];
-        for (int g2 = 1; g2 < kDirGroups; g2++) s += ps[g2 * 32 + lane];
-        const int j = jc >> 1, c = jc & 1;
+    // rows r0 .. r0 + rows of the mode's (weighted) inverse applied to bhat
+    const int rows = (N2 + kRowBlocks - 1) / kRowBlocks, r0 = rb * rows;
+    const int lane = threadIdx.x % kRowLanes;
+    for (int rr = threadIdx.x / kRowLanes; rr < rows; rr += kModeThreads / kRowLanes) {
+        const int r = r0 + rr;
+        const bool act = r < N2;
+        T re = 0, im = 0;
+        if (act) {
+            const T *g = Gh + (size_t)2 * r * N2;
+            for (int q = lane; q < N2; q += kRowLanes) {
+                const T gr = g[2 * q], gi = g[2 * q + 1], br = bh[2 * q], bi = bh[2 * q + 1];
+                re = fmav(gr, br, re);
+                re = fmav(-gi, bi, re);
+                im = fmav(gr, bi, im);
+                im = fmav(gi, br, im);
+            }
+        }
+        // fixed-order sum over the row's lanes (consecutive lanes of one wave)
+#pragma unroll
+        for (int off = kRowLanes / 2; off > 0; off >>= 1) {
+            re += __shfl_xor(re, off, kRowLanes);
+            im += __shfl_xor(im, off, kRowLanes);
+        }
+        if (act && lane == 0) {
+            Xh[2 * r] = re;
+            Xh[2 * r + 1] = im;
+        }
+    }
+}
+
+// inverse launch body: workgroup r = 2 j + c; z(i, r) = Re sum_k Xh[k][r]
+// e^{i theta_k i} (the weights 1/M and 2/M are folded into Gh)
+template <typename T>
+__device__ __forceinline__ static void direct_inverse_body(int M, int Nn, const T *__restrict__ D, T *__restrict__ z,
+                                                           int r) {
+    const int N2 = 2 * Nn, K = M / 2 + 1;
+    const T *tw = D + gh_elems(M, Nn);
+    const T *Xh = tw + tw_elems(M);
+    const int j = r >> 1, c = r & 1;
+    for (int i = threadIdx.x; i < M; i += kInvThreads) {
+        T s = 0;
+        for (int k = 0; k < K; k++) {
+            const T xr = Xh[2 * (k * N2 + r)], xi = Xh[2 * (k * N2 + r) + 1];
+            const T co = tw[2 * (k * M + i)], si = tw[2 * (k * M + i) + 1];
+            s = fmav(xr, co, s);
+            s = fmav(-xi, si, s);
+        }
         z[2 * (i + M * j) + c] = s;
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kDirThreads) direct_solve_kernel(int M, int Nn, const T *__restrict__ G,
-                                                                   const T *__restrict__ b, T *__restrict__ z) {
-    direct_solve_body(M, Nn, G, b, z, blockIdx.x);
+__global__ void __launch_bounds__(kModeThreads) direct_mode_kernel(int M, int Nn, T *D, const T *b) {
+    direct_mode_body(M, Nn, D, b, blockIdx.x);
+}
+template <typename T>
+__global__ void __launch_bounds__(kInvThreads) direct_inverse_kernel(int M, int Nn, const T *D, T *z) {
+    direct_inverse_body(M, Nn, D, z, blockIdx.x);
 }
 
 // the same over the members of an ensemble (batch.h); members whose slot
 // flag is clear return at once
 template <typename T>
-__global__ void __launch_bounds__(kDirThreads) direct_solve_batch_kernel(const DirectEntry<T> *E, const int *map,
+__global__ void __launch_bounds__(kModeThreads) direct_mode_batch_kernel(const DirectEntry<T> *E, const int *map,
                                                                          int ne, const EnsState *St) {
     const int2 el = batch_slot(map, blockIdx.x);
     const DirectEntry<T> &x = E[el.x];
     if (St && !St[x.member].active) return;
-    direct_solve_body(x.M, x.Nn, x.G, x.b, x.z, el.y);
+    direct_mode_body(x.M, x.Nn, x.D, x.b, el.y);
+}
+template <typename T>
+__global__ void __launch_bounds__(kInvThreads) direct_inverse_batch_kernel(const DirectEntry<T> *E, const int *map,
+                                                                           int ne, const EnsState *St) {
+    const int2 el = batch_slot(map, blockIdx.x);
+    const DirectEntry<T> &x = E[el.x];
+    if (St && !St[x.member].active) return;
+    direct_inverse_body(x.M, x.Nn, x.D, x.z, el.y);
 }
 
-template <typename T> size_t direct_lds_bytes(int M, int Nn) {
-    return (size_t)(2 * M * 2 * Nn + kDirGroups * 32) * sizeof(T);
+int direct_mode_workgroups(int M, int Nn) {
+    (void)Nn;
+    return (M / 2 + 1) * kRowBlocks;
+}
+int direct_inverse_workgroups(int M, int Nn) {
+    (void)M;
+    return 2 * Nn;
 }
 
-template <typename T> int mg_direct_solve(const LevelArgs<T> &L, const T *G, hipStream_t st) {
+template <typename T> int mg_direct_solve(const LevelArgs<T> &L, T *D, hipStream_t st) {
     const int M = L.m, Nn = L.n;
-    hipLaunchKernelGGL(direct_solve_kernel<T>, dim3(2 * Nn * ((M + 31) / 32)), dim3(kDirThreads),
-                       direct_lds_bytes<T>(M, Nn), st, M, Nn, G, L.b, L.z);
+    hipLaunchKernelGGL(direct_mode_kernel<T>, dim3(direct_mode_workgroups(M, Nn)), dim3(kModeThreads),
+                       direct_lds_bytes<T>(M, Nn), st, M, Nn, D, L.b);
+    hipLaunchKernelGGL(direct_inverse_kernel<T>, dim3(direct_inverse_workgroups(M, Nn)), dim3(kInvThreads), 0, st,
+                       M, Nn, D, L.z);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
@@ -111,37 +194,34 @@ template <typename T> int mg_direct_solve(const LevelArgs<T> &L, const T *G, hip
 template <typename T> int mg_direct_set_lds_limit(size_t bytes) {
     static size_t set = 0;
     if (bytes <= set) return STZ_OK;
-    STZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&direct_solve_kernel<T>),
+    STZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&direct_mode_kernel<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    STZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&direct_solve_batch_kernel<T>),
+    STZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&direct_mode_batch_kernel<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     set = bytes;
     return STZ_OK;
 }
 
 template <typename T>
-int batch_direct(const DirectEntry<T> *E, const int *map, int ne, int total, size_t lds, const EnsState *St,
-                 hipStream_t st) {
+int batch_direct(const DirectEntry<T> *E, const int *mode_map, int mode_total, const int *inv_map, int inv_total,
+                 int ne, size_t lds, const EnsState *St, hipStream_t st) {
     if (ne == 0) return STZ_OK;
-    hipLaunchKernelGGL(direct_solve_batch_kernel<T>, dim3(total), dim3(kDirThreads), lds, st, E, map, ne, St);
+    hipLaunchKernelGGL(direct_mode_batch_kernel<T>, dim3(mode_total), dim3(kModeThreads), lds, st, E, mode_map, ne,
+                       St);
+    hipLaunchKernelGGL(direct_inverse_batch_kernel<T>, dim3(inv_total), dim3(kInvThreads), 0, st, E, inv_map, ne,
+                       St);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
-template int batch_direct<double>(const DirectEntry<double> *, const int *, int, int, size_t, const EnsState *,
-                                  hipStream_t);
-template int batch_direct<float>(const DirectEntry<float> *, const int *, int, int, size_t, const EnsState *,
-                                 hipStream_t);
-int direct_workgroups(int M, int Nn) { return 2 * Nn * ((M + 31) / 32); }
 
-// host: the block-circulant inverse of an x-invariant level with row
-// stencils S (40 per row: blocks dl,dc,dr,cl,cc,cr,ul,uc,ur, then the
-// centre inverse), M nodes per row (even), Nn rows.  Layout of G:
-// [(2 j + c) * M + delta] * (2 Nn) + (2 j' + c').  Returns false if a mode's
-// system is singular.
-bool direct_inverse(const double *S, int M, int Nn, std::vector<double> &G) {
+// host: the device buffer of an x-invariant level with row stencils S (40
+// per row: blocks dl,dc,dr,cl,cc,cr,ul,uc,ur, then the centre inverse), M
+// nodes per row (even), Nn rows, in the layout above (Xh zero).  Returns
+// false if a mode's system is singular.
+bool direct_inverse(const double *S, int M, int Nn, std::vector<double> &D) {
     typedef std::complex<double> cd;
     const int N2 = 2 * Nn, K = M / 2 + 1;
-    std::vector<cd> Gk((size_t)K * N2 * N2);
+    D.assign(direct_elems(M, Nn), 0.0);
     std::vector<cd> A((size_t)N2 * N2), I((size_t)N2 * N2);
     const double pi = 3.14159265358979323846;
     for (int k = 0; k < K; k++) {
@@ -188,32 +268,32 @@ bool direct_inverse(const double *S, int M, int Nn, std::vector<double> &G) {
                 }
             }
         }
-        std::copy(I.begin(), I.end(), Gk.begin() + (size_t)k * N2 * N2);
+        // z = (1/M) [X_0 + X_{M/2} (-1)^i] + (2/M) sum_{0<k<M/2} Re(X_k e^{i theta_k i})
+        const double w = (k == 0 || 2 * k == M) ? 1.0 / M : 2.0 / M;
+        double *g = D.data() + (size_t)2 * k * N2 * N2;
+        for (size_t t = 0; t < (size_t)N2 * N2; t++) {
+            g[2 * t] = w * I[t].real();
+            g[2 * t + 1] = w * I[t].imag();
+        }
+        double *tw = D.data() + gh_elems(M, Nn) + (size_t)2 * k * M;
+        for (int i = 0; i < M; i++) {
+            const double a = 2 * pi * (double)((k * i) % M) / M;
+            tw[2 * i] = std::cos(a);
+            tw[2 * i + 1] = std::sin(a);
+        }
     }
-    // G[delta] = (1/M) sum_k Gk e^{i theta_k delta}, conjugate pairs folded
-    G.assign((size_t)N2 * M * N2, 0.0);
-    std::vector<cd> ph((size_t)K * M);
-    for (int k = 0; k < K; k++)
-        for (int d = 0; d < M; d++) ph[(size_t)k * M + d] = std::polar(1.0, 2 * pi * (double)((k * d) % M) / M);
-    for (int r = 0; r < N2; r++)
-        for (int d = 0; d < M; d++)
-            for (int q = 0; q < N2; q++) {
-                double sum = 0;
-                for (int k = 0; k < K; k++) {
-                    const cd v = Gk[((size_t)k * N2 + r) * N2 + q] * ph[(size_t)k * M + d];
-                    const bool self = k == 0 || 2 * k == M;
-                    sum += self ? v.real() : 2 * v.real();
-                }
-                G[((size_t)r * M + d) * N2 + q] = sum / M;
-            }
     return true;
 }
 
-template int mg_direct_solve<double>(const LevelArgs<double> &, const double *, hipStream_t);
-template int mg_direct_solve<float>(const LevelArgs<float> &, const float *, hipStream_t);
+template int mg_direct_solve<double>(const LevelArgs<double> &, double *, hipStream_t);
+template int mg_direct_solve<float>(const LevelArgs<float> &, float *, hipStream_t);
 template size_t direct_lds_bytes<double>(int, int);
 template size_t direct_lds_bytes<float>(int, int);
 template int mg_direct_set_lds_limit<double>(size_t);
 template int mg_direct_set_lds_limit<float>(size_t);
+template int batch_direct<double>(const DirectEntry<double> *, const int *, int, const int *, int, int, size_t,
+                                  const EnsState *, hipStream_t);
+template int batch_direct<float>(const DirectEntry<float> *, const int *, int, const int *, int, int, size_t,
+                                 const EnsState *, hipStream_t);
 
 }  // namespace stz

@@ -362,8 +362,8 @@ static int set_level_stencils(stz_sim *s, DevMG<T> &mg, const std::vector<std::v
     return STZ_OK;
 }
 
-// the direct level of hierarchy mg: its block-circulant inverse in precision
-// T, or none
+// the direct level of hierarchy mg: its mode inverses in precision T, or
+// none
 template <typename T> static int set_direct(DevMG<T> &mg, int ld, const std::vector<double> &G) {
     mg.args.ld = ld;
     mg.args.Gd = nullptr;
@@ -381,8 +381,8 @@ template <typename T> static int set_direct(DevMG<T> &mg, int ld, const std::vec
 }
 
 // the finest level l in 1..lc with at most cfg.direct_nodes nodes, an
-// x-invariant operator and an LDS image that fits is solved exactly; the
-// V-cycle stops there (both hierarchies)
+// x-invariant operator of even width and a source that fits the mode
+// launch's LDS is solved exactly; the V-cycle stops there (both hierarchies)
 static int set_direct_level(stz_sim *s) {
     int ld = -1;
     const int lc = s->mg64.args.lc;
@@ -585,7 +585,15 @@ static inline double bdry_pos(const stz_sim *s) {
 
 static int advection_impl(stz_sim *s, double dt) {
     int e;
-    if ((e = launch_advect(s->c, s->cur, s->alt, dt, s->rec_dec ? s->dec : nullptr, s->stream))) return e;
+    const uint32_t *rep = nullptr;
+    if (s->rep) {
+        if (s->rep_next >= s->rep_steps) {
+            set_error("decision replay: all %d recorded steps are used", s->rep_steps);
+            return STZ_ERR_STATE;
+        }
+        rep = s->rep + (size_t)s->rep_next++ * s->c.m * s->c.n;
+    }
+    if ((e = launch_advect(s->c, s->cur, s->alt, dt, s->rec_dec ? s->dec : nullptr, s->stream, rep))) return e;
     if (s->c.chi_diff != 0 || s->c.chi_len != 0)
         if ((e = launch_chi_diffusion(s->c, s->cur, s->alt, dt, s->stream))) return e;
     std::swap(s->cur, s->alt);
@@ -706,6 +714,7 @@ int stz_sim_destroy(stz_sim *s) {
     if (s->planes) (void)hipFree(s->planes);
     if (s->vel) (void)hipFree(s->vel);
     if (s->dec) (void)hipFree(s->dec);
+    if (s->rep) (void)hipFree(s->rep);
     if (s->partial) (void)hipFree(s->partial);
     if (s->hnorm) (void)hipHostFree(s->hnorm);
     if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
@@ -1211,6 +1220,25 @@ int stz_sim_initialize_chi_bicubic(stz_sim *s, int mm, int nn, const double *arr
 int stz_sim_record_decisions(stz_sim *s, int enable) {
     if (!s) { set_error("null simulation"); return STZ_ERR_ARG; }
     s->rec_dec = enable != 0;
+    return STZ_OK;
+}
+
+int stz_sim_replay_decisions(stz_sim *s, int nsteps, const uint32_t *h) {
+    CHECK_SIM(s);
+    if (nsteps < 0 || (nsteps > 0 && !h)) { set_error("need nsteps >= 0 and a record"); return STZ_ERR_ARG; }
+    STZ_HIP(hipStreamSynchronize(s->stream));  // a step in flight may still read the old record
+    if (s->rep) (void)hipFree(s->rep);
+    s->rep = nullptr;
+    s->rep_steps = s->rep_next = 0;
+    if (nsteps == 0) return STZ_OK;
+    const size_t bytes = (size_t)nsteps * s->c.m * s->c.n * sizeof(uint32_t);
+    if (hipMalloc(&s->rep, bytes) != hipSuccess) {
+        s->rep = nullptr;
+        set_error("out of device memory for a %d-step decision record", nsteps);
+        return STZ_ERR_HIP;
+    }
+    STZ_HIP(hipMemcpy(s->rep, h, bytes, hipMemcpyHostToDevice));
+    s->rep_steps = nsteps;
     return STZ_OK;
 }
 

@@ -31,7 +31,7 @@ template <typename T> struct DevMG {
     T *pack = nullptr;       // constant part of the workgroup solver's LDS image
     size_t pack_elems = 0;
     size_t lds = 0;
-    T *G = nullptr;          // block-circulant inverse of the direct level
+    T *G = nullptr;          // the direct level's mode inverses (mg_direct.hip)
     size_t G_elems = 0;
 
     void free_all() {
@@ -70,6 +70,8 @@ struct stz_sim {
     double *vel = nullptr, *src = nullptr;
     uint32_t *dec = nullptr;
     bool rec_dec = false;
+    uint32_t *rep = nullptr;  // decision record being replayed (rep_steps x m*n)
+    int rep_steps = 0, rep_next = 0;
     // multigrid
     std::vector<stz::HostLevel> hl;
     double viscdt_prev = -1;

@@ -30,9 +30,12 @@ __host__ __device__ static double dpl_common(const stz_model_params &mp, double 
     return Dpl;
 }
 
+// yield: -1 evaluates the athermal laws' yield test sbar < 1; 0 or 1 takes
+// the recorded branch (below or at/above yield) instead, whatever sbar is
 __host__ __device__ static double dplastic(const stz_model_params &mp, double thetainv, double sbar, double chi,
-                                  double &dchi1, double &dchi2) {
+                                           double &dchi1, double &dchi2, int yield = -1) {
     const double s0 = 1;
+    const bool below = yield < 0 ? sbar < 1 : yield == 0;
     switch (mp.kind) {
     case STZ_MODEL_LINEAR: {  // stz_model.hh:58-67
         double sC1 = exp(-mp.Delta * thetainv);
@@ -48,7 +51,7 @@ __host__ __device__ static double dplastic(const stz_model_params &mp, double th
     }
     case STZ_MODEL_NONLINEAR_ATHERMAL: {  // stz_model.hh:179-206
         dchi2 = 0;
-        if (sbar < 1) { dchi1 = 0; return 0; }
+        if (below) { dchi1 = 0; return 0; }
         double x = mp.Omegaeps0 * sbar / mp.Delta,
                sC = 0.5 * (exp(-mp.Delta * thetainv * exp(x)) + exp(-mp.Delta * thetainv * exp(-x))),
                sT = tanh(mp.Delta * sinh(x) * thetainv);
@@ -60,7 +63,7 @@ __host__ __device__ static double dplastic(const stz_model_params &mp, double th
     default: {  // linear athermal, stz_model.hh:103-151
         double tmp, sC, sT;
         dchi2 = 0;
-        if (sbar < 1) { dchi1 = 0; return 0; }
+        if (below) { dchi1 = 0; return 0; }
         if (mp.t_model == -1 || mp.Delta >= mp.Omegaeps0 * sbar)
             sC = 0.5 * (exp((mp.Omegaeps0 * sbar - mp.Delta) * thetainv) +
                         exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv));
@@ -91,16 +94,19 @@ __host__ __device__ static double dplastic(const stz_model_params &mp, double th
 }
 
 // shear_sim.cc:513-548 (adaptive_plastic_term); trips counts loop iterations
+// and last_yield is the athermal laws' yield test (sbar >= 1) on the last one
 template <class C>
 __device__ static double plastic_term(const C &c, double sbar, double &chiv, double &ddev, double dt,
-                                      int &trips) {
+                                      int &trips, bool &last_yield) {
     const double small_dev_cutoff = 1e-12;
     const double theta = 0;
     bool adapt = true;
     int t = 0;
     double dtl = dt, osbar = sbar, dplas, dchi1, dchi2, adt;
+    last_yield = false;
     if (sbar < small_dev_cutoff) { ddev = 0; trips = 0; return 0; }
     do {
+        last_yield = !(sbar < 1);
         dplas = 2 * c.t_scale * c.mu * dplastic(c.stz, c.thetainv, sbar, chiv, dchi1, dchi2);
         if (fabs(dtl * dplas) > c.adapt_fac) {
             adt = c.adapt_fac / fabs(dplas);
@@ -114,6 +120,35 @@ __device__ static double plastic_term(const C &c, double sbar, double &chiv, dou
         t++;
     } while (adapt);
     trips = t;
+    ddev = osbar - sbar;
+    return ddev / osbar;
+}
+
+// plastic_term with its decisions taken from a record of an earlier
+// evaluation: the small-deviator cut-off, the number of loop trips (every
+// trip but the last a substep of adapt_fac / |dplas|, which a nonzero dplas
+// needs, so at or above yield) and the yield branch of the last trip.
+// Where the record matches what plastic_term would decide, the result is
+// plastic_term's bit for bit; nearby, it is the smooth continuation of the
+// recorded branch.
+template <class C>
+__device__ static double plastic_term_replay(const C &c, double sbar, double &chiv, double &ddev, double dt,
+                                             bool small, int trips, bool last_yield) {
+    const double theta = 0;
+    double dtl = dt, osbar = sbar, dplas, dchi1, dchi2, adt;
+    if (small) { ddev = 0; return 0; }
+    for (int t = 0; t < trips; t++) {
+        const int yield = t + 1 < trips ? 1 : (last_yield ? 1 : 0);
+        dplas = 2 * c.t_scale * c.mu * dplastic(c.stz, c.thetainv, sbar, chiv, dchi1, dchi2, yield);
+        if (t + 1 < trips) {
+            adt = c.adapt_fac / fabs(dplas);
+            dtl -= adt;
+        } else {
+            adt = dtl;
+        }
+        sbar -= adt * dplas;
+        chiv += (dchi1 * (c.chi_inf - chiv) + dchi2 * (theta - chiv)) * c.t_scale * adt;
+    }
     ddev = osbar - sbar;
     return ddev / osbar;
 }

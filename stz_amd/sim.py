@@ -293,6 +293,17 @@ class ShearSim:
     def record_decisions(self, on=True):
         check(self._lib.stz_sim_record_decisions(self._h, 1 if on else 0))
 
+    def replay_decisions(self, records):
+        """take the decisions of the next len(records) advection steps from
+        records (each an (n, m) uint32 array from get_decisions) instead of
+        evaluating them; an empty list ends the replay"""
+        recs = [np.ascontiguousarray(r, dtype=np.uint32) for r in records]
+        for r in recs:
+            if r.shape != (self.n, self.m):
+                raise ValueError("decision record of shape %s, want %s" % (r.shape, (self.n, self.m)))
+        a = np.stack(recs) if recs else np.zeros((1, self.n, self.m), np.uint32)
+        check(self._lib.stz_sim_replay_decisions(self._h, len(recs), a))
+
     def get_decisions(self):
         a = np.empty((self.n, self.m), dtype=np.uint32)
         check(self._lib.stz_sim_download_decisions(self._h, a))

@@ -71,6 +71,21 @@ def test_members_equal_single_simulations(form, graphs):
     ens.close()
 
 
+@pytest.mark.parametrize("form", ["mixed", "fp64"])
+def test_large_members_equal_single_simulations(form):
+    """members of the workload's largest sides, 384 and 512, beside a 64 one:
+    their V-cycles stop at the direct levels 48x49 and 64x65 (the 48-wide
+    level of a 384 side is not smoothed, so the FP64 form takes it too)"""
+    print("case ensemble large", form, flush=True)
+    ens, singles = setup(form, sizes=[384, 512, 64])
+    dt = 0.5
+    ens.step_forward_quasistatic(dt, 3)
+    for g in singles:
+        g.step_forward_quasistatic(dt, 3)
+    assert_same(ens, singles, "3 steps, sides 384 and 512")
+    ens.close()
+
+
 def test_extra_slots_and_table_rebuilds():
     """A member whose predictor underestimates (velocity reset to zero after
     its count has decayed) needs slots beyond the planned ones; a new dt, a
