@@ -240,6 +240,22 @@ int stz_mg_vcycle_launch(stz_sim *sim, int level, int mode, int count);
 /* Upload a source array (m*(n+1) vec2) for a standalone solve. */
 int stz_mg_upload_src(stz_sim *sim, const double *host_vec2);
 
+/* The mean-square residual of stz_mg_l2_error, allocating nothing.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  vel and src are device arrays of
+ * m*(n+1) interleaved vec2 nodes; the operator is qs_multi's
+ * (qs_multi.cc:31-54) for the grid and moduli of p at viscosity/dt =
+ * viscdt; the result, bit-identical to stz_mg_l2_error's on the same arrays,
+ * goes to the device double *out.  Asynchronous on hip_stream (NULL: the
+ * null stream).  Partial sums live in a fixed device array of the library
+ * (one per GPU, held until the process ends), so m*(n+1) may not exceed
+ * 16,777,216 (STZ_ERR_ARG beyond), and calls that overlap on one GPU race.
+ * The first call on each device looks the array up (hipGetSymbolAddress);
+ * make that call outside any graph capture. */
+int stz_qs_l2_error_noalloc(const stz_sim_params *p, double viscdt, const double *vel, const double *src,
+                            double *out, void *hip_stream);
+
 /* ---- direct step, diagnostics, initial conditions ------------------------ */
 
 /* shear_sim::step_forward (shear_sim.cc:185-276), the direct (dynamic)

@@ -192,6 +192,8 @@ _SIGS = {
     "stz_mg_fmg_pre": [C.c_void_p],
     "stz_mg_clear_z": [C.c_void_p],
     "stz_mg_vcycle_launch": [C.c_void_p, C.c_int, C.c_int, C.c_int],
+    "stz_qs_l2_error_noalloc": [C.POINTER(SimParams), C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p],
 }
 _RESTYPE = {"stz_last_error": C.c_char_p}
 

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libstz_amd.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 SOURCES = ("explicit.hip", "diag.hip", "mg.hip", "mg_tiled.hip", "mg_direct.hip", "sim.hip",
            "bd_levelset.hip", "bd_step.hip", "bd_mg.hip", "bd_sim.hip", "ensemble.hip",
-           "adjoint.hip", "adjoint_run.hip", "law.hip")
+           "adjoint.hip", "adjoint_run.hip", "law.hip", "noalloc.hip")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
