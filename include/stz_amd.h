@@ -121,11 +121,13 @@ typedef struct stz_mg_config {
                                     kernel per colour pass everywhere */
     int32_t direct_nodes;        /* the finest coarse level with at most this
                                     many nodes whose operator is invariant
-                                    along x is solved exactly by its
-                                    block-circulant inverse, and the V-cycle
-                                    stops there (default 4200: the 64x65 level
-                                    of a 1024^2 grid); 0: the V-cycle runs to
-                                    the coarsest level, solved exactly */
+                                    along x is solved exactly (a DFT along x
+                                    and one inverse per Fourier mode), and
+                                    the V-cycle stops there (default 4200:
+                                    the 64x65 level of a 1024^2 grid; levels
+                                    of even width up to 128 and at most 72
+                                    rows qualify); 0: the V-cycle runs to the
+                                    coarsest level, solved exactly */
     int32_t reserved[5];
 } stz_mg_config;
 

@@ -82,7 +82,7 @@ template <typename T> struct CoarseEntry {
 template <typename T> struct DirectEntry {
     int member;
     int M, Nn;
-    T *D;          // the level's mode inverses, twiddles and mode scratch
+    double *D;     // the level's mode inverses, twiddles and mode scratch (FP64)
     const T *b;
     T *z;
 };

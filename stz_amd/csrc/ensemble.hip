@@ -228,7 +228,7 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
             direct.push_back(de);
             dwgs.push_back(direct_mode_workgroups(de.M, de.Nn));
             iwgs.push_back(direct_inverse_workgroups(de.M, de.Nn));
-            dlds = std::max(dlds, direct_lds_bytes<T>(de.M, de.Nn));
+            dlds = std::max(dlds, direct_lds_bytes(de.M, de.Nn));
             continue;
         }
         CoarseEntry<T> ce;

@@ -46,11 +46,11 @@ template <typename T> struct CoarseArgs {
     int zoff[kMaxLevels], boff[kMaxLevels], soff[kMaxLevels], roff, binv_off;
     int lds_elems;
     // ld >= 0: level ld (an x-invariant level, lc or finer) is solved
-    // exactly through its Fourier modes (buffer Gd, mg_direct.hip) and the
-    // levels below it are not visited; -1: the workgroup solver runs
-    // levels lc..L-1
+    // exactly through its Fourier modes (FP64 buffer Gd, mg_direct.hip, in
+    // both hierarchies) and the levels below it are not visited; -1: the
+    // workgroup solver runs levels lc..L-1
     int ld;
-    T *Gd;
+    double *Gd;
 };
 
 // the level a V-cycle bottoms out at, and its solve
@@ -58,13 +58,17 @@ template <typename T> inline int bottom_level(const CoarseArgs<T> &A) { return A
 template <typename T> int mg_bottom(const CoarseArgs<T> &A, hipStream_t st);
 
 // mg_direct.hip: exact solve of an x-invariant level through its Fourier
-// modes; D is the level's device buffer of direct_elems(M, Nn) values
-// (host: direct_inverse fills it), with the LDS the mode launch needs
-template <typename T> int mg_direct_solve(const LevelArgs<T> &L, T *D, hipStream_t st);
-template <typename T> size_t direct_lds_bytes(int M, int Nn);
+// modes, in FP64 whatever the level's T; D is the level's device buffer of
+// direct_elems(M, Nn) doubles (host: direct_inverse fills it), with the LDS
+// the mode launch needs
+template <typename T> int mg_direct_solve(const LevelArgs<T> &L, double *D, hipStream_t st);
+size_t direct_lds_bytes(int M, int Nn);
 template <typename T> int mg_direct_set_lds_limit(size_t bytes);
 size_t direct_elems(int M, int Nn);
 bool direct_inverse(const double *S, int M, int Nn, std::vector<double> &D);
+// whether the two-launch solve covers an M x Nn level (M even, 2 Nn and M
+// within the launches' fixed register and LDS tiles)
+bool direct_fits(int M, int Nn);
 
 // LDS layout for levels lc..L-1; returns the number of elements of T
 template <typename T> int coarse_layout(CoarseArgs<T> &A);

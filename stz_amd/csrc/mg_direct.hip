@@ -21,10 +21,18 @@
 // (M/2 + 1) N2^2 for the modes, instead of a V-cycle's chain of
 // barrier-separated smoothing passes on ever smaller levels.
 //
+// Both launches are latency bound (a 64 x 65 level: 66 KB of source, 9 MB of
+// mode inverses in FP64 over 132 workgroups), so each is laid out for as few
+// dependent memory round trips as possible: the mode launch issues its rows
+// of the inverse before it stages the source, stages the source with all of
+// a thread's loads in flight at once, and transforms it from LDS (transposed,
+// so a wave reads consecutive addresses); the twiddle factors come from a
+// table of the M roots of unity indexed by k i mod M.
+//
 // Device buffer of a direct level (direct_elems values of T):
 //   Gh  [k][r][q]  complex inverse of mode k, weighted for the inverse DFT
-//   tw  [k][i]     (cos, sin) of theta_k i
-//   Xh  [k][r]     the solved modes (written by the mode launch)
+//   tw  [m]        (cos, sin) of 2 pi m / M, m < M
+//   Xh  [r][k]     the solved modes (written by the mode launch)
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -39,117 +47,169 @@
 namespace stz {
 
 constexpr int kModeThreads = 256;
+constexpr int kModeWaves = kModeThreads / 64;
 constexpr int kRowBlocks = 4;      // row blocks per mode in the mode launch
-constexpr int kRowLanes = 8;       // lanes that share one row of a mode's matvec
+constexpr int kMaxRows = 9;        // rows of a mode's inverse per wave
+constexpr int kMaxQ = 3;           // entries of a row per lane (q = lane + 64 s)
+constexpr int kStage = 17;         // source nodes staged per thread per pass
 constexpr int kInvThreads = 64;
+template <typename T> struct Pair;
+template <> struct Pair<double> { typedef double2 type; };
+template <> struct Pair<float> { typedef float2 type; };
+// the largest N2 = 2 Nn the mode launch covers
+constexpr int kMaxN2 = kRowBlocks * kModeWaves * kMaxRows < 64 * kMaxQ ? kRowBlocks * kModeWaves * kMaxRows
+                                                                        : 64 * kMaxQ;
 
 __host__ __device__ static size_t gh_elems(int M, int Nn) { return (size_t)2 * (M / 2 + 1) * (2 * Nn) * (2 * Nn); }
-__host__ __device__ static size_t tw_elems(int M) { return (size_t)2 * (M / 2 + 1) * M; }
+__host__ __device__ static size_t tw_elems(int M) { return (size_t)2 * M; }
 __host__ __device__ static size_t xh_elems(int M, int Nn) { return (size_t)2 * (M / 2 + 1) * (2 * Nn); }
 size_t direct_elems(int M, int Nn) { return gh_elems(M, Nn) + tw_elems(M) + xh_elems(M, Nn); }
 
-template <typename T> size_t direct_lds_bytes(int M, int Nn) {
-    return (size_t)(M * 2 * Nn + 2 * M + 2 * 2 * Nn) * sizeof(T);
+// LDS of the mode launch: the source transposed (M rows of N2 + 1), the
+// twiddle table, the transformed source, all FP64
+size_t direct_lds_bytes(int M, int Nn) { return (size_t)(M * (2 * Nn + 1) + 2 * M + 2 * 2 * Nn) * sizeof(double); }
+
+bool direct_fits(int M, int Nn) {
+    return M >= 2 && M % 2 == 0 && M <= 2 * kInvThreads && Nn >= 1 && 2 * Nn <= kMaxN2 &&
+           direct_lds_bytes(M, Nn) <= 150 * 1024;
 }
 
-// mode launch body: workgroup blk = k * kRowBlocks + row block
+// mode launch body: workgroup blk = k * kRowBlocks + row block; the
+// level's source is in T, everything else in FP64
 template <typename T>
-__device__ __forceinline__ static void direct_mode_body(int M, int Nn, T *__restrict__ D, const T *__restrict__ b,
-                                                        int blk) {
+__device__ __forceinline__ static void direct_mode_body(int M, int Nn, double *__restrict__ D,
+                                                        const T *__restrict__ b, int blk) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int N2 = 2 * Nn, nb = M * N2;
+    const int N2 = 2 * Nn, K = M / 2 + 1, P = N2 + 1, nodes = M * Nn;
     const int k = blk / kRowBlocks, rb = blk % kRowBlocks;
-    const T *Gh = D + (size_t)2 * k * N2 * N2;
-    const T *tw = D + gh_elems(M, Nn) + (size_t)2 * k * M;
-    T *Xh = D + gh_elems(M, Nn) + tw_elems(M) + (size_t)2 * k * N2;
-    T *bs = reinterpret_cast<T *>(smem_raw);  // b, natural order: 2 (i + M j') + c'
-    T *ts = bs + nb;                          // (cos, sin) of theta_k i
-    T *bh = ts + 2 * M;                       // DFT of the source, per q = 2 j' + c'
-    for (int t = threadIdx.x; t < nb; t += kModeThreads) bs[t] = b[t];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const double *Gh = D + (size_t)2 * k * N2 * N2;
+    const double *tw = D + gh_elems(M, Nn);
+    double *Xh = D + gh_elems(M, Nn) + tw_elems(M);
+    double *bt = reinterpret_cast<double *>(smem_raw);  // bt[i P + q] = b(i, q), q = 2 j' + c'
+    double *ts = bt + M * P;                            // (cos, sin) of 2 pi m / M
+    double *bh = ts + 2 * M;                            // transformed source, per q
+
+    typedef typename Pair<T>::type V2;
+    // this wave's rows of the mode's inverse: r = r0 + wave + 4 s
+    const int rows = (N2 + kRowBlocks - 1) / kRowBlocks, r0 = rb * rows, r1 = min(r0 + rows, N2);
+    double g[kMaxRows][kMaxQ][2];
+#pragma unroll
+    for (int s = 0; s < kMaxRows; s++) {
+        const int r = r0 + wave + kModeWaves * s;
+#pragma unroll
+        for (int u = 0; u < kMaxQ; u++) {
+            const int q = lane + 64 * u;
+            const bool ok = r < r1 && q < N2;
+            const double2 gv = reinterpret_cast<const double2 *>(Gh)[(size_t)(ok ? r : 0) * N2 + (ok ? q : 0)];
+            g[s][u][0] = ok ? gv.x : 0.0;
+            g[s][u][1] = ok ? gv.y : 0.0;
+        }
+    }
+    // stage the source (vec2 nodes) transposed, and the twiddle table
+    const V2 *bv = reinterpret_cast<const V2 *>(b);
+    V2 zero;
+    zero.x = 0;
+    zero.y = 0;
+    for (int base = 0; base < nodes; base += kStage * kModeThreads) {
+        V2 v[kStage];
+#pragma unroll
+        for (int u = 0; u < kStage; u++) {
+            const int nd = base + threadIdx.x + kModeThreads * u;
+            v[u] = nd < nodes ? bv[nd] : zero;
+        }
+#pragma unroll
+        for (int u = 0; u < kStage; u++) {
+            const int nd = base + threadIdx.x + kModeThreads * u;
+            if (nd < nodes) {
+                const int i = nd % M, jj = nd / M;
+                bt[i * P + 2 * jj] = (double)v[u].x;
+                bt[i * P + 2 * jj + 1] = (double)v[u].y;
+            }
+        }
+    }
     for (int t = threadIdx.x; t < 2 * M; t += kModeThreads) ts[t] = tw[t];
     __syncthreads();
-    // bhat(q) = sum_i b(i, q) e^{-i theta_k i}
-    // (two threads per q, each summing half of the i; kModeThreads >= 2 N2
-    // is not required: the loop strides over q)
-    for (int t = threadIdx.x; t < 2 * ((N2 + kModeThreads / 2 - 1) / (kModeThreads / 2)) * (kModeThreads / 2);
-         t += kModeThreads) {
-        const int q = t >> 1, h = t & 1;
-        T re = 0, im = 0;
-        if (q < N2) {
-            const T *bq = bs + 2 * M * (q >> 1) + (q & 1);
-            const int i0 = h * (M / 2), i1 = h ? M : M / 2;
+    // bhat(q) = sum_i b(i, q) e^{-i theta_k i}, i ascending
+    for (int q = threadIdx.x; q < N2; q += kModeThreads) {
+        double re = 0, im = 0;
+        int m = 0;
 #pragma unroll 8
-            for (int i = i0; i < i1; i++) {
-                const T v = bq[2 * i];
-                re = fmav(v, ts[2 * i], re);
-                im = fmav(-v, ts[2 * i + 1], im);
-            }
+        for (int i = 0; i < M; i++) {
+            const double v = bt[i * P + q];
+            re = fmav(v, ts[2 * m], re);
+            im = fmav(-v, ts[2 * m + 1], im);
+            m += k;
+            if (m >= M) m -= M;
         }
-        re += __shfl_xor(re, 1, 2);
-        im += __shfl_xor(im, 1, 2);
-        if (q < N2 && h == 0) {
-            bh[2 * q] = re;
-            bh[2 * q + 1] = im;
-        }
+        bh[2 * q] = re;
+        bh[2 * q + 1] = im;
     }
     __syncthreads();
-    // rows r0 .. r0 + rows of the mode's (weighted) inverse applied to bhat
-    const int rows = (N2 + kRowBlocks - 1) / kRowBlocks, r0 = rb * rows;
-    const int lane = threadIdx.x % kRowLanes;
-    for (int rr = threadIdx.x / kRowLanes; rr < rows; rr += kModeThreads / kRowLanes) {
-        const int r = r0 + rr;
-        const bool act = r < N2;
-        T re = 0, im = 0;
-        if (act) {
-            const T *g = Gh + (size_t)2 * r * N2;
-            for (int q = lane; q < N2; q += kRowLanes) {
-                const T gr = g[2 * q], gi = g[2 * q + 1], br = bh[2 * q], bi = bh[2 * q + 1];
-                re = fmav(gr, br, re);
-                re = fmav(-gi, bi, re);
-                im = fmav(gr, bi, im);
-                im = fmav(gi, br, im);
-            }
-        }
-        // fixed-order sum over the row's lanes (consecutive lanes of one wave)
+    // rows of the mode's (weighted) inverse applied to bhat: each lane sums
+    // its q in ascending order, then a fixed xor tree over the wave
+    double br[kMaxQ], bi[kMaxQ];
 #pragma unroll
-        for (int off = kRowLanes / 2; off > 0; off >>= 1) {
-            re += __shfl_xor(re, off, kRowLanes);
-            im += __shfl_xor(im, off, kRowLanes);
+    for (int u = 0; u < kMaxQ; u++) {
+        const int q = lane + 64 * u;
+        br[u] = q < N2 ? bh[2 * q] : 0.0;
+        bi[u] = q < N2 ? bh[2 * q + 1] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < kMaxRows; s++) {
+        const int r = r0 + wave + kModeWaves * s;
+        if (r0 + wave + kModeWaves * s >= r1) break;  // uniform over the wave
+        double re = 0, im = 0;
+#pragma unroll
+        for (int u = 0; u < kMaxQ; u++) {
+            re = fmav(g[s][u][0], br[u], re);
+            re = fmav(-g[s][u][1], bi[u], re);
+            im = fmav(g[s][u][0], bi[u], im);
+            im = fmav(g[s][u][1], br[u], im);
         }
-        if (act && lane == 0) {
-            Xh[2 * r] = re;
-            Xh[2 * r + 1] = im;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            re += __shfl_xor(re, off);
+            im += __shfl_xor(im, off);
         }
+        if (lane == 0) reinterpret_cast<double2 *>(Xh)[(size_t)r * K + k] = make_double2(re, im);
     }
 }
 
-// inverse launch body: workgroup r = 2 j + c; z(i, r) = Re sum_k Xh[k][r]
-// e^{i theta_k i} (the weights 1/M and 2/M are folded into Gh)
+// inverse launch body: workgroup r = 2 j + c; z(i, r) = Re sum_k Xh[r][k]
+// e^{i theta_k i}, k ascending, in FP64, rounded to the level's T (the
+// weights 1/M and 2/M are folded into Gh)
 template <typename T>
-__device__ __forceinline__ static void direct_inverse_body(int M, int Nn, const T *__restrict__ D, T *__restrict__ z,
-                                                           int r) {
-    const int N2 = 2 * Nn, K = M / 2 + 1;
-    const T *tw = D + gh_elems(M, Nn);
-    const T *Xh = tw + tw_elems(M);
+__device__ __forceinline__ static void direct_inverse_body(int M, int Nn, const double *__restrict__ D,
+                                                           T *__restrict__ z, int r) {
+    __shared__ double xs[2 * (kInvThreads + 1)];
+    __shared__ double ts[2 * 2 * kInvThreads];
+    const int K = M / 2 + 1;
+    const double *tw = D + gh_elems(M, Nn);
+    const double *Xh = tw + tw_elems(M) + (size_t)2 * r * K;
     const int j = r >> 1, c = r & 1;
+    for (int t = threadIdx.x; t < 2 * K; t += kInvThreads) xs[t] = Xh[t];
+    for (int t = threadIdx.x; t < 2 * M; t += kInvThreads) ts[t] = tw[t];
+    __syncthreads();
     for (int i = threadIdx.x; i < M; i += kInvThreads) {
-        T s = 0;
-        for (int k = 0; k < K; k++) {
-            const T xr = Xh[2 * (k * N2 + r)], xi = Xh[2 * (k * N2 + r) + 1];
-            const T co = tw[2 * (k * M + i)], si = tw[2 * (k * M + i) + 1];
-            s = fmav(xr, co, s);
-            s = fmav(-xi, si, s);
+        double s = 0;
+        int m = 0;
+#pragma unroll 8
+        for (int kk = 0; kk < K; kk++) {
+            s = fmav(xs[2 * kk], ts[2 * m], s);
+            s = fmav(-xs[2 * kk + 1], ts[2 * m + 1], s);
+            m += i;
+            if (m >= M) m -= M;
         }
-        z[2 * (i + M * j) + c] = s;
+        z[2 * (i + M * j) + c] = (T)s;
     }
 }
-
 template <typename T>
-__global__ void __launch_bounds__(kModeThreads) direct_mode_kernel(int M, int Nn, T *D, const T *b) {
+__global__ void __launch_bounds__(kModeThreads) direct_mode_kernel(int M, int Nn, double *D, const T *b) {
     direct_mode_body(M, Nn, D, b, blockIdx.x);
 }
 template <typename T>
-__global__ void __launch_bounds__(kInvThreads) direct_inverse_kernel(int M, int Nn, const T *D, T *z) {
+__global__ void __launch_bounds__(kInvThreads) direct_inverse_kernel(int M, int Nn, const double *D, T *z) {
     direct_inverse_body(M, Nn, D, z, blockIdx.x);
 }
 
@@ -181,10 +241,10 @@ int direct_inverse_workgroups(int M, int Nn) {
     return 2 * Nn;
 }
 
-template <typename T> int mg_direct_solve(const LevelArgs<T> &L, T *D, hipStream_t st) {
+template <typename T> int mg_direct_solve(const LevelArgs<T> &L, double *D, hipStream_t st) {
     const int M = L.m, Nn = L.n;
     hipLaunchKernelGGL(direct_mode_kernel<T>, dim3(direct_mode_workgroups(M, Nn)), dim3(kModeThreads),
-                       direct_lds_bytes<T>(M, Nn), st, M, Nn, D, L.b);
+                       direct_lds_bytes(M, Nn), st, M, Nn, D, L.b);
     hipLaunchKernelGGL(direct_inverse_kernel<T>, dim3(direct_inverse_workgroups(M, Nn)), dim3(kInvThreads), 0, st,
                        M, Nn, D, L.z);
     STZ_HIP(hipGetLastError());
@@ -275,20 +335,18 @@ bool direct_inverse(const double *S, int M, int Nn, std::vector<double> &D) {
             g[2 * t] = w * I[t].real();
             g[2 * t + 1] = w * I[t].imag();
         }
-        double *tw = D.data() + gh_elems(M, Nn) + (size_t)2 * k * M;
-        for (int i = 0; i < M; i++) {
-            const double a = 2 * pi * (double)((k * i) % M) / M;
-            tw[2 * i] = std::cos(a);
-            tw[2 * i + 1] = std::sin(a);
-        }
+    }
+    double *tw = D.data() + gh_elems(M, Nn);
+    for (int i = 0; i < M; i++) {
+        const double a = 2 * pi * (double)i / M;
+        tw[2 * i] = std::cos(a);
+        tw[2 * i + 1] = std::sin(a);
     }
     return true;
 }
 
 template int mg_direct_solve<double>(const LevelArgs<double> &, double *, hipStream_t);
-template int mg_direct_solve<float>(const LevelArgs<float> &, float *, hipStream_t);
-template size_t direct_lds_bytes<double>(int, int);
-template size_t direct_lds_bytes<float>(int, int);
+template int mg_direct_solve<float>(const LevelArgs<float> &, double *, hipStream_t);
 template int mg_direct_set_lds_limit<double>(size_t);
 template int mg_direct_set_lds_limit<float>(size_t);
 template int batch_direct<double>(const DirectEntry<double> *, const int *, int, const int *, int, int, size_t,

@@ -362,8 +362,8 @@ static int set_level_stencils(stz_sim *s, DevMG<T> &mg, const std::vector<std::v
     return STZ_OK;
 }
 
-// the direct level of hierarchy mg: its mode inverses in precision T, or
-// none
+// the direct level of hierarchy mg: its mode inverses (FP64 in both
+// hierarchies), or none
 template <typename T> static int set_direct(DevMG<T> &mg, int ld, const std::vector<double> &G) {
     mg.args.ld = ld;
     mg.args.Gd = nullptr;
@@ -372,24 +372,22 @@ template <typename T> static int set_direct(DevMG<T> &mg, int ld, const std::vec
         (void)hipFree(mg.G);
         mg.G = nullptr;
     }
-    if (!mg.G) STZ_HIP(hipMalloc(&mg.G, G.size() * sizeof(T)));
+    if (!mg.G) STZ_HIP(hipMalloc(&mg.G, G.size() * sizeof(double)));
     mg.G_elems = G.size();
-    std::vector<T> tmp(G.begin(), G.end());
-    STZ_HIP(hipMemcpy(mg.G, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    STZ_HIP(hipMemcpy(mg.G, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
     mg.args.Gd = mg.G;
-    return mg_direct_set_lds_limit<T>(direct_lds_bytes<T>(mg.args.lv[ld].m, mg.args.lv[ld].n));
+    return mg_direct_set_lds_limit<T>(direct_lds_bytes(mg.args.lv[ld].m, mg.args.lv[ld].n));
 }
 
 // the finest level l in 1..lc with at most cfg.direct_nodes nodes, an
-// x-invariant operator of even width and a source that fits the mode
-// launch's LDS is solved exactly; the V-cycle stops there (both hierarchies)
+// x-invariant operator and a shape the direct launches cover (direct_fits)
+// is solved exactly; the V-cycle stops there (both hierarchies)
 static int set_direct_level(stz_sim *s) {
     int ld = -1;
     const int lc = s->mg64.args.lc;
     for (int l = 1; l <= lc && l < (int)s->hl.size() && s->cfg.direct_nodes > 0; l++) {
         const HostLevel &H = s->hl[l];
-        if (H.xinv && H.m % 2 == 0 && (long long)H.m * H.n <= s->cfg.direct_nodes &&
-            direct_lds_bytes<double>(H.m, H.n) <= 160 * 1024) {
+        if (H.xinv && (long long)H.m * H.n <= s->cfg.direct_nodes && direct_fits(H.m, H.n)) {
             ld = l;
             break;
         }

@@ -31,7 +31,7 @@ template <typename T> struct DevMG {
     T *pack = nullptr;       // constant part of the workgroup solver's LDS image
     size_t pack_elems = 0;
     size_t lds = 0;
-    T *G = nullptr;          // the direct level's mode inverses (mg_direct.hip)
+    double *G = nullptr;     // the direct level's mode inverses (mg_direct.hip, FP64)
     size_t G_elems = 0;
 
     void free_all() {
