@@ -13,11 +13,21 @@ The headline value is the mixed-precision form (FP32 V-cycles inside FP64
 iterative refinement); the plain FP64 form runs the same steps from the same
 state, and the two are compared field by field.
 
+Both forms use the reference's adaptive V-cycle count (tgmg::solve with a
+tgmg_predict, tgmg.in.cc:266-301: run the predicted number of cycles, test
+the residual, add cycles in triangular steps until it is below the
+threshold, then one extra cycle) with predictor constants for this solver:
+first test after 1 cycle, decay 2, against the reference's compile-time 8
+and 32 (tgmg_config.hh:26-37), which suit a CPU solver whose cycles each
+gain less and whose tests cost a residual sweep.  Every solve still stops
+only below the same threshold.  The same steps with the reference's
+constants are timed as well and reported under `with_reference_predictor`.
+
 `roofline` is for the kernel that takes the largest share of a mixed step
 (top-level smoothing launches, advection), each timed live with HIP events
 on the simulation's own stream; its algorithmic bytes follow the scope
 table's per-node figures (24 B FP32 / 48 B FP64 per node and sweep, 4.5 words
-per fine node for a restriction or a prolongation, 32 B for the FP64 read-
+(18 B FP32, 36 B FP64) per fine node for a restriction or a prolongation, 32 B for the FP64 read-
 modify-write of vel += e, 136 B per cell for the advection).  `roofline_step`
 is the whole step: cells x bytes per cell-step (312 + 176 k + 80 r mixed,
 312 + 352 k FP64, for k V-cycles and r refinements per solve) / time.
@@ -43,6 +53,7 @@ METRIC = "cell-steps/s at 1024^2: FP32-MG+FP64 refine vs FP64 @1e-8 rel; % HBM p
 HBM_PEAK_GBS = 8000.0
 YIELD_T = 425.0      # first yield of the seeded 1024^2 workload (elastic before)
 PLASTIC_T = 600.0    # plastic-flow sample time, when the run reaches it
+REFERENCE_PREDICT = (8, 32)  # tgmg_predict_init, tgmg_predict_decay (tgmg_config.hh:26,37)
 TIGHT = 1e-15        # threshold multiple of the full-size parity check: the
                      # reference's own solution moves < 2e-11 (stress) from there to 1e-16 x
 
@@ -60,7 +71,7 @@ def errs_of(fa, fb):
             "velocity": group_err(fa, fb, [0, 1]), "refmap": group_err(fa, fb, [7, 8])}
 
 
-def run_form(params, planes, form, dt, warmup, steps, nu, tol, stream, keep=()):
+def run_form(params, planes, form, dt, warmup, steps, nu, tol, stream, keep=(), predict=(1, 2)):
     import torch
     from stz_amd import ShearSim
     sim = ShearSim(params, form)
@@ -68,7 +79,7 @@ def run_form(params, planes, form, dt, warmup, steps, nu, tol, stream, keep=()):
     sim.set_fields(planes)
     sim.set_vel(np.zeros((params["n"] + 1, params["m"], 2)))
     sim.time = 0.0
-    sim.set_mg_config(nu1=nu, nu2=nu, tol_mult=tol)
+    sim.set_mg_config(nu1=nu, nu2=nu, tol_mult=tol, predict_init=predict[0], predict_decay=predict[1])
     kept = {}
     if 0 in keep:
         kept[0] = (sim.get_fields(), sim.get_vel(), sim.time)
@@ -120,7 +131,7 @@ def kernel_rooflines(sims, params, dt, nu, stream, launches=50):
     for form, sim in sims.items():
         w = 4 if form == "mixed" else 8        # bytes per word of the V-cycle
         sweep = 6 * w                          # read z, b; write z (vec2)
-        transfer = 9 * w                       # 4.5 vec2-component words per fine node
+        transfer = 4.5 * w                     # restriction or prolongation: 4.5 words per fine node
         ks = {
             "top_pre": ("smooth_tile_kernel<%s,64,32,%d,%d,true,true> (top level: %d sweeps + fused restriction)"
                         % ("float" if w == 4 else "double", nu, 1 if w == 4 else 0, nu),
@@ -214,8 +225,13 @@ def main():
     ap.add_argument("--nu", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-4,
                     help="residual threshold as a multiple of the reference's")
+    ap.add_argument("--predict-init", type=int, default=1,
+                    help="V-cycles before the first convergence test of the first solve (tgmg_predict_init)")
+    ap.add_argument("--predict-decay", type=int, default=2,
+                    help="the predictor's decay after a converged solve (tgmg_predict_decay)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
+    predict = (args.predict_init, args.predict_decay)
     if args.gpus != 1:
         print("bench.py: this benchmark runs on one GPU only (N=%d requested); the step "
               "would shard into y-slabs across GPUs, which is not built" % args.gpus)
@@ -232,14 +248,19 @@ def main():
     stream = torch.cuda.Stream()
     cells = N * N
 
-    log("bench: %dx%d, warmup %d, steps %d, V(%d,%d), threshold %g x reference" %
-        (N, N, args.warmup, args.steps, args.nu, args.nu, args.tol))
+    log("bench: %dx%d, warmup %d, steps %d, V(%d,%d), threshold %g x reference, predictor init %d decay %d" %
+        (N, N, args.warmup, args.steps, args.nu, args.nu, args.tol, predict[0], predict[1]))
     total = args.warmup + args.steps
     checks = [c for c in (500, 1000, 2000, 3000, 3500, 3750, 4000, 4500) if c < total]
-    # phase samples by simulation time: the start of the timed steps (elastic
-    # while t < YIELD_T), and t = PLASTIC_T when the timed steps reach it
+    # phase samples by simulation time: the end of the timed steps when it is
+    # still elastic (t < YIELD_T), else their start if that is; and
+    # t = PLASTIC_T when the timed steps reach it.  (Right after the wall ramp,
+    # t ~ 1, the stresses are so small that a relative comparison measures the
+    # solvers' rounding floor rather than the step.)
     samples_at = {}
-    if args.warmup * dt < YIELD_T:
+    if total * dt < YIELD_T:
+        samples_at["elastic"] = total
+    elif args.warmup * dt < YIELD_T:
         samples_at["elastic"] = args.warmup
     plastic_step = int(round(PLASTIC_T / dt))
     if args.warmup <= plastic_step < total:
@@ -250,11 +271,22 @@ def main():
     res = {}
     for form in ("fp64", "mixed"):
         sim, el, cyc, info, kept = run_form(params, planes, form, dt, args.warmup, args.steps, args.nu, args.tol,
-                                            stream, keep)
+                                            stream, keep, predict)
         res[form] = dict(sim=sim, elapsed=el, cycles=cyc, info=info, kept=kept,
                          value=cells * args.steps / el, ms=1e3 * el / args.steps)
         log("bench: %-5s %.3f ms/step, %.3e cell-steps/s, %.2f cycles/solve, last %s" %
             (form, res[form]["ms"], res[form]["value"], cyc, info))
+    # the same steps with the reference's predictor constants (tgmg_config.hh:26-45)
+    ref_pred = {}
+    if predict != REFERENCE_PREDICT:
+        for form in ("fp64", "mixed"):
+            sim, el, cyc, info, _ = run_form(params, planes, form, dt, args.warmup, args.steps, args.nu, args.tol,
+                                             stream, (), REFERENCE_PREDICT)
+            sim.close()
+            ref_pred[form] = {"value": cells * args.steps / el, "ms_per_step": 1e3 * el / args.steps,
+                              "cycles_per_solve": cyc}
+            log("bench: %-5s with the reference's predictor: %.3f ms/step, %.3e cell-steps/s, %.2f cycles/solve" %
+                (form, ref_pred[form]["ms_per_step"], ref_pred[form]["value"], cyc))
 
     lockstep = {k: errs_of(res["mixed"]["kept"][k][0], res["fp64"]["kept"][k][0])
                 for k in checks if k in res["mixed"]["kept"] and k in res["fp64"]["kept"]}
@@ -272,7 +304,8 @@ def main():
             g.set_fields(fields)
             g.set_vel(vel)
             g.time = t
-            g.set_mg_config(nu1=args.nu, nu2=args.nu, tol_mult=args.tol)
+            g.set_mg_config(nu1=args.nu, nu2=args.nu, tol_mult=args.tol, predict_init=predict[0],
+                            predict_decay=predict[1])
             g.step_forward_quasistatic(dt, 1)
             one[form] = g.get_fields()
             g.close()
@@ -351,6 +384,8 @@ def main():
             "mixed_over_fp64": mixed["value"] / fp64["value"],
             "mixed_vs_fp64_rel_err": errs,
             "cycles_per_solve": {"mixed": mixed["cycles"], "fp64": fp64["cycles"]},
+            "predictor": {"init": predict[0], "decay": predict[1], "mult": 16},
+            "with_reference_predictor": ref_pred or None,
             "fp64_vs_reference_at_1e-15": vs_ref,
             "kernels": kern,
         },

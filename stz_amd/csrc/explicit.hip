@@ -43,66 +43,112 @@ __device__ __forceinline__ static double eno2(double p0, double p1, double p2, d
 // earlier step, instead of being evaluated.  A substep count of 255 means the
 // record saturated; such a cell evaluates its plastic loop as usual.
 // ---------------------------------------------------------------------------
+// advection tile: 64 x 4 cells per workgroup (one wave per row), staged
+// with a halo of two columns and two rows on each side (the ENO stencils
+// reach +-2, the staggered velocity +1)
+constexpr int kAdvTileX = 64, kAdvTileY = 4;
+constexpr int kAdvRows = kAdvTileY + 4, kAdvCols = kAdvTileX + 4, kAdvPlanes = 9;
+constexpr int kAdvPlaneElems = kAdvRows * kAdvCols;
+constexpr int kAdvThreads = kAdvTileX * kAdvTileY;
+constexpr int kAdvPasses = (kAdvPlaneElems + kAdvThreads - 1) / kAdvThreads;
+enum { kU = 0, kV, kX, kY, kP, kQ, kS, kTau, kChi };
+
+// stage rows j0-2 .. j0+5 and columns i0-2 .. i0+65 (x periodic) of u, v,
+// X, Y, p, q, s, tau, chi into tile[plane][row][column]; rows outside -1..n
+// hold a copy of the nearest one and are never read
+__device__ __forceinline__ static void stage_advect_tile(const FieldPtrs &o, int m, int n, int i0, int j0,
+                                                         double *tile) {
+    const int t = threadIdx.x + kAdvTileX * threadIdx.y;
+    int off[kAdvPasses];
+#pragma unroll
+    for (int w = 0; w < kAdvPasses; w++) {
+        const int e = min(t + kAdvThreads * w, kAdvPlaneElems - 1);
+        const int row = e / kAdvCols, col = e - row * kAdvCols;
+        const int r = min(max(j0 - 2 + row, -1), n);
+        int ca = i0 - 2 + col;
+        if (ca < 0) ca += m;
+        while (ca >= m) ca -= m;
+        off[w] = ca + m * (r + 1);
+    }
+    const double *const P[kAdvPlanes] = {o.u, o.v, o.X, o.Y, o.p, o.q, o.s, o.tau, o.chi};
+    double v[kAdvPlanes][kAdvPasses];
+#pragma unroll
+    for (int k = 0; k < kAdvPlanes; k++)
+#pragma unroll
+        for (int w = 0; w < kAdvPasses; w++) v[k][w] = P[k][off[w]];
+#pragma unroll
+    for (int k = 0; k < kAdvPlanes; k++)
+#pragma unroll
+        for (int w = 0; w < kAdvPasses; w++)
+            if (t + kAdvThreads * w < kAdvPlaneElems) tile[k * kAdvPlaneElems + t + kAdvThreads * w] = v[k][w];
+    __syncthreads();
+}
+
 template <bool REPLAY>
 __device__ __forceinline__ static void advect_body(const SimConst &c, const FieldPtrs &o, const FieldPtrs &w,
                                                    double dt, uint32_t *dec, const uint32_t *rep, int bx,
                                                    int by) {
     const int m = c.m, n = c.n;
-    const int i = bx * blockDim.x + threadIdx.x;
-    const int j = by * blockDim.y + threadIdx.y;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = bx * kAdvTileX + tx;
+    const int j = by * kAdvTileY + ty;
+    // the block's tile of the nine input planes with its +-2 halo, staged
+    // with every load of a thread in flight at once; each read below is an
+    // LDS access at a fixed offset from the thread's own cell
+    __shared__ double tile[kAdvPlanes * kAdvRows * kAdvCols];
+    stage_advect_tile(o, m, n, bx * kAdvTileX, by * kAdvTileY, tile);
     if (i >= m || j >= n) return;
+#define RD(k, dx, dy) tile[((k) * kAdvRows + ty + 2 + (dy)) * kAdvCols + tx + 2 + (dx)]
     const uint32_t rb = REPLAY ? rep[i + m * j] : 0u;
     // the recorded branch of decision bit b, or -1 (evaluate) without replay
 #define RB(b) (REPLAY ? (int)((rb >> (b)) & 1u) : -1)
 #define UP(natural, mask) (REPLAY ? (rb & (mask)) != 0 : (natural))
     const double hx = 0.5 * c.xsp * dt, hy = 0.5 * c.ysp * dt, lx = c.lx;
-    const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
-    const int ill = i <= 1 ? i + m - 2 : i - 2, irr = i >= m - 2 ? i + 2 - m : i + 2;
     uint32_t bits = 0, br;
     double uc, vc, two_omega, dtmp;
 
     if (j > 0) {
         double Xx, Xy, Yx, Yy;
-        uc = PL(o.u, i, j);
-        vc = PL(o.v, i, j);
-        const double X0 = PL(o.X, i, j), Y0 = PL(o.Y, i, j);
+        uc = RD(kU, 0, 0);
+        vc = RD(kV, 0, 0);
+        const double X0 = RD(kX, 0, 0), Y0 = RD(kY, 0, 0);
         if (UP(uc > 0, STZ_D_NODE_UPX)) {
             bits |= STZ_D_NODE_UPX;
-            Xx = hx * eno2(PL(o.X, ir, j) + (i == m - 1 ? lx : 0), X0, PL(o.X, il, j) + (i == 0 ? -lx : 0),
-                           PL(o.X, ill, j) + (i <= 1 ? -lx : 0), br, RB(2));
+            Xx = hx * eno2(RD(kX, 1, 0) + (i == m - 1 ? lx : 0), X0, RD(kX, -1, 0) + (i == 0 ? -lx : 0),
+                           RD(kX, -2, 0) + (i <= 1 ? -lx : 0), br, RB(2));
             bits |= br ? STZ_D_NODE_ENO_XX : 0;
-            Yx = hx * eno2(PL(o.Y, ir, j), Y0, PL(o.Y, il, j), PL(o.Y, ill, j), br, RB(3));
+            Yx = hx * eno2(RD(kY, 1, 0), Y0, RD(kY, -1, 0), RD(kY, -2, 0), br, RB(3));
             bits |= br ? STZ_D_NODE_ENO_YX : 0;
         } else {
-            Xx = -hx * eno2(PL(o.X, il, j) + (i == 0 ? -lx : 0), X0, PL(o.X, ir, j) + (i == m - 1 ? lx : 0),
-                            PL(o.X, irr, j) + (i >= m - 2 ? lx : 0), br, RB(2));
+            Xx = -hx * eno2(RD(kX, -1, 0) + (i == 0 ? -lx : 0), X0, RD(kX, 1, 0) + (i == m - 1 ? lx : 0),
+                            RD(kX, 2, 0) + (i >= m - 2 ? lx : 0), br, RB(2));
             bits |= br ? STZ_D_NODE_ENO_XX : 0;
-            Yx = -hx * eno2(PL(o.Y, il, j), Y0, PL(o.Y, ir, j), PL(o.Y, irr, j), br, RB(3));
+            Yx = -hx * eno2(RD(kY, -1, 0), Y0, RD(kY, 1, 0), RD(kY, 2, 0), br, RB(3));
             bits |= br ? STZ_D_NODE_ENO_YX : 0;
         }
         if (UP(vc > 0, STZ_D_NODE_UPY)) {
             bits |= STZ_D_NODE_UPY;
             if (j > 1) {
-                Xy = hy * eno2(PL(o.X, i, j + 1), X0, PL(o.X, i, j - 1), PL(o.X, i, j - 2), br, RB(4));
+                Xy = hy * eno2(RD(kX, 0, 1), X0, RD(kX, 0, -1), RD(kX, 0, -2), br, RB(4));
                 bits |= br ? STZ_D_NODE_ENO_XY : 0;
-                Yy = hy * eno2(PL(o.Y, i, j + 1), Y0, PL(o.Y, i, j - 1), PL(o.Y, i, j - 2), br, RB(5));
+                Yy = hy * eno2(RD(kY, 0, 1), Y0, RD(kY, 0, -1), RD(kY, 0, -2), br, RB(5));
                 bits |= br ? STZ_D_NODE_ENO_YY : 0;
             } else {
                 const double hs = hy * 2;
-                Xy = hs * (X0 - PL(o.X, i, j - 1));
-                Yy = hs * (Y0 - PL(o.Y, i, j - 1));
+                Xy = hs * (X0 - RD(kX, 0, -1));
+                Yy = hs * (Y0 - RD(kY, 0, -1));
                 bits |= STZ_D_NODE_ONESIDED;
             }
         } else {
             if (j < n - 1) {
-                Xy = -hy * eno2(PL(o.X, i, j - 1), X0, PL(o.X, i, j + 1), PL(o.X, i, j + 2), br, RB(4));
+                Xy = -hy * eno2(RD(kX, 0, -1), X0, RD(kX, 0, 1), RD(kX, 0, 2), br, RB(4));
                 bits |= br ? STZ_D_NODE_ENO_XY : 0;
-                Yy = -hy * eno2(PL(o.Y, i, j - 1), Y0, PL(o.Y, i, j + 1), PL(o.Y, i, j + 2), br, RB(5));
+                Yy = -hy * eno2(RD(kY, 0, -1), Y0, RD(kY, 0, 1), RD(kY, 0, 2), br, RB(5));
                 bits |= br ? STZ_D_NODE_ENO_YY : 0;
             } else {
                 const double hs = -hy * 2;
-                Xy = hs * (X0 - PL(o.X, i, j + 1));
-                Yy = hs * (Y0 - PL(o.Y, i, j + 1));
+                Xy = hs * (X0 - RD(kX, 0, 1));
+                Yy = hs * (Y0 - RD(kY, 0, 1));
                 bits |= STZ_D_NODE_ONESIDED;
             }
         }
@@ -112,52 +158,51 @@ __device__ __forceinline__ static void advect_body(const SimConst &c, const Fiel
     } else {
         // row 0 reference map is left to set_boundaries; keep it, and the
         // ghost row -1, in the new plane set
-        PL(w.X, i, 0) = PL(o.X, i, 0);
-        PL(w.Y, i, 0) = PL(o.Y, i, 0);
-        PL(w.X, i, -1) = PL(o.X, i, -1);
-        PL(w.Y, i, -1) = PL(o.Y, i, -1);
-        PL(w.p, i, -1) = PL(o.p, i, -1);
-        PL(w.q, i, -1) = PL(o.q, i, -1);
-        PL(w.s, i, -1) = PL(o.s, i, -1);
-        PL(w.tau, i, -1) = PL(o.tau, i, -1);
-        PL(w.chi, i, -1) = PL(o.chi, i, -1);
+        PL(w.X, i, 0) = RD(kX, 0, 0);
+        PL(w.Y, i, 0) = RD(kY, 0, 0);
+        PL(w.X, i, -1) = RD(kX, 0, -1);
+        PL(w.Y, i, -1) = RD(kY, 0, -1);
+        PL(w.p, i, -1) = RD(kP, 0, -1);
+        PL(w.q, i, -1) = RD(kQ, 0, -1);
+        PL(w.s, i, -1) = RD(kS, 0, -1);
+        PL(w.tau, i, -1) = RD(kTau, 0, -1);
+        PL(w.chi, i, -1) = RD(kChi, 0, -1);
     }
     if (j == n - 1) {
-        PL(w.X, i, n) = PL(o.X, i, n);
-        PL(w.Y, i, n) = PL(o.Y, i, n);
-        PL(w.p, i, n) = PL(o.p, i, n);
-        PL(w.q, i, n) = PL(o.q, i, n);
-        PL(w.s, i, n) = PL(o.s, i, n);
-        PL(w.tau, i, n) = PL(o.tau, i, n);
-        PL(w.chi, i, n) = PL(o.chi, i, n);
+        PL(w.X, i, n) = RD(kX, 0, 1);
+        PL(w.Y, i, n) = RD(kY, 0, 1);
+        PL(w.p, i, n) = RD(kP, 0, 1);
+        PL(w.q, i, n) = RD(kQ, 0, 1);
+        PL(w.s, i, n) = RD(kS, 0, 1);
+        PL(w.tau, i, n) = RD(kTau, 0, 1);
+        PL(w.chi, i, n) = RD(kChi, 0, 1);
     }
 
     // staggered velocity and spin, shear_sim.cc:330-333
     {
-        const double u00 = PL(o.u, i, j), u10 = PL(o.u, ir, j), u01 = PL(o.u, i, j + 1), u11 = PL(o.u, ir, j + 1);
-        const double v00 = PL(o.v, i, j), v10 = PL(o.v, ir, j), v01 = PL(o.v, i, j + 1), v11 = PL(o.v, ir, j + 1);
+        const double u00 = RD(kU, 0, 0), u10 = RD(kU, 1, 0), u01 = RD(kU, 0, 1), u11 = RD(kU, 1, 1);
+        const double v00 = RD(kV, 0, 0), v10 = RD(kV, 1, 0), v01 = RD(kV, 0, 1), v11 = RD(kV, 1, 1);
         uc = 0.25 * (u00 + u10 + u01 + u11);
         vc = 0.25 * (v00 + v10 + v01 + v11);
         two_omega = hx * (-v00 + v10 - v01 + v11) - hy * (-u00 - u10 + u01 + u11);
     }
 
     // ENO derivatives of p,q,s,tau,chi, shear_sim.cc:336-338
-    const double *F[5] = {o.p, o.q, o.s, o.tau, o.chi};
     double f0[5], f1[5], f2[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++) f0[k] = PL(F[k], i, j);
+    for (int k = 0; k < 5; k++) f0[k] = RD(kP + k, 0, 0);
     if (UP(uc > 0, STZ_D_CELL_UPX)) {
         bits |= STZ_D_CELL_UPX;
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            f1[k] = hx * eno2(PL(F[k], ir, j), f0[k], PL(F[k], il, j), PL(F[k], ill, j), br,
+            f1[k] = hx * eno2(RD(kP + k, 1, 0), f0[k], RD(kP + k, -1, 0), RD(kP + k, -2, 0), br,
                               RB(STZ_D_CELL_ENO_X0 + k));
             bits |= br << (STZ_D_CELL_ENO_X0 + k);
         }
     } else {
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            f1[k] = -hx * eno2(PL(F[k], il, j), f0[k], PL(F[k], ir, j), PL(F[k], irr, j), br,
+            f1[k] = -hx * eno2(RD(kP + k, -1, 0), f0[k], RD(kP + k, 1, 0), RD(kP + k, 2, 0), br,
                                RB(STZ_D_CELL_ENO_X0 + k));
             bits |= br << (STZ_D_CELL_ENO_X0 + k);
         }
@@ -167,28 +212,28 @@ __device__ __forceinline__ static void advect_body(const SimConst &c, const Fiel
         if (j > 0) {
 #pragma unroll
             for (int k = 0; k < 5; k++) {
-                f2[k] = hy * eno2(PL(F[k], i, j + 1), f0[k], PL(F[k], i, j - 1), PL(F[k], i, j - 2), br,
+                f2[k] = hy * eno2(RD(kP + k, 0, 1), f0[k], RD(kP + k, 0, -1), RD(kP + k, 0, -2), br,
                                   RB(STZ_D_CELL_ENO_Y0 + k));
                 bits |= br << (STZ_D_CELL_ENO_Y0 + k);
             }
         } else {
             const double hs = hy * 2;
 #pragma unroll
-            for (int k = 0; k < 5; k++) f2[k] = hs * (f0[k] - PL(F[k], i, j - 1));
+            for (int k = 0; k < 5; k++) f2[k] = hs * (f0[k] - RD(kP + k, 0, -1));
             bits |= STZ_D_CELL_ONESIDED;
         }
     } else {
         if (j < n - 1) {
 #pragma unroll
             for (int k = 0; k < 5; k++) {
-                f2[k] = -hy * eno2(PL(F[k], i, j - 1), f0[k], PL(F[k], i, j + 1), PL(F[k], i, j + 2), br,
+                f2[k] = -hy * eno2(RD(kP + k, 0, -1), f0[k], RD(kP + k, 0, 1), RD(kP + k, 0, 2), br,
                                    RB(STZ_D_CELL_ENO_Y0 + k));
                 bits |= br << (STZ_D_CELL_ENO_Y0 + k);
             }
         } else {
             const double hs = -hy * 2;
 #pragma unroll
-            for (int k = 0; k < 5; k++) f2[k] = hs * (f0[k] - PL(F[k], i, j + 1));
+            for (int k = 0; k < 5; k++) f2[k] = hs * (f0[k] - RD(kP + k, 0, 1));
             bits |= STZ_D_CELL_ONESIDED;
         }
     }
@@ -225,14 +270,33 @@ __device__ __forceinline__ static void advect_body(const SimConst &c, const Fiel
     }
 #undef RB
 #undef UP
+#undef RD
 }
+// The dispatcher hands workgroups to the eight XCDs round robin (linear id
+// mod 8), and each XCD has its own L2.  The row halo a 64x4 block reads (+-2
+// rows for the advection's ENO stencils, one row for the source and the
+// correction) is the interior of its vertical neighbours, so the block id is
+// remapped to give each XCD one contiguous band of tile rows: a halo row is
+// then read from HBM once per band rather than once per block.
+__device__ __forceinline__ static void xcd_band_tile(int &bx, int &by) {
+    const int gx = gridDim.x, total = gx * gridDim.y, b = blockIdx.x + gx * blockIdx.y;
+    const int q = total >> 3, r = total & 7, x = b & 7, k = b >> 3;
+    const int t = x * q + min(x, r) + k;
+    bx = t % gx;
+    by = t / gx;
+}
+
 __global__ void __launch_bounds__(256) advect_kernel(SimConst c, FieldPtrs o, FieldPtrs w, double dt,
                                                      uint32_t *dec) {
-    advect_body<false>(c, o, w, dt, dec, nullptr, blockIdx.x, blockIdx.y);
+    int bx, by;
+    xcd_band_tile(bx, by);
+    advect_body<false>(c, o, w, dt, dec, nullptr, bx, by);
 }
 __global__ void __launch_bounds__(256) advect_replay_kernel(SimConst c, FieldPtrs o, FieldPtrs w, double dt,
                                                             uint32_t *dec, const uint32_t *rep) {
-    advect_body<true>(c, o, w, dt, dec, rep, blockIdx.x, blockIdx.y);
+    int bx, by;
+    xcd_band_tile(bx, by);
+    advect_body<true>(c, o, w, dt, dec, rep, bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -478,7 +542,9 @@ __device__ __forceinline__ static void source_body(const SimConst &c, const Fiel
 }
 __global__ void __launch_bounds__(256) source_kernel(SimConst c, FieldPtrs f, double xfac, double yfac,
                                                      double xbc, double bvel, double *src) {
-    source_body(c, f, xfac, yfac, xbc, bvel, src, blockIdx.x, blockIdx.y);
+    int bx, by;
+    xcd_band_tile(bx, by);
+    source_body(c, f, xfac, yfac, xbc, bvel, src, bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -542,7 +608,9 @@ __device__ __forceinline__ static void correct_body(const SimConst &c, const Fie
 }
 __global__ void __launch_bounds__(256) correct_kernel(SimConst c, FieldPtrs f, const double *vel, double dt,
                                                       double bvel, double bpos) {
-    correct_body(c, f, vel, dt, bvel, bpos, blockIdx.x, blockIdx.y);
+    int bx, by;
+    xcd_band_tile(bx, by);
+    correct_body(c, f, vel, dt, bvel, bpos, bx, by);
 }
 
 // set_boundaries on its own, shear_sim.cc:552-581

@@ -108,15 +108,14 @@ __device__ __forceinline__ static void direct_mode_body(int M, int Nn, double *_
     }
     // stage the source (vec2 nodes) transposed, and the twiddle table
     const V2 *bv = reinterpret_cast<const V2 *>(b);
-    V2 zero;
-    zero.x = 0;
-    zero.y = 0;
     for (int base = 0; base < nodes; base += kStage * kModeThreads) {
         V2 v[kStage];
+        // unconditional loads (of node 0 past the end), so that all of a
+        // thread's loads are in flight together
 #pragma unroll
         for (int u = 0; u < kStage; u++) {
             const int nd = base + threadIdx.x + kModeThreads * u;
-            v[u] = nd < nodes ? bv[nd] : zero;
+            v[u] = bv[nd < nodes ? nd : 0];
         }
 #pragma unroll
         for (int u = 0; u < kStage; u++) {

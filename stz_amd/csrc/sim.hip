@@ -452,19 +452,6 @@ static int refine_resid(stz_sim *s, bool norm = true) {
     return mg_resid64_to32(t, f.b, f.z, s->partial, s->dnorm, s->stream);
 }
 
-// mean-square residual of the top level (tgmg.hh:146) -> host
-static int top_l2(stz_sim *s, double *out) {
-    int e;
-    if (s->precision == STZ_PREC_MIXED) {
-        if ((e = refine_resid(s))) return e;
-    } else {
-        if ((e = mg_resid_norm(s->top64(), s->partial, s->dnorm, s->stream))) return e;
-    }
-    STZ_HIP(hipStreamSynchronize(s->stream));
-    *out = *s->hnorm / ((double)s->c.m * (s->c.n + 1));
-    return STZ_OK;
-}
-
 // the tiled V-cycle may leave the top FP64 level's result in its second
 // buffer; move it back into vel so the public pointer stays valid
 static int top_home(stz_sim *s) {
@@ -519,26 +506,55 @@ static int run_cycles(stz_sim *s, int k, bool resid_after) {
     return STZ_OK;
 }
 
-// tgmg::solve(type=4, tgmg_predict&) (tgmg.in.cc:266-301)
+// enqueue the FP64 form's convergence norm (the mixed form's comes with the
+// residual of its last cycle) and mark where it ends
+static int enqueue_test(stz_sim *s) {
+    int e;
+    if (s->precision == STZ_PREC_FP64 && (e = mg_resid_norm(s->top64(), s->partial, s->dnorm, s->stream))) return e;
+    STZ_HIP(hipEventRecord(s->norm_ev, s->stream));
+    return STZ_OK;
+}
+
+// wait for the norm enqueued by enqueue_test (work enqueued after it may
+// still run) and read it
+static int read_test(stz_sim *s, double *acc) {
+    STZ_HIP(hipEventSynchronize(s->norm_ev));
+    *acc = *s->hnorm / ((double)s->c.m * (s->c.n + 1));
+    return STZ_OK;
+}
+
+// tgmg::solve(type=4, tgmg_predict&) (tgmg.in.cc:266-301).  Every test is
+// followed by at least one more cycle: the extra cycle when it passes, the
+// first of the next k when it fails.  So with extra_cycles >= 1 that cycle is
+// enqueued before the host waits for the test's norm, and the device stays
+// busy through the wait; the kernels and their order are those of the
+// plain schedule, so the result is the same bit for bit.
 static int mg_solve_impl(stz_sim *s) {
     int e;
     const stz_mg_config &g = s->cfg;
     const double tol = s->acc_base * g.tol_mult;
     const int max_thresh = g.predict_mult * g.predict_max_iters;
+    const bool mixed = s->precision == STZ_PREC_MIXED, ahead = g.extra_cycles > 0;
     std::memset(&s->last, 0, sizeof s->last);
     s->last.tolerance = tol;
     double acc;
     int n = s->lim / g.predict_mult;
-    if (s->precision == STZ_PREC_MIXED) {  // first source of the correction equation
+    if (mixed) {  // first source of the correction equation
         // (its norm is the first test's when no cycle comes before it)
         if ((e = refine_resid(s, n == 0))) return e;
         s->last.refinements = 0;
     }
     if ((e = run_cycles(s, n, true))) return e;
-    if (s->precision == STZ_PREC_MIXED) {
-        STZ_HIP(hipStreamSynchronize(s->stream));
-        acc = *s->hnorm / ((double)s->c.m * (s->c.n + 1));
-    } else if ((e = top_l2(s, &acc))) return e;
+    if ((e = enqueue_test(s))) return e;
+    // a test whose failure would end the solve (the predictor past its
+    // limit) gets no cycle ahead, so a failed solve leaves the state the
+    // plain schedule leaves
+    bool spec = ahead && s->lim + g.predict_mult <= max_thresh;
+    if (spec) {
+        if ((e = one_cycle(s, false))) return e;
+        s->last.total_cycles++;
+    }
+    if ((e = read_test(s, &acc))) return e;
     if (acc < tol) {
         if (s->lim > 0) s->lim -= 1 + s->lim / g.predict_decay;
     } else {
@@ -553,16 +569,22 @@ static int mg_solve_impl(stz_sim *s) {
                 return STZ_ERR_NOT_CONVERGED;
             }
             n += k;
-            if ((e = run_cycles(s, k, true))) return e;
-            if (s->precision == STZ_PREC_MIXED) {
-                STZ_HIP(hipStreamSynchronize(s->stream));
-                acc = *s->hnorm / ((double)s->c.m * (s->c.n + 1));
-            } else if ((e = top_l2(s, &acc))) return e;
+            // the cycle enqueued ahead is the first of these k
+            const int rest = spec ? k - 1 : k;
+            if (spec && mixed && (e = refine_resid(s, rest == 0))) return e;
+            if ((e = run_cycles(s, rest, true))) return e;
+            if ((e = enqueue_test(s))) return e;
+            spec = ahead && s->lim + (k + 1) * g.predict_mult <= max_thresh;
+            if (spec) {
+                if ((e = one_cycle(s, false))) return e;
+                s->last.total_cycles++;
+            }
+            if ((e = read_test(s, &acc))) return e;
         } while (acc >= tol);
     }
     s->solves++;
     s->vcount += n;
-    if ((e = run_cycles(s, g.extra_cycles, false))) return e;
+    if ((e = run_cycles(s, spec ? g.extra_cycles - 1 : g.extra_cycles, false))) return e;
     s->last.cycles = n;
     s->last.residual = acc;
     s->last.status = std::isnan(acc) ? STZ_ERR_NAN : STZ_OK;
@@ -689,6 +711,10 @@ int stz_sim_create(const stz_sim_params *p, int precision, stz_sim **out) {
     // written by the summing kernels straight into coherent pinned host
     // memory: no copy launch before each convergence test
     if (hipHostMalloc(&s->hnorm, 8 * sizeof(double), hipHostMallocCoherent) != hipSuccess) return fail(STZ_ERR_HIP);
+    if (hipEventCreateWithFlags(&s->norm_ev, hipEventDisableTiming) != hipSuccess) {
+        s->norm_ev = nullptr;
+        return fail(STZ_ERR_HIP);
+    }
     s->dnorm = s->hnorm;
 
     build_dims(s);
@@ -715,6 +741,7 @@ int stz_sim_destroy(stz_sim *s) {
     if (s->rep) (void)hipFree(s->rep);
     if (s->partial) (void)hipFree(s->partial);
     if (s->hnorm) (void)hipHostFree(s->hnorm);
+    if (s->norm_ev) (void)hipEventDestroy(s->norm_ev);
     if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return STZ_OK;

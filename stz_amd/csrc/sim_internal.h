@@ -78,6 +78,7 @@ struct stz_sim {
     stz::DevMG<double> mg64;
     stz::DevMG<float> mg32;
     double *partial = nullptr, *dnorm = nullptr, *hnorm = nullptr;
+    hipEvent_t norm_ev = nullptr;  // marks the end of a convergence test's norm
     int nblocks_norm = 0;
     double acc_base = 0;
     stz_mg_config cfg{};
