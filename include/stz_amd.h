@@ -128,7 +128,13 @@ typedef struct stz_mg_config {
                                     of even width up to 128 and at most 72
                                     rows qualify); 0: the V-cycle runs to the
                                     coarsest level, solved exactly */
-    int32_t reserved[5];
+    int32_t predict_unit;        /* 0: after a failed test run k more cycles
+                                    (k = 1, 2, 3, ...) before the next, as
+                                    tgmg.in.cc:282-292 (default); 1: one more
+                                    cycle per test, for solvers whose test is
+                                    cheap next to a cycle (the predicted
+                                    count still adapts as tgmg_predict's) */
+    int32_t reserved[4];
 } stz_mg_config;
 
 /* Result of the last projection solve. */

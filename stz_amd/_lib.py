@@ -42,7 +42,7 @@ class MGConfig(C.Structure):
                 ("predict_decay", C.c_int32), ("predict_init", C.c_int32),
                 ("predict_max_iters", C.c_int32), ("tol_mult", C.c_double),
                 ("max_refine", C.c_int32), ("kernels", C.c_int32), ("direct_nodes", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("predict_unit", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class SolveInfo(C.Structure):

@@ -36,6 +36,7 @@ struct EnsState {
 // predictor settings shared by all members (tgmg_config.hh:26-45)
 struct EnsPredict {
     int mult, decay, max_thresh, extra;
+    int unit;   // one more cycle per failed test (stz_mg_config.predict_unit)
     int mixed;  // residual after every cycle but the last (mixed) or only before a test (FP64)
 };
 

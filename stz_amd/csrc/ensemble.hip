@@ -336,6 +336,7 @@ int build_tables(stz_ensemble *e, double dt) {
     e->pred.mult = e->cfg.predict_mult;
     e->pred.decay = e->cfg.predict_decay;
     e->pred.max_thresh = e->cfg.predict_mult * e->cfg.predict_max_iters;
+    e->pred.unit = e->cfg.predict_unit;
     e->pred.extra = e->cfg.extra_cycles;
     e->pred.mixed = e->precision == STZ_PREC_MIXED ? 1 : 0;
     // operators of every member for this dt (projection_impl)

@@ -931,7 +931,7 @@ __device__ static void ens_resolve(EnsState &st, double acc, const EnsPredict &P
                 st.remaining = P.extra;
                 continue;
             }
-            st.k += 1;
+            st.k = P.unit ? 1 : st.k + 1;
             st.lim += st.k * P.mult;
             if (st.lim > P.max_thresh) {
                 st.phase = ENS_FAILED;

@@ -550,6 +550,7 @@ static int mg_solve_impl(stz_sim *s) {
     // limit) gets no cycle ahead, so a failed solve leaves the state the
     // plain schedule leaves
     bool spec = ahead && s->lim + g.predict_mult <= max_thresh;
+    const bool unit = g.predict_unit != 0;
     if (spec) {
         if ((e = one_cycle(s, false))) return e;
         s->last.total_cycles++;
@@ -560,7 +561,8 @@ static int mg_solve_impl(stz_sim *s) {
     } else {
         int k = 0;
         do {
-            s->lim += ++k * g.predict_mult;
+            k = unit ? 1 : k + 1;
+            s->lim += k * g.predict_mult;
             if (s->lim > max_thresh) {
                 s->last.status = STZ_ERR_NOT_CONVERGED;
                 s->last.cycles = n;
@@ -574,7 +576,7 @@ static int mg_solve_impl(stz_sim *s) {
             if (spec && mixed && (e = refine_resid(s, rest == 0))) return e;
             if ((e = run_cycles(s, rest, true))) return e;
             if ((e = enqueue_test(s))) return e;
-            spec = ahead && s->lim + (k + 1) * g.predict_mult <= max_thresh;
+            spec = ahead && s->lim + (unit ? 1 : k + 1) * g.predict_mult <= max_thresh;
             if (spec) {
                 if ((e = one_cycle(s, false))) return e;
                 s->last.total_cycles++;
@@ -919,9 +921,9 @@ int stz_sim_set_mg_config(stz_sim *s, const stz_mg_config *g) {
         set_error("invalid multigrid configuration");
         return STZ_ERR_ARG;
     }
-    if (g->direct_nodes < 0 || g->max_refine < 1) {
-        set_error("invalid multigrid configuration (direct_nodes %d, max_refine %d)", g->direct_nodes,
-                  g->max_refine);
+    if (g->direct_nodes < 0 || g->max_refine < 1 || (g->predict_unit != 0 && g->predict_unit != 1)) {
+        set_error("invalid multigrid configuration (direct_nodes %d, max_refine %d, predict_unit %d)",
+                  g->direct_nodes, g->max_refine, g->predict_unit);
         return STZ_ERR_ARG;
     }
     const bool reset = g->predict_mult != s->cfg.predict_mult || g->predict_init != s->cfg.predict_init;

@@ -72,6 +72,24 @@ def test_members_equal_single_simulations(form, graphs):
 
 
 @pytest.mark.parametrize("form", ["mixed", "fp64"])
+def test_unit_predictor_members_equal_single_simulations(form):
+    """with predict_unit = 1 (one more cycle per failed test) and a fast
+    decay, from a cold predictor: the device state machine and the single
+    simulation's loop (which enqueues the cycle after each test before it
+    waits) take the same cycles, and more than one test per solve"""
+    print("case ensemble unit predictor", form, flush=True)
+    ens, singles = setup(form, tol_mult=1e-6)
+    for g in [ens] + singles:
+        g.set_mg_config(predict_unit=1, predict_init=1, predict_decay=2)
+    dt = 0.5
+    for g in [ens] + singles:
+        g.step_forward_quasistatic(dt, 4)
+    assert_same(ens, singles, "4 steps, unit predictor")
+    assert max(g.last_solve()["cycles"] for g in singles) > 1
+    ens.close()
+
+
+@pytest.mark.parametrize("form", ["mixed", "fp64"])
 def test_large_members_equal_single_simulations(form):
     """members of the workload's largest sides, 384 and 512, beside a 64 one:
     their V-cycles stop at the direct levels 48x49 and 64x65 (the 48-wide
