@@ -205,6 +205,14 @@ int stz_sim_set_boundaries(stz_sim *sim);
 /* Multigrid controls (tgmg.hh:331-441, tgmg_predict.hh). */
 int stz_sim_get_mg_config(stz_sim *sim, stz_mg_config *cfg);
 int stz_sim_set_mg_config(stz_sim *sim, const stz_mg_config *cfg);
+
+/* The adaptive V-cycle count's state, tgmg_predict::lim (tgmg_predict.hh:
+ * 12-14): the predicted cycles of the next solve times predict_mult.  A
+ * caller that restores a simulation's state (a tuner probing settings from
+ * one state) restores this with it. */
+int stz_sim_get_predictor(stz_sim *sim, int32_t *lim);
+int stz_sim_set_predictor(stz_sim *sim, int32_t lim);
+
 int stz_sim_last_solve(stz_sim *sim, stz_solve_info *info);
 /* shear_sim::avg_iters -> tgmg_predict::avg_iters (tgmg_predict.hh:28-33):
  * mean V-cycles per solve since the last call; resets the counters. */
@@ -520,8 +528,9 @@ int stz_ensemble_tile_choice(stz_ensemble *e, int32_t *out2);
  * the multigrid solve is differentiated as an exact solve (the adjoint
  * solve reuses the forward operator, whose interior block is symmetric);
  * states are kept every `checkpoint_every` steps and each segment is
- * recomputed during the reverse sweep.  FP64 form, linear athermal STZ law
- * with t_model = -1, no chi diffusion. */
+ * recomputed during the reverse sweep.  FP64 form; every STZ law and
+ * truncation model (the plastic loop is differentiated in dual numbers
+ * through the step's own law code) and chi diffusion. */
 typedef struct stz_adjoint stz_adjoint;   /* opaque */
 
 /* Replace the STZ law's parameters (c0, Omegaeps0, ...; stz_model.hh:22-26)

@@ -141,6 +141,8 @@ _SIGS = {
     "stz_sim_record_decisions": [C.c_void_p, C.c_int],
     "stz_sim_download_decisions": [C.c_void_p, _up],
     "stz_sim_replay_decisions": [C.c_void_p, C.c_int, _up],
+    "stz_sim_get_predictor": [C.c_void_p, C.POINTER(C.c_int32)],
+    "stz_sim_set_predictor": [C.c_void_p, C.c_int32],
     "stz_bd_create": [C.POINTER(BdParams), C.POINTER(C.c_void_p)],
     "stz_bd_destroy": [C.c_void_p],
     "stz_bd_set_stream": [C.c_void_p, C.c_void_p],

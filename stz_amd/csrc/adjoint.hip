@@ -28,6 +28,7 @@
 // node vectors as vel (m*(n+1) interleaved vec2).  The advection adjoint
 // scatters into the adjoint of the pre-step fields with FP64 atomics.
 #include "adjoint.h"
+#include "dual.h"
 #include "stz_law.h"
 
 #pragma clang fp contract(off)
@@ -55,8 +56,10 @@ __device__ __forceinline__ static void aadd(double *p, double v) {
 // ---------------------------------------------------------------------------
 // adaptive plastic term (stz_law.h plastic_term) with its Jacobian.
 // Inputs: 0 sbar, 1 chi, 2 c0, 3 Omegaeps0, 4 chi_inf.  Outputs dtmp =
-// ddev/sbar, chi after the substeps, ddev.  Linear athermal law with the
-// reference's default truncation (t_model = -1), the model the drivers use.
+// ddev/sbar, chi after the substeps, ddev.  The loop runs once in dual
+// numbers (dual.h) through the same law code as the step (dplastic_t), so
+// every STZ law and truncation model is covered and the values, hence the
+// yield branch and the substep count, are the forward's own.
 // ---------------------------------------------------------------------------
 struct PlasticJac {
     double dtmp, chi, ddev;
@@ -64,6 +67,7 @@ struct PlasticJac {
 };
 
 __device__ static void plastic_jac(const SimConst &c, double sbar0, double chi0, double dt, PlasticJac &P) {
+    typedef Dual<5> D5;
     const double small_dev_cutoff = 1e-12;
     const double theta = 0;
     for (int r = 0; r < 3; r++)
@@ -76,64 +80,31 @@ __device__ static void plastic_jac(const SimConst &c, double sbar0, double chi0,
         return;
     }
     const stz_model_params &mp = c.stz;
-    const double ti = c.thetainv, tsm = 2 * c.t_scale * c.mu;
-    double s = sbar0, chi = chi0, dtl = dt, dplas, dchi1, dchi2, adt;
-    double ds[5] = {1, 0, 0, 0, 0}, dc[5] = {0, 1, 0, 0, 0}, dd[5] = {0, 0, 0, 0, 0};
+    const D5 s0 = D5::var(sbar0, 0), c0 = D5::var(mp.c0, 2), Oe = D5::var(mp.Omegaeps0, 3),
+             chi_inf = D5::var(c.chi_inf, 4);
+    D5 s = s0, chi = D5::var(chi0, 1), dtl(dt), dplas, dchi1, dchi2, adt;
     bool adapt = true;
     do {
-        const double D = dplastic(mp, ti, s, chi, dchi1, dchi2);
-        dplas = tsm * D;
-        // dD/d(s, chi, Omegaeps0) of the linear athermal law (zero below yield)
-        double Ds = 0, Dc = 0, Do = 0;
-        if (!(s < 1)) {
-            const double a = mp.Omegaeps0 * ti;
-            const double ep = exp((mp.Omegaeps0 * s - mp.Delta) * ti), em = exp((-mp.Omegaeps0 * s - mp.Delta) * ti);
-            const double sC = 0.5 * (ep + em), sT = tanh(mp.Omegaeps0 * s * ti), bf = exp(-1 / chi) / mp.tau0;
-            const double dsC_s = 0.5 * a * (ep - em), dsC_o = 0.5 * s * ti * (ep - em);
-            const double dsT_s = a * (1 - sT * sT), dsT_o = s * ti * (1 - sT * sT);
-            Ds = bf * (dsC_s * (sT - 1 / s) + sC * (dsT_s + 1 / (s * s)));
-            Dc = D / (chi * chi);
-            Do = bf * (dsC_o * (sT - 1 / s) + sC * dsT_o);
-        }
-        double dD[5], ddp[5], ddc1[5], dadt[5];
-        for (int k = 0; k < 5; k++) {
-            dD[k] = Ds * ds[k] + Dc * dc[k] + (k == 3 ? Do : 0);
-            ddp[k] = tsm * dD[k];
-            // dchi1 = 2 s D / c0 (stz_model.hh:148-149)
-            ddc1[k] = 2 * (D * ds[k] + s * dD[k]) / mp.c0 - (k == 2 ? dchi1 / mp.c0 : 0);
-        }
+        // the forward's own expressions and order (stz_law.h plastic_term)
+        dplas = 2 * c.t_scale * c.mu * dplastic_t(mp, c.thetainv, s, chi, Oe, c0, dchi1, dchi2);
         if (fabs(dtl * dplas) > c.adapt_fac) {
             adt = c.adapt_fac / fabs(dplas);
             dtl -= adt;
-            for (int k = 0; k < 5; k++) {
-                dadt[k] = -(adt / dplas) * ddp[k];
-                dd[k] -= dadt[k];
-            }
         } else {
             adapt = false;
             adt = dtl;
-            for (int k = 0; k < 5; k++) dadt[k] = dd[k];
         }
-        const double gap = c.chi_inf - chi;
-        for (int k = 0; k < 5; k++) {
-            const double dgap = (k == 4 ? 1 : 0) - dc[k];
-            const double nds = ds[k] - dadt[k] * dplas - adt * ddp[k];
-            const double ndc = dc[k] + c.t_scale * (ddc1[k] * gap * adt + dchi1 * dgap * adt + dchi1 * gap * dadt[k]);
-            ds[k] = nds;
-            dc[k] = ndc;
-        }
-        // the forward's own updates, same order (stz_law.h)
         s -= adt * dplas;
-        chi += (dchi1 * (c.chi_inf - chi) + dchi2 * (theta - chi)) * c.t_scale * adt;
+        chi += (dchi1 * (chi_inf - chi) + dchi2 * (theta - chi)) * c.t_scale * adt;
     } while (adapt);
-    P.ddev = sbar0 - s;
-    P.dtmp = P.ddev / sbar0;
-    P.chi = chi;
+    const D5 ddev = s0 - s, dtmp = ddev / s0;
+    P.ddev = ddev.v;
+    P.dtmp = dtmp.v;
+    P.chi = chi.v;
     for (int k = 0; k < 5; k++) {
-        const double dddev = (k == 0 ? 1 : 0) - ds[k];
-        P.J[2][k] = dddev;
-        P.J[0][k] = (dddev * sbar0 - (k == 0 ? P.ddev : 0)) / (sbar0 * sbar0);
-        P.J[1][k] = dc[k];
+        P.J[0][k] = dtmp.d[k];
+        P.J[1][k] = chi.d[k];
+        P.J[2][k] = ddev.d[k];
     }
 }
 
@@ -499,6 +470,47 @@ __global__ void __launch_bounds__(256) adj_advect(SimConst c, FieldPtrs o, Field
     if (tid < 3) atomicAdd(grad + tid, (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]));
 }
 
+// ---------------------------------------------------------------------------
+// adjoint of chi_diffusion (explicit.hip, shear_sim.cc:364-386): chi after
+// the step = advected chi + the constant and ddev-weighted diffusion of the
+// pre-step chi, with the advection's new ddev as weights.  o: the pre-step
+// state; dnew: the advection's ddev plane.  The pre-step chi's adjoint goes
+// into G (before adj_advect adds its own part); the new ddev's into A.ddev,
+// which adj_advect then reads.  One thread per cell, scattered with atomics.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) adj_chi_diffusion(SimConst c, FieldPtrs o, const double *dnew, FieldPtrs A,
+                                                         FieldPtrs G, double dt) {
+    const int m = c.m, n = c.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= m || j >= n) return;
+    const double r = PL(A.chi, i, j);
+    if (r == 0) return;
+    const double hx = 0.5 * c.xsp * dt, hy = 0.5 * c.ysp * dt;
+    const double hxxcd = 2 * c.xsp * hx * c.chi_diff, hyycd = 2 * c.ysp * hy * c.chi_diff,
+                 hccd = 2 * (hxxcd + hyycd);
+    const double dfac = 0.25 * c.chi_len * c.chi_len * c.mu_inv, dfacx = dfac * c.xsp * c.xsp,
+                 dfacy = dfac * c.ysp * c.ysp;
+    const int il = i == 0 ? m - 1 : i - 1, ir = i == m - 1 ? 0 : i + 1;
+    const double D0 = PL(dnew, i, j);
+    const double Dd = D0 + PL(dnew, i, j - 1), Dl = D0 + PL(dnew, il, j), Dr = D0 + PL(dnew, ir, j),
+                 Du = D0 + PL(dnew, i, j + 1);
+    const double cf = PL(o.chi, i, j), cl = PL(o.chi, il, j), cr = PL(o.chi, ir, j), cd = PL(o.chi, i, j - 1),
+                 cu = PL(o.chi, i, j + 1);
+    aadd(&PL(G.chi, i, j), r * (-hccd - dfacx * (Dr + Dl) - dfacy * (Du + Dd)));
+    aadd(&PL(G.chi, ir, j), r * (hxxcd + dfacx * Dr));
+    aadd(&PL(G.chi, il, j), r * (hxxcd + dfacx * Dl));
+    aadd(&PL(G.chi, i, j + 1), r * (hyycd + dfacy * Du));
+    aadd(&PL(G.chi, i, j - 1), r * (hyycd + dfacy * Dd));
+    const double gr = r * dfacx * (cr - cf), gl = r * dfacx * (cl - cf), gu = r * dfacy * (cu - cf),
+                 gd = r * dfacy * (cd - cf);
+    aadd(&PL(A.ddev, i, j), gr + gl + gu + gd);
+    aadd(&PL(A.ddev, ir, j), gr);
+    aadd(&PL(A.ddev, il, j), gl);
+    aadd(&PL(A.ddev, i, j + 1), gu);
+    aadd(&PL(A.ddev, i, j - 1), gd);
+}
+
 __global__ void __launch_bounds__(256) adj_fill(double *p, size_t count) {
     const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (k < count) p[k] = 0;
@@ -527,6 +539,14 @@ int adj_launch_correct(const SimConst &c, const FieldPtrs &L, const FieldPtrs &A
 int adj_launch_source(const SimConst &c, const FieldPtrs &A, const double *w, double dt, hipStream_t st) {
     dim3 b(64, 4);
     hipLaunchKernelGGL(adj_source, grid2(c.m, c.n, b), b, 0, st, c, A, w, dt);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+int adj_launch_chi_diffusion(const SimConst &c, const FieldPtrs &o, const double *dnew, const FieldPtrs &A,
+                             const FieldPtrs &G, double dt, hipStream_t st) {
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(adj_chi_diffusion, grid2(c.m, c.n, b), b, 0, st, c, o, dnew, A, G, dt);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }

@@ -52,11 +52,12 @@ struct stz_adjoint {
     double *seg = nullptr;             // pre-step states of one segment
     double *lam = nullptr;             // three adjoint plane sets: L, A, G
     double *coef = nullptr, *g = nullptr, *w = nullptr, *keep = nullptr, *grad = nullptr;
+    double *adv = nullptr;             // the recomputed advection output (chi diffusion only)
     int adj_lim = 0;
     double stats[6] = {0, 0, 0, 0, 0, 0};
 
     void free_all() {
-        for (double **p : {&ckpt, &seg, &lam, &coef, &g, &w, &keep, &grad}) {
+        for (double **p : {&ckpt, &seg, &lam, &coef, &g, &w, &keep, &grad, &adv}) {
             if (*p) (void)hipFree(*p);
             *p = nullptr;
         }
@@ -181,6 +182,7 @@ int stz_adjoint_create(stz_sim *s, int every, stz_adjoint **out) {
     if (hipMalloc(&a->g, a->nv * sizeof(double)) != hipSuccess || hipMalloc(&a->w, a->nv * sizeof(double)) != hipSuccess ||
         hipMalloc(&a->keep, 2 * a->nv * sizeof(double)) != hipSuccess || hipMalloc(&a->grad, 3 * sizeof(double)) != hipSuccess)
         return fail("vectors");
+    if (hipMalloc(&a->adv, kPlanes * a->pe * sizeof(double)) != hipSuccess) return fail("advection output");
     a->adj_lim = s->cfg.predict_mult * s->cfg.predict_init;
     *out = a;
     return STZ_OK;
@@ -212,14 +214,6 @@ int stz_adjoint_forward(stz_adjoint *a, double dt, int nsteps) {
     stz_sim *s = a->s;
     STZ_HIP(hipSetDevice(s->device));
     const SimConst &c = s->c;
-    if (c.stz.kind != STZ_MODEL_LINEAR_ATHERMAL || c.stz.t_model != -1) {
-        set_error("the adjoint covers the linear athermal STZ law with t_model = -1");
-        return STZ_ERR_ARG;
-    }
-    if (c.chi_diff != 0 || c.chi_len != 0) {
-        set_error("the adjoint does not cover chi diffusion (chi_diff = chi_len = 0 required)");
-        return STZ_ERR_ARG;
-    }
     if (s->cfg.kernels != 0 && s->cfg.kernels != 1) { set_error("bad kernel setting"); return STZ_ERR_ARG; }
     if (s->rep) {
         // the adjoint's own forward and recomputation take their decisions
@@ -305,6 +299,14 @@ int stz_adjoint_backward(stz_adjoint *a, const double *lam_final, double *lam_in
             if ((e = adjoint_solve(a))) return e;
             if ((e = adj_launch_source(s->c, A, a->w, a->dt, st))) return e;
             if ((e = adj_zero(lG, PB, st))) return e;
+            if (s->c.chi_diff != 0 || s->c.chi_len != 0) {
+                // the advection's ddev output, recomputed from the pre-step
+                // state (its ghost rows, which the advection leaves, first)
+                const FieldPtrs W = fields_at(a->adv, a->pe);
+                STZ_HIP(hipMemcpyAsync(W.ddev, o.ddev, a->pe * sizeof(double), hipMemcpyDeviceToDevice, st));
+                if ((e = launch_advect(s->c, o, W, a->dt, nullptr, st))) return e;
+                if ((e = adj_launch_chi_diffusion(s->c, o, W.ddev, A, G, a->dt, st))) return e;
+            }
             if ((e = adj_launch_advect(s->c, o, A, G, a->dt, a->grad, st))) return e;
             std::swap(lL, lG);
         }

@@ -908,6 +908,19 @@ int stz_sim_set_boundaries(stz_sim *s) {
     return STZ_OK;
 }
 
+int stz_sim_get_predictor(stz_sim *s, int32_t *lim) {
+    if (!s || !lim) { set_error("null argument"); return STZ_ERR_ARG; }
+    *lim = s->lim;
+    return STZ_OK;
+}
+
+int stz_sim_set_predictor(stz_sim *s, int32_t lim) {
+    if (!s) { set_error("null simulation"); return STZ_ERR_ARG; }
+    if (lim < 0) { set_error("predictor limit %d < 0", lim); return STZ_ERR_ARG; }
+    s->lim = lim;
+    return STZ_OK;
+}
+
 int stz_sim_get_mg_config(stz_sim *s, stz_mg_config *g) {
     if (!s || !g) { set_error("null argument"); return STZ_ERR_ARG; }
     *g = s->cfg;

@@ -15,16 +15,23 @@ namespace stz {
 // STZ law (stz_model.hh:9-207)
 // ---------------------------------------------------------------------------
 
-__host__ __device__ static double dpl_common(const stz_model_params &mp, double sbar, double chi, double &dchi1,
-                                    double &dchi2, double sC1, double sC2, double sC, double sT) {
+// The law is written once over the scalar type S: double for the step, or a
+// Dual (dual.h) for the adjoint's derivatives with respect to s-bar, chi,
+// Omega*eps0 and c0 (passed separately as Oe and c0 so they can carry
+// derivatives; for S = double they are mp.Omegaeps0 and mp.c0).  Every
+// expression is the same for both, so the values agree bit for bit.
+template <class S>
+__host__ __device__ static S dpl_common_t(const stz_model_params &mp, const S &sbar, const S &chi, S &dchi1,
+                                          S &dchi2, const S &sC1, const S &sC2, const S &sC, const S &sT,
+                                          const S &c0) {
     // stz_model.hh:30-47
     const double s0 = 1;
-    double sbart = sbar > 1e-12 ? sbar : 1e-12, bfac = exp(-1 / chi),
-           pM = 1 + sbart * sT / s0 + mp.rho / (2 * sC2),
-           M = s0 / (2 * sbart) * (pM - sqrt(pM * pM - 4 * sbart * sT / s0)),
-           Dpl = bfac * sC * (sT - M) / mp.tau0,
-           Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
-    bfac /= mp.tau0 * mp.c0;
+    S sbart = sbar > 1e-12 ? sbar : S(1e-12), bfac = exp(-1 / chi),
+      pM = 1 + sbart * sT / s0 + mp.rho / (2 * sC2),
+      M = s0 / (2 * sbart) * (pM - sqrt(pM * pM - 4 * sbart * sT / s0)),
+      Dpl = bfac * sC * (sT - M) / mp.tau0,
+      Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
+    bfac /= mp.tau0 * c0;
     dchi1 = bfac * Gamma;
     dchi2 = bfac * mp.kappa * sC1 * mp.rho;
     return Dpl;
@@ -32,65 +39,68 @@ __host__ __device__ static double dpl_common(const stz_model_params &mp, double 
 
 // yield: -1 evaluates the athermal laws' yield test sbar < 1; 0 or 1 takes
 // the recorded branch (below or at/above yield) instead, whatever sbar is
-__host__ __device__ static double dplastic(const stz_model_params &mp, double thetainv, double sbar, double chi,
-                                           double &dchi1, double &dchi2, int yield = -1) {
+template <class S>
+__host__ __device__ static S dplastic_t(const stz_model_params &mp, double thetainv, const S &sbar, const S &chi,
+                                        const S &Oe, const S &c0, S &dchi1, S &dchi2, int yield = -1) {
     const double s0 = 1;
     const bool below = yield < 0 ? sbar < 1 : yield == 0;
     switch (mp.kind) {
     case STZ_MODEL_LINEAR: {  // stz_model.hh:58-67
-        double sC1 = exp(-mp.Delta * thetainv);
-        double sC2 = cosh(mp.Omegaeps0 * sbar * thetainv), sC = sC1 * sC2,
-               sT = tanh(mp.Omegaeps0 * sbar * thetainv);
-        return dpl_common(mp, sbar, chi, dchi1, dchi2, sC1, sC2, sC, sT);
+        S sC1 = exp(-mp.Delta * thetainv);
+        S sC2 = cosh(Oe * sbar * thetainv), sC = sC1 * sC2, sT = tanh(Oe * sbar * thetainv);
+        return dpl_common_t(mp, sbar, chi, dchi1, dchi2, sC1, sC2, sC, sT, c0);
     }
     case STZ_MODEL_NONLINEAR: {  // stz_model.hh:76-87
-        double x = mp.Omegaeps0 * sbar / mp.Delta, sC1 = exp(-mp.Delta * thetainv),
-               sC = exp(-mp.Delta * cosh(x) * thetainv) * cosh(mp.Delta * sinh(x) * thetainv), sC2 = sC / sC1,
-               sT = tanh(mp.Delta * sinh(x) * thetainv);
-        return dpl_common(mp, sbar, chi, dchi1, dchi2, sC1, sC2, sC, sT);
+        S x = Oe * sbar / mp.Delta, sC1 = exp(-mp.Delta * thetainv),
+          sC = exp(-mp.Delta * cosh(x) * thetainv) * cosh(mp.Delta * sinh(x) * thetainv), sC2 = sC / sC1,
+          sT = tanh(mp.Delta * sinh(x) * thetainv);
+        return dpl_common_t(mp, sbar, chi, dchi1, dchi2, sC1, sC2, sC, sT, c0);
     }
     case STZ_MODEL_NONLINEAR_ATHERMAL: {  // stz_model.hh:179-206
-        dchi2 = 0;
-        if (below) { dchi1 = 0; return 0; }
-        double x = mp.Omegaeps0 * sbar / mp.Delta,
-               sC = 0.5 * (exp(-mp.Delta * thetainv * exp(x)) + exp(-mp.Delta * thetainv * exp(-x))),
-               sT = tanh(mp.Delta * sinh(x) * thetainv);
-        double bfac = exp(-1 / chi), Dpl = bfac * sC * (sT - s0 / sbar) / mp.tau0,
-               Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
-        dchi1 = bfac * Gamma / (mp.tau0 * mp.c0);
+        dchi2 = S(0);
+        if (below) { dchi1 = S(0); return S(0); }
+        S x = Oe * sbar / mp.Delta,
+          sC = 0.5 * (exp(-mp.Delta * thetainv * exp(x)) + exp(-mp.Delta * thetainv * exp(-x))),
+          sT = tanh(mp.Delta * sinh(x) * thetainv);
+        S bfac = exp(-1 / chi), Dpl = bfac * sC * (sT - s0 / sbar) / mp.tau0,
+          Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
+        dchi1 = bfac * Gamma / (mp.tau0 * c0);
         return Dpl;
     }
     default: {  // linear athermal, stz_model.hh:103-151
-        double tmp, sC, sT;
-        dchi2 = 0;
-        if (below) { dchi1 = 0; return 0; }
-        if (mp.t_model == -1 || mp.Delta >= mp.Omegaeps0 * sbar)
-            sC = 0.5 * (exp((mp.Omegaeps0 * sbar - mp.Delta) * thetainv) +
-                        exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv));
+        S tmp, sC, sT;
+        dchi2 = S(0);
+        if (below) { dchi1 = S(0); return S(0); }
+        if (mp.t_model == -1 || mp.Delta >= Oe * sbar)
+            sC = 0.5 * (exp((Oe * sbar - mp.Delta) * thetainv) + exp((-Oe * sbar - mp.Delta) * thetainv));
         else switch (mp.t_model) {
-            case 0: sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) + 1); break;
+            case 0: sC = 0.5 * (exp((-Oe * sbar - mp.Delta) * thetainv) + 1); break;
             case 1:
-                tmp = mp.Omegaeps0 * sbar / mp.Delta;
-                sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) + sqrt(0.5 * (1 + tmp * tmp)));
+                tmp = Oe * sbar / mp.Delta;
+                sC = 0.5 * (exp((-Oe * sbar - mp.Delta) * thetainv) + sqrt(0.5 * (1 + tmp * tmp)));
                 break;
             case 2:
-                tmp = mp.Omegaeps0 * sbar / mp.Delta;
-                sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) + (0.5 * (1 + tmp * tmp)));
+                tmp = Oe * sbar / mp.Delta;
+                sC = 0.5 * (exp((-Oe * sbar - mp.Delta) * thetainv) + (0.5 * (1 + tmp * tmp)));
                 break;
             case 3: sC = mp.t_icpt + sbar * mp.t_grad; break;
             case 4:
-                sC = 0.5 * (exp((-mp.Omegaeps0 * sbar - mp.Delta) * thetainv) +
-                            (mp.Omegaeps0 * sbar - mp.Delta) * thetainv + 1);
+                sC = 0.5 * (exp((-Oe * sbar - mp.Delta) * thetainv) + (Oe * sbar - mp.Delta) * thetainv + 1);
                 break;
-            default: sC = 0;
+            default: sC = S(0);
         }
-        sT = mp.t_model != 3 ? tanh(mp.Omegaeps0 * sbar * thetainv) : 1;
-        double bfac = exp(-1 / chi), Dpl = bfac * sC * (sT - s0 / sbar) / mp.tau0,
-               Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
-        dchi1 = bfac * Gamma / (mp.tau0 * mp.c0);
+        sT = mp.t_model != 3 ? tanh(Oe * sbar * thetainv) : S(1);
+        S bfac = exp(-1 / chi), Dpl = bfac * sC * (sT - s0 / sbar) / mp.tau0,
+          Gamma = 2 * mp.tau0 * Dpl * sbar / (s0 * bfac);
+        dchi1 = bfac * Gamma / (mp.tau0 * c0);
         return Dpl;
     }
     }
+}
+
+__host__ __device__ static inline double dplastic(const stz_model_params &mp, double thetainv, double sbar,
+                                                  double chi, double &dchi1, double &dchi2, int yield = -1) {
+    return dplastic_t<double>(mp, thetainv, sbar, chi, mp.Omegaeps0, mp.c0, dchi1, dchi2, yield);
 }
 
 // shear_sim.cc:513-548 (adaptive_plastic_term); trips counts loop iterations

@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--serial-steps", type=int, default=3)
     ap.add_argument("--block", type=int, default=10, help="steps per block of one launch mode")
+    ap.add_argument("--tune-sweeps", action="store_true",
+                    help="pick V(nu, nu), nu in 1..4, with the ensemble's sweep-count tuner after the warm-up")
     ap.add_argument("--one-mode", action="store_true",
                     help="time every step replayed from graphs (no direct-launch comparison)")
     args = ap.parse_args()
@@ -81,6 +83,13 @@ def main():
     # that both see the same phases of the run
     if args.warmup:
         timed(args.warmup, True)
+    tuned = None
+    if args.tune_sweeps:
+        t = time.perf_counter()
+        tuned = ens.tune_sweeps(dt)
+        print("ens_bench: sweep tuner %s in %.1f s" % (tuned, time.perf_counter() - t), file=sys.stderr, flush=True)
+        tuned = {"choice": list(tuned["choice"]),
+                 "ms_per_step": {"V(%d,%d)" % k: v for k, v in tuned["ms_per_step"].items()}}
     block = args.steps if args.one_mode else max(1, min(args.block, args.steps // 2))
     rounds = 1 if args.one_mode else max(1, args.steps // (2 * block))
     el, el_d, cyc_g = 0.0, 0.0, []
@@ -125,11 +134,13 @@ def main():
         "dtype": "f32+f64" if args.form == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": "ensemble of %d simple-shear simulations, sides %s, chi 600K+20K (corr 2 cells), "
                                "dt %g, V(%d,%d), threshold %g x reference" %
-                               (len(sides), hist, dt, args.nu, args.nu, args.tol),
+                               (len(sides), hist, dt, *(tuned["choice"] if tuned else (args.nu, args.nu)),
+                                args.tol),
                    "cells": cells, "cycles_per_solve": cyc, "slots_per_step": slots,
                    "slots_beyond_plan_per_step": extra,
                    "direct_launch_ms_per_step": None if args.one_mode else 1e3 * el_d / nsteps,
                    "tiles": ens.tile_choice(),
+                   "sweep_tuner": tuned,
                    "one_at_a_time_cycles_per_solve": cyc_s,
                    "one_at_a_time_cell_steps_per_s": serial,
                    "batched_over_one_at_a_time": value / serial}}))

@@ -84,6 +84,43 @@ class Ensemble:
         check(self._lib.stz_ensemble_stats(self._h, out))
         return dict(zip(("steps", "slots", "extra_slots", "captures"), (int(v) for v in out)))
 
+    def tune_sweeps(self, dt, probe_steps=6, candidates=((1, 1), (2, 2), (3, 3), (4, 4)), reps=1):
+        """The sweep-count tuner: time `probe_steps` batched steps of dt from
+        the members' current state (fields, velocity, time and predictor)
+        with each V(nu1, nu2) of `candidates` after one untimed step (the
+        tables, the tile-shape tuner and the graph captures of the new
+        setting), keep the fastest and restore the state.  Sweep counts
+        change how many V-cycles the solves take to reach their threshold,
+        never the threshold.  Returns {"choice": (nu1, nu2), "ms_per_step":
+        {candidate: ms}}."""
+        import time
+        saved = [(m.get_fields(), m.get_vel(), m.time, m.predictor_limit) for m in self.members]
+
+        def restore(nu1, nu2):
+            self.set_mg_config(nu1=nu1, nu2=nu2)
+            for m, (f, v, t, lim) in zip(self.members, saved):
+                m.set_fields(f)
+                m.set_vel(v)
+                m.time = t
+                m.predictor_limit = lim
+
+        times = {}
+        for cand in candidates:
+            best = None
+            for _ in range(reps):
+                restore(*cand)
+                self.step_forward_quasistatic(dt, 1)     # tables, tile tuner, graph captures
+                self.members[0].synchronize()
+                t0 = time.perf_counter()
+                self.step_forward_quasistatic(dt, probe_steps)
+                self.members[0].synchronize()
+                el = (time.perf_counter() - t0) / probe_steps
+                best = el if best is None else min(best, el)
+            times[tuple(cand)] = best * 1e3
+        choice = min(times, key=times.get)
+        restore(*choice)
+        return {"choice": choice, "ms_per_step": times}
+
     def tile_choice(self):
         """{launches kept at 32x16 tiles, at 64x32} by the tuner"""
         out = np.zeros(2, dtype=np.int32)

@@ -173,6 +173,17 @@ class ShearSim:
             setattr(g, k, v)
         check(self._lib.stz_sim_set_mg_config(self._h, C.byref(g)))
 
+    @property
+    def predictor_limit(self):
+        """tgmg_predict::lim: the next solve's predicted V-cycles x predict_mult"""
+        v = C.c_int32()
+        check(self._lib.stz_sim_get_predictor(self._h, C.byref(v)))
+        return v.value
+
+    @predictor_limit.setter
+    def predictor_limit(self, lim):
+        check(self._lib.stz_sim_set_predictor(self._h, int(lim)))
+
     def last_solve(self):
         info = _lib.SolveInfo()
         check(self._lib.stz_sim_last_solve(self._h, C.byref(info)))

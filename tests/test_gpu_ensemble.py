@@ -89,6 +89,32 @@ def test_unit_predictor_members_equal_single_simulations(form):
     ens.close()
 
 
+def test_sweep_tuner_restores_the_state():
+    """the sweep-count tuner times its candidates from the members' state,
+    keeps one of them, and leaves every member where it found it (predictor
+    included); the ensemble then still equals single simulations with the
+    chosen sweeps"""
+    print("case ensemble sweep tuner", flush=True)
+    ens, singles = setup("mixed", sizes=[64, 128, 96])
+    for g in [ens] + singles:
+        g.step_forward_quasistatic(0.5, 2)
+    before = [(m.get_fields(), m.get_vel(), m.time, m.predictor_limit) for m in ens]
+    res = ens.tune_sweeps(0.5, probe_steps=2, candidates=((1, 1), (2, 2), (3, 3)))
+    print("sweep tuner", res)
+    assert res["choice"] in ((1, 1), (2, 2), (3, 3))
+    assert set(res["ms_per_step"]) == {(1, 1), (2, 2), (3, 3)}
+    for m, (f, v, t, lim) in zip(ens, before):
+        assert np.array_equal(m.get_fields(), f) and np.array_equal(m.get_vel(), v) and m.time == t
+        assert m.predictor_limit == lim
+    nu1, nu2 = res["choice"]
+    for g in singles:
+        g.set_mg_config(nu1=nu1, nu2=nu2)
+    for g in [ens] + singles:
+        g.step_forward_quasistatic(0.5, 3)
+    assert_same(ens, singles, "after the sweep tuner")
+    ens.close()
+
+
 @pytest.mark.parametrize("form", ["mixed", "fp64"])
 def test_large_members_equal_single_simulations(form):
     """members of the workload's largest sides, 384 and 512, beside a 64 one:

@@ -14,6 +14,8 @@ checked against its central differences on a reduced calibration run.
   without the replay cross many decisions in this state (v = 0 up to
   rounding, ENO comparisons between rounding-level second differences): the
   number crossed is printed, and no bar is put on those differences.
+* Every STZ law and truncation model, and chi diffusion, at 32^2: the
+  adjoint against frozen-decision central differences within 1e-5.
 """
 import numpy as np
 import pytest
@@ -168,3 +170,81 @@ def test_adjoint_matches_frozen_decision_differences(plastic_start):
         assert abs(diffs[0] - diffs[1]) <= 1e-6 * abs(diffs[0])
         assert rel < 1e-5, (k, diffs, ad)
     print("worst relative disagreement %.2e" % worst)
+
+
+# every STZ law and truncation model, and chi diffusion (constant and
+# ddev-weighted), on a 32^2 random state: 3 steps of dt 0.3, lam random
+LAW_CASES = [(0, -1, 0.0, 0.0, 0.0), (0, 1, 0.0, 0.0, 0.0), (0, 3, 0.0, 0.0, 0.0), (0, 4, 0.0, 0.0, 0.0),
+             (1, -1, 0.1, 0.0, 0.0), (2, -1, 0.1, 0.0, 0.0), (3, -1, 0.0, 0.0, 0.0),
+             (0, -1, 0.0, 2e-4, 0.02)]
+
+
+@pytest.mark.parametrize("kind,t_model,rho,chi_diff,chi_len", LAW_CASES)
+def test_adjoint_every_law_and_chi_diffusion(kind, t_model, rho, chi_diff, chi_len):
+    """The adjoint's plastic Jacobian runs the law code of the step in dual
+    numbers, and the chi-diffusion adjoint scatters the transposed stencil;
+    both checked against central differences of the frozen-decision forward
+    (h = 1e-6 and 4e-6) along a chi direction and a parameter direction,
+    within 1e-5."""
+    import copy
+    from stategen import random_state
+    print("case adjoint law", kind, t_model, rho, chi_diff, chi_len, flush=True)
+    p = simple_shear(32, u_bdry=2e-3, chi_diff=chi_diff, chi_len=chi_len)
+    st = copy.deepcopy(p["stz"])
+    st["kind"], st["t_model"], st["rho"] = kind, t_model, rho
+    p["stz"] = st
+    planes, vel = random_state(p, 21, stress=1.4, ties=False)
+    dt, steps, t0 = 0.3, 3, 1.6
+    th = np.array([st[k] for k in PARAMS])
+
+    def fwd(pl, theta, records=None):
+        g = ShearSim(p, "fp64")
+        g.set_mg_config(tol_mult=1e-12)
+        set_params(g, *theta)
+        g.set_fields(pl)
+        g.set_vel(vel)
+        g.time = t0
+        g.record_decisions(True)
+        if records is not None:
+            g.replay_decisions(records)
+        bits = []
+        for _ in range(steps):
+            g.step_forward_quasistatic(dt, 1)
+            bits.append(g.get_decisions().copy())
+        F = g.get_fields()
+        g.close()
+        return F, bits
+
+    F0, base = fwd(planes, th)
+    yielding = int(np.count_nonzero(base[0] & np.uint32(1 << 20)))
+    lam = np.random.default_rng(3).standard_normal(F0.shape)
+    g = ShearSim(p, "fp64")
+    g.set_mg_config(tol_mult=1e-12)
+    set_params(g, *th)
+    g.set_fields(planes)
+    g.set_vel(vel)
+    g.time = t0
+    a = Adjoint(g, 2)
+    a.set_solve_accuracy(1e-12)
+    a.forward(dt, steps)
+    assert np.array_equal(g.get_fields(), F0)
+    lam0, gth = a.backward(lam)
+    a.close()
+    g.close()
+    TZ = st["TZ"]
+    r = np.random.default_rng(17)
+    for label, dx, dth in (("chi", np.where(np.arange(10)[:, None, None] == 6,
+                                             r.standard_normal(planes.shape) * (10.0 / TZ), 0.0), np.zeros(3)),
+                           ("params", np.zeros_like(planes), r.standard_normal(3) * th * 0.1)):
+        ad = float(np.sum(lam0 * dx)) + float(sum(gth[q] * dth[i] for i, q in enumerate(PARAMS)))
+        diffs = []
+        for h in (1e-6, 4e-6):
+            fp = float(np.sum(lam * fwd(planes + h * dx, th + h * dth, base)[0]))
+            fm = float(np.sum(lam * fwd(planes - h * dx, th - h * dth, base)[0]))
+            diffs.append((fp - fm) / (2 * h))
+        rel = abs(diffs[0] - ad) / max(abs(diffs[0]), abs(ad))
+        print("law %d t_model %d rho %g chi_diff %g chi_len %g (%d yielding): %s fd %.12e %.12e adjoint %.12e "
+              "rel %.2e" % (kind, t_model, rho, chi_diff, chi_len, yielding, label, diffs[0], diffs[1], ad, rel),
+              flush=True)
+        assert abs(diffs[0] - diffs[1]) <= 1e-6 * abs(diffs[0])
+        assert rel < 1e-5, (label, diffs, ad)
