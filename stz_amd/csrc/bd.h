@@ -52,7 +52,8 @@ int bd_ls_march(const BdConst &c, double *phi, int8_t *st, double delta, unsigne
 int bd_ls_finish(const BdConst &c, double *phi, double *vp, int8_t *st, int build, hipStream_t s);
 int bd_ls_node_status(const BdConst &c, const double *phi, const int8_t *st, int8_t *nst, hipStream_t s);
 // extrapolation waves (traverse.hh:176-240, 62-174)
-// counter: 4 ints of device scratch; list: 2 max(cells, nodes) ints
+// counter: 16 ints of device scratch; list: 3 max(cells, nodes) ints; hflag:
+// 16 ints of host-visible memory
 int bd_extrap_cells(const BdConst &c, const double *phi, const int8_t *st, const BdCells &f, int *done,
                     int *counter, int *list, volatile int *hflag, int *waves, hipStream_t s);
 int bd_extrap_nodes(const BdConst &c, const double *phi, const int8_t *st, const int8_t *nst, const BdNodes &f,

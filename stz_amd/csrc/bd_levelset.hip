@@ -868,67 +868,89 @@ __global__ void __launch_bounds__(kThreads) node_status_kernel(BdConst c, const 
 }
 
 // ---------------------------------------------------------------------------
-// extrapolation of the cell fields (traverse.hh:176-240, extrap.hh:49-72)
-// done[]: 0 source (inside band), -1 pending (outside band), INT_MAX not in
-// the band; a finished cell holds the wave that computed it
+// extrapolation of the cell and node fields (traverse.hh:62-240,
+// extrap.hh:26-72)
+//
+// The reference walks the entries outside the interface in one order
+// (cells: increasing phi; nodes: the key below), and each takes a weighted
+// linear least-squares fit over the entries of its window that precede it.
+// Here the walk is a dependency graph: a pending entry depends on the
+// pending entries of its window that precede it.  An entry is set once
+// everything it depends on is set (Kahn's topological order), so each fit
+// reads exactly the values the reference's reads and sums them in the same
+// order.  The walk goes in waves, one launch each: one wavefront (a team)
+// per ready entry sets it and releases the entries that depend on it into
+// the next wave's list.  The number of waves is the longest dependency
+// chain.
+//
+// done[]: 0 source, -1 pending, INT_MAX not in the band; a set entry holds
+// the wave that set it.  deg[] (pending entries): the count of unset
+// dependencies in the low 16 bits and, for cells, the window half-width
+// above them.  counter: [2] failed fits, [3] pending entries, [5] order
+// violations (an entry fitted before one it depends on, which would be a
+// fault of this code: reported as an error), [8] waves, [9] entries set,
+// [10] the largest cell window half-width beyond 3, [12..14] the wave list
+// counts.
 // ---------------------------------------------------------------------------
 __global__ void zero_counters_kernel(int *counter) {
-    if (threadIdx.x < 4) counter[threadIdx.x] = 0;
+    const int t = threadIdx.x;
+    if ((t >= 2 && t <= 5) || (t >= 8 && t <= 14)) counter[t] = 0;
 }
 
-// keep the still pending entries of a list (counter[3] = new length)
-__global__ void __launch_bounds__(kThreads) compact_kernel(const int *__restrict__ done,
-                                                           const int *__restrict__ list, int count,
-                                                           int *__restrict__ out, int *counter) {
-    const int q = blockIdx.x * kThreads + threadIdx.x;
-    if (q >= count) return;
-    const int k = list[q];
-    if (done[k] == -1) out[atomicAdd(&counter[3], 1)] = k;
+// *cnt += the number of lanes with pred, one atomic per wavefront
+__device__ __forceinline__ void wave_count(bool pred, int *cnt) {
+    const unsigned long long act = __ballot(1), msk = __ballot(pred);
+    if (msk && (int)__lane_id() == __builtin_ctzll(act)) atomicAdd(cnt, (int)__popcll(msk));
 }
 
-__global__ void zero_one_kernel(int *p) {
-    if (threadIdx.x == 0) *p = 0;
+// list[(*cnt)++] = v for the lanes with pred, one atomic per wavefront
+__device__ __forceinline__ void wave_push(bool pred, int v, int *list, int *cnt) {
+    const unsigned long long act = __ballot(1), msk = __ballot(pred);
+    if (!msk) return;
+    const int leader = __builtin_ctzll(act);
+    int base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(cnt, (int)__popcll(msk));
+    base = __shfl(base, leader);
+    if (pred)
+        list[base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u))] = v;
 }
 
-// the pending cells are also listed (counter[3] entries of list) so that the
-// waves run over them only
-__global__ void __launch_bounds__(kThreads) xc_prepare_kernel(BdConst c, const int8_t *__restrict__ st, int *done,
-                                                              int *counter, int *list) {
-    const int ij = blockIdx.x * kThreads + threadIdx.x;
-    if (ij >= c.m * c.n) return;
-    const int s = code(st[ij]);
-    const int d = (s >= 2 && s <= 3) ? 0 : ((s >= 4 && s <= 5) ? -1 : INT_MAX);
-    done[ij] = d;
-    if (d == -1) list[atomicAdd(&counter[3], 1)] = ij;
+// LDS written by some lanes of a wavefront and read by others
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// one wave over the listed pending cells; counter[2] counts failed fits
-__global__ void __launch_bounds__(kThreads) xc_wave_kernel(BdConst c, const double *__restrict__ phi, BdCells f,
-                                                           int *done, int wave, int *counter,
-                                                           const int *__restrict__ list, int count) {
-    const int q = blockIdx.x * kThreads + threadIdx.x, m = c.m, n = c.n;
-    if (q >= count) return;
-    const int ij = list[q];
-    if (done[ij] != -1) return;
-    const int i = ij % m, j = ij / m;
+__device__ __forceinline__ void window(int i, int j, int m, int n, int r, int &ilo, int &ihi, int &jlo, int &jhi) {
+    ilo = i <= r ? -i : -r;
+    ihi = i + r >= m ? m - 1 - i : r;
+    jlo = j <= r ? -j : -r;
+    jhi = j + r >= n ? n - 1 - j : r;
+}
+
+// the determinant of the fit's 3x3 system from its weighted sums
+__device__ __forceinline__ double fit_det(double s, double sx, double sy, double sxx, double sxy, double syy) {
+    return s * (sxy * sxy - sxx * syy) + sx * sx * syy - 2 * sx * sxy * sy + sy * sy * sxx;
+}
+
+// The window half-width of pending cell ij (traverse.hh:189-214): the
+// smallest r in 3..7 whose band cells with a smaller phi give a nonsingular
+// system, 8 if none does (the fit fails).  The system depends on phi and
+// band membership only, so this is known before the walk.
+__device__ static int xc_radius(const BdConst &c, const double *__restrict__ phi, const int8_t *__restrict__ st,
+                                int ij) {
+    const int m = c.m, n = c.n, i = ij % m, j = ij / m;
     const double ph = phi[ij];
-    double s, sx, sy, sxx, sxy, syy, det = 0, p[15];
-    int r;
-    for (r = 3; r <= 7; r++) {
-        const int ilo = i <= r ? -i : -r, ihi = i + r >= m ? m - 1 - i : r, jlo = j <= r ? -j : -r,
-                  jhi = j + r >= n ? n - 1 - j : r;
+    for (int r = 3; r <= 7; r++) {
+        int ilo, ihi, jlo, jhi;
+        window(i, j, m, n, r, ilo, ihi, jlo, jhi);
+        double s = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0;
         for (int dj = jlo; dj <= jhi; dj++)
             for (int di = ilo; di <= ihi; di++) {
-                const int k = ij + dj * m + di, d = done[k];
-                if (d == INT_MAX || !(phi[k] < ph)) continue;
-                if (d < 0 || d >= wave) return;
-            }
-        s = sx = sy = sxx = sxy = syy = 0;
-        for (int q = 0; q < 15; q++) p[q] = 0;
-        for (int dj = jlo; dj <= jhi; dj++)
-            for (int di = ilo; di <= ihi; di++) {
-                const int k = ij + dj * m + di;
-                if (done[k] == INT_MAX) continue;
+                const int k = ij + dj * m + di, sk = code(st[k]);
+                if (sk < 2 || sk > 5) continue;
                 double w = phi[k];
                 if (!(w < ph)) continue;
                 w = ph - w;
@@ -938,36 +960,203 @@ __global__ void __launch_bounds__(kThreads) xc_wave_kernel(BdConst c, const doub
                 sxx += di * di * w;
                 sxy += di * dj * w;
                 syy += dj * dj * w;
-                const double x = di, y = dj;
-                const double v[5] = {f.p[k], f.q[k], f.s[k], f.tau[k], f.chi[k]};
-                for (int q = 0; q < 5; q++) {
-                    p[3 * q] += w * v[q];
-                    p[3 * q + 1] += w * x * v[q];
-                    p[3 * q + 2] += w * y * v[q];
-                }
             }
-        det = s * (sxy * sxy - sxx * syy) + sx * sx * syy - 2 * sx * sxy * sy + sy * sy * sxx;
-        if (!(fabs(det) < 1e-10)) break;
+        if (!(fabs(fit_det(s, sx, sy, sxx, sxy, syy)) < 1e-10)) return r;
     }
-    if (r > 7) {
-        atomicAdd(&counter[2], 1);
-        done[ij] = wave;
-        return;
+    return 8;
+}
+
+// statuses 2-3: source, 4-5: pending, others outside the band; a pending
+// cell gets its window, its dependency count, and the ready list if that is
+// zero
+__global__ void __launch_bounds__(kThreads) xc_prepare_kernel(BdConst c, const double *__restrict__ phi,
+                                                              const int8_t *__restrict__ st, int *done, int *deg,
+                                                              int *counter, int *ready) {
+    const int ij = blockIdx.x * kThreads + threadIdx.x, m = c.m, n = c.n;
+    int d = INT_MAX;
+    if (ij < m * n) {
+        const int s = code(st[ij]);
+        d = (s >= 2 && s <= 3) ? 0 : ((s >= 4 && s <= 5) ? -1 : INT_MAX);
+        done[ij] = d;
     }
+    const bool pend = d == -1;
+    wave_count(pend, &counter[3]);
+    int cnt = 0;
+    if (pend) {
+        const int r = xc_radius(c, phi, st, ij), rw = r > 7 ? 7 : r;
+        if (rw > 3) atomicMax(&counter[10], rw);
+        const int i = ij % m, j = ij / m;
+        const double ph = phi[ij];
+        int ilo, ihi, jlo, jhi;
+        window(i, j, m, n, rw, ilo, ihi, jlo, jhi);
+        for (int dj = jlo; dj <= jhi; dj++)
+            for (int di = ilo; di <= ihi; di++) {
+                const int k = ij + dj * m + di, sk = code(st[k]);
+                cnt += (sk >= 4 && sk <= 5 && phi[k] < ph) ? 1 : 0;
+            }
+        deg[ij] = cnt | r << 16;
+    }
+    wave_push(pend && cnt == 0, ij, ready, &counter[13]);
+}
+
+// one team's staging for a fit: the addend of each of its (at most 21)
+// sums for each of 64 window entries, rows padded against LDS bank
+// conflicts
+constexpr int kSums = 21, kPad = 65;
+struct TeamBuf {
+    double a[kSums][kPad];
+};
+
+// The addends of one window entry, formed as the sequential loop forms them
+// (traverse.hh:196-212 for cells, 145-152 for nodes): 0..5 the weight sums
+// s, sx, sy, sxx, sxy, syy (cells di * w, nodes w * di) and 6 + 3q + e field
+// q times w, w x, w y.  Zero for an entry not used: a sum that starts at +0
+// never holds -0, so adding +0 leaves it as it was.
+template <bool CELL, int NV>
+__device__ __forceinline__ void stage_addends(TeamBuf &B, int lane, bool used, int di, int dj, double w,
+                                              const double *v) {
+    double t[6];
+    if (CELL) {
+        t[0] = w;
+        t[1] = di * w;
+        t[2] = dj * w;
+        t[3] = di * di * w;
+        t[4] = di * dj * w;
+        t[5] = dj * dj * w;
+    } else {
+        t[0] = w;
+        t[1] = w * di;
+        t[2] = w * dj;
+        t[3] = w * di * di;
+        t[4] = w * di * dj;
+        t[5] = w * dj * dj;
+    }
+#pragma unroll
+    for (int a = 0; a < 6; a++) B.a[a][lane] = used ? t[a] : 0.0;
+    const double wx = w * di, wy = w * dj;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        B.a[6 + 3 * q][lane] = used ? w * v[q] : 0.0;
+        B.a[7 + 3 * q][lane] = used ? wx * v[q] : 0.0;
+        B.a[8 + 3 * q][lane] = used ? wy * v[q] : 0.0;
+    }
+}
+
+// lane a < nsums adds up row a of the staging over cnt entries, in order
+__device__ __forceinline__ double team_sum(const TeamBuf &B, int lane, int nsums, int cnt, double acc) {
+    if (lane < nsums) {
+        const double *row = B.a[lane];
+        int u = 0;
+        for (; u + 8 <= cnt; u += 8) {
+            double x[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) x[e] = row[u + e];
+#pragma unroll
+            for (int e = 0; e < 8; e++) acc += x[e];
+        }
+        for (; u < cnt; u++) acc += row[u];
+    }
+    return acc;
+}
+
+// the fit from the team's sums: lane 6 + 3q returns field q, and false if
+// the system is singular
+__device__ __forceinline__ bool team_solve(double acc, int lane, double &out) {
+    const double s = __shfl(acc, 0), sx = __shfl(acc, 1), sy = __shfl(acc, 2), sxx = __shfl(acc, 3),
+                 sxy = __shfl(acc, 4), syy = __shfl(acc, 5);
+    const double p1 = __shfl(acc, (lane + 1) & 63), p2 = __shfl(acc, (lane + 2) & 63);
+    double det = fit_det(s, sx, sy, sxx, sxy, syy);
+    if (fabs(det) < 1e-10) return false;
     det = 1 / det;
     const double al = (sxy * sxy - sxx * syy) * det, be = (sx * syy - sxy * sy) * det,
                  ga = (sxx * sy - sxy * sx) * det;
-    f.p[ij] = p[0] * al + p[1] * be + p[2] * ga;
-    f.q[ij] = p[3] * al + p[4] * be + p[5] * ga;
-    f.s[ij] = p[6] * al + p[7] * be + p[8] * ga;
-    f.tau[ij] = p[9] * al + p[10] * be + p[11] * ga;
-    f.chi[ij] = p[12] * al + p[13] * be + p[14] * ga;
-    done[ij] = wave;
+    out = acc * al + p1 * be + p2 * ga;
+    return true;
 }
 
-// ---------------------------------------------------------------------------
-// extrapolation of the node fields (traverse.hh:62-174, extrap.hh:26-47)
-// ---------------------------------------------------------------------------
+// what a lane of a team reads about one neighbour of the entry it releases,
+// loaded before the entry's fit so that the two loads overlap (none of it
+// changes in between: a dependant stays pending until its dependencies are
+// released)
+struct CellProbe {
+    int k, dx, dy, d, g;
+    double p;
+    bool in;
+};
+
+// the cell walk (traverse.hh:176-240): a cell depends on the pending cells
+// of its window with a smaller phi
+struct CellWalk {
+    BdConst c;
+    const double *phi;
+    BdCells f;
+    int *done, *deg, *counter;
+    int R;  // the largest window half-width
+
+    __device__ void fit(int wave, int ij, int lane, TeamBuf &B) const {
+        const int m = c.m, n = c.n, i = ij % m, j = ij / m, r = deg[ij] >> 16;
+        if (r > 7) {  // no window gives a fit (traverse.hh:214-218)
+            if (lane == 0) {
+                atomicAdd(&counter[2], 1);
+                done[ij] = wave;
+            }
+            return;
+        }
+        const double ph = phi[ij];
+        int ilo, ihi, jlo, jhi;
+        window(i, j, m, n, r, ilo, ihi, jlo, jhi);
+        const int wd = ihi - ilo + 1, nwin = wd * (jhi - jlo + 1);
+        double acc = 0;
+        bool bad = false;
+        for (int base = 0; base < nwin; base += 64) {
+            const int t = base + lane, tc = t < nwin ? t : base;
+            const int di = ilo + tc % wd, dj = jlo + tc / wd, k = ij + dj * m + di, d = done[k];
+            const double pk = phi[k], v[5] = {f.p[k], f.q[k], f.s[k], f.tau[k], f.chi[k]};
+            const bool used = t < nwin && d != INT_MAX && pk < ph;
+            bad |= used && (d < 0 || d >= wave);
+            stage_addends<true, 5>(B, lane, used, di, dj, ph - pk, v);
+            wave_sync();
+            acc = team_sum(B, lane, 21, nwin - base < 64 ? nwin - base : 64, acc);
+            wave_sync();
+        }
+        double out = 0;
+        bad |= !team_solve(acc, lane, out);  // the window was chosen nonsingular
+        if (lane >= 6 && lane < 21 && (lane - 6) % 3 == 0) {
+            switch ((lane - 6) / 3) {
+            case 0: f.p[ij] = out; break;
+            case 1: f.q[ij] = out; break;
+            case 2: f.s[ij] = out; break;
+            case 3: f.tau[ij] = out; break;
+            default: f.chi[ij] = out;
+            }
+        }
+        if (__any(bad) && lane == 0) atomicAdd(&counter[5], 1);
+        if (lane == 0) done[ij] = wave;
+    }
+
+    // neighbour base + lane of the (2R+1)^2 square around ij
+    __device__ CellProbe probe(int ij, int lane, int base) const {
+        const int m = c.m, n = c.n, i = ij % m, j = ij / m, span = 2 * R + 1;
+        const int t = base + lane, dx = t % span - R, dy = t / span - R;
+        const bool in = t < span * span && (dx | dy) != 0 && i + dx >= 0 && i + dx < m && j + dy >= 0 && j + dy < n;
+        const int k = in ? ij + dy * m + dx : ij;
+        return CellProbe{k, dx, dy, done[k], deg[k], phi[k], in};
+    }
+
+    // a pending cell k depends on ij if ij lies in its window and has the
+    // smaller phi: one count less, and k is ready at zero
+    __device__ void release(int ij, int lane, const CellProbe &first, int *next, int *nnext) const {
+        const int span = 2 * R + 1, area = span * span;
+        const double ph = phi[ij];
+        for (int base = 0; base < area; base += 64) {
+            const CellProbe P = base == 0 ? first : probe(ij, lane, base);
+            const int rk = (P.g >> 16) > 7 ? 7 : P.g >> 16;
+            const bool dep = P.in && P.d == -1 && ph < P.p && abs(P.dx) <= rk && abs(P.dy) <= rk;
+            const bool ready = dep && (atomicSub(&deg[P.k], 1) & 0xffff) == 1;
+            wave_push(ready, P.k, next, nnext);
+        }
+    }
+};
 
 // traverse::staggered_phi (traverse.hh:109-117)
 __device__ __forceinline__ double staggered_phi(const BdConst &c, const double *phi, int ie, int je) {
@@ -987,125 +1176,214 @@ __device__ __forceinline__ double staggered_phi(const BdConst &c, const double *
 // nodes completed by one cell go in increasing staggered phi, then in corner
 // order.  key = (phi of that cell, its index, staggered phi, corner).
 // done: 0 source (status 0), -1 pending, INT_MAX never set.
-struct NodeKey {
-    double cphi, sphi;
-    int cidx, corner;
-};
-
 __global__ void __launch_bounds__(kThreads) xn_prepare_kernel(BdConst c, const double *__restrict__ phi,
                                                               const int8_t *__restrict__ st,
                                                               const int8_t *__restrict__ nst, int *done,
-                                                              double *keyd, int *keyi, int *counter, int *list) {
+                                                              double *keyd, int *keyi, int *counter) {
     const int k = blockIdx.x * kThreads + threadIdx.x, mne = c.me * c.ne;
-    if (k >= mne) return;
-    const int ie = k % c.me, je = k / c.me, m = c.m, n = c.n;
-    const double spt = -1.2 * sqrt(c.dx * c.dx + c.dy * c.dy);
-    keyd[2 * (size_t)k + 1] = staggered_phi(c, phi, ie, je);
-    if (nst[k] == 0) {
-        done[k] = 0;
-        return;
-    }
-    bool ok = true;
-    double best = 0;
-    int bidx = -1, corner = 0;
-    for (int q = 0; q < 4; q++) {  // cells (ie-1,je-1), (ie,je-1), (ie-1,je), (ie,je)
-        const int ci = ie - 1 + (q & 1), cj = je - 1 + (q >> 1);
-        if (ci < 0 || ci >= m || cj < 0 || cj >= n) continue;
-        const int cidx = ci + m * cj, s = code(st[cidx]);
-        const double cp = phi[cidx];
-        if (!(s >= 2 && s <= 5) || cp < spt) {
-            ok = false;
-            break;
-        }
-        if (bidx < 0 || cp < best || (cp == best && cidx < bidx)) {
-            best = cp;
-            bidx = cidx;
-            corner = 3 - q;  // the node is this corner of the cell
-        }
-    }
-    if (!ok || bidx < 0) {
-        done[k] = INT_MAX;
-        return;
-    }
-    done[k] = -1;
-    list[atomicAdd(&counter[3], 1)] = k;
-    keyd[2 * (size_t)k] = best;
-    keyi[2 * (size_t)k] = bidx;
-    keyi[2 * (size_t)k + 1] = corner;
-}
-
-__device__ __forceinline__ bool key_less(const double *keyd, const int *keyi, int a, int b) {
-    const double ca = keyd[2 * (size_t)a], cb = keyd[2 * (size_t)b];
-    if (ca != cb) return ca < cb;
-    const int ia = keyi[2 * (size_t)a], ib = keyi[2 * (size_t)b];
-    if (ia != ib) return ia < ib;
-    const double sa = keyd[2 * (size_t)a + 1], sb = keyd[2 * (size_t)b + 1];
-    if (sa != sb) return sa < sb;
-    return keyi[2 * (size_t)a + 1] < keyi[2 * (size_t)b + 1];
-}
-
-__global__ void __launch_bounds__(kThreads) xn_wave_kernel(BdConst c, BdNodes f, int *done,
-                                                           const double *__restrict__ keyd,
-                                                           const int *__restrict__ keyi, int wave, int *counter,
-                                                           const int *__restrict__ list, int count) {
-    const int q0 = blockIdx.x * kThreads + threadIdx.x, me = c.me, ne = c.ne;
-    if (q0 >= count) return;
-    const int k = list[q0];
-    if (done[k] != -1) return;
-    const int ie = k % me, je = k / me;
-    const double sp = keyd[2 * (size_t)k + 1];
-    const int r = 2;
-    const int ilo = ie <= r ? -ie : -r, ihi = ie + r >= me ? me - 1 - ie : r, jlo = je <= r ? -je : -r,
-              jhi = je + r >= ne ? ne - 1 - je : r;
-    // a neighbour is used if it is a source or set before this node, and has
-    // a smaller staggered phi (traverse.hh:133-145)
-    auto used = [&](int q) {
-        const int d = done[q];
-        if (d == INT_MAX) return 0;
-        if (!(keyd[2 * (size_t)q + 1] < sp)) return 0;
-        if (d == 0) return 1;
-        if (!key_less(keyd, keyi, q, k)) return 0;
-        return (d > 0 && d < wave) ? 1 : -1;
-    };
-    for (int dj = jlo; dj <= jhi; dj++)
-        for (int di = ilo; di <= ihi; di++)
-            if (used(k + dj * me + di) < 0) return;
-    double s = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0, p[12];
-    for (int q = 0; q < 12; q++) p[q] = 0;
-    for (int dj = jlo; dj <= jhi; dj++)
-        for (int di = ilo; di <= ihi; di++) {
-            const int q = k + dj * me + di;
-            if (used(q) != 1) continue;
-            double w = keyd[2 * (size_t)q + 1];
-            w = sp - w;
-            s += w;
-            sx += w * di;
-            sy += w * dj;
-            sxx += w * di * di;
-            sxy += w * di * dj;
-            syy += w * dj * dj;
-            const double x = di, y = dj;
-            const double v[4] = {f.u[q], f.v[q], f.X[q], f.Y[q]};
-            for (int t = 0; t < 4; t++) {
-                p[3 * t] += w * v[t];
-                p[3 * t + 1] += w * x * v[t];
-                p[3 * t + 2] += w * y * v[t];
+    bool pend = false;
+    if (k < mne) {
+        const int ie = k % c.me, je = k / c.me, m = c.m, n = c.n;
+        const double spt = -1.2 * sqrt(c.dx * c.dx + c.dy * c.dy);
+        keyd[2 * (size_t)k + 1] = staggered_phi(c, phi, ie, je);
+        int d = 0;
+        if (nst[k] != 0) {
+            bool ok = true;
+            double best = 0;
+            int bidx = -1, corner = 0;
+            for (int q = 0; q < 4; q++) {  // cells (ie-1,je-1), (ie,je-1), (ie-1,je), (ie,je)
+                const int ci = ie - 1 + (q & 1), cj = je - 1 + (q >> 1);
+                if (ci < 0 || ci >= m || cj < 0 || cj >= n) continue;
+                const int cidx = ci + m * cj, s = code(st[cidx]);
+                const double cp = phi[cidx];
+                if (!(s >= 2 && s <= 5) || cp < spt) {
+                    ok = false;
+                    break;
+                }
+                if (bidx < 0 || cp < best || (cp == best && cidx < bidx)) {
+                    best = cp;
+                    bidx = cidx;
+                    corner = 3 - q;  // the node is this corner of the cell
+                }
+            }
+            if (!ok || bidx < 0) {
+                d = INT_MAX;
+            } else {
+                d = -1;
+                keyd[2 * (size_t)k] = best;
+                keyi[2 * (size_t)k] = bidx;
+                keyi[2 * (size_t)k + 1] = corner;
             }
         }
-    double det = s * (sxy * sxy - sxx * syy) + sx * sx * syy - 2 * sx * sxy * sy + sy * sy * sxx;
-    if (fabs(det) < 1e-10) {
-        atomicAdd(&counter[2], 1);
-        done[k] = wave;
-        return;
+        done[k] = d;
+        pend = d == -1;
     }
-    det = 1 / det;
-    const double al = (sxy * sxy - sxx * syy) * det, be = (sx * syy - sxy * sy) * det,
-                 ga = (sxx * sy - sxy * sx) * det;
-    f.u[k] = p[0] * al + p[1] * be + p[2] * ga;
-    f.v[k] = p[3] * al + p[4] * be + p[5] * ga;
-    f.X[k] = p[6] * al + p[7] * be + p[8] * ga;
-    f.Y[k] = p[9] * al + p[10] * be + p[11] * ga;
-    done[k] = wave;
+    wave_count(pend, &counter[3]);
+}
+
+// key order (cell phi, cell index, staggered phi, corner) of two nodes
+__device__ __forceinline__ bool key_lt(double ca, int ia, double sa, int oa, double cb, int ib, double sb, int ob) {
+    if (ca != cb) return ca < cb;
+    if (ia != ib) return ia < ib;
+    if (sa != sb) return sa < sb;
+    return oa < ob;
+}
+__device__ __forceinline__ bool key_less(const double *keyd, const int *keyi, int a, int b) {
+    return key_lt(keyd[2 * (size_t)a], keyi[2 * (size_t)a], keyd[2 * (size_t)a + 1], keyi[2 * (size_t)a + 1],
+                  keyd[2 * (size_t)b], keyi[2 * (size_t)b], keyd[2 * (size_t)b + 1], keyi[2 * (size_t)b + 1]);
+}
+
+// A node uses the neighbours of its 5x5 window with a smaller staggered phi
+// that are sources or precede it in key order (traverse.hh:133-145); the
+// pending ones among the latter are its dependencies.
+__global__ void __launch_bounds__(kThreads) xn_deps_kernel(BdConst c, const int *__restrict__ done,
+                                                           const double *__restrict__ keyd,
+                                                           const int *__restrict__ keyi, int *deg, int *counter,
+                                                           int *ready) {
+    const int k = blockIdx.x * kThreads + threadIdx.x, me = c.me, ne = c.ne;
+    const bool pend = k < me * ne && done[k] == -1;
+    int cnt = 0;
+    if (pend) {
+        const int ie = k % me, je = k / me;
+        const double sp = keyd[2 * (size_t)k + 1];
+        int ilo, ihi, jlo, jhi;
+        window(ie, je, me, ne, 2, ilo, ihi, jlo, jhi);
+        for (int dj = jlo; dj <= jhi; dj++)
+            for (int di = ilo; di <= ihi; di++) {
+                const int q = k + dj * me + di;
+                cnt += (done[q] == -1 && keyd[2 * (size_t)q + 1] < sp && key_less(keyd, keyi, q, k)) ? 1 : 0;
+            }
+        deg[k] = cnt;
+    }
+    wave_push(pend && cnt == 0, k, ready, &counter[13]);
+}
+
+struct NodeProbe {
+    int q, d, qi, qo;
+    double qc, qs;
+    bool in;
+};
+
+// the node walk (traverse.hh:119-174)
+struct NodeWalk {
+    BdConst c;
+    BdNodes f;
+    int *done, *deg, *counter;
+    const double *keyd;
+    const int *keyi;
+
+    __device__ void fit(int wave, int k, int lane, TeamBuf &B) const {
+        const int me = c.me, ne = c.ne, ie = k % me, je = k / me;
+        const double sp = keyd[2 * (size_t)k + 1], kc = keyd[2 * (size_t)k];
+        const int ki = keyi[2 * (size_t)k], ko = keyi[2 * (size_t)k + 1];
+        int ilo, ihi, jlo, jhi;
+        window(ie, je, me, ne, 2, ilo, ihi, jlo, jhi);
+        const int wd = ihi - ilo + 1, nwin = wd * (jhi - jlo + 1);  // at most 25
+        const int lc = lane < nwin ? lane : 0, di = ilo + lc % wd, dj = jlo + lc / wd, q = k + dj * me + di;
+        const int d = done[q], qi = keyi[2 * (size_t)q], qo = keyi[2 * (size_t)q + 1];
+        const double sq = keyd[2 * (size_t)q + 1], qc = keyd[2 * (size_t)q],
+                     v[4] = {f.u[q], f.v[q], f.X[q], f.Y[q]};
+        // used: a source, or set before k in key order (traverse.hh:133-145)
+        const bool cand = lane < nwin && d != INT_MAX && sq < sp, prec = key_lt(qc, qi, sq, qo, kc, ki, sp, ko);
+        const bool used = cand && (d == 0 || prec), bad = cand && d != 0 && prec && !(d > 0 && d < wave);
+        stage_addends<false, 4>(B, lane, used, di, dj, sp - sq, v);
+        wave_sync();
+        const double acc = team_sum(B, lane, 18, nwin, 0.0);
+        wave_sync();
+        double out = 0;
+        const bool ok = team_solve(acc, lane, out);
+        if (__any(bad) && lane == 0) atomicAdd(&counter[5], 1);
+        if (!ok) {  // singular fit (traverse.hh:153-164)
+            if (lane == 0) atomicAdd(&counter[2], 1);
+        } else if (lane >= 6 && lane < 18 && (lane - 6) % 3 == 0) {
+            switch ((lane - 6) / 3) {
+            case 0: f.u[k] = out; break;
+            case 1: f.v[k] = out; break;
+            case 2: f.X[k] = out; break;
+            default: f.Y[k] = out;
+            }
+        }
+        if (lane == 0) done[k] = wave;
+    }
+
+    // neighbour lane of the 5x5 square around k
+    __device__ NodeProbe probe(int k, int lane, int) const {
+        const int me = c.me, ne = c.ne, ie = k % me, je = k / me, dx = lane % 5 - 2, dy = lane / 5 - 2;
+        const bool in = lane < 25 && (dx | dy) != 0 && ie + dx >= 0 && ie + dx < me && je + dy >= 0 && je + dy < ne;
+        const int q = in ? k + dy * me + dx : k;
+        return NodeProbe{q, done[q], keyi[2 * (size_t)q], keyi[2 * (size_t)q + 1], keyd[2 * (size_t)q],
+                         keyd[2 * (size_t)q + 1], in};
+    }
+
+    // a pending node q depends on k if k lies in its window, has the smaller
+    // staggered phi and precedes it in key order
+    __device__ void release(int k, int lane, const NodeProbe &P, int *next, int *nnext) const {
+        const double sp = keyd[2 * (size_t)k + 1], kc = keyd[2 * (size_t)k];
+        const int ki = keyi[2 * (size_t)k], ko = keyi[2 * (size_t)k + 1];
+        const bool dep = P.in && P.d == -1 && sp < P.qs && key_lt(kc, ki, sp, ko, P.qc, P.qi, P.qs, P.qo);
+        const bool ready = dep && (atomicSub(&deg[P.q], 1) & 0xffff) == 1;
+        wave_push(ready, P.q, next, nnext);
+    }
+};
+
+constexpr int kWaveThreads = 256, kWaveTeams = kWaveThreads / 64, kWaveBlocks = 256, kMaxWaves = 1 << 20;
+
+// One wave of a walk: the teams of the grid set the entries of this wave's
+// list and release their dependants into the next one.  Lists alternate by
+// wave parity; the counts sit in three slots, wave w reading slot w % 3,
+// filling slot (w + 1) % 3 and clearing slot (w + 2) % 3 for wave w + 1 (the
+// wave before read it, and the wave after fills it, each in a launch of its
+// own).  A launch after the last wave finds a zero count and returns.
+template <class Walk>
+__device__ __forceinline__ static void walk_wave(const Walk &W, int wave, int *lists, long span, int *counter) {
+    __shared__ TeamBuf buf[kWaveTeams];
+    int *slot = counter + 12;
+    const int count = slot[wave % 3], lane = threadIdx.x & 63, team = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        slot[(wave + 2) % 3] = 0;
+        if (count > 0) {
+            counter[8] = wave;
+            counter[9] += count;
+        }
+    }
+    if (count == 0) return;
+    const int *cur = lists + (wave & 1) * span;
+    int *nxt = lists + ((wave + 1) & 1) * span;
+    for (int t = blockIdx.x * kWaveTeams + team; t < count; t += gridDim.x * kWaveTeams) {
+        const int e = cur[t];
+        const auto first = W.probe(e, lane, 0);
+        W.fit(wave, e, lane, buf[team]);
+        W.release(e, lane, first, nxt, slot + (wave + 1) % 3);
+    }
+}
+
+__global__ void __launch_bounds__(kWaveThreads) xc_wave_kernel(BdConst c, const double *__restrict__ phi, BdCells f,
+                                                               int *done, int *deg, int *counter, int *lists,
+                                                               long span, int wave) {
+    const int R = counter[10] > 3 ? counter[10] : 3;
+    walk_wave(CellWalk{c, phi, f, done, deg, counter, R}, wave, lists, span, counter);
+}
+
+__global__ void __launch_bounds__(kWaveThreads) xn_wave_kernel(BdConst c, BdNodes f, int *done,
+                                                               const double *__restrict__ keyd,
+                                                               const int *__restrict__ keyi, int *deg, int *counter,
+                                                               int *lists, long span, int wave) {
+    walk_wave(NodeWalk{c, f, done, deg, counter, keyd, keyi}, wave, lists, span, counter);
+}
+
+// the count of wave w (zero once the walk has ended) for the host
+__global__ void wave_count_kernel(const int *counter, int wave, volatile int *hflag) {
+    if (threadIdx.x == 0) hflag[12] = counter[12 + wave % 3];
+}
+
+// a walk's counts for the host: pending, order violations, waves, set
+__global__ void walk_result_kernel(const int *counter, volatile int *hflag) {
+    if (threadIdx.x == 0) {
+        hflag[8] = counter[3];
+        hflag[9] = counter[5];
+        hflag[10] = counter[8];
+        hflag[11] = counter[9];
+    }
 }
 
 inline dim3 blocks(long n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
@@ -1195,79 +1473,68 @@ int bd_ls_node_status(const BdConst &c, const double *phi, const int8_t *st, int
     return STZ_OK;
 }
 
-// Waves in batches that double from 4 to 64 launches; after each batch the
-// list keeps only the cells still pending (so late waves, when a few long
-// dependency chains remain, touch a few cells).  list has room for twice
-// the longest list.
+// Waves in batches of launches that double from 16 to 256; after each
+// batch the host reads whether the walk has ended.
 template <class Launch>
-static int run_waves(Launch launch, const int *done, int *counter, int *list, int count, volatile int *hflag,
-                     int *waves, const char *what, hipStream_t s) {
-    const int max_waves = 1 << 16;
-    int w = 1, batch = 4;
-    int *cur = list, *nxt = list + count;
+static int walk(Launch launch, const int *counter, volatile int *hflag, int *waves, const char *what,
+                hipStream_t s) {
+    int w = 1, batch = 16;
     for (;;) {
-        for (int q = 0; q < batch; q++, w++) launch(w, cur, count);
-        hipLaunchKernelGGL(zero_one_kernel, dim3(1), dim3(64), 0, s, counter + 3);
-        hipLaunchKernelGGL(compact_kernel, blocks(count), dim3(kThreads), 0, s, done, cur, count, nxt, counter);
-        hipLaunchKernelGGL(flag_kernel, dim3(1), dim3(64), 0, s, counter + 3, hflag);
+        for (int q = 0; q < batch; q++, w++) launch(w);
+        hipLaunchKernelGGL(wave_count_kernel, dim3(1), dim3(64), 0, s, counter, w, hflag);
         STZ_HIP(hipGetLastError());
         STZ_HIP(hipStreamSynchronize(s));
-        count = *hflag;
-        if (count == 0) break;
-        if (w >= max_waves) {
-            set_error("%s extrapolation did not finish in %d waves", what, max_waves);
-            return STZ_ERR_NOT_CONVERGED;
-        }
-        std::swap(cur, nxt);
-        batch = batch < 64 ? 2 * batch : 64;
+        if (hflag[12] == 0) break;
+        if (w >= kMaxWaves) break;
+        batch = batch < 256 ? 2 * batch : 256;
     }
-    *waves = w - 1;
-    return STZ_OK;
-}
-
-static int pending_count(const int *counter, volatile int *hflag, hipStream_t s, int *count) {
-    hipLaunchKernelGGL(flag_kernel, dim3(1), dim3(64), 0, s, counter + 3, hflag);
+    hipLaunchKernelGGL(walk_result_kernel, dim3(1), dim3(64), 0, s, counter, hflag);
+    STZ_HIP(hipGetLastError());
     STZ_HIP(hipStreamSynchronize(s));
-    *count = *hflag;
+    if (hflag[9] != 0 || hflag[11] != hflag[8]) {
+        set_error("%s extrapolation: %d of %d entries set in %d waves, %d fitted out of order", what, hflag[11],
+                  hflag[8], hflag[10], hflag[9]);
+        return STZ_ERR_NOT_CONVERGED;
+    }
+    *waves = hflag[10];
     return STZ_OK;
 }
 
-// counter: [0],[1] pending counts by wave parity, [2] failed fits, [3] list length
+// list: 3 max(cells, nodes) ints (dependency counts, two wave lists)
 int bd_extrap_cells(const BdConst &c, const double *phi, const int8_t *st, const BdCells &f, int *done,
                     int *counter, int *list, volatile int *hflag, int *waves, hipStream_t s) {
-    const dim3 g = blocks((long)c.m * c.n), b(kThreads);
+    const long mn = (long)c.m * c.n, span = std::max(mn, (long)c.me * c.ne);
+    int *lists = list + span;
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counter);
-    hipLaunchKernelGGL(xc_prepare_kernel, g, b, 0, s, c, st, done, counter, list);
+    // the first wave's list: lists + span (wave 1 is odd), its count slot 13
+    hipLaunchKernelGGL(xc_prepare_kernel, blocks(mn), dim3(kThreads), 0, s, c, phi, st, done, list, counter,
+                       lists + span);
     STZ_HIP(hipGetLastError());
-    int count = 0, e = pending_count(counter, hflag, s, &count);
-    if (e) return e;
-    *waves = 0;
-    if (count == 0) return STZ_OK;
-    return run_waves(
-        [&](int w, const int *l, int n) {
-            hipLaunchKernelGGL(xc_wave_kernel, blocks(n), b, 0, s, c, phi, f, done, w, counter, l, n);
+    return walk(
+        [&](int w) {
+            hipLaunchKernelGGL(xc_wave_kernel, dim3(kWaveBlocks), dim3(kWaveThreads), 0, s, c, phi, f, done, list,
+                               counter, lists, span, w);
         },
-        done, counter, list, count, hflag, waves, "cell", s);
+        counter, hflag, waves, "cell", s);
 }
 
 int bd_extrap_nodes(const BdConst &c, const double *phi, const int8_t *st, const int8_t *nst, const BdNodes &f,
                     int *done, double *key, int *counter, int *list, volatile int *hflag, int *waves,
                     hipStream_t s) {
-    const long mne = (long)c.me * c.ne;
-    const dim3 g = blocks(mne), b(kThreads);
-    int *keyi = reinterpret_cast<int *>(key + 2 * mne);
+    const long mne = (long)c.me * c.ne, span = std::max((long)c.m * c.n, mne);
+    int *keyi = reinterpret_cast<int *>(key + 2 * mne), *lists = list + span;
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counter);
-    hipLaunchKernelGGL(xn_prepare_kernel, g, b, 0, s, c, phi, st, nst, done, key, keyi, counter, list);
+    hipLaunchKernelGGL(xn_prepare_kernel, blocks(mne), dim3(kThreads), 0, s, c, phi, st, nst, done, key, keyi,
+                       counter);
+    hipLaunchKernelGGL(xn_deps_kernel, blocks(mne), dim3(kThreads), 0, s, c, done, key, keyi, list, counter,
+                       lists + span);
     STZ_HIP(hipGetLastError());
-    int count = 0, e = pending_count(counter, hflag, s, &count);
-    if (e) return e;
-    *waves = 0;
-    if (count == 0) return STZ_OK;
-    return run_waves(
-        [&](int w, const int *l, int n) {
-            hipLaunchKernelGGL(xn_wave_kernel, blocks(n), b, 0, s, c, f, done, key, keyi, w, counter, l, n);
+    return walk(
+        [&](int w) {
+            hipLaunchKernelGGL(xn_wave_kernel, dim3(kWaveBlocks), dim3(kWaveThreads), 0, s, c, f, done, key, keyi,
+                               list, counter, lists, span, w);
         },
-        done, counter, list, count, hflag, waves, "node", s);
+        counter, hflag, waves, "node", s);
 }
 
 }  // namespace stz

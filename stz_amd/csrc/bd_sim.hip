@@ -267,7 +267,7 @@ int stz_bd_create(const stz_bd_params *p, stz_bd **out) {
     if (hipMalloc(&b->st, mn) != hipSuccess || hipMalloc(&b->nst, mne) != hipSuccess ||
         hipMalloc(&b->rowt, mne * sizeof(int)) != hipSuccess || hipMalloc(&b->done_c, mn * sizeof(int)) != hipSuccess ||
         hipMalloc(&b->done_n, mne * sizeof(int)) != hipSuccess ||
-        hipMalloc(&b->xlist, 2 * std::max(mn, mne) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&b->xlist, 3 * std::max(mn, mne) * sizeof(int)) != hipSuccess ||
         hipMalloc(&b->counters, 16 * sizeof(int)) != hipSuccess ||
         hipMalloc(&b->dmin, sizeof(unsigned long long)) != hipSuccess) {
         set_error("out of device memory");
@@ -475,6 +475,11 @@ int stz_bd_init_bar(stz_bd *b, int ca) {
     if ((e = bd_extrap_cells(b->c, b->phi, b->st, b->cells(b->cc), b->done_c, b->counters, b->xlist, (volatile int *)b->hint,
                              &b->waves_c, b->stream)))
         return e;
+    if ((e = read_int(b, b->counters + 2, &fails))) return e;
+    if (fails) {
+        set_error("cell extrapolation: singular fit at %d cells (traverse.hh:214-230)", fails);
+        return STZ_ERR_STATE;
+    }
     if ((e = bd_extrap_nodes(b->c, b->phi, b->st, b->nst, b->nodes(b->nc), b->done_n, b->nkey, b->counters, b->xlist,
                              (volatile int *)b->hint, &b->waves_n, b->stream)))
         return e;
