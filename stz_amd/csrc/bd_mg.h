@@ -30,7 +30,7 @@ template <typename T> struct BdLevelT {
 };
 using BdLevel = BdLevelT<double>;
 
-enum { kR = 0, kRhat, kP, kV, kS, kT, kPhat, kShat, kTmp, kBdVecs };
+enum { kR = 0, kRhat, kP, kV, kS, kT, kPhat, kShat, kTmp, kPre, kBdVecs };
 constexpr int kMaxPartial = 4096;
 
 struct BdSolver {
@@ -45,7 +45,7 @@ struct BdSolver {
     // the levels; BiCGStab, its residuals and the convergence test stay FP64
     bool mixed = false;
     BdLevelT<float> lv32[kMaxLevels];
-    float *top32[3] = {};      // top-level preconditioned vectors p^, s^ and the residual
+    float *top32[4] = {};      // top level: preconditioned p^, s^, the residual, the pre-smoothed iterate
 
     static int dims_for(int tm, int tn, std::vector<std::pair<int, int>> &out);
     int init(int tm, int tn);
@@ -59,9 +59,14 @@ struct BdSolver {
   private:
     int alloc_mixed();
     void free_mixed();
-    template <typename T, typename TB> int vcycle_t(const TB *rhs, T *z, BdLevelT<T> *L, T *r0, hipStream_t s);
-    template <typename T, typename TB> int smooth_top(T *z, const TB *b, int sweeps, hipStream_t s);
-    template <typename T> int smooth_level(const BdLevelT<T> &L, bool bottom, int sweeps, hipStream_t s);
+    template <typename T, typename TB>
+    int vcycle_t(const TB *rhs, T *z, BdLevelT<T> *L, T *r0, T *zpre, hipStream_t s);
+    template <typename T, typename TB>
+    int smooth_top(T *zout, const TB *b, int sweeps, int mode, const T *zin, const T *zc, int cm, int cn,
+                   hipStream_t s);
+    template <typename T>
+    int smooth_level(const BdLevelT<T> &L, bool bottom, int sweeps, int mode, const T *zin, T *zout, const T *zc,
+                     int cm, int cn, hipStream_t s);
     int dot(const double *a, const double *b, int slot, int op, int host_slot, hipStream_t s);
 };
 

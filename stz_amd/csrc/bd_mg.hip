@@ -70,11 +70,11 @@ __device__ __forceinline__ void top_entry(const BdTop &t, int ij, int k, double 
 // off-diagonal part of row ij applied to z (qs_multi_bd::mul_a,
 // qs_multi_bd.hh:41-56) and the diagonal block; in T, with the operator's
 // FP64 constants rounded to T
+// zp: the row's own pair of z, in rows of `m` pairs (the vector itself, or
+// a tile of it in LDS)
 template <typename T, typename TZ = T>
-__device__ __forceinline__ V2T<T> top_offdiag(const BdTop &t, const TZ *z, int ij, int d) {
-    const int m = t.m;
+__device__ __forceinline__ V2T<T> top_offdiag_at(const BdTop &t, const TZ *zp, int m, int d) {
     if (d == -1) {
-        const TZ *zp = z + 2 * (size_t)ij;
         const TZ *zl = zp - 2, *zr = zp + 2, *zd = zp - 2 * m, *zu = zp + 2 * m;
         const TZ *zdl = zd - 2, *zdr = zd + 2, *zul = zu - 2, *zur = zu + 2;
         const T Kmx = (T)t.Kmx, Kmy = (T)t.Kmy, mx = (T)t.mx, my = (T)t.my, qK = (T)t.qK;
@@ -88,11 +88,15 @@ __device__ __forceinline__ V2T<T> top_offdiag(const BdTop &t, const TZ *z, int i
     const int off[8] = {-m - 1, -m, -m + 1, -1, 1, m - 1, m, m + 1};
     T rx = 0, ry = 0;
     for (int q = 0; q < 8; q++) {
-        const TZ *zz = z + 2 * (size_t)(ij + off[q]);
+        const TZ *zz = zp + 2 * (ptrdiff_t)off[q];
         rx += (T)a[4 * q] * (T)zz[0] + (T)a[4 * q + 1] * (T)zz[1];
         ry += (T)a[4 * q + 2] * (T)zz[0] + (T)a[4 * q + 3] * (T)zz[1];
     }
     return V2T<T>{rx, ry};
+}
+template <typename T, typename TZ = T>
+__device__ __forceinline__ V2T<T> top_offdiag(const BdTop &t, const TZ *z, int ij, int d) {
+    return top_offdiag_at<T, TZ>(t, z + 2 * (size_t)ij, t.m, d);
 }
 __device__ __forceinline__ double top_diag_scale(const BdTop &t, int d) {
     return d == -1 ? 1.0 : (d == -2 ? t.bcs : (d == -3 ? t.ocs : t.ncs));
@@ -102,18 +106,31 @@ template <typename T> __device__ __forceinline__ const T *lvl_stencil(const BdLe
     return L.type[ij] < 0 ? L.Sc : L.S + 40 * (size_t)ij;
 }
 
-template <typename T> __device__ __forceinline__ V2T<T> lvl_offdiag(const BdLevelT<T> &L, const T *z, int i, int j) {
-    const T *s = lvl_stencil(L, i + L.m * j);
+// off-diagonal part of a coarse row: stencil s, zp the row's own pair of z
+// in rows of `stride` pairs, (i,j) the row in the m x n level
+// (the loads are unconditional, from the row itself where a neighbour is
+// missing, so that they are all in flight at once)
+template <typename T>
+__device__ __forceinline__ V2T<T> lvl_offdiag_at(const T *s, const T *zp, int stride, int i, int j, int m, int n) {
     T rx = 0, ry = 0;
+#pragma unroll
     for (int k = 0; k < 9; k++) {
         if (k == 4) continue;
-        const int ii = i + k % 3 - 1, jj = j + k / 3 - 1;
-        if (ii < 0 || ii >= L.m || jj < 0 || jj >= L.n) continue;
-        const T *zz = z + 2 * (size_t)(ii + L.m * jj);
-        rx += s[4 * k] * zz[0] + s[4 * k + 1] * zz[1];
-        ry += s[4 * k + 2] * zz[0] + s[4 * k + 3] * zz[1];
+        const int di = k % 3 - 1, dj = k / 3 - 1;
+        const bool in = i + di >= 0 && i + di < m && j + dj >= 0 && j + dj < n;
+        const T *zz = zp + 2 * (in ? di + stride * dj : 0);
+        const T z0 = zz[0], z1 = zz[1];
+        if (!in) continue;
+        rx += s[4 * k] * z0 + s[4 * k + 1] * z1;
+        ry += s[4 * k + 2] * z0 + s[4 * k + 3] * z1;
     }
     return V2T<T>{rx, ry};
+}
+// the 40 entries of a coarse row's stencil, loaded together
+template <typename T> __device__ __forceinline__ void lvl_row(const BdLevelT<T> &L, int ij, T (&s)[40]) {
+    const T *p = lvl_stencil(L, ij);
+#pragma unroll
+    for (int q = 0; q < 40; q++) s[q] = p[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -138,33 +155,59 @@ template <typename T> __global__ void __launch_bounds__(kThreads) gs_level_kerne
     const int q = blockIdx.x * kThreads + threadIdx.x;
     if (q >= hm * hn) return;
     const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + L.m * j;
-    const V2T<T> o = lvl_offdiag(L, L.z, i, j);
+    T s[40];
+    lvl_row(L, ij, s);
+    const V2T<T> o = lvl_offdiag_at(s, L.z + 2 * (size_t)ij, L.m, i, j, L.m, L.n);
     const T rx = L.b[2 * (size_t)ij] - o.x, ry = L.b[2 * (size_t)ij + 1] - o.y;
-    const T *iv = lvl_stencil(L, ij) + 36;
-    L.z[2 * (size_t)ij] = iv[0] * rx + iv[1] * ry;
-    L.z[2 * (size_t)ij + 1] = iv[2] * rx + iv[3] * ry;
+    L.z[2 * (size_t)ij] = s[36] * rx + s[37] * ry;
+    L.z[2 * (size_t)ij + 1] = s[38] * rx + s[39] * ry;
 }
 
-// the bottom level in one workgroup: z in LDS, `sweeps` four-colour sweeps
+// The bottom level in one workgroup: z in LDS, `sweeps` four-colour sweeps.
+// Up to 1024 nodes, each thread keeps its node's stencil and right-hand side
+// in registers for all the sweeps.
 template <typename T> __global__ void __launch_bounds__(1024) gs_bottom_kernel(BdLevelT<T> L, int sweeps) {
     extern __shared__ __align__(16) unsigned char bd_smem[];
     T *zs = reinterpret_cast<T *>(bd_smem);
     const int N = L.m * L.n;
     for (int q = threadIdx.x; q < 2 * N; q += blockDim.x) zs[q] = L.z[q];
-    __syncthreads();
-    for (int sw = 0; sw < sweeps; sw++)
-        for (int colour = 0; colour < 4; colour++) {
-            const int hm = (L.m + 1 - (colour & 1)) >> 1, hn = (L.n + 1 - (colour >> 1)) >> 1;
-            for (int q = threadIdx.x; q < hm * hn; q += blockDim.x) {
-                const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + L.m * j;
-                const V2T<T> o = lvl_offdiag(L, zs, i, j);
-                const T rx = L.b[2 * ij] - o.x, ry = L.b[2 * ij + 1] - o.y;
-                const T *iv = lvl_stencil(L, ij) + 36;
-                zs[2 * ij] = iv[0] * rx + iv[1] * ry;
-                zs[2 * ij + 1] = iv[2] * rx + iv[3] * ry;
-            }
-            __syncthreads();
+    if (N <= (int)blockDim.x) {
+        const int q = threadIdx.x, own = q < N;
+        const int i = own ? q % L.m : 0, j = own ? q / L.m : 0, col = (i & 1) + 2 * (j & 1);
+        T s[40], bx = 0, by = 0;
+        lvl_row(L, own ? q : 0, s);
+        if (own) {
+            bx = L.b[2 * q];
+            by = L.b[2 * q + 1];
         }
+        __syncthreads();
+        for (int sw = 0; sw < sweeps; sw++)
+            for (int colour = 0; colour < 4; colour++) {
+                if (own && col == colour) {
+                    const V2T<T> o = lvl_offdiag_at(s, zs + 2 * q, L.m, i, j, L.m, L.n);
+                    const T rx = bx - o.x, ry = by - o.y;
+                    zs[2 * q] = s[36] * rx + s[37] * ry;
+                    zs[2 * q + 1] = s[38] * rx + s[39] * ry;
+                }
+                __syncthreads();
+            }
+    } else {
+        __syncthreads();
+        for (int sw = 0; sw < sweeps; sw++)
+            for (int colour = 0; colour < 4; colour++) {
+                const int hm = (L.m + 1 - (colour & 1)) >> 1, hn = (L.n + 1 - (colour >> 1)) >> 1;
+                for (int q = threadIdx.x; q < hm * hn; q += blockDim.x) {
+                    const int i = 2 * (q % hm) + (colour & 1), j = 2 * (q / hm) + (colour >> 1), ij = i + L.m * j;
+                    T s[40];
+                    lvl_row(L, ij, s);
+                    const V2T<T> o = lvl_offdiag_at(s, zs + 2 * ij, L.m, i, j, L.m, L.n);
+                    const T rx = L.b[2 * ij] - o.x, ry = L.b[2 * ij + 1] - o.y;
+                    zs[2 * ij] = s[36] * rx + s[37] * ry;
+                    zs[2 * ij + 1] = s[38] * rx + s[39] * ry;
+                }
+                __syncthreads();
+            }
+    }
     for (int q = threadIdx.x; q < 2 * N; q += blockDim.x) L.z[q] = zs[q];
 }
 
@@ -198,29 +241,16 @@ __global__ void __launch_bounds__(kThreads) resid_top_kernel(BdTop t, const T *_
     }
 }
 
-// y = A x on the top level, in FP64 (x FP64, or the mixed form's FP32
-// preconditioned vector)
-template <typename TX>
-__global__ void __launch_bounds__(kThreads) apply_top_kernel(BdTop t, const TX *__restrict__ x,
-                                                             double *__restrict__ y) {
-    const int ij = blockIdx.x * kThreads + threadIdx.x;
-    if (ij >= t.m * t.n) return;
-    const int d = t.rowt[ij];
-    const V2 o = top_offdiag<double, TX>(t, x, ij, d);
-    const double sc = top_diag_scale(t, d);
-    y[2 * (size_t)ij] = o.x + sc * t.cxx * (double)x[2 * (size_t)ij];
-    y[2 * (size_t)ij + 1] = o.y + sc * t.cyy * (double)x[2 * (size_t)ij + 1];
-}
-
 template <typename T> __global__ void __launch_bounds__(kThreads) resid_level_kernel(BdLevelT<T> L) {
     const int ij = blockIdx.x * kThreads + threadIdx.x;
     if (ij >= L.m * L.n) return;
     const int i = ij % L.m, j = ij / L.m;
-    const V2T<T> o = lvl_offdiag(L, L.z, i, j);
-    const T *s = lvl_stencil(L, ij) + 16;
+    T s[40];
+    lvl_row(L, ij, s);
+    const V2T<T> o = lvl_offdiag_at(s, L.z + 2 * (size_t)ij, L.m, i, j, L.m, L.n);
     const T zx = L.z[2 * (size_t)ij], zy = L.z[2 * (size_t)ij + 1];
-    L.r[2 * (size_t)ij] = L.b[2 * (size_t)ij] - o.x - (s[0] * zx + s[1] * zy);
-    L.r[2 * (size_t)ij + 1] = L.b[2 * (size_t)ij + 1] - o.y - (s[2] * zx + s[3] * zy);
+    L.r[2 * (size_t)ij] = L.b[2 * (size_t)ij] - o.x - (s[16] * zx + s[17] * zy);
+    L.r[2 * (size_t)ij + 1] = L.b[2 * (size_t)ij + 1] - o.y - (s[18] * zx + s[19] * zy);
 }
 
 // 1D prolongation parents on a bounded axis (tgmg.in.cc:552-621)
@@ -282,6 +312,188 @@ __global__ void __launch_bounds__(kThreads) prolong_kernel(const T *__restrict__
         }
     zf[2 * (size_t)ij] += sx;
     zf[2 * (size_t)ij + 1] += sy;
+}
+
+// ---------------------------------------------------------------------------
+// Tiled Gauss-Seidel: one four-colour sweep of a tile in one launch, with
+// the result of the colour-by-colour launches.  The tile (64 x 32 nodes in
+// FP32, 32 x 32 in FP64) is staged in LDS with a halo of 4 nodes: z, the
+// right-hand side (in T) and the row type of every node; colour pass p
+// (p = 0..3) updates the nodes of its colour within 3 - p of the tile, which
+// read only nodes within 4 - p, all brought up to date by the passes
+// before.  z starts from memory (MODE 0), from zero (MODE 1, pre-smoothing a
+// zero guess), or from memory plus the coarse correction (MODE 2,
+// z_f += P z_c before post-smoothing, tgmg.in.cc:552-621, from the coarse
+// nodes under the tile, staged first).  The result goes to another vector
+// (zout), since the tiles around read the input's halo while this one
+// writes.
+// ---------------------------------------------------------------------------
+constexpr int kTileThreads = 256, kHalo = 4;
+template <typename T> struct Tile {
+    static constexpr int X = sizeof(T) == 4 ? 64 : 32, Y = 32;
+    static constexpr int LW = X + 2 * kHalo, LH = Y + 2 * kHalo, N = LW * LH;
+    static constexpr int P = (N + kTileThreads - 1) / kTileThreads;  // staged nodes per thread
+    // the coarse nodes under the tile and its halo start at (x0/2 - 2, y0/2 - 2)
+    static constexpr int CW = LW / 2 + 3, CH = LH / 2 + 3, NC = CW * CH;
+    static size_t lds(int mode) {
+        return (4 * (size_t)N + (mode == 2 ? 2 * NC : 0) + 40) * sizeof(T) + (size_t)N * sizeof(int);
+    }
+};
+
+// the top level: z in T, b in TB staged as T (qs_multi_bd::mul_a rows)
+template <typename T, typename TB> struct TopRows {
+    BdTop t;
+    const TB *b;
+    using Src = TB;
+    static constexpr int kOutside = -2;  // no row: nothing is read
+    __device__ int m() const { return t.m; }
+    __device__ int n() const { return t.n; }
+    __device__ const TB *rhs() const { return b; }
+    __device__ const int *types() const { return t.rowt; }
+    __device__ void begin(T *) const {}
+    __device__ void update(int, int, int l, T *zs, const T *bs, const int *ts, const T *, int LW) const {
+        const int d = ts[l];
+        const V2T<T> o = top_offdiag_at<T>(t, zs + 2 * l, LW, d);
+        const double sc = top_diag_scale(t, d);
+        zs[2 * l] = (bs[2 * l] - o.x) / (T)(sc * t.cxx);
+        zs[2 * l + 1] = (bs[2 * l + 1] - o.y) / (T)(sc * t.cyy);
+    }
+};
+
+// a coarse level; its shared stencil is staged too
+template <typename T> struct LevelRows {
+    BdLevelT<T> L;
+    using Src = T;
+    static constexpr int kOutside = -1;
+    __device__ int m() const { return L.m; }
+    __device__ int n() const { return L.n; }
+    __device__ const T *rhs() const { return L.b; }
+    __device__ const int *types() const { return L.type; }
+    __device__ void begin(T *sc) const {
+        for (int q = threadIdx.x; q < 40; q += blockDim.x) sc[q] = L.Sc[q];
+    }
+    // the 40 stencil entries are loaded together before they are used
+    __device__ void relax(const T *s0, int i, int j, int l, T *zs, const T *bs, int LW) const {
+        T s[40];
+#pragma unroll
+        for (int q = 0; q < 40; q++) s[q] = s0[q];
+        const V2T<T> o = lvl_offdiag_at(s, zs + 2 * l, LW, i, j, L.m, L.n);
+        const T rx = bs[2 * l] - o.x, ry = bs[2 * l + 1] - o.y;
+        zs[2 * l] = s[36] * rx + s[37] * ry;
+        zs[2 * l + 1] = s[38] * rx + s[39] * ry;
+    }
+    __device__ void update(int i, int j, int l, T *zs, const T *bs, const int *ts, const T *sc, int LW) const {
+        if (ts[l] < 0) relax(sc, i, j, l, zs, bs, LW);
+        else relax(L.S + 40 * (i + (size_t)L.m * j), i, j, l, zs, bs, LW);
+    }
+};
+
+template <typename T, int MODE, class Rows>
+__global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const T *__restrict__ z,
+                                                               T *__restrict__ zout, const T *__restrict__ zc,
+                                                               int cm, int cn) {
+    using D = Tile<T>;
+    using TB = typename Rows::Src;
+    constexpr int P = D::P;
+    extern __shared__ __align__(16) unsigned char bd_tile_smem[];
+    T *zs = reinterpret_cast<T *>(bd_tile_smem), *bs = zs + 2 * D::N, *cs = bs + 2 * D::N,
+      *sc = cs + (MODE == 2 ? 2 * D::NC : 0);
+    int *ts = reinterpret_cast<int *>(sc + 40);
+    const int m = rows.m(), n = rows.n(), tid = threadIdx.x;
+    const int x0 = blockIdx.x * D::X, y0 = blockIdx.y * D::Y, cx0 = x0 / 2 - 2, cy0 = y0 / 2 - 2;
+    const TB *b = rows.rhs();
+    const int *tp = rows.types();
+    // staged nodes q = tid + k kTileThreads; every load of a thread is issued
+    // before the first is used
+    T vx[P], vy[P];
+    TB bx[P], by[P];
+    int tt[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+        const int q = tid + k * kTileThreads, i = x0 - kHalo + q % D::LW, j = y0 - kHalo + q / D::LW;
+        const bool in = q < D::N && i >= 0 && i < m && j >= 0 && j < n;
+        const size_t ij = in ? i + (size_t)m * j : 0;
+        if (MODE != 1) {
+            const T a0 = z[2 * ij], a1 = z[2 * ij + 1];
+            vx[k] = in ? a0 : (T)0;
+            vy[k] = in ? a1 : (T)0;
+        } else {
+            vx[k] = vy[k] = 0;
+        }
+        const TB b0 = b[2 * ij], b1 = b[2 * ij + 1];
+        const int t0 = tp[ij];
+        bx[k] = in ? b0 : (TB)0;
+        by[k] = in ? b1 : (TB)0;
+        tt[k] = in ? t0 : Rows::kOutside;
+    }
+    if (MODE == 2) {
+        constexpr int PC = (D::NC + kTileThreads - 1) / kTileThreads;
+        T cx[PC], cy[PC];
+#pragma unroll
+        for (int k = 0; k < PC; k++) {
+            const int q = tid + k * kTileThreads, I = cx0 + q % D::CW, J = cy0 + q / D::CW;
+            const bool in = q < D::NC && I >= 0 && I < cm && J >= 0 && J < cn;
+            const size_t IJ = in ? I + (size_t)cm * J : 0;
+            const T a0 = zc[2 * IJ], a1 = zc[2 * IJ + 1];
+            cx[k] = in ? a0 : (T)0;
+            cy[k] = in ? a1 : (T)0;
+        }
+#pragma unroll
+        for (int k = 0; k < PC; k++) {
+            const int q = tid + k * kTileThreads;
+            if (q < D::NC) {
+                cs[2 * q] = cx[k];
+                cs[2 * q + 1] = cy[k];
+            }
+        }
+        __syncthreads();
+    }
+    rows.begin(sc);
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+        const int q = tid + k * kTileThreads, i = x0 - kHalo + q % D::LW, j = y0 - kHalo + q / D::LW;
+        if (q >= D::N) continue;
+        T wxv = vx[k], wyv = vy[k];
+        if (MODE == 2 && i >= 0 && i < m && j >= 0 && j < n) {  // as prolong_kernel
+            int px[2], py[2];
+            double wx[2], wy[2];
+            const int nx = parents(i, m, cm, px, wx), ny = parents(j, n, cn, py, wy);
+            T sx = 0, sy = 0;
+            for (int bb = 0; bb < ny; bb++)
+                for (int aa = 0; aa < nx; aa++) {
+                    const T w = (T)(wx[aa] * wy[bb]);
+                    const T *zz = cs + 2 * ((px[aa] - cx0) + D::CW * (py[bb] - cy0));
+                    sx += w * zz[0];
+                    sy += w * zz[1];
+                }
+            wxv += sx;
+            wyv += sy;
+        }
+        zs[2 * q] = wxv;
+        zs[2 * q + 1] = wyv;
+        bs[2 * q] = (T)bx[k];
+        bs[2 * q + 1] = (T)by[k];
+        ts[q] = tt[k];
+    }
+    __syncthreads();
+    for (int c = 0; c < 4; c++) {
+        const int h = kHalo - 1 - c;
+        const int ax = max(x0 - h, 0), ex = min(x0 + D::X + h, m), ay = max(y0 - h, 0), ey = min(y0 + D::Y + h, n);
+        const int ix = ax + ((ax ^ c) & 1), jy = ay + ((ay ^ (c >> 1)) & 1);
+        const int nx = ex > ix ? (ex - ix + 1) >> 1 : 0, ny = ey > jy ? (ey - jy + 1) >> 1 : 0;
+        for (int q = tid; q < nx * ny; q += kTileThreads) {
+            const int i = ix + 2 * (q % nx), j = jy + 2 * (q / nx);
+            rows.update(i, j, (i - x0 + kHalo) + D::LW * (j - y0 + kHalo), zs, bs, ts, sc, D::LW);
+        }
+        __syncthreads();
+    }
+    for (int q = tid; q < D::X * D::Y; q += kTileThreads) {
+        const int i = x0 + q % D::X, j = y0 + q / D::X;
+        if (i >= m || j >= n) continue;
+        const int l = (i - x0 + kHalo) + D::LW * (j - y0 + kHalo);
+        zout[2 * (i + (size_t)m * j)] = zs[2 * l];
+        zout[2 * (i + (size_t)m * j) + 1] = zs[2 * l + 1];
+    }
 }
 
 // Sources of fine-level entries for the Galerkin product: the top operator,
@@ -449,6 +661,88 @@ __global__ void __launch_bounds__(kThreads) finish_dot_kernel(const double *__re
     if (host) *host = s;
 }
 
+// Products fused into the kernels that form one of their factors.  Block
+// sums go to partial[block] and partial[kMaxPartial + block].
+__device__ __forceinline__ void block_sums2(double a1, double a2, double *partial) {
+    __shared__ double red[2][kThreads / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        a1 += __shfl_down(a1, o, 64);
+        a2 += __shfl_down(a2, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a1;
+        red[1][threadIdx.x >> 6] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s1 = 0, s2 = 0;
+        for (int w = 0; w < kThreads / 64; w++) {
+            s1 += red[0][w];
+            s2 += red[1][w];
+        }
+        partial[blockIdx.x] = s1;
+        partial[kMaxPartial + blockIdx.x] = s2;
+    }
+}
+
+// y = A x on the top level in FP64 (x FP64, or the mixed form's FP32
+// preconditioned vector), with the sums of (u, y) and, when yy, of (y, y)
+template <typename TX>
+__global__ void __launch_bounds__(kThreads) apply_top_dot_kernel(BdTop t, const TX *__restrict__ x,
+                                                                 double *__restrict__ y,
+                                                                 const double *__restrict__ u, int yy,
+                                                                 double *__restrict__ partial) {
+    double a1 = 0, a2 = 0;
+    const int mn = t.m * t.n;
+    for (int ij = blockIdx.x * kThreads + threadIdx.x; ij < mn; ij += gridDim.x * kThreads) {
+        const int d = t.rowt[ij];
+        const V2 o = top_offdiag<double, TX>(t, x, ij, d);
+        const double sc = top_diag_scale(t, d);
+        const double y0 = o.x + sc * t.cxx * (double)x[2 * (size_t)ij];
+        const double y1 = o.y + sc * t.cyy * (double)x[2 * (size_t)ij + 1];
+        y[2 * (size_t)ij] = y0;
+        y[2 * (size_t)ij + 1] = y1;
+        a1 += u[2 * (size_t)ij] * y0 + u[2 * (size_t)ij + 1] * y1;
+        if (yy) a2 += y0 * y0 + y1 * y1;
+    }
+    block_sums2(a1, a2, partial);
+}
+
+// the two sums of a fused kernel into sc[slot1] and (slot2 >= 0) sc[slot2],
+// host copies where given, then the step's scalar update (op as
+// finish_dot_kernel)
+__global__ void __launch_bounds__(kThreads) finish_dots_kernel(const double *__restrict__ partial, int np,
+                                                               double *sc, int slot1, int slot2, int op,
+                                                               volatile double *host1, volatile double *host2) {
+    __shared__ double red[2][kThreads / 64];
+    double a1 = 0, a2 = 0;
+    for (int q = threadIdx.x; q < np; q += kThreads) {
+        a1 += partial[q];
+        a2 += partial[kMaxPartial + q];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a1 += __shfl_down(a1, o, 64);
+        a2 += __shfl_down(a2, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a1;
+        red[1][threadIdx.x >> 6] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double s1 = 0, s2 = 0;
+    for (int w = 0; w < kThreads / 64; w++) {
+        s1 += red[0][w];
+        s2 += red[1][w];
+    }
+    sc[slot1] = s1;
+    if (slot2 >= 0) sc[slot2] = s2;
+    if (op == 2) sc[2] = sc[0] / sc[4];
+    else if (op == 3) sc[3] = sc[5] / sc[6];
+    if (host1) *host1 = s1;
+    if (host2) *host2 = s2;
+}
+
 // p = r + beta (p - omega v), beta = (rho/rho_prev)(alpha/omega)
 __global__ void __launch_bounds__(kThreads) bicg_p_kernel(double *p, const double *__restrict__ r,
                                                           const double *__restrict__ v, const double *sc, long n,
@@ -464,18 +758,51 @@ __global__ void __launch_bounds__(kThreads) bicg_s_kernel(double *s, const doubl
     const long q = (long)blockIdx.x * kThreads + threadIdx.x;
     if (q < n) s[q] = r[q] - sc[2] * v[q];
 }
-// x += alpha phat + omega shat; r = s - omega t; rho_prev = rho
+// x += alpha phat + omega shat; r = s - omega t; rho_prev = rho; with the
+// sums of (r, r) and (r_hat, r)
 template <typename TP>
-__global__ void __launch_bounds__(kThreads) bicg_x_kernel(double *x, double *r, const TP *__restrict__ ph,
-                                                          const TP *__restrict__ sh,
-                                                          const double *__restrict__ s,
-                                                          const double *__restrict__ t, double *sc, long n) {
-    const long q = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (q == 0) sc[1] = sc[0];
-    if (q >= n) return;
+__global__ void __launch_bounds__(kThreads) bicg_x_dot_kernel(double *x, double *r, const TP *__restrict__ ph,
+                                                              const TP *__restrict__ sh,
+                                                              const double *__restrict__ s,
+                                                              const double *__restrict__ t,
+                                                              const double *__restrict__ rh, double *sc, long n,
+                                                              double *__restrict__ partial) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[1] = sc[0];
     const double a = sc[2], w = sc[3];
-    x[q] += a * (double)ph[q] + w * (double)sh[q];
-    r[q] = s[q] - w * t[q];
+    double a1 = 0, a2 = 0;
+    const long stride = (long)gridDim.x * kThreads;
+    long q = (long)blockIdx.x * kThreads + threadIdx.x;
+    // four elements at a time, their loads issued together
+    for (; q + 3 * stride < n; q += 4 * stride) {
+        double xv[4], pv[4], sv[4], s_[4], tv[4], hv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const long e = q + u * stride;
+            xv[u] = x[e];
+            pv[u] = (double)ph[e];
+            sv[u] = (double)sh[e];
+            s_[u] = s[e];
+            tv[u] = t[e];
+            hv[u] = rh[e];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const long e = q + u * stride;
+            x[e] = xv[u] + (a * pv[u] + w * sv[u]);
+            const double rq = s_[u] - w * tv[u];
+            r[e] = rq;
+            a1 += rq * rq;
+            a2 += hv[u] * rq;
+        }
+    }
+    for (; q < n; q += stride) {
+        x[q] += a * (double)ph[q] + w * (double)sh[q];
+        const double rq = s[q] - w * t[q];
+        r[q] = rq;
+        a1 += rq * rq;
+        a2 += rh[q] * rq;
+    }
+    block_sums2(a1, a2, partial);
 }
 
 }  // namespace
@@ -508,7 +835,7 @@ int BdSolver::init(int tm, int tn) {
     nl = dims_for(tm, tn, d);
     const size_t nv = 2 * (size_t)tm * tn;
     for (int k = 0; k < kBdVecs; k++) STZ_HIP(hipMalloc(&vec[k], nv * sizeof(double)));
-    STZ_HIP(hipMalloc(&partial, (size_t)kMaxPartial * sizeof(double)));
+    STZ_HIP(hipMalloc(&partial, 2 * (size_t)kMaxPartial * sizeof(double)));
     STZ_HIP(hipMalloc(&sc, 16 * sizeof(double)));
     STZ_HIP(hipHostMalloc((void **)&hscal, 4 * sizeof(double), hipHostMallocCoherent));
     for (int l = 0; l < nl; l++) {
@@ -528,7 +855,7 @@ int BdSolver::init(int tm, int tn) {
 
 int BdSolver::alloc_mixed() {
     const size_t nv = 2 * (size_t)top_m * top_n;
-    for (int k = 0; k < 3; k++) STZ_HIP(hipMalloc(&top32[k], nv * sizeof(float)));
+    for (int k = 0; k < 4; k++) STZ_HIP(hipMalloc(&top32[k], nv * sizeof(float)));
     for (int l = 0; l < nl; l++) {
         BdLevelT<float> &L = lv32[l];
         L.m = lv[l].m;
@@ -545,7 +872,7 @@ int BdSolver::alloc_mixed() {
 }
 
 void BdSolver::free_mixed() {
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 4; k++) {
         if (top32[k]) (void)hipFree(top32[k]);
         top32[k] = nullptr;
     }
@@ -624,76 +951,128 @@ int BdSolver::setup(const BdTop &t, hipStream_t s) {
     return STZ_OK;
 }
 
-template <typename T, typename TB> int BdSolver::smooth_top(T *z, const TB *b, int sweeps, hipStream_t s) {
+// one sweep of the tiled kernel over an m x n level from zin into zout;
+// mode 1 starts from zero, mode 2 adds the correction of the cm x cn level
+// zc first
+template <typename T, class Rows>
+static int tile_sweep(const Rows &rows, int m, int n, const T *zin, T *zout, int mode, const T *zc, int cm, int cn,
+                      hipStream_t s) {
+    using D = Tile<T>;
+    const dim3 g((m + D::X - 1) / D::X, (n + D::Y - 1) / D::Y), b(kTileThreads);
+    if (mode == 1)
+        hipLaunchKernelGGL((gs_tile_kernel<T, 1, Rows>), g, b, D::lds(1), s, rows, zin, zout, zc, cm, cn);
+    else if (mode == 2)
+        hipLaunchKernelGGL((gs_tile_kernel<T, 2, Rows>), g, b, D::lds(2), s, rows, zin, zout, zc, cm, cn);
+    else
+        hipLaunchKernelGGL((gs_tile_kernel<T, 0, Rows>), g, b, D::lds(0), s, rows, zin, zout, zc, cm, cn);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+// the start of a smoothing that the tiled kernel does not take, in zout:
+// zero (mode 1), zin (mode 0), zin + P zc (mode 2)
+template <typename T>
+static int start_in_place(T *zout, const T *zin, int m, int n, int mode, const T *zc, int cm, int cn, hipStream_t s) {
+    const long n2 = 2L * m * n;
+    if (mode == 1) {
+        hipLaunchKernelGGL(fill_kernel<T>, blocks(n2), dim3(kThreads), 0, s, zout, n2, (T)0);
+    } else if (zin != zout) {
+        STZ_HIP(hipMemcpyAsync(zout, zin, n2 * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
+    if (mode == 2)
+        hipLaunchKernelGGL(prolong_kernel<T>, blocks((long)m * n), dim3(kThreads), 0, s, zc, cm, cn, zout, m, n);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
+// Smoothing of the top level from zin (mode as tile_sweep) into zout
+// (zout != zin): one sweep by the tiled kernel, more by one launch per
+// colour in place
+template <typename T, typename TB>
+int BdSolver::smooth_top(T *zout, const TB *b, int sweeps, int mode, const T *zin, const T *zc, int cm, int cn,
+                         hipStream_t s) {
+    if (sweeps == 1) return tile_sweep(TopRows<T, TB>{top, b}, top_m, top_n, zin, zout, mode, zc, cm, cn, s);
+    const int e = start_in_place(zout, zin, top_m, top_n, mode, zc, cm, cn, s);
+    if (e) return e;
     for (int k = 0; k < sweeps; k++)
         for (int c = 0; c < 4; c++) {
             const long cnt = (long)((top_m + 1 - (c & 1)) >> 1) * ((top_n + 1 - (c >> 1)) >> 1);
-            hipLaunchKernelGGL((gs_top_kernel<T, TB>), blocks(cnt), dim3(kThreads), 0, s, top, z, b, c);
+            hipLaunchKernelGGL((gs_top_kernel<T, TB>), blocks(cnt), dim3(kThreads), 0, s, top, zout, b, c);
         }
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
 
-template <typename T> int BdSolver::smooth_level(const BdLevelT<T> &L, bool bottom, int sweeps, hipStream_t s) {
+// Smoothing of a coarse level from zin into zout, as smooth_top; the bottom
+// level, when it fits in LDS, in one workgroup in place
+template <typename T>
+int BdSolver::smooth_level(const BdLevelT<T> &L, bool bottom, int sweeps, int mode, const T *zin, T *zout,
+                           const T *zc, int cm, int cn, hipStream_t s) {
     const size_t lds = 2 * (size_t)L.m * L.n * sizeof(T);
     if (bottom && lds <= 64 * 1024) {
-        hipLaunchKernelGGL(gs_bottom_kernel<T>, dim3(1), dim3(1024), lds, s, L, sweeps);
+        int e = start_in_place(zout, zin, L.m, L.n, mode, zc, cm, cn, s);
+        if (e) return e;
+        BdLevelT<T> B = L;
+        B.z = zout;
+        hipLaunchKernelGGL(gs_bottom_kernel<T>, dim3(1), dim3(1024), lds, s, B, sweeps);
         STZ_HIP(hipGetLastError());
         return STZ_OK;
     }
+    if (sweeps == 1) return tile_sweep(LevelRows<T>{L}, L.m, L.n, zin, zout, mode, zc, cm, cn, s);
+    const int e = start_in_place(zout, zin, L.m, L.n, mode, zc, cm, cn, s);
+    if (e) return e;
+    BdLevelT<T> B = L;
+    B.z = zout;
     for (int k = 0; k < sweeps; k++)
         for (int c = 0; c < 4; c++) {
             const long cnt = (long)((L.m + 1 - (c & 1)) >> 1) * ((L.n + 1 - (c >> 1)) >> 1);
-            hipLaunchKernelGGL(gs_level_kernel<T>, blocks(cnt), dim3(kThreads), 0, s, L, c);
+            hipLaunchKernelGGL(gs_level_kernel<T>, blocks(cnt), dim3(kThreads), 0, s, B, c);
         }
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
 
-// one V-cycle with zero start on the top level, in precision T (the source
-// read in its own precision TB): z ~= A^-1 rhs
+// One V-cycle with zero start on the top level, in precision T (the source
+// read in its own precision TB): z ~= A^-1 rhs.  zpre holds the top level's
+// pre-smoothed iterate.  A coarse level keeps its pre-smoothed iterate in z
+// and its post-smoothed one in r (the residual it held is restricted by
+// then); the bottom level solves in z.
 template <typename T, typename TB>
-int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, hipStream_t s) {
-    const long n2 = 2L * top_m * top_n;
-    hipLaunchKernelGGL(fill_kernel<T>, blocks(n2), dim3(kThreads), 0, s, z, n2, (T)0);
-    int e = smooth_top(z, rhs, nu1, s);
+int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, T *zpre, hipStream_t s) {
+    const T *none = nullptr;
+    int e = smooth_top(zpre, rhs, nu1, 1, none, none, 0, 0, s);
     if (e) return e;
-    if (nl > 0) {
-        hipLaunchKernelGGL((resid_top_kernel<T, TB>), blocks((long)top_m * top_n), dim3(kThreads), 0, s, top, z, rhs,
-                           r0, (double *)nullptr);
-        const T *rf = r0;
-        int fm = top_m, fn = top_n;
-        for (int l = 0; l < nl; l++) {
-            BdLevelT<T> &L = Lv[l];
-            hipLaunchKernelGGL(restrict_kernel<T>, blocks((long)L.m * L.n), dim3(kThreads), 0, s, rf, fm, fn, L.b, L.m,
-                               L.n);
-            const long c2 = 2L * L.m * L.n;
-            hipLaunchKernelGGL(fill_kernel<T>, blocks(c2), dim3(kThreads), 0, s, L.z, c2, (T)0);
-            if (l == nl - 1) break;
-            if ((e = smooth_level(L, false, nu1, s))) return e;
-            hipLaunchKernelGGL(resid_level_kernel<T>, blocks((long)L.m * L.n), dim3(kThreads), 0, s, L);
-            rf = L.r;
-            fm = L.m;
-            fn = L.n;
-        }
-        if ((e = smooth_level(Lv[nl - 1], true, 2 * bottom_half, s))) return e;
-        for (int l = nl - 1; l >= 0; l--) {
-            const BdLevelT<T> &C = Lv[l];
-            T *zf = l == 0 ? z : Lv[l - 1].z;
-            const int fm2 = l == 0 ? top_m : Lv[l - 1].m, fn2 = l == 0 ? top_n : Lv[l - 1].n;
-            hipLaunchKernelGGL(prolong_kernel<T>, blocks((long)fm2 * fn2), dim3(kThreads), 0, s, C.z, C.m, C.n, zf,
-                               fm2, fn2);
-            if (l > 0 && (e = smooth_level(Lv[l - 1], false, nu2, s))) return e;
-        }
-        STZ_HIP(hipGetLastError());
+    if (nl == 0) return smooth_top(z, rhs, nu2, 0, (const T *)zpre, none, 0, 0, s);
+    hipLaunchKernelGGL((resid_top_kernel<T, TB>), blocks((long)top_m * top_n), dim3(kThreads), 0, s, top, zpre, rhs,
+                       r0, (double *)nullptr);
+    const T *rf = r0;
+    int fm = top_m, fn = top_n;
+    for (int l = 0; l < nl; l++) {
+        BdLevelT<T> &L = Lv[l];
+        hipLaunchKernelGGL(restrict_kernel<T>, blocks((long)L.m * L.n), dim3(kThreads), 0, s, rf, fm, fn, L.b, L.m,
+                           L.n);
+        if (l == nl - 1) break;
+        if ((e = smooth_level(L, false, nu1, 1, none, L.z, none, 0, 0, s))) return e;
+        hipLaunchKernelGGL(resid_level_kernel<T>, blocks((long)L.m * L.n), dim3(kThreads), 0, s, L);
+        rf = L.r;
+        fm = L.m;
+        fn = L.n;
     }
-    return smooth_top(z, rhs, nu2, s);
+    BdLevelT<T> &B = Lv[nl - 1];
+    if ((e = smooth_level(B, true, 2 * bottom_half, 1, none, B.z, none, 0, 0, s))) return e;
+    const T *res = B.z;  // the corrected iterate of the level below
+    for (int l = nl - 1; l > 0; l--) {
+        const BdLevelT<T> &C = Lv[l], &F = Lv[l - 1];
+        if ((e = smooth_level(F, false, nu2, 2, (const T *)F.z, F.r, res, C.m, C.n, s))) return e;
+        res = F.r;
+    }
+    return smooth_top(z, rhs, nu2, 2, (const T *)zpre, res, Lv[0].m, Lv[0].n, s);
 }
 
 // the FP64 preconditioner (the mixed form applies vcycle_t<float, double>
 // inside solve and keeps its result in FP32)
 int BdSolver::vcycle(const double *rhs, double *z, hipStream_t s) {
-    return vcycle_t<double, double>(rhs, z, lv, vec[kTmp], s);
+    return vcycle_t<double, double>(rhs, z, lv, vec[kTmp], vec[kPre], s);
 }
 
 int BdSolver::dot(const double *a, const double *b, int slot, int op, int host_slot, hipStream_t s) {
@@ -714,6 +1093,10 @@ int BdSolver::dot(const double *a, const double *b, int slot, int op, int host_s
 int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_solve_info &info, hipStream_t s) {
     const long mn = (long)top_m * top_n, n2 = 2 * mn;
     const dim3 g = blocks(n2), b1(kThreads);
+    // grids of the kernels with fused products: at most kMaxPartial blocks
+    const int nd = (int)std::min<long>((mn + kThreads - 1) / kThreads, kMaxPartial),
+              nx = (int)std::min<long>((n2 + kThreads - 1) / kThreads, kMaxPartial);
+    const dim3 gd(nd), gx(nx);
     double *r = vec[kR], *rh = vec[kRhat], *p = vec[kP], *v = vec[kV], *sv = vec[kS], *t = vec[kT],
            *ph = vec[kPhat], *sh = vec[kShat];
     info = stz_solve_info{};
@@ -750,32 +1133,35 @@ int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_so
         first = false;
         // p^ = M p: FP64, or (mixed) an FP32 V-cycle reading p in FP64 and
         // leaving p^ in FP32 for the FP64 kernels that use it
+        // v = A p^ with (r_hat, v), alpha
         if (mixed) {
-            if ((e = vcycle_t<float, double>(p, top32[0], lv32, top32[2], s))) return e;
-            hipLaunchKernelGGL(apply_top_kernel<float>, blocks(mn), b1, 0, s, top, top32[0], v);
+            if ((e = vcycle_t<float, double>(p, top32[0], lv32, top32[2], top32[3], s))) return e;
+            hipLaunchKernelGGL(apply_top_dot_kernel<float>, gd, b1, 0, s, top, top32[0], v, rh, 0, partial);
         } else {
             if ((e = vcycle(p, ph, s))) return e;
-            hipLaunchKernelGGL(apply_top_kernel<double>, blocks(mn), b1, 0, s, top, ph, v);
+            hipLaunchKernelGGL(apply_top_dot_kernel<double>, gd, b1, 0, s, top, ph, v, rh, 0, partial);
         }
-        if ((e = dot(rh, v, 4, 2, -1, s))) return e;
+        hipLaunchKernelGGL(finish_dots_kernel, dim3(1), b1, 0, s, partial, nd, sc, 4, -1, 2, nullptr, nullptr);
         hipLaunchKernelGGL(bicg_s_kernel, g, b1, 0, s, sv, r, v, sc, n2);
+        // t = A s^ with (t, s) and (t, t), omega
         if (mixed) {
-            if ((e = vcycle_t<float, double>(sv, top32[1], lv32, top32[2], s))) return e;
-            hipLaunchKernelGGL(apply_top_kernel<float>, blocks(mn), b1, 0, s, top, top32[1], t);
+            if ((e = vcycle_t<float, double>(sv, top32[1], lv32, top32[2], top32[3], s))) return e;
+            hipLaunchKernelGGL(apply_top_dot_kernel<float>, gd, b1, 0, s, top, top32[1], t, sv, 1, partial);
         } else {
             if ((e = vcycle(sv, sh, s))) return e;
-            hipLaunchKernelGGL(apply_top_kernel<double>, blocks(mn), b1, 0, s, top, sh, t);
+            hipLaunchKernelGGL(apply_top_dot_kernel<double>, gd, b1, 0, s, top, sh, t, sv, 1, partial);
         }
-        if ((e = dot(t, sv, 5, 0, -1, s))) return e;
-        if ((e = dot(t, t, 6, 3, -1, s))) return e;
+        hipLaunchKernelGGL(finish_dots_kernel, dim3(1), b1, 0, s, partial, nd, sc, 5, 6, 3, nullptr, nullptr);
+        // x, r with (r, r) -> host[0] and the next rho = (r_hat, r) -> sc[0], host[1]
         if (mixed)
-            hipLaunchKernelGGL(bicg_x_kernel<float>, g, b1, 0, s, x, r, top32[0], top32[1], sv, t, sc, n2);
+            hipLaunchKernelGGL(bicg_x_dot_kernel<float>, gx, b1, 0, s, x, r, top32[0], top32[1], sv, t, rh, sc, n2,
+                               partial);
         else
-            hipLaunchKernelGGL(bicg_x_kernel<double>, g, b1, 0, s, x, r, ph, sh, sv, t, sc, n2);
+            hipLaunchKernelGGL(bicg_x_dot_kernel<double>, gx, b1, 0, s, x, r, ph, sh, sv, t, rh, sc, n2, partial);
+        hipLaunchKernelGGL(finish_dots_kernel, dim3(1), b1, 0, s, partial, nx, sc, 7, 0, 0,
+                           (volatile double *)hscal, (volatile double *)hscal + 1);
         STZ_HIP(hipGetLastError());
         info.cycles = info.total_cycles = 2 * total;
-        if ((e = dot(r, r, 7, 0, 0, s))) return e;      // (r,r) -> host[0]
-        if ((e = dot(rh, r, 0, 0, 1, s))) return e;     // next rho -> sc[0], host[1]
         STZ_HIP(hipStreamSynchronize(s));
         rr = hscal[0];
         rhr = hscal[1];
