@@ -3,7 +3,7 @@ simulations of different sizes stepped together on one GPU.
 
     python -m stz_amd.ens_bench [--members 256] [--steps 200] [--warmup 60]
                                 [--form mixed] [--seed 0] [--serial-steps 3]
-                                [--one-mode]
+                                [--one-mode] [--direct-nodes 0]
 
 `--warmup 0 --steps 1000 --one-mode` times the whole 1000-step run of
 every member from t = 0.
@@ -13,6 +13,11 @@ Workload: `members` simulations, the side of each drawn uniformly from
 Gaussian field (correlation length 2 cells, seed per member), zero initial
 stress and velocity, dt = 0.2 (the 1024^2 workload's step), V(2,2), residual
 threshold 1e-4 x the reference's, FP32 multigrid inside FP64 refinement.
+The V-cycles go down to each member's workgroup coarse solve
+(`--direct-nodes 0`): the exact Fourier-mode level, which a simulation on its
+own stops at, reads each member's mode inverses (9 MB at 64 x 65) and stages
+its source once per mode and row block, which costs a batch of 256 members
+more than the coarse levels it saves.
 
 Prints one JSON line: aggregate cell-steps/s of the batched step replayed
 from HIP graphs (the library's default), the same with direct launches, and
@@ -29,7 +34,7 @@ import numpy as np
 SIDES = (64, 128, 256, 384, 512)
 
 
-def build(members, seed, form, nu, tol):
+def build(members, seed, form, nu, tol, direct_nodes=None):
     from . import problems
     from .ensemble import Ensemble
     from .initial import simple_shear_state
@@ -41,7 +46,8 @@ def build(members, seed, form, nu, tol):
         ens[k].set_fields(simple_shear_state(p, 600.0, 20.0, 2.0, seed=1000 + seed * members + k))
         ens[k].set_vel(np.zeros((p["n"] + 1, p["m"], 2)))
         ens[k].time = 0.0
-    ens.set_mg_config(nu1=nu, nu2=nu, tol_mult=tol)
+    kw = {} if direct_nodes is None else {"direct_nodes": direct_nodes}
+    ens.set_mg_config(nu1=nu, nu2=nu, tol_mult=tol, **kw)
     return ens, sides
 
 
@@ -56,6 +62,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--serial-steps", type=int, default=3)
     ap.add_argument("--block", type=int, default=10, help="steps per block of one launch mode")
+    ap.add_argument("--direct-nodes", type=int, default=0,
+                    help="largest coarse level solved exactly (stz_mg_config.direct_nodes); 0 (the default "
+                         "here): none, the V-cycles go down to the workgroup coarse solve, which batches "
+                         "better than the exact level's mode inverses")
     ap.add_argument("--tune-sweeps", action="store_true",
                     help="pick V(nu, nu), nu in 1..4, with the ensemble's sweep-count tuner after the warm-up")
     ap.add_argument("--one-mode", action="store_true",
@@ -65,7 +75,7 @@ def main():
     dt = problems.T_END / 5000
 
     t0 = time.perf_counter()
-    ens, sides = build(args.members, args.seed, args.form, args.nu, args.tol)
+    ens, sides = build(args.members, args.seed, args.form, args.nu, args.tol, args.direct_nodes)
     cells = sum(s * s for s in sides)
     print("ens_bench: %d members, %.2fM cells, set-up %.1f s" % (len(sides), cells / 1e6, time.perf_counter() - t0),
           file=sys.stderr, flush=True)
@@ -136,7 +146,8 @@ def main():
                                "dt %g, V(%d,%d), threshold %g x reference" %
                                (len(sides), hist, dt, *(tuned["choice"] if tuned else (args.nu, args.nu)),
                                 args.tol),
-                   "cells": cells, "cycles_per_solve": cyc, "slots_per_step": slots,
+                   "cells": cells, "direct_nodes": ens[0].mg_config().direct_nodes,
+                   "cycles_per_solve": cyc, "slots_per_step": slots,
                    "slots_beyond_plan_per_step": extra,
                    "direct_launch_ms_per_step": None if args.one_mode else 1e3 * el_d / nsteps,
                    "tiles": ens.tile_choice(),
