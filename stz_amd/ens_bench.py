@@ -13,8 +13,8 @@ Workload: `members` simulations, the side of each drawn uniformly from
 Gaussian field (correlation length 2 cells, seed per member), zero initial
 stress and velocity, dt = 0.2 (the 1024^2 workload's step), V(2,2), residual
 threshold 1e-4 x the reference's, FP32 multigrid inside FP64 refinement.
-The V-cycles go down to each member's workgroup coarse solve
-(`--direct-nodes 0`): the exact Fourier-mode level, which a simulation on its
+In the mixed form the V-cycles go down to each member's workgroup coarse
+solve (`--direct-nodes 0`): the exact Fourier-mode level, which a simulation on its
 own stops at, reads each member's mode inverses (9 MB at 64 x 65) and stages
 its source once per mode and row block, which costs a batch of 256 members
 more than the coarse levels it saves.
@@ -62,10 +62,11 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--serial-steps", type=int, default=3)
     ap.add_argument("--block", type=int, default=10, help="steps per block of one launch mode")
-    ap.add_argument("--direct-nodes", type=int, default=0,
-                    help="largest coarse level solved exactly (stz_mg_config.direct_nodes); 0 (the default "
-                         "here): none, the V-cycles go down to the workgroup coarse solve, which batches "
-                         "better than the exact level's mode inverses")
+    ap.add_argument("--direct-nodes", type=int, default=None,
+                    help="largest coarse level solved exactly (stz_mg_config.direct_nodes); 0: none, the "
+                         "V-cycles go down to the workgroup coarse solve, which batches better than the "
+                         "exact level's mode inverses (default 0 in the mixed form; the FP64 form keeps "
+                         "the library's default, its 384-side members need their exact 48x49 level)")
     ap.add_argument("--tune-sweeps", action="store_true",
                     help="pick V(nu, nu), nu in 1..4, with the ensemble's sweep-count tuner after the warm-up")
     ap.add_argument("--one-mode", action="store_true",
@@ -75,7 +76,8 @@ def main():
     dt = problems.T_END / 5000
 
     t0 = time.perf_counter()
-    ens, sides = build(args.members, args.seed, args.form, args.nu, args.tol, args.direct_nodes)
+    direct = args.direct_nodes if args.direct_nodes is not None else (0 if args.form == "mixed" else None)
+    ens, sides = build(args.members, args.seed, args.form, args.nu, args.tol, direct)
     cells = sum(s * s for s in sides)
     print("ens_bench: %d members, %.2fM cells, set-up %.1f s" % (len(sides), cells / 1e6, time.perf_counter() - t0),
           file=sys.stderr, flush=True)
