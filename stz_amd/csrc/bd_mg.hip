@@ -328,8 +328,13 @@ __global__ void __launch_bounds__(kThreads) prolong_kernel(const T *__restrict__
 // (zout), since the tiles around read the input's halo while this one
 // writes.
 // ---------------------------------------------------------------------------
-constexpr int kTileThreads = 256, kHalo = 4;
-template <typename T> struct Tile {
+constexpr int kTileThreads = 256;
+// MODE 3: MODE 1 and then the residual on the tile and one node around it
+// (which needs z final two nodes out: a halo of 6) and its restriction to
+// the coarse nodes under the tile, written to the coarse right-hand side
+// cb (tgmg.in.cc:515-551)
+constexpr int halo_of(int mode) { return mode == 3 ? 6 : 4; }
+template <typename T, int kHalo> struct Tile {
     static constexpr int X = sizeof(T) == 4 ? 64 : 32, Y = 32;
     static constexpr int LW = X + 2 * kHalo, LH = Y + 2 * kHalo, N = LW * LH;
     static constexpr int P = (N + kTileThreads - 1) / kTileThreads;  // staged nodes per thread
@@ -357,6 +362,14 @@ template <typename T, typename TB> struct TopRows {
         const double sc = top_diag_scale(t, d);
         zs[2 * l] = (bs[2 * l] - o.x) / (T)(sc * t.cxx);
         zs[2 * l + 1] = (bs[2 * l + 1] - o.y) / (T)(sc * t.cyy);
+    }
+    // b - A z, as resid_top_kernel
+    __device__ V2T<T> resid(int, int, int l, const T *zs, const T *bs, const int *ts, const T *, int LW) const {
+        const int d = ts[l];
+        const V2T<T> o = top_offdiag_at<T>(t, zs + 2 * l, LW, d);
+        const double sc = top_diag_scale(t, d);
+        return V2T<T>{bs[2 * l] - o.x - (T)(sc * t.cxx) * zs[2 * l],
+                      bs[2 * l + 1] - o.y - (T)(sc * t.cyy) * zs[2 * l + 1]};
     }
 };
 
@@ -386,13 +399,27 @@ template <typename T> struct LevelRows {
         if (ts[l] < 0) relax(sc, i, j, l, zs, bs, LW);
         else relax(L.S + 40 * (i + (size_t)L.m * j), i, j, l, zs, bs, LW);
     }
+    // b - A z, as resid_level_kernel
+    __device__ V2T<T> resid_at(const T *s0, int i, int j, int l, const T *zs, const T *bs, int LW) const {
+        T s[40];
+#pragma unroll
+        for (int q = 0; q < 40; q++) s[q] = s0[q];
+        const V2T<T> o = lvl_offdiag_at(s, zs + 2 * l, LW, i, j, L.m, L.n);
+        const T zx = zs[2 * l], zy = zs[2 * l + 1];
+        return V2T<T>{bs[2 * l] - o.x - (s[16] * zx + s[17] * zy), bs[2 * l + 1] - o.y - (s[18] * zx + s[19] * zy)};
+    }
+    __device__ V2T<T> resid(int i, int j, int l, const T *zs, const T *bs, const int *ts, const T *sc, int LW) const {
+        if (ts[l] < 0) return resid_at(sc, i, j, l, zs, bs, LW);
+        return resid_at(L.S + 40 * (i + (size_t)L.m * j), i, j, l, zs, bs, LW);
+    }
 };
 
 template <typename T, int MODE, class Rows>
 __global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const T *__restrict__ z,
                                                                T *__restrict__ zout, const T *__restrict__ zc,
-                                                               int cm, int cn) {
-    using D = Tile<T>;
+                                                               T *__restrict__ cb, int cm, int cn) {
+    constexpr int kHalo = halo_of(MODE);
+    using D = Tile<T, kHalo>;
     using TB = typename Rows::Src;
     constexpr int P = D::P;
     extern __shared__ __align__(16) unsigned char bd_tile_smem[];
@@ -413,7 +440,7 @@ __global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const 
         const int q = tid + k * kTileThreads, i = x0 - kHalo + q % D::LW, j = y0 - kHalo + q / D::LW;
         const bool in = q < D::N && i >= 0 && i < m && j >= 0 && j < n;
         const size_t ij = in ? i + (size_t)m * j : 0;
-        if (MODE != 1) {
+        if (MODE == 0 || MODE == 2) {
             const T a0 = z[2 * ij], a1 = z[2 * ij + 1];
             vx[k] = in ? a0 : (T)0;
             vy[k] = in ? a1 : (T)0;
@@ -493,6 +520,46 @@ __global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const 
         const int l = (i - x0 + kHalo) + D::LW * (j - y0 + kHalo);
         zout[2 * (i + (size_t)m * j)] = zs[2 * l];
         zout[2 * (i + (size_t)m * j) + 1] = zs[2 * l + 1];
+    }
+    if (MODE != 3) return;
+    // the residual on the tile and one node around it, over b in LDS (a
+    // node's residual reads its own b and the z around it)
+    {
+        constexpr int RW = D::X + 2, RH = D::Y + 2;
+        for (int q = tid; q < RW * RH; q += kTileThreads) {
+            const int i = x0 - 1 + q % RW, j = y0 - 1 + q / RW;
+            if (i < 0 || i >= m || j < 0 || j >= n) continue;
+            const int l = (i - x0 + kHalo) + D::LW * (j - y0 + kHalo);
+            const V2T<T> r = rows.resid(i, j, l, zs, bs, ts, sc, D::LW);
+            bs[2 * l] = r.x;
+            bs[2 * l + 1] = r.y;
+        }
+    }
+    __syncthreads();
+    // the coarse nodes under the tile (the last tile of a row or column also
+    // takes the coarse node past the fine grid's end, when there is one), as
+    // restrict_kernel: b_c = P^T r
+    const int I0 = x0 / 2, I1 = blockIdx.x == gridDim.x - 1 ? cm : min(cm, I0 + D::X / 2);
+    const int J0 = y0 / 2, J1 = blockIdx.y == gridDim.y - 1 ? cn : min(cn, J0 + D::Y / 2);
+    const int nI = I1 - I0;
+    for (int q = tid; q < nI * (J1 - J0); q += kTileThreads) {
+        const int I = I0 + q % nI, J = J0 + q / nI;
+        T sx = 0, sy = 0;
+        int c[2];
+        double w[2];
+        for (int fy = 2 * J - 1; fy <= 2 * J + 1; fy++) {
+            const double wy = weight_of(parents(fy, n, cn, c, w), c, w, J);
+            if (wy == 0) continue;
+            for (int fx = 2 * I - 1; fx <= 2 * I + 1; fx++) {
+                const double wx = weight_of(parents(fx, m, cm, c, w), c, w, I);
+                if (wx == 0) continue;
+                const T *r = bs + 2 * ((fx - x0 + kHalo) + D::LW * (fy - y0 + kHalo));
+                sx += (T)(wx * wy) * r[0];
+                sy += (T)(wx * wy) * r[1];
+            }
+        }
+        cb[2 * (I + (size_t)cm * J)] = sx;
+        cb[2 * (I + (size_t)cm * J) + 1] = sy;
     }
 }
 
@@ -956,15 +1023,24 @@ int BdSolver::setup(const BdTop &t, hipStream_t s) {
 // zc first
 template <typename T, class Rows>
 static int tile_sweep(const Rows &rows, int m, int n, const T *zin, T *zout, int mode, const T *zc, int cm, int cn,
-                      hipStream_t s) {
-    using D = Tile<T>;
+                      hipStream_t s, T *cb = nullptr) {
+    using D = Tile<T, 4>;
     const dim3 g((m + D::X - 1) / D::X, (n + D::Y - 1) / D::Y), b(kTileThreads);
-    if (mode == 1)
-        hipLaunchKernelGGL((gs_tile_kernel<T, 1, Rows>), g, b, D::lds(1), s, rows, zin, zout, zc, cm, cn);
+    if (mode == 3) {
+        // above 64 KiB of LDS: raise the kernel's limit once
+        static const hipError_t lim = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&gs_tile_kernel<T, 3, Rows>), hipFuncAttributeMaxDynamicSharedMemorySize,
+            (int)Tile<T, 6>::lds(1));
+        STZ_HIP(lim);
+        hipLaunchKernelGGL((gs_tile_kernel<T, 3, Rows>), g, b, (Tile<T, 6>::lds(1)), s, rows, zin, zout, zc, cb, cm,
+                           cn);
+    }
+    else if (mode == 1)
+        hipLaunchKernelGGL((gs_tile_kernel<T, 1, Rows>), g, b, D::lds(1), s, rows, zin, zout, zc, cb, cm, cn);
     else if (mode == 2)
-        hipLaunchKernelGGL((gs_tile_kernel<T, 2, Rows>), g, b, D::lds(2), s, rows, zin, zout, zc, cm, cn);
+        hipLaunchKernelGGL((gs_tile_kernel<T, 2, Rows>), g, b, D::lds(2), s, rows, zin, zout, zc, cb, cm, cn);
     else
-        hipLaunchKernelGGL((gs_tile_kernel<T, 0, Rows>), g, b, D::lds(0), s, rows, zin, zout, zc, cm, cn);
+        hipLaunchKernelGGL((gs_tile_kernel<T, 0, Rows>), g, b, D::lds(0), s, rows, zin, zout, zc, cb, cm, cn);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
@@ -1040,23 +1116,31 @@ int BdSolver::smooth_level(const BdLevelT<T> &L, bool bottom, int sweeps, int mo
 template <typename T, typename TB>
 int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, T *zpre, hipStream_t s) {
     const T *none = nullptr;
-    int e = smooth_top(zpre, rhs, nu1, 1, none, none, 0, 0, s);
-    if (e) return e;
-    if (nl == 0) return smooth_top(z, rhs, nu2, 0, (const T *)zpre, none, 0, 0, s);
-    hipLaunchKernelGGL((resid_top_kernel<T, TB>), blocks((long)top_m * top_n), dim3(kThreads), 0, s, top, zpre, rhs,
-                       r0, (double *)nullptr);
-    const T *rf = r0;
-    int fm = top_m, fn = top_n;
-    for (int l = 0; l < nl; l++) {
-        BdLevelT<T> &L = Lv[l];
-        hipLaunchKernelGGL(restrict_kernel<T>, blocks((long)L.m * L.n), dim3(kThreads), 0, s, rf, fm, fn, L.b, L.m,
-                           L.n);
-        if (l == nl - 1) break;
+    int e;
+    if (nl == 0) {
+        if ((e = smooth_top(zpre, rhs, nu1, 1, none, none, 0, 0, s))) return e;
+        return smooth_top(z, rhs, nu2, 0, (const T *)zpre, none, 0, 0, s);
+    }
+    // one pre-smoothing sweep on the top level: the tiled launch also forms
+    // the residual and restricts it (mode 3).  On the coarse levels the extra
+    // halo and stages cost more than the two passes they replace.
+    const bool fused = nu1 == 1;
+    if (fused) {
+        e = tile_sweep(TopRows<T, TB>{top, rhs}, top_m, top_n, none, zpre, 3, none, Lv[0].m, Lv[0].n, s, Lv[0].b);
+        if (e) return e;
+    } else {
+        if ((e = smooth_top(zpre, rhs, nu1, 1, none, none, 0, 0, s))) return e;
+        hipLaunchKernelGGL((resid_top_kernel<T, TB>), blocks((long)top_m * top_n), dim3(kThreads), 0, s, top, zpre,
+                           rhs, r0, (double *)nullptr);
+        hipLaunchKernelGGL(restrict_kernel<T>, blocks((long)Lv[0].m * Lv[0].n), dim3(kThreads), 0, s, r0, top_m,
+                           top_n, Lv[0].b, Lv[0].m, Lv[0].n);
+    }
+    for (int l = 0; l + 1 < nl; l++) {
+        BdLevelT<T> &L = Lv[l], &C = Lv[l + 1];
         if ((e = smooth_level(L, false, nu1, 1, none, L.z, none, 0, 0, s))) return e;
         hipLaunchKernelGGL(resid_level_kernel<T>, blocks((long)L.m * L.n), dim3(kThreads), 0, s, L);
-        rf = L.r;
-        fm = L.m;
-        fn = L.n;
+        hipLaunchKernelGGL(restrict_kernel<T>, blocks((long)C.m * C.n), dim3(kThreads), 0, s, L.r, L.m, L.n, C.b,
+                           C.m, C.n);
     }
     BdLevelT<T> &B = Lv[nl - 1];
     if ((e = smooth_level(B, true, 2 * bottom_half, 1, none, B.z, none, 0, 0, s))) return e;
