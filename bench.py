@@ -135,17 +135,25 @@ def kernel_rooflines(sims, params, dt, nu, stream, launches=50):
     per-node figures"""
     N, n1 = params["m"], params["n"] + 1
     nodes, cells = N * n1, N * params["n"]
+    # the top level's tile shape, as the library chooses it (mg_tiled.hip:
+    # big_tiles, small_tiles)
+    if N % 64 == 0 and (N // 64) * ((n1 + 31) // 32) >= 256:
+        shape = "64,34"
+    elif (N // 32) * ((n1 + 15) // 16) < 100:
+        shape = "16,16"
+    else:
+        shape = "32,16"
     out = {}
     for form, sim in sims.items():
         w = 4 if form == "mixed" else 8        # bytes per word of the V-cycle
         sweep = 6 * w                          # read z, b; write z (vec2)
         transfer = 4.5 * w                     # restriction or prolongation: 4.5 words per fine node
         ks = {
-            "top_pre": ("smooth_tile_kernel<%s,64,32,%d,%d,true,true> (top level: %d sweeps + fused restriction)"
-                        % ("float" if w == 4 else "double", nu, 1 if w == 4 else 0, nu),
+            "top_pre": ("smooth_tile_kernel<%s,%s,%d,%d,true,true> (top level: %d sweeps + fused restriction)"
+                        % ("float" if w == 4 else "double", shape, nu, 1 if w == 4 else 0, nu),
                         nodes * (nu * sweep + transfer), lambda c, s=sim: s.mg_vcycle_launch(0, 1, c)),
-            "top_post": ("smooth_tile_kernel<%s,64,32,%d,2,false,true> (top level: prolongation + %d sweeps%s)"
-                         % ("float" if w == 4 else "double", nu, nu, " + FP64 vel += e" if w == 4 else ""),
+            "top_post": ("smooth_tile_kernel<%s,%s,%d,2,false,true> (top level: prolongation + %d sweeps%s)"
+                         % ("float" if w == 4 else "double", shape, nu, nu, " + FP64 vel += e" if w == 4 else ""),
                          nodes * (nu * sweep + transfer + (32 if w == 4 else 0)),
                          lambda c, s=sim: s.mg_vcycle_launch(0, 2, c)),
         }
