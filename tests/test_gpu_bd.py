@@ -152,6 +152,27 @@ def test_projection_mixed_preconditioner(plate):
     assert rel_err(out["mixed"], out["fp64"]) < 1e-7
 
 
+@pytest.mark.parametrize("nu1,nu2,form", [(2, 2, "fp64"), (2, 2, "mixed"), (0, 1, "mixed"), (1, 0, "fp64"),
+                                           (3, 1, "mixed")])
+def test_projection_sweep_counts(plate, nu1, nu2, form):
+    """other smoothing sweep counts than the default V(1,1) take the
+    colour-by-colour launches (more than one sweep), the unfused residual
+    and restriction (pre-smoothing other than one sweep) or no smoothing on
+    one side; the solve still reaches the threshold and the reference's
+    solution"""
+    print("case bd projection sweeps", nu1, nu2, form, flush=True)
+    g = injected(plate)
+    g.set_precision(form)
+    g.set_solver(nu1, nu2, 20, 500, 1e-20)
+    g.post_process(True)
+    g.projection_step(float(plate["dt"]))
+    info = g.last_solve()
+    print(info, flush=True)
+    assert info["status"] == 0 and info["residual"] <= 1e-20
+    assert rel_err(g.get_vel(), plate["proj_vel"]) < 1e-7
+    g.close()
+
+
 def test_advection(plate):
     g = injected(plate)
     g.post_process(True)
