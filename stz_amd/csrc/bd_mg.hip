@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(kThreads) prolong_kernel(const T *__restrict__
 // (zout), since the tiles around read the input's halo while this one
 // writes.
 // ---------------------------------------------------------------------------
-constexpr int kTileThreads = 256;
+constexpr int kTileThreads = 1024;
 // MODE 3: MODE 1 and then the residual on the tile and one node around it
 // (which needs z final two nodes out: a halo of 6) and its restriction to
 // the coarse nodes under the tile, written to the coarse right-hand side
