@@ -1121,9 +1121,8 @@ int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, T *zpre, hip
         if ((e = smooth_top(zpre, rhs, nu1, 1, none, none, 0, 0, s))) return e;
         return smooth_top(z, rhs, nu2, 0, (const T *)zpre, none, 0, 0, s);
     }
-    // one pre-smoothing sweep on the top level: the tiled launch also forms
-    // the residual and restricts it (mode 3).  On the coarse levels the extra
-    // halo and stages cost more than the two passes they replace.
+    // one pre-smoothing sweep per level: the tiled launch also forms the
+    // residual and restricts it to the next level (mode 3)
     const bool fused = nu1 == 1;
     if (fused) {
         e = tile_sweep(TopRows<T, TB>{top, rhs}, top_m, top_n, none, zpre, 3, none, Lv[0].m, Lv[0].n, s, Lv[0].b);
@@ -1137,6 +1136,10 @@ int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, T *zpre, hip
     }
     for (int l = 0; l + 1 < nl; l++) {
         BdLevelT<T> &L = Lv[l], &C = Lv[l + 1];
+        if (fused) {
+            if ((e = tile_sweep(LevelRows<T>{L}, L.m, L.n, none, L.z, 3, none, C.m, C.n, s, C.b))) return e;
+            continue;
+        }
         if ((e = smooth_level(L, false, nu1, 1, none, L.z, none, 0, 0, s))) return e;
         hipLaunchKernelGGL(resid_level_kernel<T>, blocks((long)L.m * L.n), dim3(kThreads), 0, s, L);
         hipLaunchKernelGGL(restrict_kernel<T>, blocks((long)C.m * C.n), dim3(kThreads), 0, s, L.r, L.m, L.n, C.b,
