@@ -59,6 +59,10 @@ int bd_extrap_cells(const BdConst &c, const double *phi, const int8_t *st, const
 int bd_extrap_nodes(const BdConst &c, const double *phi, const int8_t *st, const int8_t *nst, const BdNodes &f,
                     int *done, double *key, int *counter, int *list, volatile int *hflag, int *waves,
                     hipStream_t s);
+// both, their waves sharing launches; each walk its own counters and lists
+int bd_extrap_both(const BdConst &c, const double *phi, const int8_t *st, const int8_t *nst, const BdCells &fc,
+                   const BdNodes &fn, int *done_c, int *done_n, double *key, int *cnt_c, int *cnt_n, int *list_c,
+                   int *list_n, volatile int *hflag, int *waves_c, int *waves_n, hipStream_t s);
 
 // explicit step, operator, correction (bd_step.hip)
 int bd_advect(const BdConst &c, const int8_t *st, const int8_t *nst, const BdCells &o, const BdNodes &on,

@@ -1334,12 +1334,13 @@ constexpr int kWaveThreads = 256, kWaveTeams = kWaveThreads / 64, kWaveBlocks = 
 // filling slot (w + 1) % 3 and clearing slot (w + 2) % 3 for wave w + 1 (the
 // wave before read it, and the wave after fills it, each in a launch of its
 // own).  A launch after the last wave finds a zero count and returns.
+// blk of nblk: this workgroup's place among the wave's workgroups
 template <class Walk>
-__device__ __forceinline__ static void walk_wave(const Walk &W, int wave, int *lists, long span, int *counter) {
-    __shared__ TeamBuf buf[kWaveTeams];
+__device__ __forceinline__ static void walk_wave(const Walk &W, int wave, int *lists, long span, int *counter,
+                                                 TeamBuf *buf, int blk, int nblk) {
     int *slot = counter + 12;
     const int count = slot[wave % 3], lane = threadIdx.x & 63, team = threadIdx.x >> 6;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blk == 0 && threadIdx.x == 0) {
         slot[(wave + 2) % 3] = 0;
         if (count > 0) {
             counter[8] = wave;
@@ -1349,7 +1350,7 @@ __device__ __forceinline__ static void walk_wave(const Walk &W, int wave, int *l
     if (count == 0) return;
     const int *cur = lists + (wave & 1) * span;
     int *nxt = lists + ((wave + 1) & 1) * span;
-    for (int t = blockIdx.x * kWaveTeams + team; t < count; t += gridDim.x * kWaveTeams) {
+    for (int t = blk * kWaveTeams + team; t < count; t += nblk * kWaveTeams) {
         const int e = cur[t];
         const auto first = W.probe(e, lane, 0);
         W.fit(wave, e, lane, buf[team]);
@@ -1360,20 +1361,43 @@ __device__ __forceinline__ static void walk_wave(const Walk &W, int wave, int *l
 __global__ void __launch_bounds__(kWaveThreads) xc_wave_kernel(BdConst c, const double *__restrict__ phi, BdCells f,
                                                                int *done, int *deg, int *counter, int *lists,
                                                                long span, int wave) {
+    __shared__ TeamBuf buf[kWaveTeams];
     const int R = counter[10] > 3 ? counter[10] : 3;
-    walk_wave(CellWalk{c, phi, f, done, deg, counter, R}, wave, lists, span, counter);
+    walk_wave(CellWalk{c, phi, f, done, deg, counter, R}, wave, lists, span, counter, buf, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kWaveThreads) xn_wave_kernel(BdConst c, BdNodes f, int *done,
                                                                const double *__restrict__ keyd,
                                                                const int *__restrict__ keyi, int *deg, int *counter,
                                                                int *lists, long span, int wave) {
-    walk_wave(NodeWalk{c, f, done, deg, counter, keyd, keyi}, wave, lists, span, counter);
+    __shared__ TeamBuf buf[kWaveTeams];
+    walk_wave(NodeWalk{c, f, done, deg, counter, keyd, keyi}, wave, lists, span, counter, buf, blockIdx.x, gridDim.x);
 }
 
-// the count of wave w (zero once the walk has ended) for the host
-__global__ void wave_count_kernel(const int *counter, int wave, volatile int *hflag) {
-    if (threadIdx.x == 0) hflag[12] = counter[12 + wave % 3];
+// a wave of the cell walk and one of the node walk in one launch (the two
+// walks do not depend on each other): the first kWaveBlocks workgroups take
+// the cells, the others the nodes
+__global__ void __launch_bounds__(kWaveThreads) both_wave_kernel(BdConst c, const double *__restrict__ phi, BdCells fc,
+                                                                 int *done_c, int *deg_c, int *counter_c,
+                                                                 int *lists_c, BdNodes fn, int *done_n,
+                                                                 const double *__restrict__ keyd,
+                                                                 const int *__restrict__ keyi, int *deg_n,
+                                                                 int *counter_n, int *lists_n, long span, int wave) {
+    __shared__ TeamBuf buf[kWaveTeams];
+    if (blockIdx.x < kWaveBlocks) {
+        const int R = counter_c[10] > 3 ? counter_c[10] : 3;
+        walk_wave(CellWalk{c, phi, fc, done_c, deg_c, counter_c, R}, wave, lists_c, span, counter_c, buf, blockIdx.x,
+                  kWaveBlocks);
+    } else {
+        walk_wave(NodeWalk{c, fn, done_n, deg_n, counter_n, keyd, keyi}, wave, lists_n, span, counter_n, buf,
+                  blockIdx.x - kWaveBlocks, gridDim.x - kWaveBlocks);
+    }
+}
+
+// the count of wave w (zero once the walk has ended) for the host; with a
+// second walk's counter, the sum of both
+__global__ void wave_count_kernel(const int *counter, const int *counter2, int wave, volatile int *hflag) {
+    if (threadIdx.x == 0) hflag[12] = counter[12 + wave % 3] + (counter2 ? counter2[12 + wave % 3] : 0);
 }
 
 // a walk's counts for the host: pending, order violations, waves, set
@@ -1476,18 +1500,23 @@ int bd_ls_node_status(const BdConst &c, const double *phi, const int8_t *st, int
 // Waves in batches of launches that double from 16 to 256; after each
 // batch the host reads whether the walk has ended.
 template <class Launch>
-static int walk(Launch launch, const int *counter, volatile int *hflag, int *waves, const char *what,
-                hipStream_t s) {
+static int run_walk(Launch launch, const int *counter, const int *counter2, volatile int *hflag, hipStream_t s) {
     int w = 1, batch = 16;
     for (;;) {
         for (int q = 0; q < batch; q++, w++) launch(w);
-        hipLaunchKernelGGL(wave_count_kernel, dim3(1), dim3(64), 0, s, counter, w, hflag);
+        hipLaunchKernelGGL(wave_count_kernel, dim3(1), dim3(64), 0, s, counter, counter2, w, hflag);
         STZ_HIP(hipGetLastError());
         STZ_HIP(hipStreamSynchronize(s));
         if (hflag[12] == 0) break;
         if (w >= kMaxWaves) break;
         batch = batch < 256 ? 2 * batch : 256;
     }
+    return STZ_OK;
+}
+
+// a finished walk's counts: entries set = entries pending, none fitted out
+// of order
+static int walk_result(const int *counter, volatile int *hflag, int *waves, const char *what, hipStream_t s) {
     hipLaunchKernelGGL(walk_result_kernel, dim3(1), dim3(64), 0, s, counter, hflag);
     STZ_HIP(hipGetLastError());
     STZ_HIP(hipStreamSynchronize(s));
@@ -1510,12 +1539,13 @@ int bd_extrap_cells(const BdConst &c, const double *phi, const int8_t *st, const
     hipLaunchKernelGGL(xc_prepare_kernel, blocks(mn), dim3(kThreads), 0, s, c, phi, st, done, list, counter,
                        lists + span);
     STZ_HIP(hipGetLastError());
-    return walk(
+    int e = run_walk(
         [&](int w) {
             hipLaunchKernelGGL(xc_wave_kernel, dim3(kWaveBlocks), dim3(kWaveThreads), 0, s, c, phi, f, done, list,
                                counter, lists, span, w);
         },
-        counter, hflag, waves, "cell", s);
+        counter, (const int *)nullptr, hflag, s);
+    return e ? e : walk_result(counter, hflag, waves, "cell", s);
 }
 
 int bd_extrap_nodes(const BdConst &c, const double *phi, const int8_t *st, const int8_t *nst, const BdNodes &f,
@@ -1529,12 +1559,39 @@ int bd_extrap_nodes(const BdConst &c, const double *phi, const int8_t *st, const
     hipLaunchKernelGGL(xn_deps_kernel, blocks(mne), dim3(kThreads), 0, s, c, done, key, keyi, list, counter,
                        lists + span);
     STZ_HIP(hipGetLastError());
-    return walk(
+    int e = run_walk(
         [&](int w) {
             hipLaunchKernelGGL(xn_wave_kernel, dim3(kWaveBlocks), dim3(kWaveThreads), 0, s, c, f, done, key, keyi,
                                list, counter, lists, span, w);
         },
-        counter, hflag, waves, "node", s);
+        counter, (const int *)nullptr, hflag, s);
+    return e ? e : walk_result(counter, hflag, waves, "node", s);
+}
+
+// both walks, their waves sharing launches; cnt_n and list_n are the node
+// walk's own counters (16 ints) and lists (3 max(cells, nodes) ints)
+int bd_extrap_both(const BdConst &c, const double *phi, const int8_t *st, const int8_t *nst, const BdCells &fc,
+                   const BdNodes &fn, int *done_c, int *done_n, double *key, int *cnt_c, int *cnt_n, int *list_c,
+                   int *list_n, volatile int *hflag, int *waves_c, int *waves_n, hipStream_t s) {
+    const long mn = (long)c.m * c.n, mne = (long)c.me * c.ne, span = std::max(mn, mne);
+    int *keyi = reinterpret_cast<int *>(key + 2 * mne), *lc = list_c + span, *ln = list_n + span;
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, cnt_c);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, cnt_n);
+    hipLaunchKernelGGL(xc_prepare_kernel, blocks(mn), dim3(kThreads), 0, s, c, phi, st, done_c, list_c, cnt_c,
+                       lc + span);
+    hipLaunchKernelGGL(xn_prepare_kernel, blocks(mne), dim3(kThreads), 0, s, c, phi, st, nst, done_n, key, keyi,
+                       cnt_n);
+    hipLaunchKernelGGL(xn_deps_kernel, blocks(mne), dim3(kThreads), 0, s, c, done_n, key, keyi, list_n, cnt_n,
+                       ln + span);
+    STZ_HIP(hipGetLastError());
+    int e = run_walk(
+        [&](int w) {
+            hipLaunchKernelGGL(both_wave_kernel, dim3(2 * kWaveBlocks), dim3(kWaveThreads), 0, s, c, phi, fc, done_c,
+                               list_c, cnt_c, lc, fn, done_n, key, keyi, list_n, cnt_n, ln, span, w);
+        },
+        cnt_c, cnt_n, hflag, s);
+    if (e || (e = walk_result(cnt_c, hflag, waves_c, "cell", s))) return e;
+    return walk_result(cnt_n, hflag, waves_n, "node", s);
 }
 
 }  // namespace stz

@@ -26,6 +26,7 @@ struct stz_bd {
     double *nkey = nullptr;         // node extrapolation keys (3 doubles per node)
     int8_t *st = nullptr, *nst = nullptr;
     int *rowt = nullptr, *done_c = nullptr, *done_n = nullptr, *counters = nullptr, *xlist = nullptr;
+    long xspan = 0;                 // max(cells, nodes): the extrapolation lists hold 6 xspan ints
     double *fmats = nullptr;
     int fcap = 0;
     unsigned long long *dmin = nullptr;
@@ -111,27 +112,35 @@ int advection(stz_bd *b, double dt) {
 }
 
 // bd_sim::post_process (bd_sim.cc:256-259)
-int post_process(stz_bd *b, bool all) {
+// the cell and (all) node extrapolations; the two walks are independent
+// and share their wave launches
+static int extrapolate(stz_bd *b, bool all) {
     int e, fails = 0;
-    if ((e = bd_extrap_cells(b->c, b->phi, b->st, b->cells(b->cc), b->done_c, b->counters, b->xlist,
-                             (volatile int *)b->hint, &b->waves_c, b->stream)))
-        return e;
+    if (all)
+        e = bd_extrap_both(b->c, b->phi, b->st, b->nst, b->cells(b->cc), b->nodes(b->nc), b->done_c, b->done_n,
+                           b->nkey, b->counters, b->counters + 16, b->xlist, b->xlist + 3 * b->xspan,
+                           (volatile int *)b->hint, &b->waves_c, &b->waves_n, b->stream);
+    else
+        e = bd_extrap_cells(b->c, b->phi, b->st, b->cells(b->cc), b->done_c, b->counters, b->xlist,
+                            (volatile int *)b->hint, &b->waves_c, b->stream);
+    if (e) return e;
     if ((e = read_int(b, b->counters + 2, &fails))) return e;
     if (fails) {
         set_error("cell extrapolation: singular fit at %d cells (traverse.hh:214-230)", fails);
         return STZ_ERR_STATE;
     }
-    if (all) {
-        if ((e = bd_extrap_nodes(b->c, b->phi, b->st, b->nst, b->nodes(b->nc), b->done_n, b->nkey, b->counters, b->xlist,
-                                 (volatile int *)b->hint, &b->waves_n, b->stream)))
-            return e;
-        if ((e = read_int(b, b->counters + 2, &fails))) return e;
-        if (fails) {
-            set_error("node extrapolation: singular fit at %d nodes (traverse.hh:153-164)", fails);
-            return STZ_ERR_STATE;
-        }
+    if (!all) return STZ_OK;
+    if ((e = read_int(b, b->counters + 16 + 2, &fails))) return e;
+    if (fails) {
+        set_error("node extrapolation: singular fit at %d nodes (traverse.hh:153-164)", fails);
+        return STZ_ERR_STATE;
     }
-    return bd_boundaries(b->c, b->nodes(b->nc), 0, b->stream);
+    return STZ_OK;
+}
+
+int post_process(stz_bd *b, bool all) {
+    const int e = extrapolate(b, all);
+    return e ? e : bd_boundaries(b->c, b->nodes(b->nc), 0, b->stream);
 }
 
 int grow_free_rows(stz_bd *b, int need) {
@@ -264,18 +273,19 @@ int stz_bd_create(const stz_bd_params *p, stz_bd **out) {
     b->force = nullptr;
     P += 6 * mne;  // vel, src, spare
     b->nkey = P;
+    b->xspan = std::max(mn, mne);
     if (hipMalloc(&b->st, mn) != hipSuccess || hipMalloc(&b->nst, mne) != hipSuccess ||
         hipMalloc(&b->rowt, mne * sizeof(int)) != hipSuccess || hipMalloc(&b->done_c, mn * sizeof(int)) != hipSuccess ||
         hipMalloc(&b->done_n, mne * sizeof(int)) != hipSuccess ||
-        hipMalloc(&b->xlist, 3 * std::max(mn, mne) * sizeof(int)) != hipSuccess ||
-        hipMalloc(&b->counters, 16 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&b->xlist, 6 * std::max(mn, mne) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&b->counters, 32 * sizeof(int)) != hipSuccess ||
         hipMalloc(&b->dmin, sizeof(unsigned long long)) != hipSuccess) {
         set_error("out of device memory");
         return fail(STZ_ERR_HIP);
     }
     (void)hipMemset(b->st, 0, mn);
     (void)hipMemset(b->nst, 0, mne);
-    (void)hipMemset(b->counters, 0, 16 * sizeof(int));
+    (void)hipMemset(b->counters, 0, 32 * sizeof(int));
     if (hipHostMalloc((void **)&b->hint, 16 * sizeof(int), hipHostMallocCoherent) != hipSuccess) {
         set_error("hipHostMalloc failed");
         return fail(STZ_ERR_HIP);
@@ -471,23 +481,7 @@ int stz_bd_init_bar(stz_bd *b, int ca) {
     if ((e = build_band(b))) return e;
     if ((e = bd_init_bar_fields(b->c, ca, b->st, b->nst, b->cells(b->cc), b->nodes(b->nc), b->stream))) return e;
     // bd_sim::extrapolate (bd_sim.hh:162-165), no boundary conditions
-    int fails = 0;
-    if ((e = bd_extrap_cells(b->c, b->phi, b->st, b->cells(b->cc), b->done_c, b->counters, b->xlist, (volatile int *)b->hint,
-                             &b->waves_c, b->stream)))
-        return e;
-    if ((e = read_int(b, b->counters + 2, &fails))) return e;
-    if (fails) {
-        set_error("cell extrapolation: singular fit at %d cells (traverse.hh:214-230)", fails);
-        return STZ_ERR_STATE;
-    }
-    if ((e = bd_extrap_nodes(b->c, b->phi, b->st, b->nst, b->nodes(b->nc), b->done_n, b->nkey, b->counters, b->xlist,
-                             (volatile int *)b->hint, &b->waves_n, b->stream)))
-        return e;
-    if ((e = read_int(b, b->counters + 2, &fails))) return e;
-    if (fails) {
-        set_error("extrapolation: singular fit at %d points", fails);
-        return STZ_ERR_STATE;
-    }
+    if ((e = extrapolate(b, true))) return e;
     STZ_HIP(hipStreamSynchronize(b->stream));
     return STZ_OK;
 }
