@@ -94,7 +94,7 @@ struct stz_ensemble {
     hipGraphExec_t g_tail[2] = {nullptr, nullptr}, g_slot = nullptr;
     std::map<std::pair<int, int>, hipGraphExec_t> g_step;  // whole step by (parity, planned slots)
     int64_t stats[4] = {0, 0, 0, 0};
-    int tuned[2] = {0, 0};        // launch positions that kept 32x16 / 64x32 tiles
+    int tuned[2] = {0, 0};        // launch positions that kept 32x16 / 64x34 tiles
 
     int nmem() const { return (int)mem.size(); }
     void free_graphs() {
@@ -156,7 +156,7 @@ int make_table(stz_ensemble *e, const std::vector<T> &E, const std::vector<int> 
 // workgroups of the tiled smoother on a level (launch_tile's grid)
 template <typename T> void tile_grid(const LevelArgs<T> &L, bool big, int &gx, int &gy) {
     gx = big ? L.m / 64 : L.m / 32;
-    gy = big ? (L.n + 31) / 32 : (L.n + 15) / 16;
+    gy = big ? (L.n + 33) / 34 : (L.n + 15) / 16;
 }
 
 // average time of one pass over `ls` (ungated launches, after one untimed
@@ -250,7 +250,7 @@ int build_plan(stz_ensemble *e, VCyclePlan<T> &P, DevMG<T> stz_sim::*mgp, bool t
                     R.push_back(&rk.first);
                     who.push_back(rk.second);
                 }
-            // candidates: 32x16 tiles for all; 64x32 tiles where the level
+            // candidates: 32x16 tiles for all; 64x34 tiles where the level
             // width allows them (the rest 32x16).  Tile shape does not change
             // what a launch computes, only how much of the halo is recomputed
             // and how many workgroups share the chip, so the faster is kept.

@@ -735,14 +735,14 @@ template int mg_smooth_level<float>(LevelArgs<float> &, int, hipStream_t);
 // ---------------------------------------------------------------------------
 // batched launchers (batch.h).  Tile shape does not change what a tiled
 // launch computes (every node sees the same colour order and operands), so
-// batches use 32x16 tiles, or 64x32 when key.big
+// batches use 32x16 tiles, or the large 64x34 ones when key.big
 // ---------------------------------------------------------------------------
 template <typename T, int S, int MODE, bool FUSE, bool SP>
 static void launch_tile_batch(bool big, const TileEntry<T> *E, const int *map, int ne, int total,
                               const EnsState *St, hipStream_t st) {
     if (big)
-        hipLaunchKernelGGL((smooth_tile_batch_kernel<T, 64, 32, S, MODE, FUSE, SP>), dim3(total),
-                           dim3(tile_threads(64, 32, tile_halo(S, FUSE))), 0, st, E, map, ne, St);
+        hipLaunchKernelGGL((smooth_tile_batch_kernel<T, 64, 34, S, MODE, FUSE, SP>), dim3(total),
+                           dim3(tile_threads(64, 34, tile_halo(S, FUSE))), 0, st, E, map, ne, St);
     else
         hipLaunchKernelGGL((smooth_tile_batch_kernel<T, 32, 16, S, MODE, FUSE, SP>), dim3(total),
                            dim3(tile_threads(32, 16, tile_halo(S, FUSE))), 0, st, E, map, ne, St);

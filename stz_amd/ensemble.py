@@ -122,7 +122,7 @@ class Ensemble:
         return {"choice": choice, "ms_per_step": times}
 
     def tile_choice(self):
-        """{launches kept at 32x16 tiles, at 64x32} by the tuner"""
+        """{launches kept at 32x16 tiles, at 64x34} by the tuner"""
         out = np.zeros(2, dtype=np.int32)
         check(self._lib.stz_ensemble_tile_choice(self._h, out))
-        return {"32x16": int(out[0]), "64x32": int(out[1])}
+        return {"32x16": int(out[0]), "64x34": int(out[1])}
