@@ -1021,28 +1021,31 @@ int BdSolver::setup(const BdTop &t, hipStream_t s) {
 // one sweep of the tiled kernel over an m x n level from zin into zout;
 // mode 1 starts from zero, mode 2 adds the correction of the cm x cn level
 // zc first
+// one launch of gs_tile_kernel<T, MODE, Rows> with `lds` bytes of LDS (the
+// kernel's limit raised once where that is above 64 KiB)
+template <typename T, int MODE, class Rows>
+static int tile_launch(dim3 g, size_t lds, hipStream_t s, const Rows &rows, const T *zin, T *zout, const T *zc,
+                       T *cb, int cm, int cn) {
+    static const hipError_t lim =
+        lds > 65536 ? hipFuncSetAttribute(reinterpret_cast<const void *>(&gs_tile_kernel<T, MODE, Rows>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
+                    : hipSuccess;
+    STZ_HIP(lim);
+    hipLaunchKernelGGL((gs_tile_kernel<T, MODE, Rows>), g, dim3(kTileThreads), lds, s, rows, zin, zout, zc, cb, cm,
+                       cn);
+    STZ_HIP(hipGetLastError());
+    return STZ_OK;
+}
+
 template <typename T, class Rows>
 static int tile_sweep(const Rows &rows, int m, int n, const T *zin, T *zout, int mode, const T *zc, int cm, int cn,
                       hipStream_t s, T *cb = nullptr) {
     using D = Tile<T, 4>;
-    const dim3 g((m + D::X - 1) / D::X, (n + D::Y - 1) / D::Y), b(kTileThreads);
-    if (mode == 3) {
-        // above 64 KiB of LDS: raise the kernel's limit once
-        static const hipError_t lim = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&gs_tile_kernel<T, 3, Rows>), hipFuncAttributeMaxDynamicSharedMemorySize,
-            (int)Tile<T, 6>::lds(1));
-        STZ_HIP(lim);
-        hipLaunchKernelGGL((gs_tile_kernel<T, 3, Rows>), g, b, (Tile<T, 6>::lds(1)), s, rows, zin, zout, zc, cb, cm,
-                           cn);
-    }
-    else if (mode == 1)
-        hipLaunchKernelGGL((gs_tile_kernel<T, 1, Rows>), g, b, D::lds(1), s, rows, zin, zout, zc, cb, cm, cn);
-    else if (mode == 2)
-        hipLaunchKernelGGL((gs_tile_kernel<T, 2, Rows>), g, b, D::lds(2), s, rows, zin, zout, zc, cb, cm, cn);
-    else
-        hipLaunchKernelGGL((gs_tile_kernel<T, 0, Rows>), g, b, D::lds(0), s, rows, zin, zout, zc, cb, cm, cn);
-    STZ_HIP(hipGetLastError());
-    return STZ_OK;
+    const dim3 g((m + D::X - 1) / D::X, (n + D::Y - 1) / D::Y);
+    if (mode == 3) return tile_launch<T, 3>(g, Tile<T, 6>::lds(1), s, rows, zin, zout, zc, cb, cm, cn);
+    if (mode == 1) return tile_launch<T, 1>(g, D::lds(1), s, rows, zin, zout, zc, cb, cm, cn);
+    if (mode == 2) return tile_launch<T, 2>(g, D::lds(2), s, rows, zin, zout, zc, cb, cm, cn);
+    return tile_launch<T, 0>(g, D::lds(0), s, rows, zin, zout, zc, cb, cm, cn);
 }
 
 // the start of a smoothing that the tiled kernel does not take, in zout:
