@@ -724,15 +724,27 @@ __global__ void __launch_bounds__(kThreads) march_seed_kernel(BdConst c, double 
     st[ij] = Z::trial;
 }
 
+// The march's per-round kernels run over the whole grid, of which only the
+// band's cells take part: the minimum and the acceptance take cells a grid
+// stride apart (kMarchBlocks workgroups), so they are bound by the one byte
+// of status per cell they read rather than by dispatching a workgroup per
+// 256 cells.  The update, which reads a cell's neighbours, keeps a thread
+// per cell.
+constexpr int kMarchBlocks = 2048;
+
 template <bool POS>
 __global__ void __launch_bounds__(kThreads) march_min_kernel(BdConst c, const double *__restrict__ phi,
                                                              const int8_t *__restrict__ st,
                                                              unsigned long long *dmin) {
     using Z = Dir<POS>;
     __shared__ unsigned long long red[kThreads / 64];
-    const int ij = blockIdx.x * kThreads + threadIdx.x;
+    const int mn = c.m * c.n;
     unsigned long long k = ~0ull;
-    if (ij < c.m * c.n && code(st[ij]) == Z::trial) k = okey(POS ? phi[ij] : -phi[ij]);
+    for (int ij = blockIdx.x * kThreads + threadIdx.x; ij < mn; ij += gridDim.x * kThreads)
+        if (code(st[ij]) == Z::trial) {
+            const unsigned long long t = okey(POS ? phi[ij] : -phi[ij]);
+            k = t < k ? t : k;
+        }
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long t = __shfl_down(k, o, 64);
         k = t < k ? t : k;
@@ -752,19 +764,23 @@ __global__ void __launch_bounds__(kThreads) march_accept_kernel(BdConst c, const
                                                                 int8_t *st, const unsigned long long *dmin,
                                                                 double delta, int *counter) {
     using Z = Dir<POS>;
-    const int ij = blockIdx.x * kThreads + threadIdx.x;
-    if (ij >= c.m * c.n) return;
-    int8_t s = st[ij];
-    if (s & kNew) {
-        st[ij] = s & 15;
-        return;
+    const int mn = c.m * c.n;
+    const unsigned long long dm = *dmin;
+    int acc = 0;
+    for (int ij = blockIdx.x * kThreads + threadIdx.x; ij < mn; ij += gridDim.x * kThreads) {
+        const int8_t s = st[ij];
+        if (s & kNew) {
+            st[ij] = s & 15;
+            continue;
+        }
+        if (code(s) != Z::trial || dm == ~0ull) continue;
+        const double kmin = okey_inv(dm), key = POS ? phi[ij] : -phi[ij];
+        if (key < kmin + delta && key < c.phi_lim) {
+            st[ij] = Z::acc | kNew;
+            acc++;
+        }
     }
-    if (code(s) != Z::trial || *dmin == ~0ull) return;
-    const double kmin = okey_inv(*dmin), key = POS ? phi[ij] : -phi[ij];
-    if (key < kmin + delta && key < c.phi_lim) {
-        st[ij] = Z::acc | kNew;
-        atomicAdd(counter, 1);
-    }
+    if (acc) atomicAdd(counter, acc);
 }
 
 // heap::update / scan after acceptances (heap.cc:166-181, 466-470):
@@ -1453,11 +1469,12 @@ static int march_dir(const BdConst &c, double *phi, int8_t *st, double delta, un
     hipLaunchKernelGGL(march_seed_kernel<POS>, g, b, 0, s, c, phi, st, dmin, counter);
     STZ_HIP(hipGetLastError());
     const int batch = 4, max_rounds = 4096;
+    const dim3 gm((unsigned)std::min<long>((long)g.x, kMarchBlocks));
     int r = 0;
     for (;;) {
         for (int q = 0; q < batch; q++) {
-            hipLaunchKernelGGL(march_min_kernel<POS>, g, b, 0, s, c, phi, st, dmin);
-            hipLaunchKernelGGL(march_accept_kernel<POS>, g, b, 0, s, c, phi, st, dmin, delta, counter);
+            hipLaunchKernelGGL(march_min_kernel<POS>, gm, b, 0, s, c, phi, st, dmin);
+            hipLaunchKernelGGL(march_accept_kernel<POS>, gm, b, 0, s, c, phi, st, dmin, delta, counter);
             hipLaunchKernelGGL(march_update_kernel<POS>, g, b, 0, s, c, phi, st, dmin, counter, last);
         }
         STZ_HIP(hipGetLastError());
@@ -1471,7 +1488,7 @@ static int march_dir(const BdConst &c, double *phi, int8_t *st, double delta, un
         }
     }
     // clear the flags of the last acceptances
-    hipLaunchKernelGGL(march_accept_kernel<POS>, g, b, 0, s, c, phi, st, dmin, 0.0, counter);
+    hipLaunchKernelGGL(march_accept_kernel<POS>, gm, b, 0, s, c, phi, st, dmin, 0.0, counter);
     STZ_HIP(hipGetLastError());
     *rounds += r;
     return STZ_OK;
