@@ -1175,6 +1175,21 @@ int BdSolver::dot(const double *a, const double *b, int slot, int op, int host_s
     return STZ_OK;
 }
 
+// Grid of a grid-stride kernel: the workgroups resident on the device at
+// once (occupancy x CUs), at most kMaxPartial and at most one per kThreads
+// elements.  Every workgroup does the same work, so a grid of several
+// resident sets leaves its last set partly empty (4096 workgroups of
+// bicg_x_dot_kernel at 7 per CU: 2.3 sets run as 3).
+template <typename K> static int resident_grid(K kernel, long elems) {
+    int dev = 0, occ = 0, cus = 0;
+    long g = kMaxPartial;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kThreads, 0) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && occ > 0 && cus > 0)
+        g = std::min<long>((long)occ * cus, kMaxPartial);
+    return (int)std::max<long>(1, std::min<long>(g, (elems + kThreads - 1) / kThreads));
+}
+
 // BiCGStab (van der Vorst 1992) with one V-cycle as right preconditioner.
 // When the shadow residual becomes orthogonal to the residual (the box-edge
 // rows are solved exactly by the first sweep, so a residual that starts
@@ -1183,9 +1198,12 @@ int BdSolver::dot(const double *a, const double *b, int slot, int op, int host_s
 int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_solve_info &info, hipStream_t s) {
     const long mn = (long)top_m * top_n, n2 = 2 * mn;
     const dim3 g = blocks(n2), b1(kThreads);
-    // grids of the kernels with fused products: at most kMaxPartial blocks
+    // grids of the kernels with fused products: at most kMaxPartial blocks;
+    // the x/r update in one resident set (417 -> 397 us at 4096^2; the
+    // operator applications, a node per thread for their stencils, lose
+    // 4 % with the same grid)
     const int nd = (int)std::min<long>((mn + kThreads - 1) / kThreads, kMaxPartial),
-              nx = (int)std::min<long>((n2 + kThreads - 1) / kThreads, kMaxPartial);
+              nx = mixed ? resident_grid(bicg_x_dot_kernel<float>, n2) : resident_grid(bicg_x_dot_kernel<double>, n2);
     const dim3 gd(nd), gx(nx);
     double *r = vec[kR], *rh = vec[kRhat], *p = vec[kP], *v = vec[kV], *sv = vec[kS], *t = vec[kT],
            *ph = vec[kPhat], *sh = vec[kShat];
