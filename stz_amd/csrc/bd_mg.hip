@@ -163,14 +163,17 @@ template <typename T> __global__ void __launch_bounds__(kThreads) gs_level_kerne
     L.z[2 * (size_t)ij + 1] = s[38] * rx + s[39] * ry;
 }
 
-// The bottom level in one workgroup: z in LDS, `sweeps` four-colour sweeps.
-// Up to 1024 nodes, each thread keeps its node's stencil and right-hand side
-// in registers for all the sweeps.
-template <typename T> __global__ void __launch_bounds__(1024) gs_bottom_kernel(BdLevelT<T> L, int sweeps) {
+// The bottom level in one workgroup: z in LDS (from zero when `zero`),
+// `sweeps` four-colour sweeps.  Up to 1024 nodes, each thread keeps its
+// node's stencil and right-hand side in registers for all the sweeps; the
+// host launches as many threads as nodes (rounded up to whole waves), so
+// each of the 4 x sweeps barriers waits on as few waves as it can.
+template <typename T>
+__global__ void __launch_bounds__(1024) gs_bottom_kernel(BdLevelT<T> L, int sweeps, int zero) {
     extern __shared__ __align__(16) unsigned char bd_smem[];
     T *zs = reinterpret_cast<T *>(bd_smem);
     const int N = L.m * L.n;
-    for (int q = threadIdx.x; q < 2 * N; q += blockDim.x) zs[q] = L.z[q];
+    for (int q = threadIdx.x; q < 2 * N; q += blockDim.x) zs[q] = zero ? (T)0 : L.z[q];
     if (N <= (int)blockDim.x) {
         const int q = threadIdx.x, own = q < N;
         const int i = own ? q % L.m : 0, j = own ? q / L.m : 0, col = (i & 1) + 2 * (j & 1);
@@ -1089,11 +1092,14 @@ int BdSolver::smooth_level(const BdLevelT<T> &L, bool bottom, int sweeps, int mo
                            const T *zc, int cm, int cn, hipStream_t s) {
     const size_t lds = 2 * (size_t)L.m * L.n * sizeof(T);
     if (bottom && lds <= 64 * 1024) {
-        int e = start_in_place(zout, zin, L.m, L.n, mode, zc, cm, cn, s);
+        // a zero start is the kernel's own; other starts are staged in zout
+        int e = mode == 1 ? STZ_OK : start_in_place(zout, zin, L.m, L.n, mode, zc, cm, cn, s);
         if (e) return e;
         BdLevelT<T> B = L;
         B.z = zout;
-        hipLaunchKernelGGL(gs_bottom_kernel<T>, dim3(1), dim3(1024), lds, s, B, sweeps);
+        const long nodes = (long)L.m * L.n;
+        const int nt = nodes <= 1024 ? (int)((nodes + 63) / 64 * 64) : 1024;
+        hipLaunchKernelGGL(gs_bottom_kernel<T>, dim3(1), dim3(nt), lds, s, B, sweeps, mode == 1 ? 1 : 0);
         STZ_HIP(hipGetLastError());
         return STZ_OK;
     }
