@@ -348,12 +348,29 @@ template <typename T, int kHalo> struct Tile {
     }
 };
 
-// the top level: z in T, b in TB staged as T (qs_multi_bd::mul_a rows)
+// the top level: z in T, b in TB staged as T (qs_multi_bd::mul_a rows).
+// In FP32 (the mixed form's preconditioner) an update multiplies by the
+// diagonal's reciprocal for its row type, formed once in FP64 and rounded
+// (ix*, iy*: interior, box edge, outside, free row); FP64 divides, as
+// qs_multi_bd's smoother.
 template <typename T, typename TB> struct TopRows {
     BdTop t;
     const TB *b;
+    T ix0 = 0, ix1 = 0, ix2 = 0, ix3 = 0, iy0 = 0, iy1 = 0, iy2 = 0, iy3 = 0;
     using Src = TB;
     static constexpr int kOutside = -2;  // no row: nothing is read
+    static TopRows make(const BdTop &t, const TB *b) {
+        TopRows r{t, b};
+        r.ix0 = (T)(1 / t.cxx);
+        r.iy0 = (T)(1 / t.cyy);
+        r.ix1 = (T)(1 / (t.bcs * t.cxx));
+        r.iy1 = (T)(1 / (t.bcs * t.cyy));
+        r.ix2 = (T)(1 / (t.ocs * t.cxx));
+        r.iy2 = (T)(1 / (t.ocs * t.cyy));
+        r.ix3 = (T)(1 / (t.ncs * t.cxx));
+        r.iy3 = (T)(1 / (t.ncs * t.cyy));
+        return r;
+    }
     __device__ int m() const { return t.m; }
     __device__ int n() const { return t.n; }
     __device__ const TB *rhs() const { return b; }
@@ -362,9 +379,16 @@ template <typename T, typename TB> struct TopRows {
     __device__ void update(int, int, int l, T *zs, const T *bs, const int *ts, const T *, int LW) const {
         const int d = ts[l];
         const V2T<T> o = top_offdiag_at<T>(t, zs + 2 * l, LW, d);
-        const double sc = top_diag_scale(t, d);
-        zs[2 * l] = (bs[2 * l] - o.x) / (T)(sc * t.cxx);
-        zs[2 * l + 1] = (bs[2 * l + 1] - o.y) / (T)(sc * t.cyy);
+        if constexpr (sizeof(T) == 4) {
+            const T rx = d == -1 ? ix0 : (d == -2 ? ix1 : (d == -3 ? ix2 : ix3));
+            const T ry = d == -1 ? iy0 : (d == -2 ? iy1 : (d == -3 ? iy2 : iy3));
+            zs[2 * l] = (bs[2 * l] - o.x) * rx;
+            zs[2 * l + 1] = (bs[2 * l + 1] - o.y) * ry;
+        } else {
+            const double sc = top_diag_scale(t, d);
+            zs[2 * l] = (bs[2 * l] - o.x) / (T)(sc * t.cxx);
+            zs[2 * l + 1] = (bs[2 * l + 1] - o.y) / (T)(sc * t.cyy);
+        }
     }
     // b - A z, as resid_top_kernel
     __device__ V2T<T> resid(int, int, int l, const T *zs, const T *bs, const int *ts, const T *, int LW) const {
@@ -1073,7 +1097,7 @@ static int start_in_place(T *zout, const T *zin, int m, int n, int mode, const T
 template <typename T, typename TB>
 int BdSolver::smooth_top(T *zout, const TB *b, int sweeps, int mode, const T *zin, const T *zc, int cm, int cn,
                          hipStream_t s) {
-    if (sweeps == 1) return tile_sweep(TopRows<T, TB>{top, b}, top_m, top_n, zin, zout, mode, zc, cm, cn, s);
+    if (sweeps == 1) return tile_sweep(TopRows<T, TB>::make(top, b), top_m, top_n, zin, zout, mode, zc, cm, cn, s);
     const int e = start_in_place(zout, zin, top_m, top_n, mode, zc, cm, cn, s);
     if (e) return e;
     for (int k = 0; k < sweeps; k++)
@@ -1134,7 +1158,7 @@ int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, T *zpre, hip
     // residual and restricts it to the next level (mode 3)
     const bool fused = nu1 == 1;
     if (fused) {
-        e = tile_sweep(TopRows<T, TB>{top, rhs}, top_m, top_n, none, zpre, 3, none, Lv[0].m, Lv[0].n, s, Lv[0].b);
+        e = tile_sweep(TopRows<T, TB>::make(top, rhs), top_m, top_n, none, zpre, 3, none, Lv[0].m, Lv[0].n, s, Lv[0].b);
         if (e) return e;
     } else {
         if ((e = smooth_top(zpre, rhs, nu1, 1, none, none, 0, 0, s))) return e;
