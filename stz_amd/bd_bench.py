@@ -10,8 +10,10 @@ the left and right edges that move apart at nominal strain rate 1e-4
 field with correlation length 4 cells inside the plate, dt = 0.5; the
 projection's V-cycle preconditioner in FP32 inside FP64 BiCGStab (mixed,
 the default) or in FP64.  Prints one JSON line: cell-steps/s over the m*n
-cells, ms per step and the work counts of the last step (march rounds,
-extrapolation waves, BiCGStab iterations).
+cells, ms per step, each timed step's BiCGStab iteration count and the ms
+per iteration (the count moves with rounding from build to build; the time
+per iteration compares builds), and the work counts of the last step (march
+rounds, extrapolation waves, BiCGStab iterations).
 """
 import argparse
 import json
@@ -55,14 +57,22 @@ def main():
     sim.set_precision(a.precision)
     sim.step_forward_quasistatic(a.dt, a.warmup)
     sim.synchronize()
+    # one step per call, so that every step's BiCGStab iteration count is
+    # read (the count moves with rounding from build to build, and the step
+    # time with it)
+    iters = []
     t0 = time.perf_counter()
-    sim.step_forward_quasistatic(a.dt, a.steps)
+    for _ in range(a.steps):
+        sim.step_forward_quasistatic(a.dt, 1)
+        iters.append(sim.step_stats()["bicgstab_iterations"])
     sim.synchronize()
     el = time.perf_counter() - t0
     out = {"workload": "plate with hole, grips, %dx%d cells" % (a.size, a.size), "nu": a.nu,
            "precision": a.precision,
            "steps": a.steps, "ms_per_step": 1e3 * el / a.steps,
            "cell_steps_per_s": a.size * a.size * a.steps / el,
+           "bicgstab_iterations": iters,
+           "ms_per_iteration": 1e3 * el / max(1, sum(iters)),
            "last_solve": sim.last_solve(), "last_step": sim.step_stats()}
     print(json.dumps(out))
 
