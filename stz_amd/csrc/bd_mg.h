@@ -18,16 +18,20 @@ struct BdTop {
 };
 
 // Coarse level.  Rows whose fine rows were all interior rows share one
-// stencil Sc (type -1); the others keep their own 40 entries in S (type 0).
-// The Galerkin hierarchy is built in FP64; the mixed form keeps an FP32 copy
-// of every level (stencils rounded once per solve) for its preconditioner.
+// stencil (type -1, Sc[0..39]); rows whose fine rows were all outside rows
+// and whose product equals the shared outside stencil bit for bit share
+// that one (type -2, Sc[40..79]); the others keep their own 40 entries in S
+// (type 0).  The Galerkin hierarchy is built in FP64; the mixed form keeps
+// an FP32 copy of every level (stencils rounded once per solve) for its
+// preconditioner.
 template <typename T> struct BdLevelT {
     int m = 0, n = 0;
     T *z = nullptr, *b = nullptr, *r = nullptr;
     T *S = nullptr;        // 40 per node, read only where type >= 0
     int *type = nullptr;   // shared by the FP64 and FP32 copies
-    T *Sc = nullptr;       // 40
+    T *Sc = nullptr;       // 80: the shared interior and outside stencils
 };
+constexpr int kSharedStencils = 80;
 using BdLevel = BdLevelT<double>;
 
 enum { kR = 0, kRhat, kP, kV, kS, kT, kPhat, kShat, kTmp, kPre, kBdVecs };

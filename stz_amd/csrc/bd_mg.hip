@@ -102,8 +102,10 @@ __device__ __forceinline__ double top_diag_scale(const BdTop &t, int d) {
     return d == -1 ? 1.0 : (d == -2 ? t.bcs : (d == -3 ? t.ocs : t.ncs));
 }
 
+// the stencil of row ij: shared (type -1 interior, -2 outside) or its own
 template <typename T> __device__ __forceinline__ const T *lvl_stencil(const BdLevelT<T> &L, int ij) {
-    return L.type[ij] < 0 ? L.Sc : L.S + 40 * (size_t)ij;
+    const int t = L.type[ij];
+    return t < 0 ? L.Sc + 40 * (-1 - t) : L.S + 40 * (size_t)ij;
 }
 
 // off-diagonal part of a coarse row: stencil s, zp the row's own pair of z
@@ -344,7 +346,7 @@ template <typename T, int kHalo> struct Tile {
     // the coarse nodes under the tile and its halo start at (x0/2 - 2, y0/2 - 2)
     static constexpr int CW = LW / 2 + 3, CH = LH / 2 + 3, NC = CW * CH;
     static size_t lds(int mode) {
-        return (4 * (size_t)N + (mode == 2 ? 2 * NC : 0) + 40) * sizeof(T) + (size_t)N * sizeof(int);
+        return (4 * (size_t)N + (mode == 2 ? 2 * NC : 0) + kSharedStencils) * sizeof(T) + (size_t)N * sizeof(int);
     }
 };
 
@@ -400,7 +402,7 @@ template <typename T, typename TB> struct TopRows {
     }
 };
 
-// a coarse level; its shared stencil is staged too
+// a coarse level; its two shared stencils are staged too
 template <typename T> struct LevelRows {
     BdLevelT<T> L;
     using Src = T;
@@ -410,7 +412,7 @@ template <typename T> struct LevelRows {
     __device__ const T *rhs() const { return L.b; }
     __device__ const int *types() const { return L.type; }
     __device__ void begin(T *sc) const {
-        for (int q = threadIdx.x; q < 40; q += blockDim.x) sc[q] = L.Sc[q];
+        for (int q = threadIdx.x; q < kSharedStencils; q += blockDim.x) sc[q] = L.Sc[q];
     }
     // the 40 stencil entries are loaded together before they are used
     __device__ void relax(const T *s0, int i, int j, int l, T *zs, const T *bs, int LW) const {
@@ -423,7 +425,8 @@ template <typename T> struct LevelRows {
         zs[2 * l + 1] = s[38] * rx + s[39] * ry;
     }
     __device__ void update(int i, int j, int l, T *zs, const T *bs, const int *ts, const T *sc, int LW) const {
-        if (ts[l] < 0) relax(sc, i, j, l, zs, bs, LW);
+        const int t = ts[l];
+        if (t < 0) relax(sc + 40 * (-1 - t), i, j, l, zs, bs, LW);
         else relax(L.S + 40 * (i + (size_t)L.m * j), i, j, l, zs, bs, LW);
     }
     // b - A z, as resid_level_kernel
@@ -436,7 +439,8 @@ template <typename T> struct LevelRows {
         return V2T<T>{bs[2 * l] - o.x - (s[16] * zx + s[17] * zy), bs[2 * l + 1] - o.y - (s[18] * zx + s[19] * zy)};
     }
     __device__ V2T<T> resid(int i, int j, int l, const T *zs, const T *bs, const int *ts, const T *sc, int LW) const {
-        if (ts[l] < 0) return resid_at(sc, i, j, l, zs, bs, LW);
+        const int t = ts[l];
+        if (t < 0) return resid_at(sc + 40 * (-1 - t), i, j, l, zs, bs, LW);
         return resid_at(L.S + 40 * (i + (size_t)L.m * j), i, j, l, zs, bs, LW);
     }
 };
@@ -452,7 +456,7 @@ __global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const 
     extern __shared__ __align__(16) unsigned char bd_tile_smem[];
     T *zs = reinterpret_cast<T *>(bd_tile_smem), *bs = zs + 2 * D::N, *cs = bs + 2 * D::N,
       *sc = cs + (MODE == 2 ? 2 * D::NC : 0);
-    int *ts = reinterpret_cast<int *>(sc + 40);
+    int *ts = reinterpret_cast<int *>(sc + kSharedStencils);
     const int m = rows.m(), n = rows.n(), tid = threadIdx.x;
     const int x0 = blockIdx.x * D::X, y0 = blockIdx.y * D::Y, cx0 = x0 / 2 - 2, cy0 = y0 / 2 - 2;
     const TB *b = rows.rhs();
@@ -591,58 +595,64 @@ __global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const 
 }
 
 // Sources of fine-level entries for the Galerkin product: the top operator,
-// a stored coarse level, or (for the shared interior stencil) a level whose
-// every row is its interior row
+// a stored coarse level, or (for a shared stencil) a level whose every row
+// is its interior row (KIND -1) or its outside row (KIND -2)
 struct TopSrc {
     BdTop t;
     __device__ int m() const { return t.m; }
     __device__ int n() const { return t.n; }
     __device__ bool interior(int f) const { return t.rowt[f] == -1; }
+    __device__ bool outside(int f) const { return t.rowt[f] == -3; }
     __device__ void entry(int f, int k, double *A) const { top_entry(t, f, k, A); }
 };
 struct LvlSrc {
     BdLevel F;
     __device__ int m() const { return F.m; }
     __device__ int n() const { return F.n; }
-    __device__ bool interior(int f) const { return F.type[f] < 0; }
+    __device__ bool interior(int f) const { return F.type[f] == -1; }
+    __device__ bool outside(int f) const { return F.type[f] == -2; }
     __device__ void entry(int f, int k, double *A) const {
         const double *s = lvl_stencil(F, f) + 4 * k;
         A[0] = s[0]; A[1] = s[1]; A[2] = s[2]; A[3] = s[3];
     }
 };
-struct TopConstSrc {
+template <int KIND> struct TopConstSrc {
     BdTop t;
     __device__ int m() const { return t.m; }
     __device__ int n() const { return t.n; }
-    __device__ bool interior(int) const { return true; }
+    __device__ bool interior(int) const { return KIND == -1; }
+    __device__ bool outside(int) const { return KIND == -2; }
     __device__ void entry(int, int k, double *A) const {
-        // rows of type -1 do not depend on the row index
+        // rows of type -1 (interior) and -3 (outside) do not depend on the
+        // row index
         BdTop u = t;
-        const int d = -1;
+        const int d = KIND == -1 ? -1 : -3;
         u.rowt = &d;
         top_entry(u, 0, k, A);
     }
 };
-struct LvlConstSrc {
+template <int KIND> struct LvlConstSrc {
     BdLevel F;
     __device__ int m() const { return F.m; }
     __device__ int n() const { return F.n; }
-    __device__ bool interior(int) const { return true; }
+    __device__ bool interior(int) const { return KIND == -1; }
+    __device__ bool outside(int) const { return KIND == -2; }
     __device__ void entry(int, int k, double *A) const {
-        const double *s = F.Sc + 4 * k;
+        const double *s = F.Sc + 40 * (-1 - KIND) + 4 * k;
         A[0] = s[0]; A[1] = s[1]; A[2] = s[2]; A[3] = s[3];
     }
 };
 
 // Galerkin coarse block row at coarse node (I,J): sum over fine nodes f
 // with weight P(f,(I,J)) of P(f,.)^T A(f,g) P(g,.), then the inverse of cc.
-// Returns whether every fine row used was an interior row.
+// Returns -1 when every fine row used was an interior row, -2 when every
+// one was an outside row, 0 otherwise.
 template <class Src>
-__device__ bool rap_node(const Src &src, int cm, int cn, int I, int J, double *o) {
+__device__ int rap_node(const Src &src, int cm, int cn, int I, int J, double *o) {
     const int fm = src.m(), fn = src.n();
     double acc[36];
     for (int q = 0; q < 36; q++) acc[q] = 0;
-    bool all_interior = true;
+    bool all_interior = true, all_outside = true;
     int c[2], cx[2], cy[2];
     double w[2], wx[2], wy[2];
     for (int fy = 2 * J - 1; fy <= 2 * J + 1; fy++) {
@@ -653,6 +663,7 @@ __device__ bool rap_node(const Src &src, int cm, int cn, int I, int J, double *o
             if (wfx == 0) continue;
             const int fij = fx + fm * fy;
             all_interior = all_interior && src.interior(fij);
+            all_outside = all_outside && src.outside(fij);
             for (int k = 0; k < 9; k++) {
                 const int gx = fx + k % 3 - 1, gy = fy + k / 3 - 1;
                 if (gx < 0 || gx >= fm || gy < 0 || gy >= fn) continue;
@@ -678,29 +689,41 @@ __device__ bool rap_node(const Src &src, int cm, int cn, int I, int J, double *o
     o[37] = -b * idet;
     o[38] = -cc * idet;
     o[39] = a * idet;
-    return all_interior;
+    return all_interior ? -1 : (all_outside ? -2 : 0);
 }
 
+// Every row's product; a row whose fine rows were all interior rows takes
+// the shared interior stencil, one whose fine rows were all outside rows
+// takes the shared outside stencil if its product equals it bit for bit
+// (near the level's edges the prolongation weights differ); the others
+// store their own.  The shared stencils are formed first (rap_const_kernel).
 template <class Src>
 __global__ void __launch_bounds__(kThreads) rap_kernel(Src src, BdLevel C, int shared) {
     const int IJ = blockIdx.x * kThreads + threadIdx.x;
     if (IJ >= C.m * C.n) return;
     double o[40];
-    const bool interior = rap_node(src, C.m, C.n, IJ % C.m, IJ / C.m, o) && shared;
-    C.type[IJ] = interior ? -1 : 0;
-    if (interior) return;
+    const int k = rap_node(src, C.m, C.n, IJ % C.m, IJ / C.m, o), kind = shared ? k : 0;
+    int type = kind == -1 ? -1 : 0;
+    if (kind == -2) {
+        bool same = true;
+        for (int q = 0; q < 40; q++) same = same && o[q] == C.Sc[40 + q];
+        type = same ? -2 : 0;
+    }
+    C.type[IJ] = type;
+    if (type < 0) return;
     double *S = C.S + 40 * (size_t)IJ;
     for (int q = 0; q < 40; q++) S[q] = o[q];
 }
 
-// the shared interior stencil of a level: the product at a node whose
-// footprint lies in the interior (same arithmetic as rap_kernel)
-template <class Src>
+// a shared stencil of a level (KIND -1 interior, -2 outside): the product
+// at a node whose footprint lies away from the level's edges, every fine row
+// of the kind (same arithmetic as rap_kernel)
+template <int KIND, class Src>
 __global__ void rap_const_kernel(Src src, BdLevel C) {
     if (threadIdx.x != 0) return;
     double o[40];
     (void)rap_node(src, C.m, C.n, 2, 2, o);
-    for (int q = 0; q < 40; q++) C.Sc[q] = o[q];
+    for (int q = 0; q < 40; q++) C.Sc[40 * (-1 - KIND) + q] = o[q];
 }
 
 template <typename T> __global__ void __launch_bounds__(kThreads) fill_kernel(T *p, long n, T v) {
@@ -942,7 +965,7 @@ int BdSolver::init(int tm, int tn) {
         STZ_HIP(hipMalloc(&L.r, n2 * sizeof(double)));
         STZ_HIP(hipMalloc(&L.S, 20 * n2 * sizeof(double)));
         STZ_HIP(hipMalloc(&L.type, (size_t)L.m * L.n * sizeof(int)));
-        STZ_HIP(hipMalloc(&L.Sc, 40 * sizeof(double)));
+        STZ_HIP(hipMalloc(&L.Sc, kSharedStencils * sizeof(double)));
     }
     return was_mixed ? set_mixed(true) : STZ_OK;
 }
@@ -959,7 +982,7 @@ int BdSolver::alloc_mixed() {
         STZ_HIP(hipMalloc(&L.b, n2 * sizeof(float)));
         STZ_HIP(hipMalloc(&L.r, n2 * sizeof(float)));
         STZ_HIP(hipMalloc(&L.S, 20 * n2 * sizeof(float)));
-        STZ_HIP(hipMalloc(&L.Sc, 40 * sizeof(float)));
+        STZ_HIP(hipMalloc(&L.Sc, kSharedStencils * sizeof(float)));
         L.type = lv[l].type;
     }
     return STZ_OK;
@@ -1023,14 +1046,23 @@ int BdSolver::setup(const BdTop &t, hipStream_t s) {
         const dim3 g = blocks((long)lv[l].m * lv[l].n);
         // the shared stencil needs a fine footprint of 7x7 nodes
         const bool cst = (l == 0 ? top_m : lv[l - 1].m) >= 7 && (l == 0 ? top_n : lv[l - 1].n) >= 7;
+        // the shared stencils first: rap_kernel compares outside rows with one
         if (l == 0) {
+            if (cst) {
+                hipLaunchKernelGGL((rap_const_kernel<-1, TopConstSrc<-1>>), dim3(1), dim3(64), 0, s,
+                                   TopConstSrc<-1>{top}, lv[0]);
+                hipLaunchKernelGGL((rap_const_kernel<-2, TopConstSrc<-2>>), dim3(1), dim3(64), 0, s,
+                                   TopConstSrc<-2>{top}, lv[0]);
+            }
             hipLaunchKernelGGL(rap_kernel<TopSrc>, g, dim3(kThreads), 0, s, TopSrc{top}, lv[0], cst ? 1 : 0);
-            if (cst) hipLaunchKernelGGL(rap_const_kernel<TopConstSrc>, dim3(1), dim3(64), 0, s, TopConstSrc{top}, lv[0]);
         } else {
+            if (cst) {
+                hipLaunchKernelGGL((rap_const_kernel<-1, LvlConstSrc<-1>>), dim3(1), dim3(64), 0, s,
+                                   LvlConstSrc<-1>{lv[l - 1]}, lv[l]);
+                hipLaunchKernelGGL((rap_const_kernel<-2, LvlConstSrc<-2>>), dim3(1), dim3(64), 0, s,
+                                   LvlConstSrc<-2>{lv[l - 1]}, lv[l]);
+            }
             hipLaunchKernelGGL(rap_kernel<LvlSrc>, g, dim3(kThreads), 0, s, LvlSrc{lv[l - 1]}, lv[l], cst ? 1 : 0);
-            if (cst)
-                hipLaunchKernelGGL(rap_const_kernel<LvlConstSrc>, dim3(1), dim3(64), 0, s, LvlConstSrc{lv[l - 1]},
-                                   lv[l]);
         }
         STZ_HIP(hipGetLastError());
     }
@@ -1039,7 +1071,7 @@ int BdSolver::setup(const BdTop &t, hipStream_t s) {
             const long ns = 40L * lv[l].m * lv[l].n;
             hipLaunchKernelGGL((convert_kernel<double, float>), blocks(ns), dim3(kThreads), 0, s, lv[l].S, lv32[l].S, ns);
             hipLaunchKernelGGL((convert_kernel<double, float>), dim3(1), dim3(kThreads), 0, s, lv[l].Sc, lv32[l].Sc,
-                               40L);
+                               (long)kSharedStencils);
             STZ_HIP(hipGetLastError());
         }
     return STZ_OK;
