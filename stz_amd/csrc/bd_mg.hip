@@ -692,18 +692,42 @@ __device__ int rap_node(const Src &src, int cm, int cn, int I, int J, double *o)
     return all_interior ? -1 : (all_outside ? -2 : 0);
 }
 
-// Every row's product; a row whose fine rows were all interior rows takes
-// the shared interior stencil, one whose fine rows were all outside rows
-// takes the shared outside stencil if its product equals it bit for bit
-// (near the level's edges the prolongation weights differ); the others
-// store their own.  The shared stencils are formed first (rap_const_kernel).
+// The kind rap_node returns, from the same fine rows, without the product
+template <class Src> __device__ int rap_kind(const Src &src, int cm, int cn, int I, int J) {
+    const int fm = src.m(), fn = src.n();
+    bool all_interior = true, all_outside = true;
+    int c[2];
+    double w[2];
+    for (int fy = 2 * J - 1; fy <= 2 * J + 1; fy++) {
+        if (weight_of(parents(fy, fn, cn, c, w), c, w, J) == 0) continue;
+        for (int fx = 2 * I - 1; fx <= 2 * I + 1; fx++) {
+            if (weight_of(parents(fx, fm, cm, c, w), c, w, I) == 0) continue;
+            const int fij = fx + fm * fy;
+            all_interior = all_interior && src.interior(fij);
+            all_outside = all_outside && src.outside(fij);
+        }
+    }
+    return all_interior ? -1 : (all_outside ? -2 : 0);
+}
+
+// Every row's stencil: a row whose fine rows were all interior rows takes
+// the shared interior stencil (its product is not formed), one whose fine
+// rows were all outside rows takes the shared outside stencil if its
+// product equals it bit for bit (near the level's edges the prolongation
+// weights differ); the others store their own product.  The shared
+// stencils are formed first (rap_const_kernel).
 template <class Src>
 __global__ void __launch_bounds__(kThreads) rap_kernel(Src src, BdLevel C, int shared) {
     const int IJ = blockIdx.x * kThreads + threadIdx.x;
     if (IJ >= C.m * C.n) return;
+    const int I = IJ % C.m, J = IJ / C.m, kind = shared ? rap_kind(src, C.m, C.n, I, J) : 0;
+    if (kind == -1) {  // the shared interior stencil, no product needed
+        C.type[IJ] = -1;
+        return;
+    }
     double o[40];
-    const int k = rap_node(src, C.m, C.n, IJ % C.m, IJ / C.m, o), kind = shared ? k : 0;
-    int type = kind == -1 ? -1 : 0;
+    (void)rap_node(src, C.m, C.n, I, J, o);
+    int type = 0;
     if (kind == -2) {
         bool same = true;
         for (int q = 0; q < 40; q++) same = same && o[q] == C.Sc[40 + q];
