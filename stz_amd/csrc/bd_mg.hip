@@ -266,10 +266,10 @@ __device__ __forceinline__ int parents(int f, int uf, int sc, int *c, double *w)
     c[0] = f / 2; w[0] = 0.5; c[1] = f / 2 + 1; w[1] = 0.5;
     return 2;
 }
+// the weight of parent `target` among the np parents (c, w); written out
+// so that c and w stay in registers
 __device__ __forceinline__ double weight_of(int np, const int *c, const double *w, int target) {
-    for (int k = 0; k < np; k++)
-        if (c[k] == target) return w[k];
-    return 0;
+    return np > 0 && c[0] == target ? w[0] : (np > 1 && c[1] == target ? w[1] : 0);
 }
 
 // b_c = P^T r_f (restriction with the unnormalised weights 1, 1/2, 1/4)
@@ -651,6 +651,7 @@ template <class Src>
 __device__ int rap_node(const Src &src, int cm, int cn, int I, int J, double *o) {
     const int fm = src.m(), fn = src.n();
     double acc[36];
+#pragma unroll
     for (int q = 0; q < 36; q++) acc[q] = 0;
     bool all_interior = true, all_outside = true;
     int c[2], cx[2], cy[2];
@@ -671,17 +672,28 @@ __device__ int rap_node(const Src &src, int cm, int cn, int I, int J, double *o)
                 src.entry(fij, k, A);
                 if (A[0] == 0 && A[1] == 0 && A[2] == 0 && A[3] == 0) continue;
                 const int ny = parents(gy, fn, cn, cy, wy), nx = parents(gx, fm, cm, cx, wx);
-                for (int q = 0; q < ny; q++)
-                    for (int p = 0; p < nx; p++) {
+                // every index below is a compile-time one after unrolling,
+                // so acc stays in registers; each entry's sum runs in the
+                // order of the parents (q, then p)
+#pragma unroll
+                for (int q = 0; q < 2; q++)
+#pragma unroll
+                    for (int p = 0; p < 2; p++) {
+                        if (q >= ny || p >= nx) continue;
                         const int dI = cx[p] - I, dJ = cy[q] - J;
                         if (dI < -1 || dI > 1 || dJ < -1 || dJ > 1) continue;
                         const double ww = wfx * wfy * wx[p] * wy[q];
-                        double *e = acc + 4 * ((dI + 1) + 3 * (dJ + 1));
-                        for (int r = 0; r < 4; r++) e[r] += ww * A[r];
+                        const int blk = (dI + 1) + 3 * (dJ + 1);
+#pragma unroll
+                        for (int bb = 0; bb < 9; bb++)
+                            if (bb == blk)
+#pragma unroll
+                                for (int r = 0; r < 4; r++) acc[4 * bb + r] += ww * A[r];
                     }
             }
         }
     }
+#pragma unroll
     for (int q = 0; q < 36; q++) o[q] = acc[q];
     const double a = acc[16], b = acc[17], cc = acc[18], d = acc[19];
     const double idet = 1. / (a * d - b * cc);
