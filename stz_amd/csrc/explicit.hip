@@ -599,7 +599,13 @@ __device__ __forceinline__ static void correct_body(const SimConst &c, const Fie
         PL(f.u, i, jw) = j == 0 ? -bvel : bvel;
         PL(f.v, i, jw) = 0;
         PL(f.X, i, jw) = j == 0 ? c.ax + i * c.dx + bpos : c.ax + i * c.dx - bpos;
-        PL(f.Y, i, jw) = j == 0 ? c.ay : c.by;
+        // both walls' y in registers before the pick: a select of two
+        // loads from the argument block is otherwise folded into one load
+        // through a selected address, which costs a private (scratch) copy
+        // of the two values in every thread of the launch
+        double ya = c.ay, yb = c.by;
+        asm volatile("" : "+v"(ya), "+v"(yb));
+        PL(f.Y, i, jw) = j == 0 ? ya : yb;
     }
     PL(f.p, i, j) = p;
     PL(f.q, i, j) = q;
