@@ -35,7 +35,18 @@ constexpr int kSharedStencils = 80;
 using BdLevel = BdLevelT<double>;
 
 enum { kR = 0, kRhat, kP, kV, kS, kT, kPhat, kShat, kTmp, kPre, kBdVecs };
-constexpr int kMaxPartial = 4096;
+constexpr int kMaxPartial = 4096;    // workgroups of the grid-stride product kernels
+constexpr int kPartialHalf = 16384;  // partial sums per product (the second at this offset)
+
+// The operator application fused into the top level's last post-smoothing
+// launch: y = A z on every node in FP64, with the sums of (u, y) and, when
+// yy, of (y, y) per tile into partial[tile] and partial[kPartialHalf + tile]
+struct ApplyArgs {
+    double *y;
+    const double *u;
+    double *partial;
+    int yy;
+};
 
 struct BdSolver {
     int top_m = 0, top_n = 0, nl = 0;
@@ -63,8 +74,11 @@ struct BdSolver {
   private:
     int alloc_mixed();
     void free_mixed();
+    // with ap, the last top-level launch also applies the operator where it
+    // can (*tiles: the partial sums it wrote, 0 where it did not)
     template <typename T, typename TB>
-    int vcycle_t(const TB *rhs, T *z, BdLevelT<T> *L, T *r0, T *zpre, hipStream_t s);
+    int vcycle_t(const TB *rhs, T *z, BdLevelT<T> *L, T *r0, T *zpre, hipStream_t s, const ApplyArgs *ap = nullptr,
+                 int *tiles = nullptr);
     template <typename T, typename TB>
     int smooth_top(T *zout, const TB *b, int sweeps, int mode, const T *zin, const T *zc, int cm, int cn,
                    hipStream_t s);

@@ -334,19 +334,50 @@ __global__ void __launch_bounds__(kThreads) prolong_kernel(const T *__restrict__
 // writes.
 // ---------------------------------------------------------------------------
 constexpr int kTileThreads = 1024;
+
+// Sums of a1 and a2 over a workgroup of NT threads in a fixed order (the
+// lanes by shuffles, then the waves in turn) into partial[idx] and
+// partial[kPartialHalf + idx]
+template <int NT> __device__ __forceinline__ void group_sums2(double a1, double a2, double *partial, int idx) {
+    __shared__ double red[2][NT / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        a1 += __shfl_down(a1, o, 64);
+        a2 += __shfl_down(a2, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a1;
+        red[1][threadIdx.x >> 6] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s1 = 0, s2 = 0;
+        for (int w = 0; w < NT / 64; w++) {
+            s1 += red[0][w];
+            s2 += red[1][w];
+        }
+        partial[idx] = s1;
+        partial[kPartialHalf + idx] = s2;
+    }
+}
 // MODE 3: MODE 1 and then the residual on the tile and one node around it
 // (which needs z final two nodes out: a halo of 6) and its restriction to
 // the coarse nodes under the tile, written to the coarse right-hand side
 // cb (tgmg.in.cc:515-551)
-constexpr int halo_of(int mode) { return mode == 3 ? 6 : 4; }
+// MODE 4: MODE 2 and then y = A z on the tile (the top level's operator in
+// FP64, ApplyArgs), which needs z final one node out: a halo of 5
+constexpr int halo_of(int mode) { return mode == 3 ? 6 : (mode == 4 ? 5 : 4); }
+// (a halo of 5 takes 30 rows, so that its region stays within three staged
+// nodes per thread: four held the launch above 64 VGPRs, one tile per CU)
 template <typename T, int kHalo> struct Tile {
-    static constexpr int X = sizeof(T) == 4 ? 64 : 32, Y = 32;
+    static constexpr int X = sizeof(T) == 4 ? 64 : 32, Y = kHalo == 5 ? 30 : 32;
     static constexpr int LW = X + 2 * kHalo, LH = Y + 2 * kHalo, N = LW * LH;
     static constexpr int P = (N + kTileThreads - 1) / kTileThreads;  // staged nodes per thread
-    // the coarse nodes under the tile and its halo start at (x0/2 - 2, y0/2 - 2)
+    // the coarse nodes under the tile and its halo start at x0/2 - (H+1)/2
+    // (the parent of fine node x0 - H) and likewise in y
     static constexpr int CW = LW / 2 + 3, CH = LH / 2 + 3, NC = CW * CH;
     static size_t lds(int mode) {
-        return (4 * (size_t)N + (mode == 2 ? 2 * NC : 0) + kSharedStencils) * sizeof(T) + (size_t)N * sizeof(int);
+        return (4 * (size_t)N + (mode == 2 || mode == 4 ? 2 * NC : 0) + kSharedStencils) * sizeof(T) +
+               (size_t)N * sizeof(int);
     }
 };
 
@@ -391,6 +422,31 @@ template <typename T, typename TB> struct TopRows {
             zs[2 * l] = (bs[2 * l] - o.x) / (T)(sc * t.cxx);
             zs[2 * l + 1] = (bs[2 * l + 1] - o.y) / (T)(sc * t.cyy);
         }
+    }
+    // y = A z in FP64 at staged node l, as apply_top_dot_kernel (the same
+    // sums in the same order; a free row's eight blocks one at a time, which
+    // keeps the launch within 64 VGPRs, two tiles per CU)
+    __device__ void apply(int l, const T *zs, const int *ts, int LW, double &y0, double &y1) const {
+        const int d = ts[l];
+        const T *zp = zs + 2 * l;
+        V2 o;
+        if (d >= 0) {
+            const double *a = t.fmats + 32 * (size_t)d;
+            const int off[8] = {-LW - 1, -LW, -LW + 1, -1, 1, LW - 1, LW, LW + 1};
+            double rx = 0, ry = 0;
+#pragma unroll 1
+            for (int q = 0; q < 8; q++) {
+                const T *zz = zp + 2 * (ptrdiff_t)off[q];
+                rx += a[4 * q] * (double)zz[0] + a[4 * q + 1] * (double)zz[1];
+                ry += a[4 * q + 2] * (double)zz[0] + a[4 * q + 3] * (double)zz[1];
+            }
+            o = V2{rx, ry};
+        } else {
+            o = top_offdiag_at<double, T>(t, zp, LW, d);
+        }
+        const double sc = top_diag_scale(t, d);
+        y0 = o.x + sc * t.cxx * (double)zp[0];
+        y1 = o.y + sc * t.cyy * (double)zp[1];
     }
     // b - A z, as resid_top_kernel
     __device__ V2T<T> resid(int, int, int l, const T *zs, const T *bs, const int *ts, const T *, int LW) const {
@@ -446,19 +502,20 @@ template <typename T> struct LevelRows {
 };
 
 template <typename T, int MODE, class Rows>
-__global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const T *__restrict__ z,
+__device__ __forceinline__ static void gs_tile_body(Rows rows, const T *__restrict__ z,
                                                                T *__restrict__ zout, const T *__restrict__ zc,
-                                                               T *__restrict__ cb, int cm, int cn) {
+                                                               T *__restrict__ cb, int cm, int cn, ApplyArgs ap) {
     constexpr int kHalo = halo_of(MODE);
     using D = Tile<T, kHalo>;
     using TB = typename Rows::Src;
     constexpr int P = D::P;
     extern __shared__ __align__(16) unsigned char bd_tile_smem[];
     T *zs = reinterpret_cast<T *>(bd_tile_smem), *bs = zs + 2 * D::N, *cs = bs + 2 * D::N,
-      *sc = cs + (MODE == 2 ? 2 * D::NC : 0);
+      *sc = cs + (MODE == 2 || MODE == 4 ? 2 * D::NC : 0);
     int *ts = reinterpret_cast<int *>(sc + kSharedStencils);
     const int m = rows.m(), n = rows.n(), tid = threadIdx.x;
-    const int x0 = blockIdx.x * D::X, y0 = blockIdx.y * D::Y, cx0 = x0 / 2 - 2, cy0 = y0 / 2 - 2;
+    const int x0 = blockIdx.x * D::X, y0 = blockIdx.y * D::Y, cx0 = x0 / 2 - (kHalo + 1) / 2,
+              cy0 = y0 / 2 - (kHalo + 1) / 2;
     const TB *b = rows.rhs();
     const int *tp = rows.types();
     // staged nodes q = tid + k kTileThreads; every load of a thread is issued
@@ -471,7 +528,7 @@ __global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const 
         const int q = tid + k * kTileThreads, i = x0 - kHalo + q % D::LW, j = y0 - kHalo + q / D::LW;
         const bool in = q < D::N && i >= 0 && i < m && j >= 0 && j < n;
         const size_t ij = in ? i + (size_t)m * j : 0;
-        if (MODE == 0 || MODE == 2) {
+        if (MODE == 0 || MODE == 2 || MODE == 4) {
             const T a0 = z[2 * ij], a1 = z[2 * ij + 1];
             vx[k] = in ? a0 : (T)0;
             vy[k] = in ? a1 : (T)0;
@@ -484,7 +541,7 @@ __global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const 
         by[k] = in ? b1 : (TB)0;
         tt[k] = in ? t0 : Rows::kOutside;
     }
-    if (MODE == 2) {
+    if (MODE == 2 || MODE == 4) {
         constexpr int PC = (D::NC + kTileThreads - 1) / kTileThreads;
         T cx[PC], cy[PC];
 #pragma unroll
@@ -512,7 +569,7 @@ __global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const 
         const int q = tid + k * kTileThreads, i = x0 - kHalo + q % D::LW, j = y0 - kHalo + q / D::LW;
         if (q >= D::N) continue;
         T wxv = vx[k], wyv = vy[k];
-        if (MODE == 2 && i >= 0 && i < m && j >= 0 && j < n) {  // as prolong_kernel
+        if ((MODE == 2 || MODE == 4) && i >= 0 && i < m && j >= 0 && j < n) {  // as prolong_kernel
             int px[2], py[2];
             double wx[2], wy[2];
             const int nx = parents(i, m, cm, px, wx), ny = parents(j, n, cn, py, wy);
@@ -545,12 +602,52 @@ __global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const 
         }
         __syncthreads();
     }
+    double a1 = 0, a2 = 0;  // MODE 4: the tile's sums of (u, y) and (y, y)
     for (int q = tid; q < D::X * D::Y; q += kTileThreads) {
         const int i = x0 + q % D::X, j = y0 + q / D::X;
         if (i >= m || j >= n) continue;
         const int l = (i - x0 + kHalo) + D::LW * (j - y0 + kHalo);
-        zout[2 * (i + (size_t)m * j)] = zs[2 * l];
-        zout[2 * (i + (size_t)m * j) + 1] = zs[2 * l + 1];
+        const size_t k = 2 * (i + (size_t)m * j);
+        zout[k] = zs[2 * l];
+        zout[k + 1] = zs[2 * l + 1];
+    }
+    if constexpr (MODE == 4) {
+#pragma unroll 1
+        for (int q = tid; q < D::X * D::Y; q += kTileThreads) {
+            const int i = x0 + q % D::X, j = y0 + q / D::X;
+            if (i >= m || j >= n) continue;
+            const int l = (i - x0 + kHalo) + D::LW * (j - y0 + kHalo);
+            const size_t k = 2 * (i + (size_t)m * j);
+            double y0v, y1v;
+            rows.apply(l, zs, ts, D::LW, y0v, y1v);
+            ap.y[k] = y0v;
+            ap.y[k + 1] = y1v;
+            a1 += ap.u[k] * y0v + ap.u[k + 1] * y1v;
+            if (ap.yy) a2 += y0v * y0v + y1v * y1v;
+        }
+        // the tile's sums in a fixed order (lanes by shuffles, then the
+        // waves in turn) through the right-hand side's LDS, which no pass
+        // reads any more
+        for (int o = 32; o > 0; o >>= 1) {
+            a1 += __shfl_down(a1, o, 64);
+            a2 += __shfl_down(a2, o, 64);
+        }
+        double *red = reinterpret_cast<double *>(bs);
+        if ((tid & 63) == 0) {
+            red[tid >> 6] = a1;
+            red[kTileThreads / 64 + (tid >> 6)] = a2;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double s1 = 0, s2 = 0;
+            for (int w = 0; w < kTileThreads / 64; w++) {
+                s1 += red[w];
+                s2 += red[kTileThreads / 64 + w];
+            }
+            const int tile = blockIdx.x + gridDim.x * blockIdx.y;
+            ap.partial[tile] = s1;
+            ap.partial[kPartialHalf + tile] = s2;
+        }
     }
     if (MODE != 3) return;
     // the residual on the tile and one node around it, over b in LDS (a
@@ -592,6 +689,23 @@ __global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const 
         cb[2 * (I + (size_t)cm * J)] = sx;
         cb[2 * (I + (size_t)cm * J) + 1] = sy;
     }
+}
+
+template <typename T, int MODE, class Rows>
+__global__ void __launch_bounds__(kTileThreads) gs_tile_kernel(Rows rows, const T *__restrict__ z,
+                                                               T *__restrict__ zout, const T *__restrict__ zc,
+                                                               T *__restrict__ cb, int cm, int cn, ApplyArgs ap) {
+    gs_tile_body<T, MODE, Rows>(rows, z, zout, zc, cb, cm, cn, ap);
+}
+// MODE 4 (the top level's last post-smoothing sweep with the operator
+// application).  Two tiles per CU need 64 VGPRs or fewer (eight waves per
+// SIMD); the hint keeps the compiler's schedule within that (104 without
+// it, one tile per CU)
+template <typename T, class Rows>
+__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
+gs_tile_apply_kernel(Rows rows, const T *__restrict__ z, T *__restrict__ zout, const T *__restrict__ zc,
+                     T *__restrict__ cb, int cm, int cn, ApplyArgs ap) {
+    gs_tile_body<T, 4, Rows>(rows, z, zout, zc, cb, cm, cn, ap);
 }
 
 // Sources of fine-level entries for the Galerkin product: the top operator,
@@ -815,27 +929,9 @@ __global__ void __launch_bounds__(kThreads) finish_dot_kernel(const double *__re
 }
 
 // Products fused into the kernels that form one of their factors.  Block
-// sums go to partial[block] and partial[kMaxPartial + block].
+// sums go to partial[block] and partial[kPartialHalf + block].
 __device__ __forceinline__ void block_sums2(double a1, double a2, double *partial) {
-    __shared__ double red[2][kThreads / 64];
-    for (int o = 32; o > 0; o >>= 1) {
-        a1 += __shfl_down(a1, o, 64);
-        a2 += __shfl_down(a2, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = a1;
-        red[1][threadIdx.x >> 6] = a2;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s1 = 0, s2 = 0;
-        for (int w = 0; w < kThreads / 64; w++) {
-            s1 += red[0][w];
-            s2 += red[1][w];
-        }
-        partial[blockIdx.x] = s1;
-        partial[kMaxPartial + blockIdx.x] = s2;
-    }
+    group_sums2<kThreads>(a1, a2, partial, blockIdx.x);
 }
 
 // y = A x on the top level in FP64 (x FP64, or the mixed form's FP32
@@ -871,7 +967,7 @@ __global__ void __launch_bounds__(kThreads) finish_dots_kernel(const double *__r
     double a1 = 0, a2 = 0;
     for (int q = threadIdx.x; q < np; q += kThreads) {
         a1 += partial[q];
-        a2 += partial[kMaxPartial + q];
+        a2 += partial[kPartialHalf + q];
     }
     for (int o = 32; o > 0; o >>= 1) {
         a1 += __shfl_down(a1, o, 64);
@@ -988,7 +1084,7 @@ int BdSolver::init(int tm, int tn) {
     nl = dims_for(tm, tn, d);
     const size_t nv = 2 * (size_t)tm * tn;
     for (int k = 0; k < kBdVecs; k++) STZ_HIP(hipMalloc(&vec[k], nv * sizeof(double)));
-    STZ_HIP(hipMalloc(&partial, 2 * (size_t)kMaxPartial * sizeof(double)));
+    STZ_HIP(hipMalloc(&partial, 2 * (size_t)kPartialHalf * sizeof(double)));
     STZ_HIP(hipMalloc(&sc, 16 * sizeof(double)));
     STZ_HIP(hipHostMalloc((void **)&hscal, 4 * sizeof(double), hipHostMallocCoherent));
     for (int l = 0; l < nl; l++) {
@@ -1120,14 +1216,16 @@ int BdSolver::setup(const BdTop &t, hipStream_t s) {
 // kernel's limit raised once where that is above 64 KiB)
 template <typename T, int MODE, class Rows>
 static int tile_launch(dim3 g, size_t lds, hipStream_t s, const Rows &rows, const T *zin, T *zout, const T *zc,
-                       T *cb, int cm, int cn) {
+                       T *cb, int cm, int cn, const ApplyArgs &ap = ApplyArgs{}) {
+    void (*kern)(Rows, const T *, T *, const T *, T *, int, int, ApplyArgs);
+    if constexpr (MODE == 4) kern = &gs_tile_apply_kernel<T, Rows>;
+    else kern = &gs_tile_kernel<T, MODE, Rows>;
     static const hipError_t lim =
-        lds > 65536 ? hipFuncSetAttribute(reinterpret_cast<const void *>(&gs_tile_kernel<T, MODE, Rows>),
+        lds > 65536 ? hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
                     : hipSuccess;
     STZ_HIP(lim);
-    hipLaunchKernelGGL((gs_tile_kernel<T, MODE, Rows>), g, dim3(kTileThreads), lds, s, rows, zin, zout, zc, cb, cm,
-                       cn);
+    hipLaunchKernelGGL(kern, g, dim3(kTileThreads), lds, s, rows, zin, zout, zc, cb, cm, cn, ap);
     STZ_HIP(hipGetLastError());
     return STZ_OK;
 }
@@ -1141,6 +1239,18 @@ static int tile_sweep(const Rows &rows, int m, int n, const T *zin, T *zout, int
     if (mode == 1) return tile_launch<T, 1>(g, D::lds(1), s, rows, zin, zout, zc, cb, cm, cn);
     if (mode == 2) return tile_launch<T, 2>(g, D::lds(2), s, rows, zin, zout, zc, cb, cm, cn);
     return tile_launch<T, 0>(g, D::lds(0), s, rows, zin, zout, zc, cb, cm, cn);
+}
+
+// the top level's last post-smoothing sweep (mode 2) with the operator
+// application fused (mode 4); returns the partial sums written through
+// *tiles
+template <typename T, typename TB>
+static int tile_sweep_apply(const TopRows<T, TB> &rows, int m, int n, const T *zin, T *zout, const T *zc, int cm,
+                            int cn, const ApplyArgs &ap, int *tiles, hipStream_t s) {
+    using D = Tile<T, 5>;
+    const dim3 g((m + D::X - 1) / D::X, (n + D::Y - 1) / D::Y);
+    *tiles = (int)(g.x * g.y);
+    return tile_launch<T, 4>(g, Tile<T, 5>::lds(4), s, rows, zin, zout, zc, (T *)nullptr, cm, cn, ap);
 }
 
 // the start of a smoothing that the tiled kernel does not take, in zout:
@@ -1215,9 +1325,11 @@ int BdSolver::smooth_level(const BdLevelT<T> &L, bool bottom, int sweeps, int mo
 // and its post-smoothed one in r (the residual it held is restricted by
 // then); the bottom level solves in z.
 template <typename T, typename TB>
-int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, T *zpre, hipStream_t s) {
+int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, T *zpre, hipStream_t s, const ApplyArgs *ap,
+                       int *tiles) {
     const T *none = nullptr;
     int e;
+    if (tiles) *tiles = 0;
     if (nl == 0) {
         if ((e = smooth_top(zpre, rhs, nu1, 1, none, none, 0, 0, s))) return e;
         return smooth_top(z, rhs, nu2, 0, (const T *)zpre, none, 0, 0, s);
@@ -1253,6 +1365,15 @@ int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, T *zpre, hip
         const BdLevelT<T> &C = Lv[l], &F = Lv[l - 1];
         if ((e = smooth_level(F, false, nu2, 2, (const T *)F.z, F.r, res, C.m, C.n, s))) return e;
         res = F.r;
+    }
+    // the operator application fused into the last launch: FP32 tiles, one
+    // sweep, a partial sum per tile within the buffer
+    if constexpr (sizeof(T) == 4) {
+        using D = Tile<T, 5>;
+        const long nt = (long)((top_m + D::X - 1) / D::X) * ((top_n + D::Y - 1) / D::Y);
+        if (ap && tiles && nu2 == 1 && nt <= kPartialHalf)
+            return tile_sweep_apply(TopRows<T, TB>::make(top, rhs), top_m, top_n, (const T *)zpre, z, res, Lv[0].m,
+                                    Lv[0].n, *ap, tiles, s);
     }
     return smooth_top(z, rhs, nu2, 2, (const T *)zpre, res, Lv[0].m, Lv[0].n, s);
 }
@@ -1340,24 +1461,32 @@ int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_so
         // p^ = M p: FP64, or (mixed) an FP32 V-cycle reading p in FP64 and
         // leaving p^ in FP32 for the FP64 kernels that use it
         // v = A p^ with (r_hat, v), alpha
+        int np = nd;  // partial sums of the product
         if (mixed) {
-            if ((e = vcycle_t<float, double>(p, top32[0], lv32, top32[2], top32[3], s))) return e;
-            hipLaunchKernelGGL(apply_top_dot_kernel<float>, gd, b1, 0, s, top, top32[0], v, rh, 0, partial);
+            const ApplyArgs ap{v, rh, partial, 0};
+            int tiles = 0;
+            if ((e = vcycle_t<float, double>(p, top32[0], lv32, top32[2], top32[3], s, &ap, &tiles))) return e;
+            if (tiles) np = tiles;
+            else hipLaunchKernelGGL(apply_top_dot_kernel<float>, gd, b1, 0, s, top, top32[0], v, rh, 0, partial);
         } else {
             if ((e = vcycle(p, ph, s))) return e;
             hipLaunchKernelGGL(apply_top_dot_kernel<double>, gd, b1, 0, s, top, ph, v, rh, 0, partial);
         }
-        hipLaunchKernelGGL(finish_dots_kernel, dim3(1), b1, 0, s, partial, nd, sc, 4, -1, 2, nullptr, nullptr);
+        hipLaunchKernelGGL(finish_dots_kernel, dim3(1), b1, 0, s, partial, np, sc, 4, -1, 2, nullptr, nullptr);
         hipLaunchKernelGGL(bicg_s_kernel, g, b1, 0, s, sv, r, v, sc, n2);
         // t = A s^ with (t, s) and (t, t), omega
+        np = nd;
         if (mixed) {
-            if ((e = vcycle_t<float, double>(sv, top32[1], lv32, top32[2], top32[3], s))) return e;
-            hipLaunchKernelGGL(apply_top_dot_kernel<float>, gd, b1, 0, s, top, top32[1], t, sv, 1, partial);
+            const ApplyArgs ap{t, sv, partial, 1};
+            int tiles = 0;
+            if ((e = vcycle_t<float, double>(sv, top32[1], lv32, top32[2], top32[3], s, &ap, &tiles))) return e;
+            if (tiles) np = tiles;
+            else hipLaunchKernelGGL(apply_top_dot_kernel<float>, gd, b1, 0, s, top, top32[1], t, sv, 1, partial);
         } else {
             if ((e = vcycle(sv, sh, s))) return e;
             hipLaunchKernelGGL(apply_top_dot_kernel<double>, gd, b1, 0, s, top, sh, t, sv, 1, partial);
         }
-        hipLaunchKernelGGL(finish_dots_kernel, dim3(1), b1, 0, s, partial, nd, sc, 5, 6, 3, nullptr, nullptr);
+        hipLaunchKernelGGL(finish_dots_kernel, dim3(1), b1, 0, s, partial, np, sc, 5, 6, 3, nullptr, nullptr);
         // x, r with (r, r) -> host[0] and the next rho = (r_hat, r) -> sc[0], host[1]
         if (mixed)
             hipLaunchKernelGGL(bicg_x_dot_kernel<float>, gx, b1, 0, s, x, r, top32[0], top32[1], sv, t, rh, sc, n2,
