@@ -36,7 +36,7 @@ using BdLevel = BdLevelT<double>;
 
 enum { kR = 0, kRhat, kP, kV, kS, kT, kPhat, kShat, kTmp, kPre, kBdVecs };
 constexpr int kMaxPartial = 4096;    // workgroups of the grid-stride product kernels
-constexpr int kPartialHalf = 16384;  // partial sums per product (the second at this offset)
+constexpr int kPartialHalf = 32768;  // partial sums per product (the second at this offset)
 
 // The operator application fused into the top level's last post-smoothing
 // launch: y = A z on every node in FP64, with the sums of (u, y) and, when

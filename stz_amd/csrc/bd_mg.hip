@@ -1366,9 +1366,9 @@ int BdSolver::vcycle_t(const TB *rhs, T *z, BdLevelT<T> *Lv, T *r0, T *zpre, hip
         if ((e = smooth_level(F, false, nu2, 2, (const T *)F.z, F.r, res, C.m, C.n, s))) return e;
         res = F.r;
     }
-    // the operator application fused into the last launch: FP32 tiles, one
-    // sweep, a partial sum per tile within the buffer
-    if constexpr (sizeof(T) == 4) {
+    // the operator application fused into the last launch: one sweep, a
+    // partial sum per tile within the buffer
+    {
         using D = Tile<T, 5>;
         const long nt = (long)((top_m + D::X - 1) / D::X) * ((top_n + D::Y - 1) / D::Y);
         if (ap && tiles && nu2 == 1 && nt <= kPartialHalf)
@@ -1469,8 +1469,11 @@ int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_so
             if (tiles) np = tiles;
             else hipLaunchKernelGGL(apply_top_dot_kernel<float>, gd, b1, 0, s, top, top32[0], v, rh, 0, partial);
         } else {
-            if ((e = vcycle(p, ph, s))) return e;
-            hipLaunchKernelGGL(apply_top_dot_kernel<double>, gd, b1, 0, s, top, ph, v, rh, 0, partial);
+            const ApplyArgs ap{v, rh, partial, 0};
+            int tiles = 0;
+            if ((e = vcycle_t<double, double>(p, ph, lv, vec[kTmp], vec[kPre], s, &ap, &tiles))) return e;
+            if (tiles) np = tiles;
+            else hipLaunchKernelGGL(apply_top_dot_kernel<double>, gd, b1, 0, s, top, ph, v, rh, 0, partial);
         }
         hipLaunchKernelGGL(finish_dots_kernel, dim3(1), b1, 0, s, partial, np, sc, 4, -1, 2, nullptr, nullptr);
         hipLaunchKernelGGL(bicg_s_kernel, g, b1, 0, s, sv, r, v, sc, n2);
@@ -1483,8 +1486,11 @@ int BdSolver::solve(double *x, const double *b, double acc, int max_iter, stz_so
             if (tiles) np = tiles;
             else hipLaunchKernelGGL(apply_top_dot_kernel<float>, gd, b1, 0, s, top, top32[1], t, sv, 1, partial);
         } else {
-            if ((e = vcycle(sv, sh, s))) return e;
-            hipLaunchKernelGGL(apply_top_dot_kernel<double>, gd, b1, 0, s, top, sh, t, sv, 1, partial);
+            const ApplyArgs ap{t, sv, partial, 1};
+            int tiles = 0;
+            if ((e = vcycle_t<double, double>(sv, sh, lv, vec[kTmp], vec[kPre], s, &ap, &tiles))) return e;
+            if (tiles) np = tiles;
+            else hipLaunchKernelGGL(apply_top_dot_kernel<double>, gd, b1, 0, s, top, sh, t, sv, 1, partial);
         }
         hipLaunchKernelGGL(finish_dots_kernel, dim3(1), b1, 0, s, partial, np, sc, 5, 6, 3, nullptr, nullptr);
         // x, r with (r, r) -> host[0] and the next rho = (r_hat, r) -> sc[0], host[1]
